@@ -1,0 +1,183 @@
+/*
+ * rbhip -- C ABI of the MI355X (gfx950) FEM hot path behind redbirdpy's API.
+ *
+ * Conventions
+ *  - Every function returns an int status: 0 ok, < 0 error (message from
+ *    rb_last_error(), thread-local).  No C++ exception crosses the ABI.
+ *  - Pointers are device pointers unless a parameter is documented as host.
+ *    All buffers are owned by the caller; indices are int32 and 0-based.
+ *  - `stream` is a hipStream_t passed as void*; work is stream-ordered.
+ *  - Real values are float64 / float32; complex values are interleaved
+ *    (re, im) pairs of float64 (complex128) or float32 (complex64).
+ *
+ * The reference (fangq/redbirdpy v0.2.0) is pure Python with no FFI; the
+ * functions below replace the compute inside the Python functions cited on
+ * each declaration.  The Python host mirror (redbirdpy_amd) keeps the
+ * reference's signatures and calls these through ctypes.
+ */
+#ifndef RBHIP_H
+#define RBHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RB_ABI_VERSION 1
+
+/* property layouts for rb_assemble */
+#define RB_PROP_ELEM 0   /* one row per element (label tables are expanded by the caller) */
+#define RB_PROP_NODE 1   /* one row per node */
+#define RB_PROP_VOLUME 2 /* no optics: every tet pair gets V_e (the Jacobian patch weights) */
+
+/* precision for rb_krylov_solve */
+#define RB_PREC_FP64 0  /* all arithmetic in float64 / complex128 */
+#define RB_PREC_MIXED 1 /* float32 / complex64 inner solves + float64 residual refinement */
+
+/* dtype codes */
+#define RB_F64 0
+#define RB_C128 1
+#define RB_F32 2
+#define RB_C64 3
+
+int rb_version(void);
+const char* rb_last_error(void);
+int rb_device_count(int* n /* host */);
+
+/* Zero `bytes` bytes with a fill kernel (never hipMemset). */
+int rb_zero(void* p, size_t bytes, void* stream);
+
+/*
+ * Node -> element incidence (replaces the per-node loops of
+ * redbirdpy utility.py:1005-1041 `_nodevolume` / `_femnz`).
+ * el: ne x nv (0-based); inc_ptr: nn+1; inc_list: ne*nv entries e*nv+local,
+ * ascending within each node.
+ */
+int rb_incidence(const int32_t* el, int64_t ne, int nv, int64_t nn, int32_t* inc_ptr, int32_t* inc_list,
+                 void* stream);
+
+/*
+ * CSR pattern of the FEM matrix (replaces the COO->CSR conversion of
+ * redbirdpy forward.py:249-254 and `_femnz`, utility.py:1020-1041):
+ * row n lists, ascending, every node sharing a tet with n.  Call with
+ * col_idx == NULL to get *nnz_out (host) only.  diag_slot (nn, optional)
+ * receives the slot of each diagonal entry.
+ */
+int rb_csr_pattern(const int32_t* el, int64_t ne, int64_t nn, const int32_t* inc_ptr, const int32_t* inc_list,
+                   int32_t* row_ptr, int32_t* col_idx, int64_t col_cap, int64_t* nnz_out /* host */,
+                   int32_t* diag_slot, void* stream);
+
+/*
+ * Assembly slot map: for each CSR slot, the ascending list of contributing
+ * (tet, i, j) / (face, i, j) codes; fixes the summation order of
+ * rb_assemble.  tinc/finc are the tet (nv=4) and face (nv=3) incidences.
+ */
+int rb_slot_map(const int32_t* el, int64_t ne, const int32_t* fc, int64_t nf, int64_t nn, const int32_t* tinc_ptr,
+                const int32_t* tinc_list, const int32_t* finc_ptr, const int32_t* finc_list, const int32_t* row_ptr,
+                const int32_t* col_idx, int64_t nnz, int32_t* slot_ptr, int32_t* codes, void* stream);
+
+/*
+ * FEM system matrix values for nsys systems sharing the pattern
+ * (redbirdpy forward.py:90-256 `femlhs`, element branch :156-176, node
+ * branch :177-228, Robin boundary :230-246).  mua/musp/nref: nsys x plen
+ * (musp already mus*(1-g)); nref rows have length nlen (1 = one value per
+ * system).  omega, bc: nsys each, bc = (1-Reff)/(12(1+Reff)).  out64 (and
+ * optional out32) receive nsys x nnz values, complex when is_complex.
+ */
+int rb_assemble(const int32_t* slot_ptr, const int32_t* codes, int64_t nnz, const int32_t* el, int64_t ne,
+                const int32_t* fc, int64_t nf, int64_t nn, const double* deldotdel, const double* evol,
+                const double* area, int nsys, int kind, const double* mua, const double* musp, const double* nref,
+                int64_t plen, int64_t nlen, const double* omega, const double* bc, int is_complex, void* out64,
+                void* out32, void* stream);
+
+/*
+ * Point location (iso2mesh.tsearchn as called by redbirdpy
+ * forward.py:316, :337): lowest-index tet with all barycentric
+ * coordinates >= -tol.  node: nn x 3, pts: npts x 3; loc: npts (INT32_MAX
+ * when outside), bary: npts x 4 (NaN when outside).
+ */
+int rb_tsearch(const double* node, const int32_t* el, int64_t ne, const double* pts, int npts, double tol,
+               int32_t* loc, double* bary, void* stream);
+
+/*
+ * Point-optode RHS columns (redbirdpy forward.py:314-348): B is a zeroed
+ * dense n x ld block (row-major); column cols[p] gets bary[p] at the four
+ * nodes of tet loc[p].
+ */
+int rb_rhs_points(const int32_t* el, const int32_t* loc, const double* bary, int npts, const int32_t* cols,
+                  int64_t ld, int is_complex, double* B, void* stream);
+
+/*
+ * Multi-RHS Krylov solve (replaces redbirdpy solver.py:385-588 `femsolve`,
+ * whose default here is SuperLU): Jacobi CG (real) / COCG (complex
+ * symmetric) per column over nsys systems x k columns, B and X row-major
+ * n x (nsys*k).  precision: RB_PREC_FP64 or RB_PREC_MIXED (needs vals32).
+ * Stops column c when ||b_c - A x_c|| <= tol ||b_c||; zero columns give 0.
+ * Host outputs: iterations, max relative residual, flag (0 converged,
+ * 1 max iterations, 4 breakdown).  work: rb_krylov_workspace() bytes.
+ */
+int rb_krylov_workspace(int64_t n, int nsys, int k, int is_complex, int precision, size_t* bytes /* host */);
+int rb_krylov_solve(int64_t n, int64_t nnz, const int32_t* row_ptr, const int32_t* col_idx,
+                    const int32_t* diag_slot, int nsys, int k, int is_complex, const void* vals64,
+                    const void* vals32, const void* B, void* X, int precision, double tol, int maxiter, int use_x0,
+                    int check_every, void* work, size_t work_bytes, int* iters_out /* host */,
+                    double* relres_out /* host */, int* flag_out /* host */, void* stream);
+
+/* Y = A_s X for each system s (dtype RB_F64/RB_C128/RB_F32/RB_C64). */
+int rb_spmm(int64_t n, int64_t nnz, const int32_t* row_ptr, const int32_t* col_idx, int nsys, int k, int dtype,
+            const void* vals, const void* X, void* Y, void* stream);
+
+/*
+ * Detector readout (redbirdpy forward.py:365-442 `femgetdet`):
+ * out[d][s] = sum over the sparse detector column d (det_ptr/det_row/
+ * det_val, CSC, real weights) of w * phi[row][src0 + s].  phi is n x ldphi.
+ */
+int rb_getdet(int ndet, const int32_t* det_ptr, const int32_t* det_row, const double* det_val, const void* phi,
+              int64_t ldphi, int src0, int nsrc, int is_complex, void* out, void* stream);
+
+/*
+ * mua Jacobian on a source x detector grid (redbirdpy forward.py:509-569
+ * `jac`, numpy branch :530-567).  For source column scol[s] and detector
+ * column dcol[d] of phi (n x ldphi), cell = d*Ns + s, output row
+ * p = pmap ? pmap[cell] : cell (pmap < 0 skips inactive pairs, as the
+ * reference drops sd rows with active == 0):
+ *   Jnode[n][p] = -0.0125 * Phi_S^T Q_n Phi_D,  Q_n = sum_{e ni n} V_e (I + 1 1^T)_e
+ * which equals 0.25 * sum_{e ni n} Jelem[p][e] with the reference's
+ * Jelem = -0.05 V_e (sum_i ps_i pd_i + S_s S_d).  JT: nn x ldj node-major;
+ * out_dtype RB_F64/RB_F32 (real phi) or RB_C128/RB_C64 (complex phi).
+ * Optional Jelem: ne x ldj element-major.  maxnp = longest CSR row.
+ */
+int rb_jac_grid(int64_t nn, int64_t ne, const int32_t* el, const double* evol, const int32_t* inc_ptr,
+                const int32_t* inc_list, const int32_t* row_ptr, const int32_t* col_idx, int maxnp,
+                const void* phi, int64_t ldphi, int is_complex, int Ns, int Nd, const int32_t* scol,
+                const int32_t* dcol, const int32_t* pmap, void* JT, int64_t ldj, int out_dtype, void* Jelem,
+                void* stream);
+
+/*
+ * Jacobian contraction g[n] = sum_{d,s} w[d*Ns+s] Jnode[n][(s,d)] without
+ * forming J: the backward of the forward map (transpose of redbirdpy
+ * `jac`, forward.py:509-569).  psi_work: nn x Ns, h_work: ne scalars of the
+ * phi type; g: nn (float64 or complex128).
+ */
+int rb_jac_contract_grid(int64_t nn, int64_t ne, const int32_t* el, const double* evol, const int32_t* inc_ptr,
+                         const int32_t* inc_list, const void* phi, int64_t ldphi, int is_complex, int Ns, int Nd,
+                         const int32_t* scol, const int32_t* dcol, const void* w, void* psi_work, void* h_work,
+                         void* g, void* stream);
+
+/*
+ * Gauss-Newton Gram G = diag(d) (J^T)^T J^T diag(d) for JT: K x m (row-major,
+ * redbirdpy recon.py:474 `Amat @ Amat.T`), G: m x m float64.
+ * mode: 0 float64 FMA, 1 float32 MFMA, 2 bf16 MFMA (split in 3 products:
+ * hi*hi + hi*lo + lo*hi), each with float32 accumulation per K-tile and
+ * float64 accumulation across tiles.  in_dtype RB_F64 or RB_F32.
+ * d (m, optional): row scaling applied before the products.
+ */
+int rb_gram(int64_t K, int64_t m, const void* JT, int64_t ldj, int in_dtype, const double* d, int mode, double* G,
+            void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,127 @@
+"""ctypes binding of librbhip.so (the C ABI declared in include/rbhip.h).
+
+The library is built in-tree by ``__graft_entry__.build()``.  There is no
+fallback: if the library or a GPU is missing, every compute entry point
+raises.
+"""
+
+import ctypes
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librbhip.so")
+
+RB_PROP_ELEM = 0
+RB_PROP_NODE = 1
+RB_PROP_VOLUME = 2
+RB_PREC_FP64 = 0
+RB_PREC_MIXED = 1
+RB_F64, RB_C128, RB_F32, RB_C64 = 0, 1, 2, 3
+
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_int = ctypes.c_int
+_dbl = ctypes.c_double
+_vp = ctypes.c_void_p
+_sz = ctypes.c_size_t
+
+# name -> argtypes (every function returns int unless listed in _RESTYPE)
+_SIGS = {
+    "rb_version": [],
+    "rb_last_error": [],
+    "rb_device_count": [ctypes.POINTER(_int)],
+    "rb_zero": [_vp, _sz, _vp],
+    "rb_incidence": [_vp, _i64, _int, _i64, _vp, _vp, _vp],
+    "rb_csr_pattern": [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _i64, ctypes.POINTER(_i64), _vp, _vp],
+    "rb_slot_map": [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp],
+    "rb_assemble": [_vp, _vp, _i64, _vp, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _int, _int, _vp, _vp, _vp,
+                    _i64, _i64, _vp, _vp, _int, _vp, _vp, _vp],
+    "rb_tsearch": [_vp, _vp, _i64, _vp, _int, _dbl, _vp, _vp, _vp],
+    "rb_rhs_points": [_vp, _vp, _vp, _int, _vp, _i64, _int, _vp, _vp],
+    "rb_krylov_workspace": [_i64, _int, _int, _int, _int, ctypes.POINTER(_sz)],
+    "rb_krylov_solve": [_i64, _i64, _vp, _vp, _vp, _int, _int, _int, _vp, _vp, _vp, _vp, _int, _dbl, _int,
+                        _int, _int, _vp, _sz, ctypes.POINTER(_int), ctypes.POINTER(_dbl), ctypes.POINTER(_int),
+                        _vp],
+    "rb_spmm": [_i64, _i64, _vp, _vp, _int, _int, _int, _vp, _vp, _vp, _vp],
+    "rb_getdet": [_int, _vp, _vp, _vp, _vp, _i64, _int, _int, _int, _vp, _vp],
+    "rb_jac_grid": [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _int, _vp, _i64, _int, _int, _int, _vp, _vp,
+                    _vp, _vp, _i64, _int, _vp, _vp],
+    "rb_jac_contract_grid": [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _int, _int, _int, _vp, _vp, _vp, _vp,
+                             _vp, _vp, _vp],
+    "rb_gram": [_i64, _i64, _vp, _i64, _int, _vp, _int, _vp, _vp],
+}
+_RESTYPE = {"rb_last_error": ctypes.c_char_p}
+
+_lib = None
+
+
+def exported_names():
+    """Names of every entry point this binding expects the library to export."""
+    return sorted(_SIGS)
+
+
+def load(path=LIB_PATH):
+    """Load the library and bind signatures (no GPU needed)."""
+    global _lib
+    if _lib is not None and path == LIB_PATH:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            "librbhip.so is not built (%s); run __graft_entry__.build() first" % path
+        )
+    L = ctypes.CDLL(path)
+    for name, args in _SIGS.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = _RESTYPE.get(name, _int)
+    if path == LIB_PATH:
+        _lib = L
+    return L
+
+
+def lib():
+    return load()
+
+
+class RbError(RuntimeError):
+    pass
+
+
+def check(rc, name):
+    if rc != 0:
+        msg = lib().rb_last_error()
+        raise RbError("%s failed (%d): %s" % (name, rc, msg.decode() if msg else ""))
+
+
+def call(name, *args):
+    """Call an entry point and raise RbError on a nonzero status."""
+    rc = getattr(lib(), name)(*args)
+    check(rc, name)
+    return rc
+
+
+def ptr(t):
+    """Device pointer of a tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream_ptr(device=None):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+_checked = False
+
+
+def require_gpu():
+    """Raise unless a HIP device is visible and the library is loadable."""
+    global _checked
+    if _checked:
+        return
+    lib()
+    if not torch.cuda.is_available():
+        raise RbError("redbirdpy_amd needs a HIP GPU (torch.cuda.is_available() is False)")
+    _checked = True

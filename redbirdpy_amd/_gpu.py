@@ -1,0 +1,427 @@
+"""Device-resident engine: mesh structures and the hot-path calls on torch tensors.
+
+Everything here runs through librbhip (hand-written HIP kernels); torch only
+allocates device memory and provides the stream.  The public redbirdpy-
+compatible functions in ``forward``/``solver``/``recon`` convert numpy/scipy
+inputs at their boundary and call into this module.
+"""
+
+import ctypes
+import numpy as np
+import torch
+
+from . import _abi
+from ._abi import ptr, call
+
+C0 = 299792458000.0  # mm/s (redbirdpy forward.py:36)
+
+
+def device():
+    _abi.require_gpu()
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _i32(a, dev):
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32), device=dev)
+
+
+def _f64(a, dev):
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=dev)
+
+
+class GpuMesh:
+    """Device copies of a tet mesh plus its CSR pattern and assembly slot map.
+
+    elem/face are taken 1-based (redbirdpy convention, __init__.py:13-15) and
+    stored 0-based int32.
+    """
+
+    def __init__(self, node, elem, face=None, evol=None, area=None, dev=None):
+        dev = dev or device()
+        self.dev = dev
+        node = np.asarray(node, dtype=np.float64)
+        e0 = np.asarray(elem)[:, :4].astype(np.int64) - 1
+        self.nn = int(node.shape[0])
+        self.ne = int(e0.shape[0])
+        self.node = _f64(node, dev)
+        self.el = _i32(e0, dev)
+        self.evol = _f64(evol, dev) if evol is not None else None
+        if face is not None and np.size(face):
+            f0 = np.asarray(face)[:, :3].astype(np.int64) - 1
+            self.nf = int(f0.shape[0])
+            self.fc = _i32(f0, dev)
+            self.area = _f64(area, dev) if area is not None else None
+        else:
+            self.nf = 0
+            self.fc = None
+            self.area = None
+        s = _abi.stream_ptr()
+        nn, ne, nf = self.nn, self.ne, self.nf
+        self.tinc_ptr = torch.empty(nn + 1, dtype=torch.int32, device=dev)
+        self.tinc = torch.empty(max(4 * ne, 1), dtype=torch.int32, device=dev)
+        call("rb_incidence", ptr(self.el), ne, 4, nn, ptr(self.tinc_ptr), ptr(self.tinc), s)
+        if nf:
+            self.finc_ptr = torch.empty(nn + 1, dtype=torch.int32, device=dev)
+            self.finc = torch.empty(3 * nf, dtype=torch.int32, device=dev)
+            call("rb_incidence", ptr(self.fc), nf, 3, nn, ptr(self.finc_ptr), ptr(self.finc), s)
+        else:
+            self.finc_ptr = None
+            self.finc = None
+        self.row_ptr = torch.empty(nn + 1, dtype=torch.int32, device=dev)
+        nnz = ctypes.c_int64(0)
+        call("rb_csr_pattern", ptr(self.el), ne, nn, ptr(self.tinc_ptr), ptr(self.tinc), ptr(self.row_ptr),
+             None, 0, ctypes.byref(nnz), None, s)
+        self.nnz = int(nnz.value)
+        self.col_idx = torch.empty(max(self.nnz, 1), dtype=torch.int32, device=dev)
+        self.diag = torch.empty(nn, dtype=torch.int32, device=dev)
+        call("rb_csr_pattern", ptr(self.el), ne, nn, ptr(self.tinc_ptr), ptr(self.tinc), ptr(self.row_ptr),
+             ptr(self.col_idx), self.nnz, ctypes.byref(nnz), ptr(self.diag), s)
+        self.slot_ptr = torch.empty(self.nnz + 1, dtype=torch.int32, device=dev)
+        self.codes = torch.empty(max(16 * ne + 9 * nf, 1), dtype=torch.int32, device=dev)
+        call("rb_slot_map", ptr(self.el), ne, ptr(self.fc), nf, nn, ptr(self.tinc_ptr), ptr(self.tinc),
+             ptr(self.finc_ptr), ptr(self.finc), ptr(self.row_ptr), ptr(self.col_idx), self.nnz,
+             ptr(self.slot_ptr), ptr(self.codes), s)
+        self.maxnp = int((self.row_ptr[1:] - self.row_ptr[:-1]).max().item()) if nn else 0
+        self._dd_src = None
+        self.dd = None
+
+    def set_geometry(self, deldotdel, evol, area):
+        """Upload element geometry (cached by object identity)."""
+        if self._dd_src is not deldotdel:
+            self.dd = _f64(np.asarray(deldotdel), self.dev)
+            self._dd_src = deldotdel
+        if evol is not None and (self.evol is None or getattr(self, "_evol_src", None) is not evol):
+            self.evol = _f64(evol, self.dev)
+            self._evol_src = evol
+        if area is not None and self.nf and (self.area is None or getattr(self, "_area_src", None) is not area):
+            self.area = _f64(area, self.dev)
+            self._area_src = area
+
+    def pattern_host(self):
+        """(indptr int64, indices int64) of the CSR pattern on the host."""
+        return (self.row_ptr.cpu().numpy().astype(np.int64), self.col_idx[: self.nnz].cpu().numpy().astype(np.int64))
+
+
+_MESH_CACHE = []  # [(key objects, GpuMesh)], small LRU
+
+
+def mesh_for(cfg):
+    """GpuMesh for cfg (node/elem/face), cached on the identity of those arrays."""
+    key = (cfg["node"], cfg["elem"], cfg.get("face"))
+    for i, (k, m) in enumerate(_MESH_CACHE):
+        if all(a is b for a, b in zip(k, key)):
+            _MESH_CACHE.insert(0, _MESH_CACHE.pop(i))
+            return m
+    m = GpuMesh(cfg["node"], cfg["elem"], cfg.get("face"), cfg.get("evol"), cfg.get("area"))
+    _MESH_CACHE.insert(0, (key, m))
+    del _MESH_CACHE[4:]
+    return m
+
+
+def mesh_from_elem(elem, nn, evol):
+    """GpuMesh without faces, for jac() which only receives elem/evol."""
+    key = (None, elem, None)
+    for i, (k, m) in enumerate(_MESH_CACHE):
+        if k[1] is elem and m.nn == nn:
+            _MESH_CACHE.insert(0, _MESH_CACHE.pop(i))
+            if m.evol is None or getattr(m, "_evol_src", None) is not evol:
+                m.evol = _f64(evol, m.dev)
+                m._evol_src = evol
+            return m
+    node = np.zeros((nn, 3))
+    m = GpuMesh(node, elem, None, evol, None)
+    m._evol_src = evol
+    _MESH_CACHE.insert(0, (key, m))
+    del _MESH_CACHE[4:]
+    return m
+
+
+# ---------------------------------------------------------------------------
+# properties (redbirdpy forward.py:105-142)
+# ---------------------------------------------------------------------------
+def props_for(cfg, wavelength="", mode=1):
+    """Return (props, reff, omega) as femlhs sees them for one wavelength/mode."""
+    if isinstance(cfg.get("prop"), dict) and wavelength:
+        props = cfg["prop"][wavelength]
+        reff = cfg["reff"][wavelength] if isinstance(cfg.get("reff"), dict) else cfg["reff"]
+        om = cfg["omega"].get(wavelength, 0) if isinstance(cfg.get("omega"), dict) else cfg.get("omega", 0)
+    else:
+        props = cfg["prop"]
+        reff = cfg.get("reff", 0.493)
+        om = cfg.get("omega", 0)
+    if mode == 2:
+        om = 0
+    return np.asarray(props, dtype=np.float64), float(reff), float(om)
+
+
+def expand_props(cfg, props):
+    """Per-element or per-node (mua, musp, nref, kind) arrays for the kernel.
+
+    Mirrors femlhs's choice (forward.py:128-142): rows == nn or ne use the
+    rows directly (element branch iff len == ne); otherwise a label table is
+    indexed by cfg['seg'] and n is taken from the first element's label.
+    """
+    nn = cfg["node"].shape[0]
+    ne = cfg["elem"].shape[0]
+    seg = cfg.get("seg", None)
+    if props.ndim == 1:
+        props = props.reshape(1, -1)
+    nc = props.shape[1]
+    if props.shape[0] == nn or props.shape[0] == ne:
+        mua = props[:, 0]
+        musp = props[:, 1] * (1 - props[:, 2]) if nc >= 3 else props[:, 1]
+        nref = props[:, 3] if nc >= 4 else np.array([1.37])
+        kind = _abi.RB_PROP_ELEM if len(mua) == ne else _abi.RB_PROP_NODE
+    elif seg is not None:
+        s = np.clip(np.asarray(seg).astype(np.int64), 0, props.shape[0] - 1)
+        mua = props[s, 0]
+        musp = props[s, 1] * (1 - props[s, 2]) if nc >= 3 else props[s, 1]
+        nref = np.array([props[s[0], 3] if nc >= 4 else 1.37])
+        kind = _abi.RB_PROP_ELEM
+    else:
+        raise ValueError("Property format not recognized")
+    return np.asarray(mua, float), np.asarray(musp, float), np.asarray(nref, float), kind
+
+
+class System:
+    """One assembled FEM system set: nsys matrices sharing the mesh pattern."""
+
+    def __init__(self, mesh, vals64, vals32, nsys, is_complex):
+        self.mesh = mesh
+        self.vals64 = vals64
+        self.vals32 = vals32
+        self.nsys = nsys
+        self.is_complex = is_complex
+
+
+def assemble(mesh, cfg, deldotdel, systems, want32=False):
+    """Assemble femlhs matrices for a list of (wavelength, mode) systems in one launch.
+
+    Returns a System with values (nsys x nnz) float64/complex128 (+ 32-bit copy).
+    """
+    dev = mesh.dev
+    mesh.set_geometry(deldotdel, cfg["evol"], cfg.get("area"))
+    muas, musps, nrefs, oms, bcs = [], [], [], [], []
+    kind = None
+    for wv, md in systems:
+        props, reff, om = props_for(cfg, wv, md)
+        mua, musp, nref, k = expand_props(cfg, props)
+        if kind is None:
+            kind = k
+        elif kind != k:
+            raise ValueError("all systems of one assembly must share the property layout")
+        muas.append(mua)
+        musps.append(musp)
+        nrefs.append(nref)
+        oms.append(om)
+        bcs.append((1 - reff) / (12.0 * (1 + reff)))
+    plen = len(muas[0])
+    nlen = max(len(x) for x in nrefs)
+    nref_m = np.stack([np.broadcast_to(x, (nlen,)) for x in nrefs])
+    nsys = len(systems)
+    is_complex = any(o > 0 for o in oms)
+    mua_d = _f64(np.stack(muas), dev)
+    musp_d = _f64(np.stack(musps), dev)
+    nref_d = _f64(nref_m, dev)
+    om_d = _f64(np.array(oms), dev)
+    bc_d = _f64(np.array(bcs), dev)
+    shape = (nsys, mesh.nnz)
+    v64 = torch.empty(shape, dtype=torch.complex128 if is_complex else torch.float64, device=dev)
+    v32 = torch.empty(shape, dtype=torch.complex64 if is_complex else torch.float32, device=dev) if want32 else None
+    call("rb_assemble", ptr(mesh.slot_ptr), ptr(mesh.codes), mesh.nnz, ptr(mesh.el), mesh.ne, ptr(mesh.fc),
+         mesh.nf, mesh.nn, ptr(mesh.dd), ptr(mesh.evol), ptr(mesh.area), nsys, kind, ptr(mua_d), ptr(musp_d),
+         ptr(nref_d), plen, nlen, ptr(om_d), ptr(bc_d), int(is_complex), ptr(v64), ptr(v32), _abi.stream_ptr())
+    return System(mesh, v64, v32, nsys, is_complex)
+
+
+def volume_weights(mesh):
+    """W[n, i] = sum of V_e over tets holding n and i (kind RB_PROP_VOLUME)."""
+    w = torch.empty(mesh.nnz, dtype=torch.float64, device=mesh.dev)
+    call("rb_assemble", ptr(mesh.slot_ptr), ptr(mesh.codes), mesh.nnz, ptr(mesh.el), mesh.ne, None, 0, mesh.nn,
+         ptr(mesh.dd), ptr(mesh.evol), None, 1, _abi.RB_PROP_VOLUME, None, None, None, 0, 1, None, None, 0,
+         ptr(w), None, _abi.stream_ptr())
+    return w
+
+
+# ---------------------------------------------------------------------------
+# solve
+# ---------------------------------------------------------------------------
+_PREC = {"fp64": _abi.RB_PREC_FP64, "mixed": _abi.RB_PREC_MIXED}
+
+
+class SolveInfo:
+    def __init__(self, iters, relres, flag):
+        self.iters = iters
+        self.relres = relres
+        self.flag = flag
+
+    def __repr__(self):
+        return "SolveInfo(iters=%d, relres=%.3e, flag=%d)" % (self.iters, self.relres, self.flag)
+
+
+def krylov_solve(row_ptr, col_idx, diag, n, nnz, vals64, vals32, B, k, nsys, is_complex, precision="fp64",
+                 tol=1e-12, maxiter=2000, X0=None, check_every=16):
+    """Solve A_s X = B for nsys systems x k columns; B is (n, nsys*k) on device."""
+    dev = B.device
+    prec = _PREC[precision]
+    C = nsys * k
+    if C > 512:
+        raise ValueError("at most 512 columns per solve call")
+    cdt = torch.complex128 if is_complex else torch.float64
+    B = B.to(cdt).contiguous()
+    if X0 is not None:
+        X = X0.to(cdt).clone().contiguous()
+        use_x0 = 1
+    else:
+        X = torch.empty_like(B)
+        use_x0 = 0
+    nbytes = ctypes.c_size_t(0)
+    call("rb_krylov_workspace", n, nsys, k, int(is_complex), prec, ctypes.byref(nbytes))
+    work = torch.empty(int(nbytes.value), dtype=torch.uint8, device=dev)
+    it = ctypes.c_int(0)
+    rel = ctypes.c_double(0)
+    fl = ctypes.c_int(0)
+    if prec == _abi.RB_PREC_MIXED and vals32 is None:
+        vals32 = vals64.to(torch.complex64 if is_complex else torch.float32)
+    call("rb_krylov_solve", n, nnz, ptr(row_ptr), ptr(col_idx), ptr(diag), nsys, k, int(is_complex), ptr(vals64),
+         ptr(vals32), ptr(B), ptr(X), prec, float(tol), int(maxiter), use_x0, int(check_every), ptr(work),
+         nbytes.value, ctypes.byref(it), ctypes.byref(rel), ctypes.byref(fl), _abi.stream_ptr())
+    return X, SolveInfo(it.value, rel.value, fl.value)
+
+
+def solve_system(sysm, B, k, precision="fp64", tol=1e-12, maxiter=2000, X0=None):
+    m = sysm.mesh
+    return krylov_solve(m.row_ptr, m.col_idx, m.diag, m.nn, m.nnz, sysm.vals64, sysm.vals32, B, k, sysm.nsys,
+                        sysm.is_complex, precision, tol, maxiter, X0)
+
+
+# ---------------------------------------------------------------------------
+# optodes and right-hand sides (redbirdpy forward.py:259-362)
+# ---------------------------------------------------------------------------
+def tsearch(mesh, pts, tol=1e-10):
+    """GPU point location; returns (loc0 int64 with -1 outside, bary (npts,4))."""
+    pts = np.atleast_2d(np.asarray(pts, dtype=np.float64))[:, :3]
+    npts = pts.shape[0]
+    if npts == 0:
+        return np.zeros(0, np.int64), np.zeros((0, 4))
+    dev = mesh.dev
+    p = _f64(pts, dev)
+    loc = torch.empty(npts, dtype=torch.int32, device=dev)
+    bary = torch.empty((npts, 4), dtype=torch.float64, device=dev)
+    call("rb_tsearch", ptr(mesh.node), ptr(mesh.el), mesh.ne, ptr(p), npts, float(tol), ptr(loc), ptr(bary),
+         _abi.stream_ptr())
+    loc = loc.cpu().numpy().astype(np.int64)
+    loc[loc == np.iinfo(np.int32).max] = -1
+    return loc, bary.cpu().numpy()
+
+
+def rhs_block(mesh, B, col0, loc0, bary, is_complex):
+    """Write point-optode columns col0.. into the dense device block B (n x ld)."""
+    ok = loc0 >= 0
+    if not np.any(ok):
+        return
+    dev = mesh.dev
+    cols = _i32(np.arange(len(loc0))[ok] + col0, dev)
+    l = _i32(loc0[ok], dev)
+    b = _f64(bary[ok], dev)
+    ld = B.shape[1]
+    call("rb_rhs_points", ptr(mesh.el), ptr(l), ptr(b), int(ok.sum()), ptr(cols), ld, int(is_complex), ptr(B),
+         _abi.stream_ptr())
+
+
+def zeros(shape, dtype, dev):
+    t = torch.empty(shape, dtype=dtype, device=dev)
+    call("rb_zero", ptr(t), t.numel() * t.element_size(), _abi.stream_ptr())
+    return t
+
+
+# ---------------------------------------------------------------------------
+# detector readout and Jacobian
+# ---------------------------------------------------------------------------
+def getdet(det_ptr, det_row, det_val, phi, src0, nsrc, is_complex):
+    """out[d, s] = sum_j w_j phi[row_j, src0 + s] for sparse detector columns."""
+    ndet = det_ptr.numel() - 1
+    dev = phi.device
+    out = torch.empty((ndet, nsrc), dtype=torch.complex128 if is_complex else torch.float64, device=dev)
+    if phi.stride(1) != 1:
+        phi = phi.contiguous()
+    call("rb_getdet", ndet, ptr(det_ptr), ptr(det_row), ptr(det_val), ptr(phi), phi.stride(0), src0, nsrc,
+         int(is_complex), ptr(out), _abi.stream_ptr())
+    return out
+
+
+def pair_grid(src_cols, det_cols):
+    """Reduce an ordered pair list to (unique srcs, unique dets, pmap, dup_rows).
+
+    pmap[d_idx * Ns + s_idx] = output row of that pair (-1 when absent);
+    dup_rows lists (row, first_row) for repeated pairs.
+    """
+    src_cols = np.asarray(src_cols, dtype=np.int64)
+    det_cols = np.asarray(det_cols, dtype=np.int64)
+    S, si = np.unique(src_cols, return_inverse=True)
+    D, di = np.unique(det_cols, return_inverse=True)
+    Ns, Nd = len(S), len(D)
+    pmap = np.full(Ns * Nd, -1, dtype=np.int64)
+    cell = di * Ns + si
+    dups = []
+    for r, c in enumerate(cell):
+        if pmap[c] < 0:
+            pmap[c] = r
+        else:
+            dups.append((r, int(pmap[c])))
+    return S, D, pmap, dups
+
+
+def jac_node(mesh, phi, src_cols, det_cols, out_dtype=torch.float64, want_elem=False):
+    """JT (nn x npairs) node-major Jacobian (and optional element Jacobian, ne x npairs)."""
+    npairs = len(src_cols)
+    dev = phi.device
+    is_complex = phi.is_complex()
+    S, D, pmap, dups = pair_grid(src_cols, det_cols)
+    full = (len(S) * len(D) == npairs) and np.array_equal(pmap, np.arange(npairs))
+    code = {torch.float64: _abi.RB_F64, torch.float32: _abi.RB_F32, torch.complex128: _abi.RB_C128,
+            torch.complex64: _abi.RB_C64}[out_dtype]
+    JT = torch.empty((mesh.nn, npairs), dtype=out_dtype, device=dev)
+    JE = torch.empty((mesh.ne, npairs), dtype=out_dtype, device=dev) if want_elem else None
+    if npairs == 0:
+        return JT, JE
+    s_d = _i32(S, dev)
+    d_d = _i32(D, dev)
+    pm = None if full else _i32(pmap, dev)
+    phi = phi.contiguous()
+    call("rb_jac_grid", mesh.nn, mesh.ne, ptr(mesh.el), ptr(mesh.evol), ptr(mesh.tinc_ptr), ptr(mesh.tinc),
+         ptr(mesh.row_ptr), ptr(mesh.col_idx), mesh.maxnp, ptr(phi), phi.shape[1], int(is_complex), len(S), len(D),
+         ptr(s_d), ptr(d_d), ptr(pm), ptr(JT), npairs, code, ptr(JE), _abi.stream_ptr())
+    for r, f in dups:
+        JT[:, r] = JT[:, f]
+        if JE is not None:
+            JE[:, r] = JE[:, f]
+    return JT, JE
+
+
+def jac_contract(mesh, phi, S, D, w):
+    """g[n] = sum_{d,s} w[d, s] Jnode[n, (s, d)] for full source x detector grids."""
+    dev = phi.device
+    is_complex = phi.is_complex()
+    Ns, Nd = len(S), len(D)
+    dt = phi.dtype
+    w = w.to(dt).contiguous()
+    psi = torch.empty((mesh.nn, Ns), dtype=dt, device=dev)
+    h = torch.empty(mesh.ne, dtype=dt, device=dev)
+    g = torch.empty(mesh.nn, dtype=dt, device=dev)
+    s_d = _i32(S, dev)
+    d_d = _i32(D, dev)
+    phi = phi.contiguous()
+    call("rb_jac_contract_grid", mesh.nn, mesh.ne, ptr(mesh.el), ptr(mesh.evol), ptr(mesh.tinc_ptr),
+         ptr(mesh.tinc), ptr(phi), phi.shape[1], int(is_complex), Ns, Nd, ptr(s_d), ptr(d_d), ptr(w), ptr(psi),
+         ptr(h), ptr(g), _abi.stream_ptr())
+    return g
+
+
+def gram(JT, d=None, mode=0):
+    """G = diag(d) JT^T JT diag(d) in float64 (m x m)."""
+    K, m = JT.shape
+    G = torch.empty((m, m), dtype=torch.float64, device=JT.device)
+    code = _abi.RB_F64 if JT.dtype == torch.float64 else _abi.RB_F32
+    call("rb_gram", K, m, ptr(JT), JT.stride(0), code, ptr(d), mode, ptr(G), _abi.stream_ptr())
+    return G

@@ -1,0 +1,793 @@
+// K2-K4: multi-RHS Krylov solves over one shared CSR pattern.
+//
+// Every column of the n x C right-hand-side block runs its own Jacobi-
+// preconditioned CG (real SPD) or COCG (complex symmetric, unconjugated
+// inner products) recurrence; all columns share one SpMM per iteration,
+// which streams the CSR arrays once for all sources and detectors.  Column
+// c belongs to system c / k, so several matrices with the same pattern
+// (wavelengths) are solved in the same launches.
+//
+// Layout: X, R, P, Q are row-major n x C (ld = C), so one CSR row gathers
+// contiguous C-wide rows of P; a group of G lanes (G = 64 when C > 16)
+// covers the columns of one row in NCH chunks.  Per-column dot products are
+// reduced in double precision: lanes -> LDS -> per-block partials -> the
+// last block of each group of 32 blocks -> the last group, in a fixed
+// order, so every run gives the same bits.
+//
+// Precision modes: plain (everything in the 64-bit type) and mixed (inner
+// solves in the 32-bit type on a 32-bit copy of A; outer FP64 residual
+// r = b - A x and refinement x += d until ||r_c|| <= tol ||b_c||).
+#include "rb_common.h"
+#include "../../include/rbhip.h"
+#include <vector>
+#include <algorithm>
+#include <math.h>
+
+#define KB 256       // threads per block
+#define RED_GS 32    // blocks per reduction group
+#define MAXND 3      // dot products per kernel
+#define NBMAX 1024   // blocks per launch (grid-stride over rows)
+
+template <typename A>
+struct ColState {
+  A* rho;
+  A* alpha;
+  A* beta;
+  double* bb;
+  double* rr;
+  int* active;
+  int* iters;
+  int* flag;
+  int* nactive;
+  A* part;    // [nblocks][MAXND][C]
+  A* gpart;   // [ngroups][MAXND][C]
+  int* cnt1;  // [ngroups]
+  int* cnt2;  // [1]
+  double tol2;
+  int maxiter;
+};
+
+struct Csr {
+  const int32_t* row_ptr;
+  const int32_t* col_idx;
+  const int32_t* diag;
+  int64_t n, nnz;
+};
+
+// ---------------------------------------------------------------------------
+// deterministic cross-block reduction (see file comment); fin(c, v[]) runs
+// once per column in the last block, which then recounts active columns.
+// ---------------------------------------------------------------------------
+template <typename A, typename Fin>
+__device__ void cross_block_finish(const ColState<A>& st, int nd, int C, Fin fin) {
+  __shared__ int s_last;
+  __shared__ int s_cnt;
+  const int nblocks = gridDim.x;
+  const int ngroups = (nblocks + RED_GS - 1) / RED_GS;
+  const int grp = blockIdx.x / RED_GS;
+  const int gsize = min(RED_GS, nblocks - grp * RED_GS);
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int old = atomicAdd(&st.cnt1[grp], 1);
+    s_last = (old == gsize - 1);
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  const int64_t bstride = (int64_t)MAXND * C;
+  for (int i = threadIdx.x; i < nd * C; i += blockDim.x) {
+    const A* p = st.part + (int64_t)(grp * RED_GS) * bstride + i;
+    A acc = coherent_load(p);
+    for (int b = 1; b < gsize; ++b) acc = acc_add(acc, coherent_load(p + b * bstride));
+    st.gpart[(int64_t)grp * bstride + i] = acc;
+  }
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    st.cnt1[grp] = 0;
+    int old = atomicAdd(st.cnt2, 1);
+    s_last = (old == ngroups - 1);
+    s_cnt = 0;
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  int loc = 0;
+  for (int c = threadIdx.x; c < C; c += blockDim.x) {
+    A v[MAXND];
+    for (int d = 0; d < nd; ++d) {
+      const A* p = st.gpart + d * C + c;
+      A acc = coherent_load(p);
+      for (int g = 1; g < ngroups; ++g) acc = acc_add(acc, coherent_load(p + g * bstride));
+      v[d] = acc;
+    }
+    fin(c, v);
+    loc += st.active[c] ? 1 : 0;
+  }
+  atomicAdd(&s_cnt, loc);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    *st.nactive = s_cnt;
+    *st.cnt2 = 0;
+  }
+}
+
+// per-thread partials pv[ch][d] (column lane + ch*G) -> part[blockIdx][d][c]
+template <typename A, int G, int NCH>
+__device__ void block_partials(const ColState<A>& st, A (&pv)[NCH][MAXND], int nd, int C) {
+  __shared__ A lds[KB * MAXND];
+  const int lane = threadIdx.x % G;
+  const int rg = threadIdx.x / G;
+  constexpr int RPB = KB / G;
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) {
+    for (int d = 0; d < nd; ++d) lds[(rg * MAXND + d) * G + lane] = pv[ch][d];
+    __syncthreads();
+    for (int i = threadIdx.x; i < nd * G; i += KB) {
+      int d = i / G, l = i - d * G;
+      int c = l + ch * G;
+      if (c < C) {
+        A acc = lds[d * G + l];
+        for (int r = 1; r < RPB; ++r) acc = acc_add(acc, lds[(r * MAXND + d) * G + l]);
+        st.part[(int64_t)blockIdx.x * MAXND * C + d * C + c] = acc;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+__device__ inline T row_dot(const int32_t* __restrict__ col_idx, const T* __restrict__ vals, int32_t rs, int32_t re,
+                            const T* __restrict__ X, int64_t ld, int c) {
+  T acc = sc<T>::zero();
+  for (int32_t j = rs; j < re; ++j) {
+    T a = vals[j];
+    T x = X[(int64_t)col_idx[j] * ld + c];
+    acc = acc + a * x;
+  }
+  return acc;
+}
+
+#define ROW_LOOP(nrows)                                                                    \
+  const int lane = threadIdx.x % G;                                                        \
+  const int rg = threadIdx.x / G;                                                          \
+  constexpr int RPB = KB / G;                                                              \
+  for (int64_t row = (int64_t)blockIdx.x * RPB + rg; row < (nrows); row += (int64_t)gridDim.x * RPB)
+
+// ---------------------------------------------------------------------------
+// iteration kernels
+// ---------------------------------------------------------------------------
+// r = b - A x0 (or b, with x = 0), z = dinv r, p = z ; dots rz, rr, bb
+template <typename T, int G, int NCH>
+__global__ void __launch_bounds__(KB) k_cg_init(Csr m, const T* __restrict__ vals, const T* __restrict__ dinv,
+                                                const T* __restrict__ B, T* __restrict__ X, int use_x0,
+                                                T* __restrict__ R, T* __restrict__ P, int C, int k,
+                                                ColState<typename sc<T>::acc> st) {
+  typedef typename sc<T>::acc A;
+  A pv[NCH][MAXND];
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch)
+    for (int d = 0; d < MAXND; ++d) pv[ch][d] = acc_zero(A());
+  ROW_LOOP(m.n) {
+    int32_t rs = m.row_ptr[row], re = m.row_ptr[row + 1];
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) {
+      int c = lane + ch * G;
+      if (c < C) {
+        int s = c / k;
+        int64_t o = row * C + c;
+        T b = B[o];
+        T r = b;
+        if (use_x0)
+          r = b - row_dot(m.col_idx, vals + (int64_t)s * m.nnz, rs, re, X, C, c);
+        else
+          X[o] = sc<T>::zero();
+        T z = dinv[(int64_t)s * m.n + row] * r;
+        R[o] = r;
+        P[o] = z;
+        pv[ch][0] = acc_add(pv[ch][0], amul(r, z));
+        pv[ch][1] = acc_add(pv[ch][1], acc_from_real(sc<T>::norm2(r), A()));
+        pv[ch][2] = acc_add(pv[ch][2], acc_from_real(sc<T>::norm2(b), A()));
+      }
+    }
+  }
+  block_partials<A, G, NCH>(st, pv, 3, C);
+  cross_block_finish<A>(st, 3, C, [&](int c, A* v) {
+    double bb = acc_re(v[2]);
+    double rr = acc_re(v[1]);
+    st.bb[c] = bb;
+    st.rr[c] = rr;
+    st.rho[c] = v[0];
+    st.iters[c] = 0;
+    st.alpha[c] = acc_zero(A());
+    st.beta[c] = acc_zero(A());
+    st.flag[c] = 0;
+    st.active[c] = (bb > 0.0) && (rr > st.tol2 * bb);
+  });
+}
+
+// q = A p ; dot pq ; alpha = rho / pq
+template <typename T, int G, int NCH>
+__global__ void __launch_bounds__(KB) k_cg_spmm(Csr m, const T* __restrict__ vals, const T* __restrict__ P,
+                                                T* __restrict__ Q, int C, int k, ColState<typename sc<T>::acc> st) {
+  typedef typename sc<T>::acc A;
+  if (coherent_load(st.nactive) == 0) return;
+  A pv[NCH][MAXND];
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) pv[ch][0] = acc_zero(A());
+  ROW_LOOP(m.n) {
+    int32_t rs = m.row_ptr[row], re = m.row_ptr[row + 1];
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) {
+      int c = lane + ch * G;
+      if (c < C) {
+        int s = c / k;
+        T q = row_dot(m.col_idx, vals + (int64_t)s * m.nnz, rs, re, P, C, c);
+        int64_t o = row * C + c;
+        Q[o] = q;
+        pv[ch][0] = acc_add(pv[ch][0], amul(P[o], q));
+      }
+    }
+  }
+  block_partials<A, G, NCH>(st, pv, 1, C);
+  cross_block_finish<A>(st, 1, C, [&](int c, A* v) {
+    A pq = v[0];
+    if (st.active[c]) {
+      if (acc_abs2(pq) == 0.0) {
+        st.flag[c] = 4;
+        st.active[c] = 0;
+        st.alpha[c] = acc_zero(A());
+      } else {
+        st.alpha[c] = acc_div(st.rho[c], pq);
+      }
+    } else {
+      st.alpha[c] = acc_zero(A());
+    }
+  });
+}
+
+// x += alpha p ; r -= alpha q ; z = dinv r ; dots rz, rr ; beta, stopping test
+template <typename T, int G, int NCH>
+__global__ void __launch_bounds__(KB) k_cg_update(int64_t n, const T* __restrict__ dinv, T* __restrict__ X,
+                                                  T* __restrict__ R, const T* __restrict__ P,
+                                                  const T* __restrict__ Q, int C, int k,
+                                                  ColState<typename sc<T>::acc> st) {
+  typedef typename sc<T>::acc A;
+  if (coherent_load(st.nactive) == 0) return;
+  A pv[NCH][MAXND];
+  A al[NCH];
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) {
+    pv[ch][0] = acc_zero(A());
+    pv[ch][1] = acc_zero(A());
+    int c = (int)(threadIdx.x % G) + ch * G;
+    al[ch] = (c < C) ? st.alpha[c] : acc_zero(A());
+  }
+  ROW_LOOP(n) {
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) {
+      int c = lane + ch * G;
+      if (c < C) {
+        int s = c / k;
+        int64_t o = row * C + c;
+        T r = R[o];
+        if (acc_abs2(al[ch]) != 0.0) {
+          X[o] = X[o] + scale_to(P[o], al[ch]);
+          r = r - scale_to(Q[o], al[ch]);
+          R[o] = r;
+        }
+        T z = dinv[(int64_t)s * n + row] * r;
+        pv[ch][0] = acc_add(pv[ch][0], amul(r, z));
+        pv[ch][1] = acc_add(pv[ch][1], acc_from_real(sc<T>::norm2(r), A()));
+      }
+    }
+  }
+  block_partials<A, G, NCH>(st, pv, 2, C);
+  cross_block_finish<A>(st, 2, C, [&](int c, A* v) {
+    double rr = acc_re(v[1]);
+    st.rr[c] = rr;
+    if (st.active[c]) {
+      int it = st.iters[c] + 1;
+      st.iters[c] = it;
+      A rz = v[0];
+      if (rr <= st.tol2 * st.bb[c]) {
+        st.active[c] = 0;
+        st.beta[c] = acc_zero(A());
+      } else if (it >= st.maxiter) {
+        st.active[c] = 0;
+        st.flag[c] = 1;
+        st.beta[c] = acc_zero(A());
+      } else if (acc_abs2(rz) == 0.0 || acc_abs2(st.rho[c]) == 0.0) {
+        st.active[c] = 0;
+        st.flag[c] = 4;
+        st.beta[c] = acc_zero(A());
+      } else {
+        st.beta[c] = acc_div(rz, st.rho[c]);
+        st.rho[c] = rz;
+      }
+    } else {
+      st.beta[c] = acc_zero(A());
+    }
+  });
+}
+
+// p = dinv r + beta p on active columns
+template <typename T, int G, int NCH>
+__global__ void __launch_bounds__(KB) k_cg_pupdate(int64_t n, const T* __restrict__ dinv, const T* __restrict__ R,
+                                                   T* __restrict__ P, int C, int k,
+                                                   ColState<typename sc<T>::acc> st) {
+  typedef typename sc<T>::acc A;
+  if (coherent_load(st.nactive) == 0) return;
+  A be[NCH];
+  int act[NCH];
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) {
+    int c = (int)(threadIdx.x % G) + ch * G;
+    be[ch] = (c < C) ? st.beta[c] : acc_zero(A());
+    act[ch] = (c < C) ? st.active[c] : 0;
+  }
+  ROW_LOOP(n) {
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) {
+      int c = lane + ch * G;
+      if (c < C && act[ch]) {
+        int s = c / k;
+        int64_t o = row * C + c;
+        T z = dinv[(int64_t)s * n + row] * R[o];
+        P[o] = z + scale_to(P[o], be[ch]);
+      }
+    }
+  }
+}
+
+// outer residual of the mixed solve: r32 = (b - A64 x64), dots rr, bb; the
+// inner right-hand side of a column that meets the tolerance is zeroed by
+// k_mask_cols afterwards
+template <typename T64, typename T32, int G, int NCH>
+__global__ void __launch_bounds__(KB) k_outer_resid(Csr m, const T64* __restrict__ vals, const T64* __restrict__ B,
+                                                    const T64* __restrict__ X, int use_x, T32* __restrict__ R32,
+                                                    int C, int k, int outer_it, int max_outer,
+                                                    ColState<typename sc<T64>::acc> st) {
+  typedef typename sc<T64>::acc A;
+  A pv[NCH][MAXND];
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) {
+    pv[ch][0] = acc_zero(A());
+    pv[ch][1] = acc_zero(A());
+  }
+  ROW_LOOP(m.n) {
+    int32_t rs = m.row_ptr[row], re = m.row_ptr[row + 1];
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) {
+      int c = lane + ch * G;
+      if (c < C) {
+        int s = c / k;
+        int64_t o = row * C + c;
+        T64 b = B[o];
+        T64 r = b;
+        if (use_x) r = b - row_dot(m.col_idx, vals + (int64_t)s * m.nnz, rs, re, X, C, c);
+        R32[o] = narrow(r, T32());
+        pv[ch][0] = acc_add(pv[ch][0], acc_from_real(sc<T64>::norm2(r), A()));
+        pv[ch][1] = acc_add(pv[ch][1], acc_from_real(sc<T64>::norm2(b), A()));
+      }
+    }
+  }
+  block_partials<A, G, NCH>(st, pv, 2, C);
+  cross_block_finish<A>(st, 2, C, [&](int c, A* v) {
+    double rr = acc_re(v[0]), bb = acc_re(v[1]);
+    st.rr[c] = rr;
+    st.bb[c] = bb;
+    int act = (bb > 0.0) && (rr > st.tol2 * bb);
+    if (act && outer_it >= max_outer) {
+      act = 0;
+      st.flag[c] = 1;
+    }
+    st.active[c] = act;
+  });
+}
+
+template <typename T32>
+__global__ void k_mask_cols(int64_t total, int C, const int* __restrict__ active, T32* __restrict__ R32) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stp = (int64_t)gridDim.x * blockDim.x;
+  for (; i < total; i += stp) {
+    int c = (int)(i % C);
+    if (!active[c]) R32[i] = sc<T32>::zero();
+  }
+}
+
+template <typename T64, typename T32>
+__global__ void k_accum(int64_t total, const T32* __restrict__ D, T64* __restrict__ X) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stp = (int64_t)gridDim.x * blockDim.x;
+  for (; i < total; i += stp) X[i] = X[i] + widen(D[i], T64());
+}
+
+template <typename T>
+__global__ void k_dinv(Csr m, const T* __restrict__ vals, int nsys, T* __restrict__ dinv) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.n * nsys) return;
+  int64_t s = i / m.n, row = i - s * m.n;
+  int32_t d = m.diag[row];
+  dinv[i] = (d >= 0) ? tdiv(one_of(T()), vals[s * m.nnz + d]) : sc<T>::zero();
+}
+
+template <typename T, int G, int NCH>
+__global__ void __launch_bounds__(KB) k_spmm(Csr m, const T* __restrict__ vals, const T* __restrict__ X,
+                                             T* __restrict__ Y, int C, int k) {
+  ROW_LOOP(m.n) {
+    int32_t rs = m.row_ptr[row], re = m.row_ptr[row + 1];
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) {
+      int c = lane + ch * G;
+      if (c < C) {
+        int s = c / k;
+        Y[row * C + c] = row_dot(m.col_idx, vals + (int64_t)s * m.nnz, rs, re, X, C, c);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static int pick_G(int C) { return C <= 4 ? 4 : (C <= 16 ? 16 : 64); }
+static int pick_NCH(int C) {
+  int g = pick_G(C);
+  int n = (C + g - 1) / g;
+  return n <= 1 ? 1 : (n <= 2 ? 2 : (n <= 4 ? 4 : 8));
+}
+static int64_t grid_for(int64_t n, int C) {
+  int64_t rpb = KB / pick_G(C);
+  int64_t g = ceil_div(n, rpb);
+  if (g > NBMAX) g = NBMAX;
+  if (g < 1) g = 1;
+  return g;
+}
+
+// dispatch a kernel template over (G, NCH)
+#define DISPATCH_GN(C, KERNEL, ...)                                       \
+  do {                                                                    \
+    int _g = pick_G(C), _n = pick_NCH(C);                                 \
+    if (_g == 4) {                                                        \
+      KERNEL(4, 1, __VA_ARGS__);                                          \
+    } else if (_g == 16) {                                                \
+      KERNEL(16, 1, __VA_ARGS__);                                         \
+    } else if (_n == 1) {                                                 \
+      KERNEL(64, 1, __VA_ARGS__);                                         \
+    } else if (_n == 2) {                                                 \
+      KERNEL(64, 2, __VA_ARGS__);                                         \
+    } else if (_n == 4) {                                                 \
+      KERNEL(64, 4, __VA_ARGS__);                                         \
+    } else {                                                              \
+      KERNEL(64, 8, __VA_ARGS__);                                         \
+    }                                                                     \
+  } while (0)
+
+struct Layout {
+  size_t off = 0;
+  size_t take(size_t bytes) {
+    size_t o = off;
+    off += (bytes + 255) & ~(size_t)255;
+    return o;
+  }
+};
+
+template <typename A>
+static void state_layout(Layout& L, int64_t C, int64_t nblocks, size_t* offs) {
+  int64_t ngroups = ceil_div(nblocks, RED_GS);
+  offs[0] = L.take(C * sizeof(A));                    // rho
+  offs[1] = L.take(C * sizeof(A));                    // alpha
+  offs[2] = L.take(C * sizeof(A));                    // beta
+  offs[3] = L.take(C * sizeof(double));               // bb
+  offs[4] = L.take(C * sizeof(double));               // rr
+  offs[5] = L.take(C * sizeof(int));                  // active
+  offs[6] = L.take(C * sizeof(int));                  // iters
+  offs[7] = L.take(C * sizeof(int));                  // flag
+  offs[8] = L.take(64);                               // nactive
+  offs[9] = L.take(nblocks * MAXND * C * sizeof(A));  // part
+  offs[10] = L.take(ngroups * MAXND * C * sizeof(A)); // gpart
+  offs[11] = L.take((ngroups + 1) * sizeof(int));     // cnt1 + cnt2
+}
+
+template <typename A>
+static ColState<A> state_at(char* base, const size_t* o, int64_t nblocks) {
+  ColState<A> s;
+  int64_t ngroups = ceil_div(nblocks, RED_GS);
+  s.rho = (A*)(base + o[0]);
+  s.alpha = (A*)(base + o[1]);
+  s.beta = (A*)(base + o[2]);
+  s.bb = (double*)(base + o[3]);
+  s.rr = (double*)(base + o[4]);
+  s.active = (int*)(base + o[5]);
+  s.iters = (int*)(base + o[6]);
+  s.flag = (int*)(base + o[7]);
+  s.nactive = (int*)(base + o[8]);
+  s.part = (A*)(base + o[9]);
+  s.gpart = (A*)(base + o[10]);
+  s.cnt1 = (int*)(base + o[11]);
+  s.cnt2 = s.cnt1 + ngroups;
+  s.tol2 = 0;
+  s.maxiter = 0;
+  return s;
+}
+
+template <typename T>
+struct SolveBufs {
+  T* R;
+  T* P;
+  T* Q;
+  T* dinv;
+};
+
+// workspace: plain -> R,P,Q,dinv (T64) + one state; mixed -> R,P,Q,D,dinv
+// (T32) + inner and outer states
+template <typename T64, typename T32>
+static size_t ws_layout(int64_t n, int64_t C, int nsys, int mixed, size_t* o) {
+  typedef typename sc<T64>::acc A;
+  Layout L;
+  int64_t nb = grid_for(n, (int)C);
+  if (!mixed) {
+    o[0] = L.take(n * C * sizeof(T64));
+    o[1] = L.take(n * C * sizeof(T64));
+    o[2] = L.take(n * C * sizeof(T64));
+    o[3] = L.take(n * nsys * sizeof(T64));
+    state_layout<A>(L, C, nb, o + 8);
+  } else {
+    o[0] = L.take(n * C * sizeof(T32));
+    o[1] = L.take(n * C * sizeof(T32));
+    o[2] = L.take(n * C * sizeof(T32));
+    o[3] = L.take(n * nsys * sizeof(T32));
+    o[4] = L.take(n * C * sizeof(T32));  // D (inner solution)
+    state_layout<A>(L, C, nb, o + 8);
+    state_layout<A>(L, C, nb, o + 20);
+  }
+  return L.off;
+}
+
+static unsigned ew_grid(int64_t total) {
+  int64_t g = ceil_div(total, 256);
+  if (g > 8192) g = 8192;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+static int* g_pinned = nullptr;  // host flag for convergence polls
+
+static int poll_nactive(const int* d_nactive, hipStream_t s, int* out) {
+  if (!g_pinned) RB_HIP(hipHostMalloc((void**)&g_pinned, 64, hipHostMallocDefault));
+  RB_HIP(hipMemcpyAsync(g_pinned, d_nactive, sizeof(int), hipMemcpyDeviceToHost, s));
+  RB_HIP(hipStreamSynchronize(s));
+  *out = *g_pinned;
+  return RB_OK;
+}
+
+#define L_INIT(G_, N_, ...) hipLaunchKernelGGL((k_cg_init<T, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
+#define L_SPMM(G_, N_, ...) hipLaunchKernelGGL((k_cg_spmm<T, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
+#define L_UPD(G_, N_, ...) hipLaunchKernelGGL((k_cg_update<T, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
+#define L_PUP(G_, N_, ...) hipLaunchKernelGGL((k_cg_pupdate<T, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
+
+// one CG/COCG solve of all columns; X is the solution (and x0 if use_x0)
+template <typename T>
+static int run_cg(const Csr& m, const T* vals, const T* dinv, const T* B, T* X, int use_x0, SolveBufs<T> w, int C,
+                  int k, ColState<typename sc<T>::acc> st, int check_every, hipStream_t s, int* iters_done) {
+  int64_t nb = grid_for(m.n, C);
+  int64_t ng = ceil_div(nb, RED_GS);
+  int rc = rb_fill_i32(st.cnt1, 0, ng + 1, s);
+  if (rc) return rc;
+  DISPATCH_GN(C, L_INIT, m, vals, dinv, B, X, use_x0, w.R, w.P, C, k, st);
+  RB_CHECK_LAUNCH();
+  int nact = 0;
+  rc = poll_nactive(st.nactive, s, &nact);
+  if (rc) return rc;
+  int it = 0;
+  while (nact > 0 && it < st.maxiter) {
+    int chunk = std::min(check_every, st.maxiter - it);
+    for (int j = 0; j < chunk; ++j) {
+      DISPATCH_GN(C, L_SPMM, m, vals, w.P, w.Q, C, k, st);
+      DISPATCH_GN(C, L_UPD, m.n, dinv, X, w.R, w.P, w.Q, C, k, st);
+      DISPATCH_GN(C, L_PUP, m.n, dinv, w.R, w.P, C, k, st);
+    }
+    RB_CHECK_LAUNCH();
+    it += chunk;
+    rc = poll_nactive(st.nactive, s, &nact);
+    if (rc) return rc;
+  }
+  *iters_done = it;
+  return RB_OK;
+}
+
+template <typename T64, typename T32>
+static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const void* vals32, const void* Bv, void* Xv,
+                      int precision, double tol, int maxiter, int use_x0, int check_every, void* work,
+                      size_t work_bytes, int* iters_out, double* relres_out, int* flag_out, hipStream_t s) {
+  typedef typename sc<T64>::acc A;
+  const int C = nsys * k;
+  const int64_t n = m.n;
+  size_t o[32];
+  size_t need = ws_layout<T64, T32>(n, C, nsys, precision, o);
+  if (work_bytes < need || !work) {
+    rb_set_error("rb_krylov_solve: workspace %zu bytes < required %zu", work_bytes, need);
+    return RB_ERR_SIZE;
+  }
+  char* base = (char*)work;
+  int64_t nb = grid_for(n, C);
+  const T64* B = (const T64*)Bv;
+  T64* X = (T64*)Xv;
+  const double tol2 = tol * tol;
+  int rc;
+  std::vector<double> hrr(C), hbb(C);
+  std::vector<int> hflag(C), hit(C);
+
+  if (!precision) {
+    SolveBufs<T64> w{(T64*)(base + o[0]), (T64*)(base + o[1]), (T64*)(base + o[2]), (T64*)(base + o[3])};
+    ColState<A> st = state_at<A>(base, o + 8, nb);
+    st.tol2 = tol2;
+    st.maxiter = maxiter;
+    const T64* A64 = (const T64*)vals64;
+    hipLaunchKernelGGL(k_dinv<T64>, dim3((unsigned)ceil_div(n * nsys, 256)), dim3(256), 0, s, m, A64, nsys, w.dinv);
+    RB_CHECK_LAUNCH();
+    int its = 0;
+    rc = run_cg<T64>(m, A64, w.dinv, B, X, use_x0, w, C, k, st, check_every, s, &its);
+    if (rc) return rc;
+    RB_HIP(hipMemcpyAsync(hrr.data(), st.rr, C * sizeof(double), hipMemcpyDeviceToHost, s));
+    RB_HIP(hipMemcpyAsync(hbb.data(), st.bb, C * sizeof(double), hipMemcpyDeviceToHost, s));
+    RB_HIP(hipMemcpyAsync(hflag.data(), st.flag, C * sizeof(int), hipMemcpyDeviceToHost, s));
+    RB_HIP(hipMemcpyAsync(hit.data(), st.iters, C * sizeof(int), hipMemcpyDeviceToHost, s));
+    RB_HIP(hipStreamSynchronize(s));
+    int mx = 0, fl = 0;
+    double rel = 0;
+    for (int c = 0; c < C; ++c) {
+      mx = std::max(mx, hit[c]);
+      fl = std::max(fl, hflag[c]);
+      if (hbb[c] > 0) rel = std::max(rel, sqrt(hrr[c] / hbb[c]));
+    }
+    if (iters_out) *iters_out = mx;
+    if (relres_out) *relres_out = rel;
+    if (flag_out) *flag_out = fl;
+    return RB_OK;
+  }
+
+  // mixed precision: inner CG/COCG in T32 on vals32, outer FP64 refinement
+  if (!vals32) {
+    rb_set_error("rb_krylov_solve: mixed precision needs the 32-bit matrix values");
+    return RB_ERR_ARG;
+  }
+  SolveBufs<T32> w{(T32*)(base + o[0]), (T32*)(base + o[1]), (T32*)(base + o[2]), (T32*)(base + o[3])};
+  T32* D = (T32*)(base + o[4]);
+  ColState<A> sin = state_at<A>(base, o + 8, nb);
+  ColState<A> sout = state_at<A>(base, o + 20, nb);
+  const double inner_tol = std::max(tol, 1e-5);
+  sin.tol2 = inner_tol * inner_tol;
+  sin.maxiter = maxiter;
+  sout.tol2 = tol2;
+  sout.maxiter = maxiter;
+  const T64* A64 = (const T64*)vals64;
+  const T32* A32 = (const T32*)vals32;
+  hipLaunchKernelGGL(k_dinv<T32>, dim3((unsigned)ceil_div(n * nsys, 256)), dim3(256), 0, s, m, A32, nsys, w.dinv);
+  RB_CHECK_LAUNCH();
+  const int max_outer = 12;
+  int total_it = 0;
+  int64_t ng = ceil_div(nb, RED_GS);
+  rc = rb_fill_i32(sout.flag, 0, C, s);
+  if (rc) return rc;
+  int inner_flag_max = 0;
+  for (int outer = 0;; ++outer) {
+    rc = rb_fill_i32(sout.cnt1, 0, ng + 1, s);
+    if (rc) return rc;
+#define L_ORES(G_, N_, ...) \
+  hipLaunchKernelGGL((k_outer_resid<T64, T32, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
+    DISPATCH_GN(C, L_ORES, m, A64, B, X, (outer > 0 || use_x0) ? 1 : 0, w.Q, C, k, outer, max_outer, sout);
+#undef L_ORES
+    RB_CHECK_LAUNCH();
+    if (outer == 0 && !use_x0) {
+      rc = rb_fill_bytes0(X, (size_t)n * C * sizeof(T64), s);
+      if (rc) return rc;
+    }
+    int nact = 0;
+    rc = poll_nactive(sout.nactive, s, &nact);
+    if (rc) return rc;
+    if (nact == 0) break;
+    // inner rhs = outer residual (in Q), zeroed on columns that are done
+    hipLaunchKernelGGL(k_mask_cols<T32>, dim3(ew_grid(n * C)), dim3(256), 0, s, (int64_t)n * C, C, sout.active, w.Q);
+    RB_CHECK_LAUNCH();
+    // run_cg reads B = Q only in its init kernel, before Q is reused as the
+    // SpMM output, so Q can serve as the inner rhs
+    int its = 0;
+    rc = run_cg<T32>(m, A32, w.dinv, w.Q, D, 0, w, C, k, sin, check_every, s, &its);
+    if (rc) return rc;
+    total_it += its;
+    RB_HIP(hipMemcpyAsync(hflag.data(), sin.flag, C * sizeof(int), hipMemcpyDeviceToHost, s));
+    RB_HIP(hipStreamSynchronize(s));
+    for (int c = 0; c < C; ++c) inner_flag_max = std::max(inner_flag_max, hflag[c] == 4 ? 4 : 0);
+    hipLaunchKernelGGL((k_accum<T64, T32>), dim3(ew_grid(n * C)), dim3(256), 0, s, (int64_t)n * C, D, X);
+    RB_CHECK_LAUNCH();
+  }
+  RB_HIP(hipMemcpyAsync(hrr.data(), sout.rr, C * sizeof(double), hipMemcpyDeviceToHost, s));
+  RB_HIP(hipMemcpyAsync(hbb.data(), sout.bb, C * sizeof(double), hipMemcpyDeviceToHost, s));
+  RB_HIP(hipMemcpyAsync(hflag.data(), sout.flag, C * sizeof(int), hipMemcpyDeviceToHost, s));
+  RB_HIP(hipStreamSynchronize(s));
+  int fl = inner_flag_max;
+  double rel = 0;
+  for (int c = 0; c < C; ++c) {
+    fl = std::max(fl, hflag[c]);
+    if (hbb[c] > 0) rel = std::max(rel, sqrt(hrr[c] / hbb[c]));
+  }
+  if (iters_out) *iters_out = total_it;
+  if (relres_out) *relres_out = rel;
+  if (flag_out) *flag_out = fl;
+  return RB_OK;
+}
+
+extern "C" int rb_krylov_workspace(int64_t n, int nsys, int k, int is_complex, int precision, size_t* bytes) {
+  if (!bytes || n <= 0 || nsys <= 0 || k <= 0) return RB_ERR_ARG;
+  size_t o[32];
+  if (is_complex)
+    *bytes = ws_layout<cplx<double>, cplx<float>>(n, (int64_t)nsys * k, nsys, precision, o);
+  else
+    *bytes = ws_layout<double, float>(n, (int64_t)nsys * k, nsys, precision, o);
+  return RB_OK;
+}
+
+extern "C" int rb_krylov_solve(int64_t n, int64_t nnz, const int32_t* row_ptr, const int32_t* col_idx,
+                               const int32_t* diag_slot, int nsys, int k, int is_complex, const void* vals64,
+                               const void* vals32, const void* B, void* X, int precision, double tol, int maxiter,
+                               int use_x0, int check_every, void* work, size_t work_bytes, int* iters_out,
+                               double* relres_out, int* flag_out, void* stream) {
+  if (n <= 0 || nsys <= 0 || k <= 0 || !vals64 || !B || !X) {
+    rb_set_error("rb_krylov_solve: bad arguments");
+    return RB_ERR_ARG;
+  }
+  int64_t C = (int64_t)nsys * k;
+  if (C > 512) {
+    rb_set_error("rb_krylov_solve: %lld columns exceed the 512-column limit of one launch", (long long)C);
+    return RB_ERR_SIZE;
+  }
+  if (check_every <= 0) check_every = 16;
+  if (maxiter <= 0) maxiter = 1000;
+  Csr m{row_ptr, col_idx, diag_slot, n, nnz};
+  hipStream_t s = (hipStream_t)stream;
+  if (is_complex)
+    return solve_impl<cplx<double>, cplx<float>>(m, nsys, k, vals64, vals32, B, X, precision, tol, maxiter, use_x0,
+                                                 check_every, work, work_bytes, iters_out, relres_out, flag_out, s);
+  return solve_impl<double, float>(m, nsys, k, vals64, vals32, B, X, precision, tol, maxiter, use_x0, check_every,
+                                   work, work_bytes, iters_out, relres_out, flag_out, s);
+}
+
+// Y = A X for every system (dtype: 0 f64, 1 c128, 2 f32, 3 c64)
+extern "C" int rb_spmm(int64_t n, int64_t nnz, const int32_t* row_ptr, const int32_t* col_idx, int nsys, int k,
+                       int dtype, const void* vals, const void* X, void* Y, void* stream) {
+  if (n <= 0 || nsys <= 0 || k <= 0) return RB_ERR_ARG;
+  int C = nsys * k;
+  if (C > 512) {
+    rb_set_error("rb_spmm: too many columns");
+    return RB_ERR_SIZE;
+  }
+  Csr m{row_ptr, col_idx, nullptr, n, nnz};
+  hipStream_t s = (hipStream_t)stream;
+  int64_t nb = grid_for(n, C);
+#define L_SP(G_, N_, TT)                                                                              \
+  hipLaunchKernelGGL((k_spmm<TT, G_, N_>), dim3(nb), dim3(KB), 0, s, m, (const TT*)vals, (const TT*)X, \
+                     (TT*)Y, C, k)
+  switch (dtype) {
+    case 0:
+      DISPATCH_GN(C, L_SP, double);
+      break;
+    case 1:
+      DISPATCH_GN(C, L_SP, cplx<double>);
+      break;
+    case 2:
+      DISPATCH_GN(C, L_SP, float);
+      break;
+    case 3:
+      DISPATCH_GN(C, L_SP, cplx<float>);
+      break;
+    default:
+      rb_set_error("rb_spmm: bad dtype %d", dtype);
+      return RB_ERR_ARG;
+  }
+#undef L_SP
+  RB_CHECK_LAUNCH();
+  return RB_OK;
+}

@@ -1,0 +1,312 @@
+"""FEM forward model on the GPU (drop-in for redbirdpy.forward).
+
+``runforward`` assembles every (wavelength, mode) system in one assembly
+launch, builds all right-hand sides in one dense device block
+[point src | wide src | point det | wide det] per system, and solves every
+column of every system in one multi-RHS Krylov solve; ``femgetdet`` and
+``jac`` read the device-resident fields.  Public functions take and return
+numpy/scipy objects like the reference; ``forward_device`` keeps results on
+the GPU for the reconstruction loop and for autograd.
+"""
+
+__all__ = ["runforward", "femlhs", "femrhs", "femgetdet", "jac", "jacchrome", "C0", "forward_device"]
+
+import numpy as np
+import torch
+from scipy import sparse
+
+from . import _gpu
+from .utility import sdmap, getoptodes, deldotdel, optode_counts
+from .property import extinction
+
+C0 = 299792458000.0  # speed of light in mm/s (forward.py:36)
+R_C0 = 1.0 / C0
+
+last_info = None
+
+
+def femlhs(cfg, deldotdel_mat, wavelength="", mode=1):
+    """FEM system matrix as scipy CSR, assembled on the GPU (forward.py:90-256)."""
+    mesh = _gpu.mesh_for(cfg)
+    sysm = _gpu.assemble(mesh, cfg, deldotdel_mat, [(wavelength, mode)])
+    indptr, indices = mesh.pattern_host()
+    data = sysm.vals64[0].cpu().numpy()
+    return sparse.csr_matrix((data, indices, indptr), shape=(mesh.nn, mesh.nn))
+
+
+def _locate(mesh, pts):
+    """GPU point location; returns (loc 1-based float with NaN, bary with NaN)."""
+    if pts is None or np.size(pts) == 0:
+        return np.zeros(0), np.zeros((0, 4))
+    l0, b = _gpu.tsearch(mesh, np.atleast_2d(pts)[:, :3])
+    loc = np.where(l0 >= 0, l0 + 1, np.nan).astype(float)
+    b = np.where((l0 >= 0)[:, None], b, np.nan)
+    return loc, b
+
+
+class _Optodes:
+    """Point-optode locations and wide-field patterns of one wavelength."""
+
+    def __init__(self, cfg, mesh, wv):
+        src, det, wsrc, wdet = getoptodes(cfg, wv)
+        self.ns, self.nd, self.ws, self.wd = optode_counts(cfg)
+        self.src = np.atleast_2d(src) if self.ns else None
+        self.det = np.atleast_2d(det) if self.nd else None
+        self.wsrc = np.asarray(wsrc) if self.ws else None
+        self.wdet = np.asarray(wdet) if self.wd else None
+        self.sloc, self.sbary = _locate(mesh, self.src)
+        self.dloc, self.dbary = _locate(mesh, self.det)
+        self.k = self.ns + self.ws + self.nd + self.wd
+
+    def rhs_csr(self, cfg, nn):
+        """RHS as scipy CSR (nn x k) in the reference's column order."""
+        e0 = cfg["elem"][:, :4].astype(np.int64) - 1
+        rows, cols, vals = [], [], []
+
+        def points(loc, bary, c0):
+            for i in range(len(loc)):
+                if not np.isnan(loc[i]):
+                    rows.append(e0[int(loc[i]) - 1])
+                    cols.append(np.full(4, c0 + i))
+                    vals.append(bary[i])
+
+        def wide(w, c0):
+            r, c = np.nonzero(w)
+            rows.append(r)
+            cols.append(c + c0)
+            vals.append(w[r, c])
+
+        points(self.sloc, self.sbary, 0)
+        if self.ws:
+            wide(self.wsrc, self.ns)
+        points(self.dloc, self.dbary, self.ns + self.ws)
+        if self.wd:
+            wide(self.wdet, self.ns + self.ws + self.nd)
+        if rows:
+            A = sparse.coo_matrix(
+                (np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(nn, self.k)
+            ).tocsr()
+            A.eliminate_zeros()
+            return A
+        return sparse.csr_matrix((nn, self.k))
+
+    def fill_block(self, mesh, B, c0, is_complex):
+        """Write this wavelength's columns into the device block B at column c0."""
+        if self.ns:
+            _gpu.rhs_block(mesh, B, c0, _loc0(self.sloc), self.sbary, is_complex)
+        if self.ws:
+            B[:, c0 + self.ns : c0 + self.ns + self.ws] = torch.as_tensor(self.wsrc, device=B.device).to(B.dtype)
+        if self.nd:
+            _gpu.rhs_block(mesh, B, c0 + self.ns + self.ws, _loc0(self.dloc), self.dbary, is_complex)
+        if self.wd:
+            c = c0 + self.ns + self.ws + self.nd
+            B[:, c : c + self.wd] = torch.as_tensor(self.wdet, device=B.device).to(B.dtype)
+
+    def det_columns(self, cfg, mesh):
+        """Detector columns as device CSC (ptr, row, val) for rb_getdet."""
+        e0 = cfg["elem"][:, :4].astype(np.int64) - 1
+        ptrs, rows, vals = [0], [], []
+        for i in range(self.nd):
+            if not np.isnan(self.dloc[i]):
+                rows.append(e0[int(self.dloc[i]) - 1])
+                vals.append(self.dbary[i])
+                ptrs.append(ptrs[-1] + 4)
+            else:
+                ptrs.append(ptrs[-1])
+        for j in range(self.wd):
+            r = np.flatnonzero(self.wdet[:, j])
+            rows.append(r)
+            vals.append(self.wdet[r, j])
+            ptrs.append(ptrs[-1] + len(r))
+        dev = mesh.dev
+        row = np.concatenate(rows) if rows else np.zeros(0, np.int64)
+        val = np.concatenate(vals) if vals else np.zeros(0)
+        return (
+            torch.as_tensor(np.array(ptrs, np.int32), device=dev),
+            torch.as_tensor(row.astype(np.int32), device=dev),
+            torch.as_tensor(val.astype(np.float64), device=dev),
+        )
+
+
+def _loc0(loc):
+    return np.where(np.isnan(loc), -1, np.nan_to_num(loc, nan=0) - 1).astype(np.int64)
+
+
+def femrhs(cfg, sd=None, wv="", md=1):
+    """RHS columns [point src | wide src | point det | wide det] (forward.py:259-362).
+
+    Returns (rhs csr, loc 1-based with NaN for wide-field, bary, optode).
+    """
+    mesh = _gpu.mesh_for(cfg)
+    op = _Optodes(cfg, mesh, wv)
+    nn = cfg["node"].shape[0]
+    if op.k == 0:
+        return sparse.csr_matrix((nn, 0)), np.array([]), np.array([]).reshape(0, 4), np.array([])
+    loc = np.full(op.k, np.nan)
+    bary = np.full((op.k, 4), np.nan)
+    loc[: op.ns] = op.sloc
+    bary[: op.ns] = op.sbary
+    d0 = op.ns + op.ws
+    loc[d0 : d0 + op.nd] = op.dloc
+    bary[d0 : d0 + op.nd] = op.dbary
+    opts = [x for x in (op.src, op.det) if x is not None]
+    optode = np.vstack(opts) if opts else np.array([])
+    return op.rhs_csr(cfg, nn), loc, bary, optode
+
+
+def femgetdet(phi, cfg, rhs, loc=None, bary=None):
+    """detval = rhs[:, det cols]^T phi[:, src cols], (ndet x nsrc) (forward.py:365-442)."""
+    ns, nd, ws, wd = optode_counts(cfg)
+    ts, td = ns + ws, nd + wd
+    if ts == 0 or td == 0:
+        return np.array([])
+    R = sparse.csc_matrix(rhs)[:, ts : ts + td] if sparse.issparse(rhs) else sparse.csc_matrix(np.asarray(rhs)[:, ts : ts + td])
+    R.sort_indices()
+    dev = _gpu.device()
+    phi_t = phi if torch.is_tensor(phi) else torch.as_tensor(np.ascontiguousarray(phi), device=dev)
+    phi_t = phi_t.to(dev)
+    is_complex = phi_t.is_complex() or np.iscomplexobj(R.data)
+    phi_t = phi_t.to(torch.complex128 if is_complex else torch.float64).contiguous()
+    if np.iscomplexobj(R.data):
+        raise NotImplementedError("complex detector weights are not supported")
+    out = _gpu.getdet(
+        torch.as_tensor(R.indptr.astype(np.int32), device=dev),
+        torch.as_tensor(R.indices.astype(np.int32), device=dev),
+        torch.as_tensor(R.data.astype(np.float64), device=dev),
+        phi_t, 0, ts, is_complex)
+    return out.cpu().numpy()
+
+
+class ForwardResult:
+    """Device-resident output of forward_device."""
+
+    def __init__(self, systems, k, X, optodes, detvals, info, is_complex):
+        self.systems = systems  # [(wv, mode)]
+        self.k = k  # columns per system
+        self.X = X  # (nn, nsys*k) device
+        self.optodes = optodes  # wv -> _Optodes
+        self.detvals = detvals  # [(ndet x nsrc) device tensors] per system
+        self.info = info
+        self.is_complex = is_complex
+
+    def phi(self, i):
+        return self.X[:, i * self.k : (i + 1) * self.k]
+
+
+def forward_device(cfg, rfcw=(1,), precision="fp64", tol=1e-12, maxiter=None, X0=None, solver_cache=None):
+    """Assemble, build RHS and solve all (wavelength, mode) systems on the GPU."""
+    if cfg.get("deldotdel") is None:
+        cfg["deldotdel"], _ = deldotdel(cfg)
+    wvs = list(cfg["prop"].keys()) if isinstance(cfg.get("prop"), dict) else [""]
+    systems = [(wv, md) for wv in wvs for md in rfcw]
+    mesh = _gpu.mesh_for(cfg)
+    sysm = _gpu.assemble(mesh, cfg, cfg["deldotdel"], systems, want32=(precision == "mixed"))
+    optodes = {wv: _Optodes(cfg, mesh, wv) for wv in wvs}
+    k = optodes[wvs[0]].k
+    nsys = len(systems)
+    dt = torch.complex128 if sysm.is_complex else torch.float64
+    B = _gpu.zeros((mesh.nn, nsys * k), dt, mesh.dev)
+    for i, (wv, md) in enumerate(systems):
+        optodes[wv].fill_block(mesh, B, i * k, sysm.is_complex)
+    if maxiter is None:
+        maxiter = max(4000, 4 * int(np.sqrt(mesh.nn)) + 1000)
+    if nsys * k <= 512:
+        X, info = _gpu.solve_system(sysm, B, k, precision, tol, maxiter, X0)
+    else:
+        X = torch.empty_like(B)
+        info = None
+        for i in range(nsys):
+            sub = _gpu.System(mesh, sysm.vals64[i : i + 1], None if sysm.vals32 is None else sysm.vals32[i : i + 1],
+                              1, sysm.is_complex)
+            x0 = None if X0 is None else X0[:, i * k : (i + 1) * k]
+            Xi, inf = _gpu.solve_system(sub, B[:, i * k : (i + 1) * k].contiguous(), k, precision, tol, maxiter, x0)
+            X[:, i * k : (i + 1) * k] = Xi
+            info = inf if info is None or inf.relres > info.relres else info
+    detvals = []
+    for i, (wv, md) in enumerate(systems):
+        op = optodes[wv]
+        ts = op.ns + op.ws
+        if ts == 0 or op.nd + op.wd == 0:
+            detvals.append(None)
+            continue
+        if not hasattr(op, "_detcols"):
+            op._detcols = op.det_columns(cfg, mesh)
+        p, r, v = op._detcols
+        detvals.append(_gpu.getdet(p, r, v, X[:, i * k : (i + 1) * k], 0, ts, sysm.is_complex))
+    return ForwardResult(systems, k, X, optodes, detvals, info, sysm.is_complex)
+
+
+def runforward(cfg, **kwargs):
+    """Forward simulation at all sources and wavelengths (forward.py:40-87).
+
+    Returns (detval, phi) with the reference's nesting: dicts by wavelength
+    when cfg['prop'] is a dict, by mode when rfcw has several entries.
+    Extra keywords: precision ('fp64' | 'mixed'), tol, maxiter.
+    """
+    global last_info
+    rfcw = kwargs.get("rfcw", [1])
+    if isinstance(rfcw, int):
+        rfcw = [rfcw]
+    res = forward_device(cfg, rfcw, kwargs.get("precision", "fp64"), kwargs.get("tol", 1e-12),
+                         kwargs.get("maxiter", None))
+    last_info = res.info
+    wvs = list(cfg["prop"].keys()) if isinstance(cfg.get("prop"), dict) else [""]
+    det = {md: {"detphi": {}} for md in rfcw}
+    phi = {md: {"phi": {}} for md in rfcw}
+    Xh = res.X.cpu().numpy()
+    for i, (wv, md) in enumerate(res.systems):
+        phi[md]["phi"][wv] = np.ascontiguousarray(Xh[:, i * res.k : (i + 1) * res.k])
+        dv = res.detvals[i]
+        det[md]["detphi"][wv] = dv.cpu().numpy() if dv is not None else np.array([])
+    if len(wvs) == 1:
+        for md in rfcw:
+            phi[md]["phi"] = phi[md]["phi"][wvs[0]]
+            det[md]["detphi"] = det[md]["detphi"][wvs[0]]
+    if len(rfcw) == 1:
+        phi = phi[rfcw[0]]["phi"]
+        det = det[rfcw[0]]["detphi"]
+    return det, phi
+
+
+def _active_pairs(sd):
+    sd = np.asarray(sd)
+    if sd.shape[1] >= 3:
+        sd = sd[sd[:, 2] == 1]
+    return sd[:, 0].astype(np.int64), sd[:, 1].astype(np.int64)
+
+
+def jac(sd, phi, deldotdel_mat, elem, evol, iselem=False, return_elem=True):
+    """mua Jacobian (forward.py:509-569): (Jnode nsd x nn, Jelem nsd x ne).
+
+    Rows are the active sd pairs in order.  ``return_elem=False`` skips the
+    element Jacobian (the reference always builds it) and returns None there.
+    """
+    s, d = _active_pairs(sd)
+    dev = _gpu.device()
+    phi_t = phi if torch.is_tensor(phi) else torch.as_tensor(np.ascontiguousarray(phi), device=dev)
+    phi_t = phi_t.to(dev)
+    if not phi_t.is_complex():
+        phi_t = phi_t.to(torch.float64)
+    mesh = _gpu.mesh_from_elem(elem, phi_t.shape[0], evol)
+    out = torch.complex128 if phi_t.is_complex() else torch.float64
+    JT, JE = _gpu.jac_node(mesh, phi_t.contiguous(), s, d, out, want_elem=return_elem)
+    Jn = JT.cpu().numpy().T
+    Je = JE.cpu().numpy().T if JE is not None else None
+    return Jn, Je
+
+
+def jacchrome(Jmua, chromophores, extin=None):
+    """Chromophore Jacobians: stack over wavelengths of E[w, c] * Jmua[w] (forward.py:572-589).
+
+    ``extin`` may replace the table with an (Nwv x Nchrome) matrix.  This
+    is a scaling and stacking of arrays already computed; the reconstruction
+    path never materialises it (it uses the Kronecker form of the Gram).
+    """
+    if not isinstance(Jmua, dict):
+        raise ValueError("Jmua must be a dict with wavelength keys")
+    wvs = list(Jmua.keys())
+    E = extinction(wvs, chromophores)[0] if extin is None else np.asarray(extin, dtype=float)
+    out = {}
+    for i, ch in enumerate(chromophores):
+        out[ch] = np.vstack([Jmua[wv] * E[j, i] for j, wv in enumerate(wvs)])
+    return out

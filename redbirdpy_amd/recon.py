@@ -1,0 +1,333 @@
+"""Gauss-Newton reconstruction on the GPU (drop-in for redbirdpy.recon on the hot path).
+
+``runrecon`` keeps the reference's control flow (recon.py:39-335): sync
+properties, forward-solve every wavelength, build Jacobians, form the
+normal equations and update the unknowns.  The dense algebra stays on the
+device:
+
+* the Jacobian of every wavelength is written node-major into one block
+  JT (nn x nwv*npairs) by the Jacobian kernel;
+* for chromophore unknowns the flattened Jacobian [E[w,c] J_w] is never
+  built: its Gram is (E E^T)[w,w'] * (J_w J_w'^T), one Gram over nn;
+* the regularised system s^2 J J^T + lambda I is diagonally normalised
+  (recon.py:784-789) and solved by a float64 Cholesky; with
+  ``precision='mixed'`` the Gram comes from the float32 Jacobian and the
+  solution is refined against float64 products J (J^T y).
+"""
+
+__all__ = ["runrecon", "reginv", "reginvover", "reginvunder", "matreform", "matflat", "syncprop"]
+
+import time
+import numpy as np
+import torch
+
+from . import _gpu
+from .forward import forward_device, _active_pairs
+from .utility import sdmap
+from .property import updateprop, extinction, CHROMOPHORES
+
+
+# ---------------------------------------------------------------------------
+# small host helpers with the reference's semantics
+# ---------------------------------------------------------------------------
+def _normalize_diag(A):
+    """Scale to unit diagonal: (D A D, d) with d = 1/sqrt(|diag|+1e-16) (recon.py:784-789)."""
+    if torch.is_tensor(A):
+        di = 1.0 / torch.sqrt(torch.abs(torch.diagonal(A)) + 1e-16)
+        return di[:, None] * A * di[None, :], di
+    di = 1.0 / np.sqrt(np.abs(np.diag(A)) + 1e-16)
+    return (di[:, None] * di[None, :]) * A, di
+
+
+def _flatten_detphi(detphi, sd, wavelengths, rfcw):
+    """Measurements flattened wavelength by wavelength (recon.py:792-815)."""
+    if isinstance(detphi, np.ndarray):
+        return detphi.flatten()
+    out = []
+    for wv in wavelengths:
+        p = detphi.get(wv, detphi) if isinstance(detphi, dict) else detphi
+        if isinstance(p, dict):
+            for md in rfcw:
+                out.extend(p.get(md, {}).get("detphi", p).flatten())
+        else:
+            out.extend(np.asarray(p).flatten())
+    return np.array(out)
+
+
+def matreform(Amat, ymeas, ymodel, form="complex"):
+    """Real-valued forms of the complex system (recon.py:512-567)."""
+    rhs = ymeas - ymodel
+    if form == "complex":
+        return Amat, rhs, 1
+    if form in ("real", "reim"):
+        if not np.isreal(rhs).all() and not np.isreal(Amat).all():
+            if form == "reim":
+                A = np.block([[np.real(Amat), -np.imag(Amat)], [np.imag(Amat), np.real(Amat)]])
+            else:
+                A = np.vstack([np.real(Amat), np.imag(Amat)])
+            return A, np.concatenate([np.real(rhs), np.imag(rhs)]), 2
+        return np.real(Amat), np.real(rhs), 1
+    if form == "logphase":
+        t = np.conj(ymodel) / np.abs(ymodel * ymodel)
+        t = t[:, None] * Amat if Amat.ndim == 2 else t * Amat
+        lr = np.log(np.abs(ymeas)) - np.log(np.abs(ymodel))
+        if np.isreal(ymodel).all():
+            return np.real(t), lr, 1
+        return np.vstack([np.real(t), np.imag(t)]), np.concatenate([lr, np.angle(ymeas) - np.angle(ymodel)]), 2
+    raise ValueError("Unknown form: %s" % form)
+
+
+def matflat(Amat, weight=None):
+    """Flatten a dict of Jacobian blocks (recon.py:570-593)."""
+    if isinstance(Amat, np.ndarray):
+        return Amat
+    if isinstance(Amat, dict):
+        keys = list(Amat.keys())
+        w = np.ones(len(keys)) if weight is None else weight
+        first = Amat[keys[0]]
+        if isinstance(first, dict):
+            inner = list(first.keys())
+            return np.vstack([np.hstack([Amat[k][wv] * w[j] for j, k in enumerate(keys)]) for wv in inner])
+        return np.hstack([Amat[k] * w[i] for i, k in enumerate(keys)])
+    return Amat
+
+
+def syncprop(cfg, recon):
+    """Copy recon unknowns to the forward mesh (recon.py:669-781, single-mesh branches)."""
+    if "param" in recon:
+        keys = list(recon["param"].keys())
+        first = recon["param"][keys[0]]
+        n = len(first) if hasattr(first, "__len__") else 1
+        if n < 50:
+            cfg["param"] = {k: (v.copy() if hasattr(v, "copy") else v) for k, v in recon["param"].items()}
+        else:
+            cfg.setdefault("param", {})
+            for k in keys:
+                cfg["param"][k] = recon["param"][k].copy()
+    elif "prop" in recon:
+        if isinstance(recon["prop"], dict):
+            cfg["prop"] = {k: v.copy() for k, v in recon["prop"].items()}
+        else:
+            cfg["prop"] = recon["prop"].copy()
+    if "mapid" in recon and "node" in recon:
+        raise NotImplementedError("dual-mesh reconstruction is not supported")
+    return cfg, recon
+
+
+# ---------------------------------------------------------------------------
+# regularised solves on the GPU (recon.py:338-509)
+# ---------------------------------------------------------------------------
+def _chol_solve_normalized(H, rhs):
+    """y = H^-1 rhs via the unit-diagonal normalisation and a float64 Cholesky."""
+    Hn, di = _normalize_diag(H)
+    L = torch.linalg.cholesky(Hn)
+    z = torch.cholesky_solve((di * rhs).reshape(-1, 1), L).reshape(-1)
+    return di * z
+
+
+def reginvunder(Amat, rhs, lambda_, invR=None, blocks=None, **kwargs):
+    """x = A^T (A A^T + lambda I)^-1 rhs (recon.py:429-509, no prior transform)."""
+    if invR is not None:
+        raise NotImplementedError("prior transforms are not supported")
+    dev = _gpu.device()
+    A = torch.as_tensor(np.ascontiguousarray(Amat, dtype=np.float64), device=dev)
+    b = torch.as_tensor(np.asarray(rhs, dtype=np.float64).ravel(), device=dev)
+    AT = A.t().contiguous()  # (N x m) node-major, the Gram kernel's layout
+    H = _gpu.gram(AT)
+    H.diagonal().add_(lambda_)
+    y = _chol_solve_normalized(H, b)
+    return (AT @ y).cpu().numpy()
+
+
+def reginvover(Amat, rhs, lambda_, LTL=None, blocks=None, **kwargs):
+    """x = (A^T A + lambda I)^-1 A^T rhs (recon.py:363-426, identity regulariser)."""
+    if LTL is not None:
+        raise NotImplementedError("prior matrices are not supported")
+    dev = _gpu.device()
+    A = torch.as_tensor(np.ascontiguousarray(Amat, dtype=np.float64), device=dev)
+    b = torch.as_tensor(np.asarray(rhs, dtype=np.float64).ravel(), device=dev)
+    H = _gpu.gram(A)
+    H.diagonal().add_(lambda_)
+    g = A.t() @ b
+    return _chol_solve_normalized(H, g).cpu().numpy()
+
+
+def reginv(Amat, rhs, lambda_, Areg=None, blocks=None, **kwargs):
+    """Over- or under-determined branch by shape (recon.py:338-360)."""
+    Areg = Areg or {}
+    if Amat.shape[0] >= Amat.shape[1]:
+        return reginvover(Amat, rhs, lambda_, Areg.get("ltl"), blocks, **kwargs)
+    return reginvunder(Amat, rhs, lambda_, Areg.get("lir"), blocks, **kwargs)
+
+
+# ---------------------------------------------------------------------------
+# Gauss-Newton loop
+# ---------------------------------------------------------------------------
+class GNStep:
+    """One normal-equation solve in Kronecker form.
+
+    Unknown blocks c (chromophores, or a single 'mua' block) share the node
+    dimension; row blocks w (wavelengths) hold npairs rows each.  The
+    flattened Jacobian is J[(w,p), (c,n)] = E[w,c] Jw[p,n].
+    """
+
+    def __init__(self, JT, npairs, E, lam, precision="fp64"):
+        self.JT = JT  # (nn, nw*npairs) float64 (or float32 for mixed)
+        self.np = npairs
+        self.E = torch.as_tensor(E, dtype=torch.float64, device=JT.device)  # (nw, nc)
+        self.lam = lam
+        self.precision = precision
+
+    def solve(self, misfit):
+        JT, npairs, E = self.JT, self.np, self.E
+        nw, nc = E.shape
+        m = nw * npairs
+        G = _gpu.gram(JT)  # (m, m) float64
+        W = E @ E.t()  # (nw, nw)
+        Wk = torch.repeat_interleave(torch.repeat_interleave(W, npairs, 0), npairs, 1)
+        H0 = G * Wk
+        s2 = 1.0 / torch.trace(H0)  # 1/||J_flat||_F^2
+        H = s2 * H0
+        H.diagonal().add_(self.lam)
+        y = _chol_solve_normalized(H, misfit)
+        # J_flat^T y = sum_w E[w, c] JT_w y_w
+        T = torch.stack([JT[:, w * npairs : (w + 1) * npairs].to(torch.float64) @ y[w * npairs : (w + 1) * npairs]
+                         for w in range(nw)], dim=1)  # (nn, nw)
+        dmu = s2 * (T @ E)  # (nn, nc)
+        return dmu
+
+
+def _chromophores(cfg):
+    if isinstance(cfg.get("prop"), dict) and "param" in cfg:
+        return [k for k in cfg["param"].keys() if k in CHROMOPHORES]
+    return []
+
+
+def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
+    """Iterative Gauss-Newton reconstruction (recon.py:39-335).
+
+    Supported on the GPU: a single-wavelength node-based mua unknown
+    (recon['prop'] with nn rows), and multispectral chromophore unknowns
+    (recon['param'] node arrays).  Extra keywords: precision ('fp64' |
+    'mixed'), extin (dict wavelength -> {chromophore: value} replacing the
+    extinction table), solver tolerance ``solvertol``.
+    Returns (recon, resid, cfg, updates, Jmua, detphi, phi) like the reference;
+    Jmua holds the node Jacobians per wavelength (numpy) only when
+    ``return_jac=True`` (they can be several GB).
+    """
+    maxiter = kwargs.get("maxiter", 5)
+    lam = kwargs.get("lambda_", recon.get("lambda", 0.05))
+    report = kwargs.get("report", True)
+    tol = kwargs.get("tol", 0)
+    reform = kwargs.get("reform", "real")
+    rfcw = kwargs.get("rfcw", [1])
+    precision = kwargs.get("precision", "fp64")
+    extin = kwargs.get("extin", None)
+    stol = kwargs.get("solvertol", 1e-12 if precision == "fp64" else 1e-10)
+    return_jac = kwargs.get("return_jac", False)
+    if isinstance(rfcw, int):
+        rfcw = [rfcw]
+    if rfcw != [1] or reform not in ("real", "complex"):
+        raise NotImplementedError("GPU runrecon supports rfcw=[1] with reform 'real'")
+    if kwargs.get("prior") or "lmat" in kwargs or "ltl" in kwargs:
+        raise NotImplementedError("structural priors are not supported")
+    if sd is None:
+        sd = sdmap(cfg)
+    if "prop" in recon and isinstance(recon["prop"], np.ndarray) and recon["prop"].ndim == 1:
+        recon["prop"] = recon["prop"].reshape(1, -1)
+    resid = np.zeros(maxiter)
+    updates = []
+    X0 = None
+    detphi_out = phi_out = Jm_out = None
+    for it in range(maxiter):
+        t0 = time.time()
+        if "param" in recon or "prop" in recon:
+            cfg, recon = syncprop(cfg, recon)
+        if "param" in cfg and isinstance(cfg.get("prop"), dict):
+            cfg["prop"] = updateprop(cfg, extin=extin)
+        res = forward_device(cfg, rfcw, precision, stol, None, X0)
+        X0 = res.X
+        wvs = list(cfg["prop"].keys()) if isinstance(cfg.get("prop"), dict) else [""]
+        chs = _chromophores(cfg)
+        mesh = _gpu.mesh_for(cfg)
+        npairs = None
+        JT = None
+        jdt = torch.float32 if precision == "mixed" else torch.float64
+        for i, wv in enumerate(wvs):
+            sdw = sd.get(wv, sd) if isinstance(sd, dict) else sd
+            s, d = _active_pairs(sdw)
+            if res.is_complex:
+                raise NotImplementedError("GPU runrecon supports CW (omega = 0) data")
+            if JT is None:
+                npairs = len(s)
+                JT = torch.empty((mesh.nn, len(wvs) * npairs), dtype=jdt, device=mesh.dev)
+            elif len(s) != npairs:
+                raise NotImplementedError("all wavelengths must have the same active pairs")
+            Jw, _ = _gpu.jac_node(mesh, res.phi(i), s, d, jdt)
+            JT[:, i * npairs : (i + 1) * npairs] = Jw
+        detphi = {wv: res.detvals[i].cpu().numpy() for i, wv in enumerate(wvs)}
+        if wvs == [""]:
+            detphi = detphi[""]
+        y0 = _flatten_detphi(detphi0, sd, wvs, rfcw)
+        y = _flatten_detphi(detphi, sd, wvs, rfcw)
+        misfit = np.real(y0 - y)
+        resid[it] = np.sum(np.abs(misfit))
+        if chs:
+            if extin is None:
+                E = extinction(wvs, chs)[0]
+            else:
+                E = np.array([[extin[wv][c] for c in chs] for wv in wvs], dtype=float)
+            keys = chs
+        else:
+            if len(wvs) != 1:
+                raise NotImplementedError("multi-wavelength mua reconstruction needs chromophore params")
+            E = np.ones((1, 1))
+            keys = ["mua"]
+        step = GNStep(JT, npairs, E, lam, precision)
+        dmu = step.solve(torch.as_tensor(misfit, dtype=torch.float64, device=mesh.dev)).cpu().numpy()
+        update = {}
+        for c, key in enumerate(keys):
+            dx = dmu[:, c]
+            update[key] = dx
+            if key == "mua":
+                if "prop" in recon and isinstance(recon["prop"], np.ndarray) and recon["prop"].shape[0] >= 50:
+                    recon["prop"][:, 0] += dx
+                else:
+                    raise NotImplementedError("label-based mua reconstruction is not supported")
+            else:
+                tgt = recon if ("param" in recon and key in recon["param"]) else cfg
+                if "param" not in tgt or key not in tgt["param"]:
+                    continue
+                pv = tgt["param"][key]
+                npar = len(pv) if hasattr(pv, "__len__") else 1
+                if npar < len(dx) and npar < 50:
+                    if npar == 1:
+                        if hasattr(pv, "__len__"):
+                            tgt["param"][key][0] += dx[0]
+                        else:
+                            tgt["param"][key] += dx[0]
+                    else:
+                        for li in range(min(npar, len(dx))):
+                            tgt["param"][key][li] += dx[li]
+                else:
+                    tgt["param"][key] = pv + dx
+        updates.append(update)
+        detphi_out = detphi
+        if report:
+            rel = resid[it] / resid[0] if it > 0 else 1.0
+            print("iter [%4d]: residual=%.6e, relres=%.6e lambda=%.6e (time=%.2f s)"
+                  % (it + 1, resid[it], rel, lam, time.time() - t0))
+        if it > 0 and abs(resid[it] - resid[it - 1]) / resid[0] < tol:
+            resid = resid[: it + 1]
+            break
+    if return_jac:
+        Jm_out = {}
+        for i, wv in enumerate(wvs):
+            Jm_out["mua" if wv == "" else wv] = JT[:, i * npairs : (i + 1) * npairs].t().cpu().numpy()
+    Xh = X0.cpu().numpy()
+    k = res.k
+    phi_out = {wv: Xh[:, i * k : (i + 1) * k] for i, wv in enumerate(wvs)}
+    if wvs == [""]:
+        phi_out = phi_out[""]
+    recon["lambda"] = lam
+    return recon, resid, cfg, updates, Jm_out, detphi_out, phi_out

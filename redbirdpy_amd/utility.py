@@ -1,0 +1,316 @@
+"""Mesh and optode utilities (drop-in for redbirdpy.utility on the hot path).
+
+elem/face stay 1-based in cfg, as in redbirdpy (utility.py:1-6).  The CSR
+connectivity (``rows``/``cols``/``idxcount``) is produced by the GPU pattern
+kernel; the remaining geometry is vectorised host code, as in the reference.
+"""
+
+__all__ = [
+    "meshprep",
+    "deldotdel",
+    "sdmap",
+    "getoptodes",
+    "getdistance",
+    "getltr",
+    "getreff",
+    "elem2node",
+    "forcearray",
+]
+
+import numpy as np
+
+_ARRAY_KEYS = ["node", "elem", "face", "srcpos", "srcdir", "detpos", "detdir", "prop", "seg", "widesrc", "widedet"]
+
+
+def forcearray(cfg, keys):
+    """List-valued entries become numpy arrays (utility.py:1112-1117)."""
+    for key in keys:
+        if key in cfg and isinstance(cfg[key], list):
+            cfg[key] = np.array(cfg[key])
+    return cfg
+
+
+def _tet_points(node, elem):
+    e0 = np.asarray(elem)[:, :4].astype(np.int64) - 1
+    return node[e0]  # (ne, 4, 3)
+
+
+def _signed_volume6(node, elem):
+    p = _tet_points(node, elem)
+    a = p[:, 1] - p[:, 0]
+    b = p[:, 2] - p[:, 0]
+    c = p[:, 3] - p[:, 0]
+    return np.einsum("ij,ij->i", np.cross(a, b), c)
+
+
+def reorient(node, elem):
+    """Swap local nodes 0 and 1 of tets with negative signed volume (utility.py:1047-1063)."""
+    out = np.array(elem, copy=True)
+    neg = _signed_volume6(node, elem) < 0
+    out[neg, 0], out[neg, 1] = elem[neg, 1], elem[neg, 0]
+    return out
+
+
+def boundary_faces(elem):
+    """Faces that belong to a single tet, outward-ordered, 1-based (utility.py:1066-1087).
+
+    Local faces (0,2,1), (0,1,3), (0,3,2), (1,2,3) are stacked face-type-major;
+    the result is ordered lexicographically by the sorted vertex triple.
+    """
+    e0 = np.asarray(elem)[:, :4].astype(np.int64) - 1
+    loc = np.array([[0, 2, 1], [0, 1, 3], [0, 3, 2], [1, 2, 3]])
+    tri = np.concatenate([e0[:, loc[k]] for k in range(4)], axis=0)
+    key = np.sort(tri, axis=1)
+    order = np.lexsort((key[:, 2], key[:, 1], key[:, 0]))
+    ks = key[order]
+    newgrp = np.ones(len(ks), dtype=bool)
+    newgrp[1:] = np.any(ks[1:] != ks[:-1], axis=1)
+    start = np.flatnonzero(newgrp)
+    cnt = np.diff(np.append(start, len(ks)))
+    single = start[cnt == 1]
+    return tri[order[single]] + 1
+
+
+def elemvolume(node, elem):
+    """|det|/6 for tets, half cross-product norm for triangles (utility.py:1090-1109)."""
+    e = np.asarray(elem)
+    if e.shape[1] >= 4:
+        return np.abs(_signed_volume6(node, e)) / 6.0
+    if e.shape[1] == 3:
+        e0 = e.astype(np.int64) - 1
+        cr = np.cross(node[e0[:, 1]] - node[e0[:, 0]], node[e0[:, 2]] - node[e0[:, 0]])
+        return 0.5 * np.sqrt(np.einsum("ij,ij->i", cr, cr))
+    raise ValueError("Unsupported element type with %d nodes" % e.shape[1])
+
+
+def nodevolume(nn, elem, evol):
+    """A quarter of each incident tet's volume (utility.py:1005-1017)."""
+    e0 = np.asarray(elem)[:, :4].astype(np.int64) - 1
+    nv = np.zeros(nn)
+    for j in range(4):
+        nv += np.bincount(e0[:, j], weights=evol, minlength=nn)
+    return nv * 0.25
+
+
+def deldotdel(cfg):
+    """V * grad(phi_i) . grad(phi_j) for the 10 upper-triangle pairs (utility.py:53-98).
+
+    Gradients are the cofactors of the vertex coordinates divided by 6V.
+    Returns (ne x 10, delphi 3 x 4 x ne).
+    """
+    node = cfg["node"]
+    evol = np.asarray(cfg["evol"], dtype=float)
+    p = _tet_points(node, cfg["elem"]).transpose(2, 1, 0)  # (3, 4, ne)
+    ne = p.shape[2]
+    cof = ((3, 1, 2, 1), (2, 0, 3, 2), (1, 3, 0, 3), (0, 2, 1, 0))
+    g = np.empty((3, 4, ne))
+    scale = 1.0 / (evol * 6.0)
+    for c in range(3):
+        u, v = [x for x in range(3) if x != c]
+        for i, (a, b, cc, d) in enumerate(cof):
+            g[c, i] = ((p[u, a] - p[u, b]) * (p[v, cc] - p[v, d]) - (p[u, cc] - p[u, d]) * (p[v, a] - p[v, b])) * scale
+    out = np.empty((ne, 10))
+    k = 0
+    for i in range(4):
+        for j in range(i, 4):
+            out[:, k] = np.einsum("cn,cn->n", g[:, i], g[:, j])
+            k += 1
+    out *= evol[:, None]
+    return out, g
+
+
+def getreff(n_in, n_out=1.0):
+    """Effective reflection coefficient, Haskell et al. 1994 (utility.py:879-904).
+
+    Riemann sums of step pi/2000 over the Fresnel reflectance, as in the reference.
+    """
+    if n_in <= n_out:
+        return 0.0
+    step = np.pi / 2000
+    crit = np.arcsin(n_out / n_in)
+    th = np.arange(0, crit, step)
+    ci = np.cos(th)
+    ct = np.sqrt(1 - (n_in * np.sin(th)) ** 2)
+    rperp = ((n_in * ct - n_out * ci) / (n_in * ct + n_out * ci)) ** 2
+    rpar = ((n_in * ci - n_out * ct) / (n_in * ci + n_out * ct)) ** 2
+    full = np.arange(0, np.pi / 2, step)
+    R = np.ones(len(full))
+    R[: len(th)] = 0.5 * rperp + 0.5 * rpar
+    s = np.sin(full)
+    c = np.cos(full)
+    r_phi = 2 * np.sum(s * c * R) * step
+    r_j = 3 * np.sum(s * c * c * R) * step
+    return (r_phi + r_j) / (2 - r_phi + r_j)
+
+
+def getltr(cfg, wv=""):
+    """Transport mean free path 1/(mua + mus(1-g)) of the bulk (utility.py:862-876)."""
+    from .property import getbulk
+
+    bk = getbulk(cfg)
+    if isinstance(bk, dict):
+        bk = bk[wv if wv else list(bk.keys())[0]]
+    return 1.0 / (bk[0] + bk[1] * (1 - bk[2]))
+
+
+def _tile_dir(d, n):
+    d = np.atleast_2d(np.asarray(d, dtype=float))
+    return np.tile(d, (n, 1)) if d.shape[0] == 1 else d
+
+
+def getoptodes(cfg, wv=""):
+    """Point optodes moved inward by dir * l_tr; wide-field patterns as given (utility.py:799-822)."""
+    ltr = getltr(cfg, wv)
+    src = det = None
+    if cfg.get("srcpos") is not None and np.size(cfg["srcpos"]) > 0:
+        sp = np.atleast_2d(cfg["srcpos"]).astype(float)
+        src = sp + _tile_dir(cfg["srcdir"], sp.shape[0]) * ltr
+    if cfg.get("detpos") is not None and np.size(cfg["detpos"]) > 0:
+        dp = np.atleast_2d(cfg["detpos"]).astype(float)
+        det = dp + _tile_dir(cfg.get("detdir", cfg["srcdir"]), dp.shape[0]) * ltr
+    return src, det, cfg.get("widesrc", None), cfg.get("widedet", None)
+
+
+def optode_counts(cfg):
+    """(point src, point det, wide src, wide det) counts as femrhs/femgetdet count them."""
+
+    def npos(key):
+        v = cfg.get(key)
+        if v is None:
+            return 0
+        v = np.atleast_2d(v)
+        return v.shape[0] if v.size > 0 else 0
+
+    def nwide(key):
+        v = cfg.get(key)
+        return v.shape[1] if v is not None and np.size(v) > 0 else 0
+
+    return npos("srcpos"), npos("detpos"), nwide("widesrc"), nwide("widedet")
+
+
+def getdistance(srcpos, detpos, badsrc=None, baddet=None, widesrc=None, widedet=None):
+    """(Ndet x Nsrc) source-detector distances, inf for excluded ones (utility.py:825-859)."""
+    badsrc = badsrc or []
+    baddet = baddet or []
+    srcpos = np.atleast_2d(srcpos)
+    detpos = np.atleast_2d(detpos)
+    ns, nd = srcpos.shape[0], detpos.shape[0]
+    ws = widesrc.shape[0] if widesrc is not None else 0
+    wd = widedet.shape[0] if widedet is not None else 0
+    dist = np.full((nd + wd, ns + ws), np.inf)
+    gs = sorted(set(range(ns)) - set(badsrc))
+    gd = sorted(set(range(nd)) - set(baddet))
+    if gs and gd:
+        diff = srcpos[gs, None, :3].transpose(1, 0, 2) - detpos[gd, None, :3]
+        dist[np.ix_(gd, gs)] = np.sqrt(np.sum(diff ** 2, axis=2))
+    return dist
+
+
+def sdmap(cfg, maxdist=np.inf, **kwargs):
+    """Source-detector pair table, row = d*Nsrc + s (utility.py:675-796).
+
+    Columns: [src column, det column (after all sources), active(, mode)];
+    multi-wavelength cfgs get a dict of 4-column tables.
+    """
+    ns, nd, ws, wd = optode_counts(cfg)
+    if ns + ws == 0 or nd + wd == 0:
+        raise ValueError("Must define at least one source and detector")
+    bads = list(kwargs.get("excludesrc", cfg.get("excludesrc", [])))
+    badd = list(kwargs.get("excludedet", cfg.get("excludedet", [])))
+    goods = sorted(set(range(ns)) - set(bads))
+    goodd = sorted(set(range(nd)) - set(badd))
+    off = ns + ws
+    srcs = goods + list(range(ns, ns + ws))
+    dets = [off + d for d in goodd] + [off + d for d in range(nd, nd + wd)]
+    ss, dd = np.meshgrid(srcs, dets)
+    pairs = np.column_stack([ss.ravel(), dd.ravel()])
+    act = np.ones(len(pairs))
+    if maxdist < np.inf and ns > 0 and nd > 0:
+        dist = getdistance(cfg["srcpos"], cfg["detpos"], bads, baddet=badd)
+        for r, (s, d) in enumerate(pairs):
+            di = int(d) - off
+            if s < ns and di < nd:
+                act[r] = 1.0 if dist[di, int(s)] < maxdist else 0.0
+    if isinstance(cfg.get("prop"), dict):
+        return {wv: np.column_stack([pairs, act, np.ones(len(pairs))]) for wv in cfg["prop"]}
+    return np.column_stack([pairs, act])
+
+
+def elem2node(elem, elemval, nodelen=None):
+    """Average element values onto nodes with weight 1/4 per incident tet (utility.py:910-929)."""
+    if isinstance(elem, dict):
+        nodelen = elem["node"].shape[0]
+        elem = elem["elem"]
+    e0 = np.asarray(elem)[:, :4].astype(np.int64) - 1
+    v = np.asarray(elemval, dtype=float)
+    v2 = v[:, None] if v.ndim == 1 else v
+    out = np.zeros((nodelen, v2.shape[1]))
+    for j in range(4):
+        np.add.at(out, e0[:, j], v2)
+    return (out * 0.25).squeeze()
+
+
+def _connectivity(cfg):
+    """rows/cols/idxcount of the off-diagonal node adjacency from the GPU CSR pattern."""
+    from . import _gpu
+
+    mesh = _gpu.mesh_for(cfg)
+    indptr, indices = mesh.pattern_host()
+    rows = np.repeat(np.arange(mesh.nn), np.diff(indptr))
+    off = indices != rows
+    return rows[off], indices[off], np.bincount(rows[off], minlength=mesh.nn)
+
+
+def meshprep(cfg):
+    """Derived mesh fields and the sd map (utility.py:101-246).
+
+    Adds reoriented elem, face, area, evol, nvol, reff, musp0, rows/cols/
+    idxcount/idxsum, deldotdel and omega to cfg (in place) and returns
+    (cfg, sd).
+    """
+    from .property import updateprop, getbulk
+
+    cfg = forcearray(cfg, _ARRAY_KEYS)
+    if "node" not in cfg or "elem" not in cfg:
+        raise ValueError("cfg must contain 'node' and 'elem'")
+    node = cfg["node"]
+    elem = cfg["elem"]
+    if elem.shape[1] > 4 and cfg.get("seg") is None:
+        cfg["seg"] = elem[:, 4].astype(int)
+    if not cfg.get("isreoriented", False):
+        ro = reorient(node, elem[:, :4])
+        cfg["elem"] = np.column_stack([ro, elem[:, 4:]]) if elem.shape[1] > 4 else ro
+        cfg["isreoriented"] = True
+    if cfg.get("face") is None:
+        cfg["face"] = boundary_faces(cfg["elem"][:, :4])
+    if cfg.get("area") is None:
+        cfg["area"] = elemvolume(node, cfg["face"])
+    if cfg.get("evol") is None:
+        cfg["evol"] = elemvolume(node, cfg["elem"][:, :4])
+    if np.any(cfg["evol"] == 0):
+        raise ValueError("Degenerate elements detected at indices: %s" % np.flatnonzero(cfg["evol"] == 0))
+    if cfg.get("nvol") is None:
+        cfg["nvol"] = nodevolume(node.shape[0], cfg["elem"], cfg["evol"])
+    if "srcpos" not in cfg:
+        raise ValueError("cfg.srcpos is required")
+    if "srcdir" not in cfg:
+        raise ValueError("cfg.srcdir is required")
+    if isinstance(cfg.get("prop"), dict) and "param" in cfg:
+        cfg["prop"] = updateprop(cfg)
+    if cfg.get("reff") is None:
+        bk = getbulk(cfg)
+        if isinstance(bk, dict):
+            cfg["reff"] = {wv: getreff(p[3], 1.0) for wv, p in bk.items()}
+            cfg["musp0"] = {wv: p[1] * (1 - p[2]) for wv, p in bk.items()}
+        else:
+            cfg["reff"] = getreff(bk[3], 1.0)
+            cfg["musp0"] = bk[1] * (1 - bk[2])
+    if cfg.get("cols") is None:
+        cfg["rows"], cfg["cols"], cfg["idxcount"] = _connectivity(cfg)
+    if cfg.get("idxsum") is None:
+        cfg["idxsum"] = np.cumsum(cfg["idxcount"])
+    if cfg.get("deldotdel") is None:
+        cfg["deldotdel"], _ = deldotdel(cfg)
+    if "omega" not in cfg:
+        cfg["omega"] = 0
+    return cfg, sdmap(cfg)

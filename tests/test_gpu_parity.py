@@ -1,0 +1,181 @@
+"""GPU parity of the HIP path against the CPU oracle (same seeded inputs).
+
+Integer work (CSR pattern, connectivity, point location) must be
+bit-exact; float64 values within the tolerances written in each test.
+"""
+
+import numpy as np
+import pytest
+from scipy import sparse
+
+from conftest import box_cfg, grid_optodes
+
+pytestmark = pytest.mark.gpu
+
+
+def _both(make):
+    import redbirdpy_amd as rb
+    from oracle import rb_oracle as orc
+
+    c1, sd1 = rb.meshprep(make())
+    c2, sd2 = orc.meshprep(make())
+    return rb, orc, c1, sd1, c2, sd2
+
+
+def _relmax(a, b):
+    return np.max(np.abs(a - b)) / np.max(np.abs(b))
+
+
+def test_meshprep_integer_bitexact():
+    rb, orc, c1, sd1, c2, sd2 = _both(lambda: box_cfg((12, 10, 6), 2.0))
+    assert np.array_equal(c1["elem"], c2["elem"])
+    assert np.array_equal(c1["face"], c2["face"])
+    assert np.array_equal(c1["idxcount"], c2["idxcount"])
+    assert np.array_equal(c1["rows"], c2["rows"])
+    assert np.array_equal(c1["cols"], c2["cols"])  # both sorted within each row
+    assert np.array_equal(sd1, sd2)
+    np.testing.assert_allclose(c1["evol"], c2["evol"], rtol=1e-14)
+    np.testing.assert_allclose(c1["area"], c2["area"], rtol=1e-14)
+    np.testing.assert_allclose(c1["nvol"], c2["nvol"], rtol=1e-13)
+    np.testing.assert_allclose(c1["deldotdel"], c2["deldotdel"], rtol=1e-12, atol=1e-14)
+    assert c1["reff"] == pytest.approx(c2["reff"], rel=1e-15)
+
+
+@pytest.mark.parametrize("branch", ["label", "elem", "node"])
+@pytest.mark.parametrize("omega", [0.0, 2 * np.pi * 1e8])
+def test_femlhs_matches_oracle(branch, omega):
+    def make():
+        cfg = box_cfg((12, 10, 6), 2.0, omega=omega)
+        rng = np.random.default_rng(7)
+        ne = cfg["elem"].shape[0]
+        nn = cfg["node"].shape[0]
+        if branch == "elem":
+            cfg["prop"] = np.column_stack([rng.uniform(0.005, 0.03, ne), rng.uniform(0.5, 1.5, ne),
+                                           np.zeros(ne), rng.uniform(1.3, 1.45, ne)])
+        elif branch == "node":
+            cfg["prop"] = np.column_stack([rng.uniform(0.005, 0.03, nn), rng.uniform(0.5, 1.5, nn),
+                                           np.zeros(nn), rng.uniform(1.3, 1.45, nn)])
+            cfg.pop("seg")
+        else:
+            lab = np.ones(ne, dtype=int)
+            lab[rng.random(ne) < 0.3] = 2
+            cfg["seg"] = lab
+            cfg["prop"] = np.array([[0, 0, 1, 1], [0.01, 1.0, 0, 1.37], [0.02, 1.2, 0, 1.37]])
+        return cfg
+
+    rb, orc, c1, sd1, c2, sd2 = _both(make)
+    A1 = rb.femlhs(c1, c1["deldotdel"])
+    A2 = orc.femlhs(c2, c2["deldotdel"])
+    A2.sort_indices()
+    assert np.array_equal(A1.indptr, A2.indptr)
+    assert np.array_equal(A1.indices, A2.indices)
+    assert np.iscomplexobj(A1.data) == np.iscomplexobj(A2.data)
+    # entry-wise, relative to the summed magnitude of the contributions
+    scale = np.max(np.abs(A2.data))
+    assert np.max(np.abs(A1.data - A2.data)) <= 1e-14 * scale
+
+
+def test_femlhs_multiwavelength():
+    def make():
+        cfg = box_cfg((12, 10, 6), 2.0)
+        cfg["prop"] = {"690": np.array([[0, 0, 1, 1], [0.012, 1.1, 0, 1.37]]),
+                       "830": np.array([[0, 0, 1, 1], [0.008, 0.9, 0, 1.37]])}
+        return cfg
+
+    rb, orc, c1, sd1, c2, sd2 = _both(make)
+    for wv in ("690", "830"):
+        A1 = rb.femlhs(c1, c1["deldotdel"], wavelength=wv)
+        A2 = orc.femlhs(c2, c2["deldotdel"], wavelength=wv)
+        assert np.max(np.abs((A1 - A2).data)) <= 1e-14 * np.max(np.abs(A2.data))
+
+
+def test_tsearch_and_femrhs():
+    def make():
+        p1 = (20, 16, 8)
+        return box_cfg(p1, 2.0, src=grid_optodes(p1, 3), det=grid_optodes(p1, 2) + [0.3, 0.7, 0])
+
+    rb, orc, c1, sd1, c2, sd2 = _both(make)
+    r1, l1, b1, o1 = rb.femrhs(c1, sd1)
+    r2, l2, b2, o2 = orc.femrhs(c2, sd2)
+    assert np.array_equal(np.isnan(l1), np.isnan(l2))
+    assert np.array_equal(l1[~np.isnan(l1)], l2[~np.isnan(l2)])
+    np.testing.assert_allclose(b1, b2, rtol=0, atol=1e-12)
+    assert abs(r1 - r2).max() <= 1e-12
+    np.testing.assert_allclose(o1, o2, rtol=1e-15)
+
+
+@pytest.mark.parametrize("omega", [0.0, 2 * np.pi * 1e8])
+def test_femsolve_vs_superlu(omega):
+    rb, orc, c1, sd1, c2, sd2 = _both(lambda: box_cfg((16, 16, 8), 2.0, omega=omega))
+    A = orc.femlhs(c2, c2["deldotdel"])
+    rhs, _, _, _ = orc.femrhs(c2, sd2)
+    xr, _ = orc.femsolve(A, rhs)
+    rhs_d = rhs.toarray()
+    rhs_d = np.column_stack([rhs_d, np.zeros(rhs_d.shape[0])])  # a zero column stays zero
+    x, flag = rb.femsolve(A, rhs_d)
+    assert flag == 0
+    assert np.all(x[:, -1] == 0)
+    err = np.linalg.norm(x[:, :-1] - xr) / np.linalg.norm(xr)
+    assert err < 1e-10, err
+
+
+def test_femsolve_mixed_precision():
+    rb, orc, c1, sd1, c2, sd2 = _both(lambda: box_cfg((16, 16, 8), 2.0, omega=2 * np.pi * 1e8))
+    A = orc.femlhs(c2, c2["deldotdel"])
+    rhs, _, _, _ = orc.femrhs(c2, sd2)
+    xr, _ = orc.femsolve(A, rhs)
+    x, flag = rb.femsolve(A, rhs, precision="mixed", tol=1e-11)
+    assert flag == 0
+    err = np.linalg.norm(x - xr) / np.linalg.norm(xr)
+    assert err < 1e-9, err
+
+
+@pytest.mark.parametrize("omega", [0.0, 2 * np.pi * 1e8])
+def test_runforward_matches_oracle(omega):
+    def make():
+        p1 = (20, 20, 10)
+        return box_cfg(p1, 2.0, src=grid_optodes(p1, 2), det=grid_optodes(p1, 3, margin=3.0), omega=omega)
+
+    rb, orc, c1, sd1, c2, sd2 = _both(make)
+    d1, p1 = rb.runforward(c1)
+    d2, p2 = orc.runforward(c2)
+    assert d1.shape == d2.shape and p1.shape == p2.shape
+    assert np.linalg.norm(p1 - p2) / np.linalg.norm(p2) < 1e-10
+    assert np.max(np.abs(d1 - d2) / np.abs(d2)) < 1e-9
+    # detector readout of the oracle field through the GPU kernel
+    r2, l2, b2, _ = orc.femrhs(c2, sd2)
+    g = rb.femgetdet(p2, c2, r2, l2, b2)
+    np.testing.assert_allclose(g, d2, rtol=1e-13)
+
+
+@pytest.mark.parametrize("omega", [0.0, 2 * np.pi * 1e8])
+def test_jac_matches_oracle(omega):
+    def make():
+        p1 = (16, 16, 8)
+        return box_cfg(p1, 2.0, src=grid_optodes(p1, 2), det=grid_optodes(p1, 3, margin=3.0), omega=omega)
+
+    rb, orc, c1, sd1, c2, sd2 = _both(make)
+    _, phi = orc.runforward(c2)
+    sd = sd2.copy()
+    sd[[1, 4], 2] = 0  # inactive pairs are dropped
+    Jn1, Je1 = rb.jac(sd, phi, c1["deldotdel"], c1["elem"], c1["evol"])
+    Jn2, Je2 = orc.jac(sd, phi, c2["deldotdel"], c2["elem"], c2["evol"])
+    assert Jn1.shape == Jn2.shape and Je1.shape == Je2.shape
+    assert np.linalg.norm(Jn1 - Jn2) / np.linalg.norm(Jn2) < 1e-12
+    assert np.linalg.norm(Je1 - Je2) / np.linalg.norm(Je2) < 1e-12
+
+
+def test_multiwavelength_forward():
+    def make():
+        p1 = (16, 16, 8)
+        cfg = box_cfg(p1, 2.0, src=grid_optodes(p1, 2), det=grid_optodes(p1, 2, margin=3.0))
+        cfg["prop"] = {"690": np.array([[0, 0, 1, 1], [0.012, 1.1, 0, 1.37]]),
+                       "830": np.array([[0, 0, 1, 1], [0.008, 0.9, 0, 1.37]])}
+        return cfg
+
+    rb, orc, c1, sd1, c2, sd2 = _both(make)
+    d1, p1 = rb.runforward(c1, sd=sd1)
+    d2, p2 = orc.runforward(c2, sd=sd2)
+    for wv in ("690", "830"):
+        assert np.linalg.norm(p1[wv] - p2[wv]) / np.linalg.norm(p2[wv]) < 1e-10
+        assert np.max(np.abs(d1[wv] - d2[wv]) / np.abs(d2[wv])) < 1e-9
