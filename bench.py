@@ -1,0 +1,188 @@
+"""Benchmark: multi-RHS frequency-domain forward solves on one MI355X.
+
+Workload (one "step" = one pass of the forward hot path): the 60x60x30 mm
+box at 1 mm spacing (115,351 nodes, 648,000 tets, CSR nnz 1,670,851) with
+a spherical inclusion, omega = 2 pi 100 MHz, 32 sources + 32 detectors.
+Each step assembles the complex FEM matrix on the GPU, fills the 64 RHS
+columns, solves all 64 columns in one multi-RHS COCG solve and reads the
+32x32 detector values.  The mesh, its CSR pattern and the located optodes
+are resident in HBM before the timed region.
+
+value   : RHS-solves/s of the mixed path (complex64 COCG + float64 residual
+          refinement), relative residual 1e-8 per column;
+extra   : the float64 path (complex128 COCG to 1e-12), the error of the
+          mixed solution against it, and, with --gn, Gauss-Newton
+          iterations/s of the 4-wavelength chromophore reconstruction.
+roofline: the SpMM kernel of the mixed path (q = A p with the p.q
+          reduction), algorithmic bytes per launch / mean launch duration
+          measured with hipEvents on its stream during the timed steps.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _args():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--h", type=float, default=1.0, help="mesh spacing in mm (1.0 = the 115,351-node mesh)")
+    p.add_argument("--tol-mixed", type=float, default=1e-8)
+    p.add_argument("--tol-fp64", type=float, default=1e-12)
+    p.add_argument("--skip-cpu", action="store_true")
+    p.add_argument("--skip-fp64", action="store_true")
+    p.add_argument("--gn", action="store_true", help="also time Gauss-Newton iterations")
+    p.add_argument("--cpu-h", type=float, default=2.0, help="mesh spacing of the bounded CPU sample")
+    return p.parse_args()
+
+
+def _spmm_bytes(nn, nnz, C, sv):
+    """Algorithmic HBM bytes of one q = A p launch: CSR once, p read once, q written once."""
+    return nnz * (sv + 4) + (nn + 1) * 4 + 2 * nn * C * sv
+
+
+def cpu_baseline(h):
+    """Oracle (reference algorithm, scipy SuperLU) on the same FD box at spacing h."""
+    from redbirdpy_amd import workloads
+    from oracle import rb_oracle as orc
+
+    cfg = workloads.fd_box(h)
+    cfg, sd = orc.meshprep(cfg)
+    rhs, loc, bary, _ = orc.femrhs(cfg, sd)
+    t0 = time.perf_counter()
+    A = orc.femlhs(cfg, cfg["deldotdel"])
+    x, _ = orc.femsolve(A, rhs)
+    det = orc.femgetdet(x, cfg, rhs, loc, bary)
+    dt = time.perf_counter() - t0
+    k = rhs.shape[1]
+    return {
+        "value": k / dt,
+        "unit": "RHS-solves/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": "oracle femlhs + SuperLU factor/solve of %d complex RHS + femgetdet on the same FD box at %.1f mm "
+                  "(%d nodes), one pass, %.1f s; SuperLU is single-threaded" % (k, h, cfg["node"].shape[0], dt),
+        "seconds": dt,
+        "nodes": int(cfg["node"].shape[0]),
+    }
+
+
+def main():
+    a = _args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (--gpus %d refused)" % a.gpus)
+        sys.exit(2)
+    import torch
+    import redbirdpy_amd as rb
+    from redbirdpy_amd import _gpu, _abi, workloads
+    from redbirdpy_amd.forward import forward_device, locate_optodes
+
+    torch.cuda.set_device(0)
+    cfg = workloads.fd_box(a.h)
+    cfg, sd = rb.meshprep(cfg)
+    mesh = _gpu.mesh_for(cfg)
+    opt = locate_optodes(cfg)
+    k = opt[""].k
+    L = _abi.lib()
+
+    def step(prec, tol):
+        return forward_device(cfg, [1], prec, tol, optodes=opt)
+
+    def timed(prec, tol, steps, warmup, prof=False):
+        for _ in range(warmup):
+            step(prec, tol)
+        torch.cuda.synchronize()
+        if prof:
+            L.rb_prof_reset()
+            L.rb_prof_enable(1)
+        t0 = time.perf_counter()
+        res = None
+        for _ in range(steps):
+            res = step(prec, tol)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        L.rb_prof_enable(0)
+        return res, dt
+
+    res_m, dt_m = timed("mixed", a.tol_mixed, a.steps, a.warmup, prof=True)
+    import ctypes
+
+    ms = ctypes.c_double(0)
+    cnt = ctypes.c_int64(0)
+    _abi.call("rb_prof_read", _abi.RB_PROF_SPMM, ctypes.byref(ms), ctypes.byref(cnt))
+    spmm_avg_s = ms.value / 1e3 / max(cnt.value, 1)
+    per_kernel = {}
+    for name, kid in (("spmm", 0), ("update", 1), ("pupdate", 2), ("resid64", 3), ("init", 4)):
+        m2 = ctypes.c_double(0)
+        c2 = ctypes.c_int64(0)
+        _abi.call("rb_prof_read", kid, ctypes.byref(m2), ctypes.byref(c2))
+        per_kernel[name] = {"launches_per_step": c2.value / a.steps, "ms_per_step": m2.value / a.steps}
+    C = k
+    nbytes = _spmm_bytes(mesh.nn, mesh.nnz, C, 8)  # complex64 inner solve
+    achieved = nbytes / spmm_avg_s / 1e9
+
+    out = {
+        "metric": "RHS-solves/s + GN iters/s @115k nodes: FP32+refine vs FP64, 1 GPU, <=1e-6",
+        "value": k / dt_m,
+        "unit": "RHS-solves/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": dt_m * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64+c128-refine",
+        "data": "synthetic",
+        "config": {
+            "workload": "FD forward, 60x60x30 mm Kuhn box at %.1f mm (%d nodes, %d tets, nnz %d), sphere inclusion, "
+                        "100 MHz, 32 src + 32 det = %d complex RHS in one multi-RHS solve" % (
+                            a.h, mesh.nn, mesh.ne, mesh.nnz, k),
+            "nodes": mesh.nn,
+            "rhs": k,
+            "tol_mixed": a.tol_mixed,
+        },
+        "mixed": {"iters": res_m.info.iters, "relres": res_m.info.relres, "flag": res_m.info.flag,
+                  "kernels": per_kernel},
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "k_cg_spmm<cplx<float>,64,1> (q = A p, p.q reduction)",
+            "bytes_per_launch": nbytes,
+            "avg_launch_us": spmm_avg_s * 1e6,
+        },
+    }
+    if not a.skip_fp64:
+        res_d, dt_d = timed("fp64", a.tol_fp64, max(1, a.steps // 2), 1)
+        phi_d = res_d.X
+        phi_m = res_m.X
+        err_phi = (torch.linalg.norm(phi_m - phi_d) / torch.linalg.norm(phi_d)).item()
+        dm, dd = res_m.detvals[0], res_d.detvals[0]
+        err_det = (torch.abs(dm - dd) / torch.abs(dd)).max().item()
+        out["fp64"] = {"rhs_solves_per_s": k / dt_d, "ms_per_step": dt_d * 1e3, "iters": res_d.info.iters,
+                       "relres": res_d.info.relres, "tol": a.tol_fp64}
+        out["mixed"]["err_phi_relL2_vs_fp64"] = err_phi
+        out["mixed"]["err_detval_maxrel_vs_fp64"] = err_det
+        out["speedup_mixed_vs_fp64"] = dt_d / dt_m
+    if not a.skip_cpu:
+        out["cpu_baseline"] = cpu_baseline(a.cpu_h)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -125,6 +125,21 @@ int rb_krylov_solve(int64_t n, int64_t nnz, const int32_t* row_ptr, const int32_
                     int check_every, void* work, size_t work_bytes, int* iters_out /* host */,
                     double* relres_out /* host */, int* flag_out /* host */, void* stream);
 
+/*
+ * Per-kernel timing of the Krylov loop: when enabled, hipEvents are
+ * recorded on the launch stream around every launch of the kernels below;
+ * rb_prof_read returns the summed milliseconds and the launch count.
+ */
+#define RB_PROF_SPMM 0    /* q = A p with the p.q reduction */
+#define RB_PROF_UPDATE 1  /* x, r update with the r.z and r.r reductions */
+#define RB_PROF_PUPDATE 2 /* p = z + beta p */
+#define RB_PROF_RESID 3   /* float64 residual of the mixed-precision refinement */
+#define RB_PROF_INIT 4    /* initial residual */
+#define RB_PROF_NKERNELS 5
+int rb_prof_enable(int on);
+int rb_prof_reset(void);
+int rb_prof_read(int id, double* total_ms /* host */, int64_t* count /* host */);
+
 /* Y = A_s X for each system s (dtype RB_F64/RB_C128/RB_F32/RB_C64). */
 int rb_spmm(int64_t n, int64_t nnz, const int32_t* row_ptr, const int32_t* col_idx, int nsys, int k, int dtype,
             const void* vals, const void* X, void* Y, void* stream);

@@ -19,6 +19,7 @@ RB_PROP_VOLUME = 2
 RB_PREC_FP64 = 0
 RB_PREC_MIXED = 1
 RB_F64, RB_C128, RB_F32, RB_C64 = 0, 1, 2, 3
+RB_PROF_SPMM, RB_PROF_UPDATE, RB_PROF_PUPDATE, RB_PROF_RESID, RB_PROF_INIT = 0, 1, 2, 3, 4
 
 _i32 = ctypes.c_int32
 _i64 = ctypes.c_int64
@@ -51,6 +52,9 @@ _SIGS = {
     "rb_jac_contract_grid": [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _int, _int, _int, _vp, _vp, _vp, _vp,
                              _vp, _vp, _vp],
     "rb_gram": [_i64, _i64, _vp, _i64, _int, _vp, _int, _vp, _vp],
+    "rb_prof_enable": [_int],
+    "rb_prof_reset": [],
+    "rb_prof_read": [_int, ctypes.POINTER(_dbl), ctypes.POINTER(_i64)],
 }
 _RESTYPE = {"rb_last_error": ctypes.c_char_p}
 

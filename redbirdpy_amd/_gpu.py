@@ -155,11 +155,11 @@ def props_for(cfg, wavelength="", mode=1):
 
 
 def expand_props(cfg, props):
-    """Per-element or per-node (mua, musp, nref, kind) arrays for the kernel.
+    """Property layout as femlhs chooses it (forward.py:128-142).
 
-    Mirrors femlhs's choice (forward.py:128-142): rows == nn or ne use the
-    rows directly (element branch iff len == ne); otherwise a label table is
-    indexed by cfg['seg'] and n is taken from the first element's label.
+    Returns (kind, plen, arrays) where arrays is ("rows", mua, musp, nref)
+    for per-node/per-element tables, or ("label", table, seg) for a label
+    table indexed by cfg['seg'] (n taken from the first element's label).
     """
     nn = cfg["node"].shape[0]
     ne = cfg["elem"].shape[0]
@@ -172,15 +172,29 @@ def expand_props(cfg, props):
         musp = props[:, 1] * (1 - props[:, 2]) if nc >= 3 else props[:, 1]
         nref = props[:, 3] if nc >= 4 else np.array([1.37])
         kind = _abi.RB_PROP_ELEM if len(mua) == ne else _abi.RB_PROP_NODE
-    elif seg is not None:
-        s = np.clip(np.asarray(seg).astype(np.int64), 0, props.shape[0] - 1)
-        mua = props[s, 0]
-        musp = props[s, 1] * (1 - props[s, 2]) if nc >= 3 else props[s, 1]
-        nref = np.array([props[s[0], 3] if nc >= 4 else 1.37])
-        kind = _abi.RB_PROP_ELEM
-    else:
-        raise ValueError("Property format not recognized")
-    return np.asarray(mua, float), np.asarray(musp, float), np.asarray(nref, float), kind
+        return kind, len(mua), ("rows", np.asarray(mua, float), np.asarray(musp, float), np.asarray(nref, float))
+    if seg is not None:
+        seg = np.asarray(seg)
+        if seg.shape[0] != ne:
+            raise ValueError("label properties need one seg label per element")
+        tab = np.zeros((props.shape[0], 3))
+        tab[:, 0] = props[:, 0]
+        tab[:, 1] = props[:, 1] * (1 - props[:, 2]) if nc >= 3 else props[:, 1]
+        s0 = int(np.clip(seg[0], 0, props.shape[0] - 1))
+        tab[:, 2] = props[s0, 3] if nc >= 4 else 1.37
+        return _abi.RB_PROP_ELEM, ne, ("label", tab, seg)
+    raise ValueError("Property format not recognized")
+
+
+def _seg_device(mesh, seg, nlabel):
+    """Clipped int64 labels on the device, cached on the seg array identity and sum."""
+    key = (id(seg), seg.shape[0], int(np.sum(seg)), nlabel)
+    c = getattr(mesh, "_seg_cache", None)
+    if c is not None and c[0] == key and c[2] is seg:
+        return c[1]
+    sd = torch.as_tensor(np.clip(seg.astype(np.int64), 0, nlabel - 1), device=mesh.dev)
+    mesh._seg_cache = (key, sd, seg)
+    return sd
 
 
 class System:
@@ -201,28 +215,37 @@ def assemble(mesh, cfg, deldotdel, systems, want32=False):
     """
     dev = mesh.dev
     mesh.set_geometry(deldotdel, cfg["evol"], cfg.get("area"))
-    muas, musps, nrefs, oms, bcs = [], [], [], [], []
-    kind = None
+    cols = {"mua": [], "musp": [], "nref": []}
+    oms, bcs = [], []
+    kind = plen = None
+    nlen = 1
     for wv, md in systems:
         props, reff, om = props_for(cfg, wv, md)
-        mua, musp, nref, k = expand_props(cfg, props)
+        k, pl, arr = expand_props(cfg, props)
         if kind is None:
-            kind = k
-        elif kind != k:
+            kind, plen = k, pl
+        elif kind != k or plen != pl:
             raise ValueError("all systems of one assembly must share the property layout")
-        muas.append(mua)
-        musps.append(musp)
-        nrefs.append(nref)
+        if arr[0] == "label":
+            tab = torch.as_tensor(arr[1], device=dev)
+            seg = _seg_device(mesh, arr[2], arr[1].shape[0])
+            rows = tab[seg]  # (ne, 3) gather on the device
+            cols["mua"].append(rows[:, 0])
+            cols["musp"].append(rows[:, 1])
+            cols["nref"].append(tab[:1, 2])  # n of the first element's label, as femlhs uses
+        else:
+            cols["mua"].append(_f64(arr[1], dev))
+            cols["musp"].append(_f64(arr[2], dev))
+            cols["nref"].append(_f64(arr[3], dev))
+            nlen = max(nlen, len(arr[3]))
         oms.append(om)
         bcs.append((1 - reff) / (12.0 * (1 + reff)))
-    plen = len(muas[0])
-    nlen = max(len(x) for x in nrefs)
-    nref_m = np.stack([np.broadcast_to(x, (nlen,)) for x in nrefs])
     nsys = len(systems)
+    nref_rows = [x if x.numel() == nlen else x.expand(nlen) for x in cols["nref"]]
+    mua_d = torch.stack(cols["mua"]).contiguous()
+    musp_d = torch.stack(cols["musp"]).contiguous()
+    nref_d = torch.stack(nref_rows).contiguous()
     is_complex = any(o > 0 for o in oms)
-    mua_d = _f64(np.stack(muas), dev)
-    musp_d = _f64(np.stack(musps), dev)
-    nref_d = _f64(nref_m, dev)
     om_d = _f64(np.array(oms), dev)
     bc_d = _f64(np.array(bcs), dev)
     shape = (nsys, mesh.nnz)

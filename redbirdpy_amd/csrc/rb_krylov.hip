@@ -563,6 +563,82 @@ static int poll_nactive(const int* d_nactive, hipStream_t s, int* out) {
   return RB_OK;
 }
 
+// ---------------------------------------------------------------------------
+// optional per-kernel timing: hipEvents recorded around each launch on the
+// launch stream, read back by rb_prof_read (RB_PROF_* ids in rbhip.h)
+// ---------------------------------------------------------------------------
+static int g_prof = 0;
+static double g_prof_ms[RB_PROF_NKERNELS];
+static int64_t g_prof_cnt[RB_PROF_NKERNELS];
+static std::vector<std::pair<hipEvent_t, hipEvent_t>> g_prof_pending[RB_PROF_NKERNELS];
+static std::vector<hipEvent_t> g_prof_pool;
+
+static hipEvent_t prof_event() {
+  hipEvent_t e = nullptr;
+  if (!g_prof_pool.empty()) {
+    e = g_prof_pool.back();
+    g_prof_pool.pop_back();
+  } else if (hipEventCreate(&e) != hipSuccess) {
+    e = nullptr;
+  }
+  return e;
+}
+
+struct ProfScope {
+  int id;
+  hipStream_t s;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(int id_, hipStream_t s_) : id(id_), s(s_) {
+    if (g_prof) {
+      a = prof_event();
+      b = prof_event();
+      if (a && b) (void)hipEventRecord(a, s);
+    }
+  }
+  ~ProfScope() {
+    if (g_prof && a && b) {
+      (void)hipEventRecord(b, s);
+      g_prof_pending[id].push_back({a, b});
+    }
+  }
+};
+
+extern "C" int rb_prof_enable(int on) {
+  g_prof = on ? 1 : 0;
+  return RB_OK;
+}
+
+extern "C" int rb_prof_reset(void) {
+  for (int i = 0; i < RB_PROF_NKERNELS; ++i) {
+    for (auto& pr : g_prof_pending[i]) {
+      (void)hipEventSynchronize(pr.second);
+      g_prof_pool.push_back(pr.first);
+      g_prof_pool.push_back(pr.second);
+    }
+    g_prof_pending[i].clear();
+    g_prof_ms[i] = 0;
+    g_prof_cnt[i] = 0;
+  }
+  return RB_OK;
+}
+
+extern "C" int rb_prof_read(int id, double* total_ms, int64_t* count) {
+  if (id < 0 || id >= RB_PROF_NKERNELS || !total_ms || !count) return RB_ERR_ARG;
+  for (auto& pr : g_prof_pending[id]) {
+    RB_HIP(hipEventSynchronize(pr.second));
+    float ms = 0;
+    RB_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
+    g_prof_ms[id] += ms;
+    g_prof_cnt[id] += 1;
+    g_prof_pool.push_back(pr.first);
+    g_prof_pool.push_back(pr.second);
+  }
+  g_prof_pending[id].clear();
+  *total_ms = g_prof_ms[id];
+  *count = g_prof_cnt[id];
+  return RB_OK;
+}
+
 #define L_INIT(G_, N_, ...) hipLaunchKernelGGL((k_cg_init<T, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
 #define L_SPMM(G_, N_, ...) hipLaunchKernelGGL((k_cg_spmm<T, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
 #define L_UPD(G_, N_, ...) hipLaunchKernelGGL((k_cg_update<T, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
@@ -576,7 +652,10 @@ static int run_cg(const Csr& m, const T* vals, const T* dinv, const T* B, T* X, 
   int64_t ng = ceil_div(nb, RED_GS);
   int rc = rb_fill_i32(st.cnt1, 0, ng + 1, s);
   if (rc) return rc;
-  DISPATCH_GN(C, L_INIT, m, vals, dinv, B, X, use_x0, w.R, w.P, C, k, st);
+  {
+    ProfScope ps(RB_PROF_INIT, s);
+    DISPATCH_GN(C, L_INIT, m, vals, dinv, B, X, use_x0, w.R, w.P, C, k, st);
+  }
   RB_CHECK_LAUNCH();
   int nact = 0;
   rc = poll_nactive(st.nactive, s, &nact);
@@ -585,9 +664,18 @@ static int run_cg(const Csr& m, const T* vals, const T* dinv, const T* B, T* X, 
   while (nact > 0 && it < st.maxiter) {
     int chunk = std::min(check_every, st.maxiter - it);
     for (int j = 0; j < chunk; ++j) {
-      DISPATCH_GN(C, L_SPMM, m, vals, w.P, w.Q, C, k, st);
-      DISPATCH_GN(C, L_UPD, m.n, dinv, X, w.R, w.P, w.Q, C, k, st);
-      DISPATCH_GN(C, L_PUP, m.n, dinv, w.R, w.P, C, k, st);
+      {
+        ProfScope ps(RB_PROF_SPMM, s);
+        DISPATCH_GN(C, L_SPMM, m, vals, w.P, w.Q, C, k, st);
+      }
+      {
+        ProfScope ps(RB_PROF_UPDATE, s);
+        DISPATCH_GN(C, L_UPD, m.n, dinv, X, w.R, w.P, w.Q, C, k, st);
+      }
+      {
+        ProfScope ps(RB_PROF_PUPDATE, s);
+        DISPATCH_GN(C, L_PUP, m.n, dinv, w.R, w.P, C, k, st);
+      }
     }
     RB_CHECK_LAUNCH();
     it += chunk;
@@ -678,7 +766,10 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
     if (rc) return rc;
 #define L_ORES(G_, N_, ...) \
   hipLaunchKernelGGL((k_outer_resid<T64, T32, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
-    DISPATCH_GN(C, L_ORES, m, A64, B, X, (outer > 0 || use_x0) ? 1 : 0, w.Q, C, k, outer, max_outer, sout);
+    {
+      ProfScope ps(RB_PROF_RESID, s);
+      DISPATCH_GN(C, L_ORES, m, A64, B, X, (outer > 0 || use_x0) ? 1 : 0, w.Q, C, k, outer, max_outer, sout);
+    }
 #undef L_ORES
     RB_CHECK_LAUNCH();
     if (outer == 0 && !use_x0) {

@@ -9,7 +9,7 @@ numpy/scipy objects like the reference; ``forward_device`` keeps results on
 the GPU for the reconstruction loop and for autograd.
 """
 
-__all__ = ["runforward", "femlhs", "femrhs", "femgetdet", "jac", "jacchrome", "C0", "forward_device"]
+__all__ = ["runforward", "femlhs", "femrhs", "femgetdet", "jac", "jacchrome", "C0"]
 
 import numpy as np
 import torch
@@ -193,7 +193,14 @@ class ForwardResult:
         return self.X[:, i * self.k : (i + 1) * self.k]
 
 
-def forward_device(cfg, rfcw=(1,), precision="fp64", tol=1e-12, maxiter=None, X0=None, solver_cache=None):
+def locate_optodes(cfg):
+    """Located optodes per wavelength (reusable by forward_device while optodes and bulk optics stay fixed)."""
+    wvs = list(cfg["prop"].keys()) if isinstance(cfg.get("prop"), dict) else [""]
+    mesh = _gpu.mesh_for(cfg)
+    return {wv: _Optodes(cfg, mesh, wv) for wv in wvs}
+
+
+def forward_device(cfg, rfcw=(1,), precision="fp64", tol=1e-12, maxiter=None, X0=None, optodes=None):
     """Assemble, build RHS and solve all (wavelength, mode) systems on the GPU."""
     if cfg.get("deldotdel") is None:
         cfg["deldotdel"], _ = deldotdel(cfg)
@@ -201,7 +208,8 @@ def forward_device(cfg, rfcw=(1,), precision="fp64", tol=1e-12, maxiter=None, X0
     systems = [(wv, md) for wv in wvs for md in rfcw]
     mesh = _gpu.mesh_for(cfg)
     sysm = _gpu.assemble(mesh, cfg, cfg["deldotdel"], systems, want32=(precision == "mixed"))
-    optodes = {wv: _Optodes(cfg, mesh, wv) for wv in wvs}
+    if optodes is None:
+        optodes = {wv: _Optodes(cfg, mesh, wv) for wv in wvs}
     k = optodes[wvs[0]].k
     nsys = len(systems)
     dt = torch.complex128 if sysm.is_complex else torch.float64
