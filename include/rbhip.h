@@ -192,6 +192,17 @@ int rb_jac_contract_grid(int64_t nn, int64_t ne, const int32_t* el, const double
 int rb_gram(int64_t K, int64_t m, const void* JT, int64_t ldj, int in_dtype, const double* d, int mode, double* G,
             void* stream);
 
+/*
+ * Gram on the matrix cores: mode 1 = bf16 (hi*hi), mode 3 = bf16x3 split
+ * (hi*hi + hi*lo + lo*hi, ~16 mantissa bits per product), fp32 MFMA
+ * accumulation flushed to fp64 every 256 k, K split in 4 chunks summed in
+ * a fixed order.  work: rb_gram_workspace() bytes (4 m^2 doubles).
+ * Same result layout as rb_gram (redbirdpy recon.py:474).
+ */
+int rb_gram_workspace(int64_t K, int64_t m, int mode, size_t* bytes /* host */);
+int rb_gram_mfma(int64_t K, int64_t m, const void* JT, int64_t ldj, int in_dtype, const double* d, int mode,
+                 double* G, void* work, size_t work_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

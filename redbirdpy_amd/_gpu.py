@@ -441,10 +441,21 @@ def jac_contract(mesh, phi, S, D, w):
     return g
 
 
-def gram(JT, d=None, mode=0):
-    """G = diag(d) JT^T JT diag(d) in float64 (m x m)."""
+GRAM_MODES = {"fp64": 0, "bf16": 1, "bf16x3": 3}
+
+
+def gram(JT, d=None, mode="fp64"):
+    """G = diag(d) JT^T JT diag(d) in float64 (m x m); mode fp64 | bf16 | bf16x3 (MFMA)."""
     K, m = JT.shape
     G = torch.empty((m, m), dtype=torch.float64, device=JT.device)
     code = _abi.RB_F64 if JT.dtype == torch.float64 else _abi.RB_F32
-    call("rb_gram", K, m, ptr(JT), JT.stride(0), code, ptr(d), mode, ptr(G), _abi.stream_ptr())
+    md = GRAM_MODES[mode] if isinstance(mode, str) else int(mode)
+    if md == 0:
+        call("rb_gram", K, m, ptr(JT), JT.stride(0), code, ptr(d), 0, ptr(G), _abi.stream_ptr())
+        return G
+    nb = ctypes.c_size_t(0)
+    call("rb_gram_workspace", K, m, md, ctypes.byref(nb))
+    work = torch.empty(int(nb.value), dtype=torch.uint8, device=JT.device)
+    call("rb_gram_mfma", K, m, ptr(JT), JT.stride(0), code, ptr(d), md, ptr(G), ptr(work), nb.value,
+         _abi.stream_ptr())
     return G

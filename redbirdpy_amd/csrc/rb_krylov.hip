@@ -10,9 +10,12 @@
 // Layout: X, R, P, Q are row-major n x C (ld = C), so one CSR row gathers
 // contiguous C-wide rows of P; a group of G lanes (G = 64 when C > 16)
 // covers the columns of one row in NCH chunks.  Per-column dot products are
-// reduced in double precision: lanes -> LDS -> per-block partials -> the
-// last block of each group of 32 blocks -> the last group, in a fixed
-// order, so every run gives the same bits.
+// reduced in double precision without atomics or fences: lanes -> LDS ->
+// per-block partials (plain stores) -> a finish kernel with one block per
+// column that sums the block partials in a fixed tree order at the next
+// kernel boundary.  Every run gives the same bits, and no kernel issues an
+// agent-scope fence (which on gfx950 writes back and invalidates the whole
+// L2 that the SpMM gather relies on).
 //
 // Precision modes: plain (everything in the 64-bit type) and mixed (inner
 // solves in the 32-bit type on a 32-bit copy of A; outer FP64 residual
@@ -24,9 +27,10 @@
 #include <math.h>
 
 #define KB 256       // threads per block
-#define RED_GS 32    // blocks per reduction group
 #define MAXND 3      // dot products per kernel
-#define NBMAX 1024   // blocks per launch (grid-stride over rows)
+#define NBMAX 2048   // blocks per launch (grid-stride over rows)
+#define FB 256       // threads of a finish block
+#define UNR 8        // CSR entries gathered per step
 
 template <typename A>
 struct ColState {
@@ -39,10 +43,8 @@ struct ColState {
   int* iters;
   int* flag;
   int* nactive;
-  A* part;    // [nblocks][MAXND][C]
-  A* gpart;   // [ngroups][MAXND][C]
-  int* cnt1;  // [ngroups]
-  int* cnt2;  // [1]
+  A* part;  // [nblocks][MAXND][C]
+  int nb;   // producer blocks whose partials are valid
   double tol2;
   int maxiter;
 };
@@ -53,65 +55,6 @@ struct Csr {
   const int32_t* diag;
   int64_t n, nnz;
 };
-
-// ---------------------------------------------------------------------------
-// deterministic cross-block reduction (see file comment); fin(c, v[]) runs
-// once per column in the last block, which then recounts active columns.
-// ---------------------------------------------------------------------------
-template <typename A, typename Fin>
-__device__ void cross_block_finish(const ColState<A>& st, int nd, int C, Fin fin) {
-  __shared__ int s_last;
-  __shared__ int s_cnt;
-  const int nblocks = gridDim.x;
-  const int ngroups = (nblocks + RED_GS - 1) / RED_GS;
-  const int grp = blockIdx.x / RED_GS;
-  const int gsize = min(RED_GS, nblocks - grp * RED_GS);
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int old = atomicAdd(&st.cnt1[grp], 1);
-    s_last = (old == gsize - 1);
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  const int64_t bstride = (int64_t)MAXND * C;
-  for (int i = threadIdx.x; i < nd * C; i += blockDim.x) {
-    const A* p = st.part + (int64_t)(grp * RED_GS) * bstride + i;
-    A acc = coherent_load(p);
-    for (int b = 1; b < gsize; ++b) acc = acc_add(acc, coherent_load(p + b * bstride));
-    st.gpart[(int64_t)grp * bstride + i] = acc;
-  }
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    st.cnt1[grp] = 0;
-    int old = atomicAdd(st.cnt2, 1);
-    s_last = (old == ngroups - 1);
-    s_cnt = 0;
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  int loc = 0;
-  for (int c = threadIdx.x; c < C; c += blockDim.x) {
-    A v[MAXND];
-    for (int d = 0; d < nd; ++d) {
-      const A* p = st.gpart + d * C + c;
-      A acc = coherent_load(p);
-      for (int g = 1; g < ngroups; ++g) acc = acc_add(acc, coherent_load(p + g * bstride));
-      v[d] = acc;
-    }
-    fin(c, v);
-    loc += st.active[c] ? 1 : 0;
-  }
-  atomicAdd(&s_cnt, loc);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    *st.nactive = s_cnt;
-    *st.cnt2 = 0;
-  }
-}
 
 // per-thread partials pv[ch][d] (column lane + ch*G) -> part[blockIdx][d][c]
 template <typename A, int G, int NCH>
@@ -137,16 +80,72 @@ __device__ void block_partials(const ColState<A>& st, A (&pv)[NCH][MAXND], int n
   }
 }
 
-template <typename T>
-__device__ inline T row_dot(const int32_t* __restrict__ col_idx, const T* __restrict__ vals, int32_t rs, int32_t re,
-                            const T* __restrict__ X, int64_t ld, int c) {
-  T acc = sc<T>::zero();
-  for (int32_t j = rs; j < re; ++j) {
-    T a = vals[j];
-    T x = X[(int64_t)col_idx[j] * ld + c];
-    acc = acc + a * x;
+// true when some column is still active (read by every producer block)
+__device__ inline bool block_any_active(const int* active, int C) {
+  __shared__ int s_any;
+  if (threadIdx.x == 0) s_any = 0;
+  __syncthreads();
+  for (int c = threadIdx.x; c < C; c += blockDim.x)
+    if (active[c]) s_any = 1;
+  __syncthreads();
+  return s_any != 0;
+}
+
+// finish-block helper: v[d] = sum over producer blocks of part[b][d][c]
+template <typename A>
+__device__ void finish_reduce(const ColState<A>& st, int nd, int C, int c, A* v) {
+  __shared__ A red[FB];
+  for (int d = 0; d < nd; ++d) {
+    A acc = acc_zero(A());
+    for (int b = threadIdx.x; b < st.nb; b += FB) acc = acc_add(acc, st.part[(int64_t)b * MAXND * C + d * C + c]);
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = FB / 2; o > 0; o >>= 1) {
+      if (threadIdx.x < o) red[threadIdx.x] = acc_add(red[threadIdx.x], red[threadIdx.x + o]);
+      __syncthreads();
+    }
+    v[d] = red[0];
+    __syncthreads();
   }
-  return acc;
+}
+
+// y[ch] = sum_j A_sj x[col_j, lane + ch*G]; UNR column indices and values are
+// fetched before their gathers so the loads of one step are in flight together
+template <typename T, int G, int NCH>
+__device__ inline void row_gather(const int32_t* __restrict__ col_idx, const T* __restrict__ vals, int32_t rs,
+                                  int32_t re, const T* __restrict__ X, int64_t ld, int lane, int C, T (&y)[NCH]) {
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) y[ch] = sc<T>::zero();
+  int32_t j = rs;
+  for (; j + UNR <= re; j += UNR) {
+    int32_t cc[UNR];
+    T a[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      cc[u] = col_idx[j + u];
+      a[u] = vals[j + u];
+    }
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) {
+      int c = lane + ch * G;
+      if (c < C) {
+        T x[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) x[u] = X[(int64_t)cc[u] * ld + c];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) y[ch] = y[ch] + a[u] * x[u];
+      }
+    }
+  }
+  for (; j < re; ++j) {
+    int32_t cj = col_idx[j];
+    T aj = vals[j];
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) {
+      int c = lane + ch * G;
+      if (c < C) y[ch] = y[ch] + aj * X[(int64_t)cj * ld + c];
+    }
+  }
 }
 
 #define ROW_LOOP(nrows)                                                                    \
@@ -156,7 +155,7 @@ __device__ inline T row_dot(const int32_t* __restrict__ col_idx, const T* __rest
   for (int64_t row = (int64_t)blockIdx.x * RPB + rg; row < (nrows); row += (int64_t)gridDim.x * RPB)
 
 // ---------------------------------------------------------------------------
-// iteration kernels
+// producer kernels
 // ---------------------------------------------------------------------------
 // r = b - A x0 (or b, with x = 0), z = dinv r, p = z ; dots rz, rr, bb
 template <typename T, int G, int NCH>
@@ -170,7 +169,11 @@ __global__ void __launch_bounds__(KB) k_cg_init(Csr m, const T* __restrict__ val
   for (int ch = 0; ch < NCH; ++ch)
     for (int d = 0; d < MAXND; ++d) pv[ch][d] = acc_zero(A());
   ROW_LOOP(m.n) {
-    int32_t rs = m.row_ptr[row], re = m.row_ptr[row + 1];
+    T ax[NCH];
+    if (use_x0) {
+      int32_t rs = m.row_ptr[row], re = m.row_ptr[row + 1];
+      row_gather<T, G, NCH>(m.col_idx, vals + 0, rs, re, X, C, lane, C, ax);
+    }
 #pragma unroll
     for (int ch = 0; ch < NCH; ++ch) {
       int c = lane + ch * G;
@@ -179,10 +182,17 @@ __global__ void __launch_bounds__(KB) k_cg_init(Csr m, const T* __restrict__ val
         int64_t o = row * C + c;
         T b = B[o];
         T r = b;
-        if (use_x0)
-          r = b - row_dot(m.col_idx, vals + (int64_t)s * m.nnz, rs, re, X, C, c);
-        else
+        if (use_x0) {
+          if (s != 0) {  // multi-system: redo the row with this system's values
+            int32_t rs = m.row_ptr[row], re = m.row_ptr[row + 1];
+            T acc = sc<T>::zero();
+            for (int32_t j = rs; j < re; ++j) acc = acc + vals[(int64_t)s * m.nnz + j] * X[(int64_t)m.col_idx[j] * C + c];
+            ax[ch] = acc;
+          }
+          r = b - ax[ch];
+        } else {
           X[o] = sc<T>::zero();
+        }
         T z = dinv[(int64_t)s * m.n + row] * r;
         R[o] = r;
         P[o] = z;
@@ -193,68 +203,62 @@ __global__ void __launch_bounds__(KB) k_cg_init(Csr m, const T* __restrict__ val
     }
   }
   block_partials<A, G, NCH>(st, pv, 3, C);
-  cross_block_finish<A>(st, 3, C, [&](int c, A* v) {
-    double bb = acc_re(v[2]);
-    double rr = acc_re(v[1]);
-    st.bb[c] = bb;
-    st.rr[c] = rr;
-    st.rho[c] = v[0];
-    st.iters[c] = 0;
-    st.alpha[c] = acc_zero(A());
-    st.beta[c] = acc_zero(A());
-    st.flag[c] = 0;
-    st.active[c] = (bb > 0.0) && (rr > st.tol2 * bb);
-  });
 }
 
-// q = A p ; dot pq ; alpha = rho / pq
-template <typename T, int G, int NCH>
+// q = A_s p ; partial p.q
+template <typename T, int G, int NCH, bool MULTI>
 __global__ void __launch_bounds__(KB) k_cg_spmm(Csr m, const T* __restrict__ vals, const T* __restrict__ P,
                                                 T* __restrict__ Q, int C, int k, ColState<typename sc<T>::acc> st) {
   typedef typename sc<T>::acc A;
-  if (coherent_load(st.nactive) == 0) return;
+  if (!block_any_active(st.active, C)) return;
   A pv[NCH][MAXND];
 #pragma unroll
   for (int ch = 0; ch < NCH; ++ch) pv[ch][0] = acc_zero(A());
   ROW_LOOP(m.n) {
     int32_t rs = m.row_ptr[row], re = m.row_ptr[row + 1];
+    T q[NCH];
+    if (!MULTI) {
+      row_gather<T, G, NCH>(m.col_idx, vals, rs, re, P, C, lane, C, q);
+    } else if (k % G == 0) {  // every chunk of G columns belongs to one system
+#pragma unroll
+      for (int ch = 0; ch < NCH; ++ch) {
+        T q1[1];
+        row_gather<T, G, 1>(m.col_idx, vals + (int64_t)((ch * G) / k) * m.nnz, rs, re, P + ch * G, C, lane,
+                            C - ch * G, q1);
+        q[ch] = q1[0];
+      }
+    } else {
+#pragma unroll
+      for (int ch = 0; ch < NCH; ++ch) {
+        int c = lane + ch * G;
+        q[ch] = sc<T>::zero();
+        if (c < C) {
+          const T* v = vals + (int64_t)(c / k) * m.nnz;
+          for (int32_t j = rs; j < re; ++j) q[ch] = q[ch] + v[j] * P[(int64_t)m.col_idx[j] * C + c];
+        }
+      }
+    }
 #pragma unroll
     for (int ch = 0; ch < NCH; ++ch) {
       int c = lane + ch * G;
       if (c < C) {
-        int s = c / k;
-        T q = row_dot(m.col_idx, vals + (int64_t)s * m.nnz, rs, re, P, C, c);
         int64_t o = row * C + c;
-        Q[o] = q;
-        pv[ch][0] = acc_add(pv[ch][0], amul(P[o], q));
+        Q[o] = q[ch];
+        pv[ch][0] = acc_add(pv[ch][0], amul(P[o], q[ch]));
       }
     }
   }
   block_partials<A, G, NCH>(st, pv, 1, C);
-  cross_block_finish<A>(st, 1, C, [&](int c, A* v) {
-    A pq = v[0];
-    if (st.active[c]) {
-      if (acc_abs2(pq) == 0.0) {
-        st.flag[c] = 4;
-        st.active[c] = 0;
-        st.alpha[c] = acc_zero(A());
-      } else {
-        st.alpha[c] = acc_div(st.rho[c], pq);
-      }
-    } else {
-      st.alpha[c] = acc_zero(A());
-    }
-  });
 }
 
-// x += alpha p ; r -= alpha q ; z = dinv r ; dots rz, rr ; beta, stopping test
+// x += alpha p ; r -= alpha q ; z = dinv r ; partials rz, rr
 template <typename T, int G, int NCH>
 __global__ void __launch_bounds__(KB) k_cg_update(int64_t n, const T* __restrict__ dinv, T* __restrict__ X,
                                                   T* __restrict__ R, const T* __restrict__ P,
                                                   const T* __restrict__ Q, int C, int k,
                                                   ColState<typename sc<T>::acc> st) {
   typedef typename sc<T>::acc A;
-  if (coherent_load(st.nactive) == 0) return;
+  if (!block_any_active(st.active, C)) return;
   A pv[NCH][MAXND];
   A al[NCH];
 #pragma unroll
@@ -284,32 +288,6 @@ __global__ void __launch_bounds__(KB) k_cg_update(int64_t n, const T* __restrict
     }
   }
   block_partials<A, G, NCH>(st, pv, 2, C);
-  cross_block_finish<A>(st, 2, C, [&](int c, A* v) {
-    double rr = acc_re(v[1]);
-    st.rr[c] = rr;
-    if (st.active[c]) {
-      int it = st.iters[c] + 1;
-      st.iters[c] = it;
-      A rz = v[0];
-      if (rr <= st.tol2 * st.bb[c]) {
-        st.active[c] = 0;
-        st.beta[c] = acc_zero(A());
-      } else if (it >= st.maxiter) {
-        st.active[c] = 0;
-        st.flag[c] = 1;
-        st.beta[c] = acc_zero(A());
-      } else if (acc_abs2(rz) == 0.0 || acc_abs2(st.rho[c]) == 0.0) {
-        st.active[c] = 0;
-        st.flag[c] = 4;
-        st.beta[c] = acc_zero(A());
-      } else {
-        st.beta[c] = acc_div(rz, st.rho[c]);
-        st.rho[c] = rz;
-      }
-    } else {
-      st.beta[c] = acc_zero(A());
-    }
-  });
 }
 
 // p = dinv r + beta p on active columns
@@ -318,7 +296,7 @@ __global__ void __launch_bounds__(KB) k_cg_pupdate(int64_t n, const T* __restric
                                                    T* __restrict__ P, int C, int k,
                                                    ColState<typename sc<T>::acc> st) {
   typedef typename sc<T>::acc A;
-  if (coherent_load(st.nactive) == 0) return;
+  if (!block_any_active(st.active, C)) return;
   A be[NCH];
   int act[NCH];
 #pragma unroll
@@ -341,14 +319,11 @@ __global__ void __launch_bounds__(KB) k_cg_pupdate(int64_t n, const T* __restric
   }
 }
 
-// outer residual of the mixed solve: r32 = (b - A64 x64), dots rr, bb; the
-// inner right-hand side of a column that meets the tolerance is zeroed by
-// k_mask_cols afterwards
+// outer residual of the mixed solve: r32 = (b - A64 x64); partials rr, bb
 template <typename T64, typename T32, int G, int NCH>
 __global__ void __launch_bounds__(KB) k_outer_resid(Csr m, const T64* __restrict__ vals, const T64* __restrict__ B,
                                                     const T64* __restrict__ X, int use_x, T32* __restrict__ R32,
-                                                    int C, int k, int outer_it, int max_outer,
-                                                    ColState<typename sc<T64>::acc> st) {
+                                                    int C, int k, ColState<typename sc<T64>::acc> st) {
   typedef typename sc<T64>::acc A;
   A pv[NCH][MAXND];
 #pragma unroll
@@ -366,7 +341,12 @@ __global__ void __launch_bounds__(KB) k_outer_resid(Csr m, const T64* __restrict
         int64_t o = row * C + c;
         T64 b = B[o];
         T64 r = b;
-        if (use_x) r = b - row_dot(m.col_idx, vals + (int64_t)s * m.nnz, rs, re, X, C, c);
+        if (use_x) {
+          T64 acc = sc<T64>::zero();
+          const T64* v = vals + (int64_t)s * m.nnz;
+          for (int32_t j = rs; j < re; ++j) acc = acc + v[j] * X[(int64_t)m.col_idx[j] * C + c];
+          r = b - acc;
+        }
         R32[o] = narrow(r, T32());
         pv[ch][0] = acc_add(pv[ch][0], acc_from_real(sc<T64>::norm2(r), A()));
         pv[ch][1] = acc_add(pv[ch][1], acc_from_real(sc<T64>::norm2(b), A()));
@@ -374,7 +354,90 @@ __global__ void __launch_bounds__(KB) k_outer_resid(Csr m, const T64* __restrict
     }
   }
   block_partials<A, G, NCH>(st, pv, 2, C);
-  cross_block_finish<A>(st, 2, C, [&](int c, A* v) {
+}
+
+// ---------------------------------------------------------------------------
+// finish kernels: one block per column
+// ---------------------------------------------------------------------------
+template <typename A>
+__global__ void __launch_bounds__(FB) k_fin_init(int C, ColState<A> st) {
+  int c = blockIdx.x;
+  A v[3];
+  finish_reduce(st, 3, C, c, v);
+  if (threadIdx.x == 0) {
+    double bb = acc_re(v[2]);
+    double rr = acc_re(v[1]);
+    st.bb[c] = bb;
+    st.rr[c] = rr;
+    st.rho[c] = v[0];
+    st.iters[c] = 0;
+    st.alpha[c] = acc_zero(A());
+    st.beta[c] = acc_zero(A());
+    st.flag[c] = 0;
+    st.active[c] = (bb > 0.0) && (rr > st.tol2 * bb);
+  }
+}
+
+template <typename A>
+__global__ void __launch_bounds__(FB) k_fin_alpha(int C, ColState<A> st) {
+  int c = blockIdx.x;
+  if (!st.active[c]) {
+    if (threadIdx.x == 0) st.alpha[c] = acc_zero(A());
+    return;
+  }
+  A v[1];
+  finish_reduce(st, 1, C, c, v);
+  if (threadIdx.x == 0) {
+    A pq = v[0];
+    if (acc_abs2(pq) == 0.0) {
+      st.flag[c] = 4;  // breakdown
+      st.active[c] = 0;
+      st.alpha[c] = acc_zero(A());
+    } else {
+      st.alpha[c] = acc_div(st.rho[c], pq);
+    }
+  }
+}
+
+template <typename A>
+__global__ void __launch_bounds__(FB) k_fin_beta(int C, ColState<A> st) {
+  int c = blockIdx.x;
+  if (!st.active[c]) {
+    if (threadIdx.x == 0) st.beta[c] = acc_zero(A());
+    return;
+  }
+  A v[2];
+  finish_reduce(st, 2, C, c, v);
+  if (threadIdx.x == 0) {
+    double rr = acc_re(v[1]);
+    st.rr[c] = rr;
+    int it = st.iters[c] + 1;
+    st.iters[c] = it;
+    A rz = v[0];
+    if (rr <= st.tol2 * st.bb[c]) {
+      st.active[c] = 0;
+      st.beta[c] = acc_zero(A());
+    } else if (it >= st.maxiter) {
+      st.active[c] = 0;
+      st.flag[c] = 1;
+      st.beta[c] = acc_zero(A());
+    } else if (acc_abs2(rz) == 0.0 || acc_abs2(st.rho[c]) == 0.0) {
+      st.active[c] = 0;
+      st.flag[c] = 4;
+      st.beta[c] = acc_zero(A());
+    } else {
+      st.beta[c] = acc_div(rz, st.rho[c]);
+      st.rho[c] = rz;
+    }
+  }
+}
+
+template <typename A>
+__global__ void __launch_bounds__(FB) k_fin_outer(int C, int outer_it, int max_outer, ColState<A> st) {
+  int c = blockIdx.x;
+  A v[2];
+  finish_reduce(st, 2, C, c, v);
+  if (threadIdx.x == 0) {
     double rr = acc_re(v[0]), bb = acc_re(v[1]);
     st.rr[c] = rr;
     st.bb[c] = bb;
@@ -384,7 +447,19 @@ __global__ void __launch_bounds__(KB) k_outer_resid(Csr m, const T64* __restrict
       st.flag[c] = 1;
     }
     st.active[c] = act;
-  });
+  }
+}
+
+// count active columns (one block), for the host's convergence polls
+__global__ void k_count_active(const int* active, int C, int* nactive) {
+  __shared__ int s;
+  if (threadIdx.x == 0) s = 0;
+  __syncthreads();
+  int loc = 0;
+  for (int c = threadIdx.x; c < C; c += blockDim.x) loc += active[c] ? 1 : 0;
+  atomicAdd(&s, loc);
+  __syncthreads();
+  if (threadIdx.x == 0) *nactive = s;
 }
 
 template <typename T32>
@@ -422,8 +497,10 @@ __global__ void __launch_bounds__(KB) k_spmm(Csr m, const T* __restrict__ vals, 
     for (int ch = 0; ch < NCH; ++ch) {
       int c = lane + ch * G;
       if (c < C) {
-        int s = c / k;
-        Y[row * C + c] = row_dot(m.col_idx, vals + (int64_t)s * m.nnz, rs, re, X, C, c);
+        const T* v = vals + (int64_t)(c / k) * m.nnz;
+        T acc = sc<T>::zero();
+        for (int32_t j = rs; j < re; ++j) acc = acc + v[j] * X[(int64_t)m.col_idx[j] * C + c];
+        Y[row * C + c] = acc;
       }
     }
   }
@@ -476,7 +553,6 @@ struct Layout {
 
 template <typename A>
 static void state_layout(Layout& L, int64_t C, int64_t nblocks, size_t* offs) {
-  int64_t ngroups = ceil_div(nblocks, RED_GS);
   offs[0] = L.take(C * sizeof(A));                    // rho
   offs[1] = L.take(C * sizeof(A));                    // alpha
   offs[2] = L.take(C * sizeof(A));                    // beta
@@ -487,14 +563,11 @@ static void state_layout(Layout& L, int64_t C, int64_t nblocks, size_t* offs) {
   offs[7] = L.take(C * sizeof(int));                  // flag
   offs[8] = L.take(64);                               // nactive
   offs[9] = L.take(nblocks * MAXND * C * sizeof(A));  // part
-  offs[10] = L.take(ngroups * MAXND * C * sizeof(A)); // gpart
-  offs[11] = L.take((ngroups + 1) * sizeof(int));     // cnt1 + cnt2
 }
 
 template <typename A>
 static ColState<A> state_at(char* base, const size_t* o, int64_t nblocks) {
   ColState<A> s;
-  int64_t ngroups = ceil_div(nblocks, RED_GS);
   s.rho = (A*)(base + o[0]);
   s.alpha = (A*)(base + o[1]);
   s.beta = (A*)(base + o[2]);
@@ -505,9 +578,7 @@ static ColState<A> state_at(char* base, const size_t* o, int64_t nblocks) {
   s.flag = (int*)(base + o[7]);
   s.nactive = (int*)(base + o[8]);
   s.part = (A*)(base + o[9]);
-  s.gpart = (A*)(base + o[10]);
-  s.cnt1 = (int*)(base + o[11]);
-  s.cnt2 = s.cnt1 + ngroups;
+  s.nb = (int)nblocks;
   s.tol2 = 0;
   s.maxiter = 0;
   return s;
@@ -521,7 +592,7 @@ struct SolveBufs {
   T* dinv;
 };
 
-// workspace: plain -> R,P,Q,dinv (T64) + one state; mixed -> R,P,Q,D,dinv
+// workspace: plain -> R,P,Q,dinv (T64) + one state; mixed -> R,P,Q,dinv,D
 // (T32) + inner and outer states
 template <typename T64, typename T32>
 static size_t ws_layout(int64_t n, int64_t C, int nsys, int mixed, size_t* o) {
@@ -555,8 +626,10 @@ static unsigned ew_grid(int64_t total) {
 
 static int* g_pinned = nullptr;  // host flag for convergence polls
 
-static int poll_nactive(const int* d_nactive, hipStream_t s, int* out) {
+static int poll_nactive(const int* active, int C, int* d_nactive, hipStream_t s, int* out) {
   if (!g_pinned) RB_HIP(hipHostMalloc((void**)&g_pinned, 64, hipHostMallocDefault));
+  hipLaunchKernelGGL(k_count_active, dim3(1), dim3(256), 0, s, active, C, d_nactive);
+  RB_CHECK_LAUNCH();
   RB_HIP(hipMemcpyAsync(g_pinned, d_nactive, sizeof(int), hipMemcpyDeviceToHost, s));
   RB_HIP(hipStreamSynchronize(s));
   *out = *g_pinned;
@@ -640,25 +713,27 @@ extern "C" int rb_prof_read(int id, double* total_ms, int64_t* count) {
 }
 
 #define L_INIT(G_, N_, ...) hipLaunchKernelGGL((k_cg_init<T, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
-#define L_SPMM(G_, N_, ...) hipLaunchKernelGGL((k_cg_spmm<T, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
+#define L_SPMM1(G_, N_, ...) hipLaunchKernelGGL((k_cg_spmm<T, G_, N_, false>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
+#define L_SPMMN(G_, N_, ...) hipLaunchKernelGGL((k_cg_spmm<T, G_, N_, true>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
 #define L_UPD(G_, N_, ...) hipLaunchKernelGGL((k_cg_update<T, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
 #define L_PUP(G_, N_, ...) hipLaunchKernelGGL((k_cg_pupdate<T, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
 
 // one CG/COCG solve of all columns; X is the solution (and x0 if use_x0)
 template <typename T>
-static int run_cg(const Csr& m, const T* vals, const T* dinv, const T* B, T* X, int use_x0, SolveBufs<T> w, int C,
-                  int k, ColState<typename sc<T>::acc> st, int check_every, hipStream_t s, int* iters_done) {
+static int run_cg(const Csr& m, int nsys, const T* vals, const T* dinv, const T* B, T* X, int use_x0,
+                  SolveBufs<T> w, int C, int k, ColState<typename sc<T>::acc> st, int check_every, hipStream_t s,
+                  int* iters_done) {
+  typedef typename sc<T>::acc A;
   int64_t nb = grid_for(m.n, C);
-  int64_t ng = ceil_div(nb, RED_GS);
-  int rc = rb_fill_i32(st.cnt1, 0, ng + 1, s);
-  if (rc) return rc;
+  st.nb = (int)nb;
   {
     ProfScope ps(RB_PROF_INIT, s);
     DISPATCH_GN(C, L_INIT, m, vals, dinv, B, X, use_x0, w.R, w.P, C, k, st);
+    hipLaunchKernelGGL(k_fin_init<A>, dim3(C), dim3(FB), 0, s, C, st);
   }
   RB_CHECK_LAUNCH();
   int nact = 0;
-  rc = poll_nactive(st.nactive, s, &nact);
+  int rc = poll_nactive(st.active, C, st.nactive, s, &nact);
   if (rc) return rc;
   int it = 0;
   while (nact > 0 && it < st.maxiter) {
@@ -666,12 +741,17 @@ static int run_cg(const Csr& m, const T* vals, const T* dinv, const T* B, T* X, 
     for (int j = 0; j < chunk; ++j) {
       {
         ProfScope ps(RB_PROF_SPMM, s);
-        DISPATCH_GN(C, L_SPMM, m, vals, w.P, w.Q, C, k, st);
+        if (nsys == 1)
+          DISPATCH_GN(C, L_SPMM1, m, vals, w.P, w.Q, C, k, st);
+        else
+          DISPATCH_GN(C, L_SPMMN, m, vals, w.P, w.Q, C, k, st);
       }
+      hipLaunchKernelGGL(k_fin_alpha<A>, dim3(C), dim3(FB), 0, s, C, st);
       {
         ProfScope ps(RB_PROF_UPDATE, s);
         DISPATCH_GN(C, L_UPD, m.n, dinv, X, w.R, w.P, w.Q, C, k, st);
       }
+      hipLaunchKernelGGL(k_fin_beta<A>, dim3(C), dim3(FB), 0, s, C, st);
       {
         ProfScope ps(RB_PROF_PUPDATE, s);
         DISPATCH_GN(C, L_PUP, m.n, dinv, w.R, w.P, C, k, st);
@@ -679,7 +759,7 @@ static int run_cg(const Csr& m, const T* vals, const T* dinv, const T* B, T* X, 
     }
     RB_CHECK_LAUNCH();
     it += chunk;
-    rc = poll_nactive(st.nactive, s, &nact);
+    rc = poll_nactive(st.active, C, st.nactive, s, &nact);
     if (rc) return rc;
   }
   *iters_done = it;
@@ -717,7 +797,7 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
     hipLaunchKernelGGL(k_dinv<T64>, dim3((unsigned)ceil_div(n * nsys, 256)), dim3(256), 0, s, m, A64, nsys, w.dinv);
     RB_CHECK_LAUNCH();
     int its = 0;
-    rc = run_cg<T64>(m, A64, w.dinv, B, X, use_x0, w, C, k, st, check_every, s, &its);
+    rc = run_cg<T64>(m, nsys, A64, w.dinv, B, X, use_x0, w, C, k, st, check_every, s, &its);
     if (rc) return rc;
     RB_HIP(hipMemcpyAsync(hrr.data(), st.rr, C * sizeof(double), hipMemcpyDeviceToHost, s));
     RB_HIP(hipMemcpyAsync(hbb.data(), st.bb, C * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -757,18 +837,16 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
   RB_CHECK_LAUNCH();
   const int max_outer = 12;
   int total_it = 0;
-  int64_t ng = ceil_div(nb, RED_GS);
   rc = rb_fill_i32(sout.flag, 0, C, s);
   if (rc) return rc;
   int inner_flag_max = 0;
   for (int outer = 0;; ++outer) {
-    rc = rb_fill_i32(sout.cnt1, 0, ng + 1, s);
-    if (rc) return rc;
 #define L_ORES(G_, N_, ...) \
   hipLaunchKernelGGL((k_outer_resid<T64, T32, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
     {
       ProfScope ps(RB_PROF_RESID, s);
-      DISPATCH_GN(C, L_ORES, m, A64, B, X, (outer > 0 || use_x0) ? 1 : 0, w.Q, C, k, outer, max_outer, sout);
+      DISPATCH_GN(C, L_ORES, m, A64, B, X, (outer > 0 || use_x0) ? 1 : 0, w.Q, C, k, sout);
+      hipLaunchKernelGGL(k_fin_outer<A>, dim3(C), dim3(FB), 0, s, C, outer, max_outer, sout);
     }
 #undef L_ORES
     RB_CHECK_LAUNCH();
@@ -777,16 +855,15 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
       if (rc) return rc;
     }
     int nact = 0;
-    rc = poll_nactive(sout.nactive, s, &nact);
+    rc = poll_nactive(sout.active, C, sout.nactive, s, &nact);
     if (rc) return rc;
     if (nact == 0) break;
-    // inner rhs = outer residual (in Q), zeroed on columns that are done
+    // inner rhs = outer residual (in Q), zeroed on columns that are done; run_cg
+    // reads it only in its init kernel, before Q becomes the SpMM output
     hipLaunchKernelGGL(k_mask_cols<T32>, dim3(ew_grid(n * C)), dim3(256), 0, s, (int64_t)n * C, C, sout.active, w.Q);
     RB_CHECK_LAUNCH();
-    // run_cg reads B = Q only in its init kernel, before Q is reused as the
-    // SpMM output, so Q can serve as the inner rhs
     int its = 0;
-    rc = run_cg<T32>(m, A32, w.dinv, w.Q, D, 0, w, C, k, sin, check_every, s, &its);
+    rc = run_cg<T32>(m, nsys, A32, w.dinv, w.Q, D, 0, w, C, k, sin, check_every, s, &its);
     if (rc) return rc;
     total_it += its;
     RB_HIP(hipMemcpyAsync(hflag.data(), sin.flag, C * sizeof(int), hipMemcpyDeviceToHost, s));

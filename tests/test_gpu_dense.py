@@ -1,0 +1,49 @@
+"""GPU Gram (fp64 FMA, bf16 / bf16x3 MFMA) and Jacobian contraction against float64 references."""
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import box_cfg, grid_optodes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("K,m", [(5000, 300), (1031, 128), (64, 70)])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_gram_modes(K, m, dtype):
+    from redbirdpy_amd import _gpu
+
+    rng = np.random.default_rng(K + m)
+    # rows spanning orders of magnitude, like Jacobian rows
+    J = rng.standard_normal((K, m)) * np.exp(rng.uniform(-6, 0, m))[None, :]
+    JT = torch.as_tensor(J, device="cuda").to(dtype)
+    ref = J.astype(np.float64) if dtype == torch.float64 else JT.double().cpu().numpy()
+    G_ref = ref.T @ ref
+    d = torch.as_tensor(1 / np.sqrt(np.diag(G_ref)), device="cuda")
+    Gn_ref = G_ref * np.outer(d.cpu().numpy(), d.cpu().numpy())
+    G64 = _gpu.gram(JT, d, "fp64").cpu().numpy()
+    assert np.max(np.abs(G64 - Gn_ref)) < 1e-12
+    G3 = _gpu.gram(JT, d, "bf16x3").cpu().numpy()
+    assert np.max(np.abs(G3 - Gn_ref)) < 2e-5  # ~16-bit products, unit diagonal
+    G1 = _gpu.gram(JT, d, "bf16").cpu().numpy()
+    assert np.max(np.abs(G1 - Gn_ref)) < 2e-2
+    assert np.array_equal(G3, G3.T)
+
+
+def test_jac_contract_matches_dense():
+    import redbirdpy_amd as rb
+    from redbirdpy_amd import _gpu
+
+    p1 = (16, 16, 8)
+    cfg = box_cfg(p1, 2.0, src=grid_optodes(p1, 2), det=grid_optodes(p1, 3, margin=3.0))
+    cfg, sd = rb.meshprep(cfg)
+    det, phi = rb.runforward(cfg)
+    Jn, _ = rb.jac(sd, phi, cfg["deldotdel"], cfg["elem"], cfg["evol"], return_elem=False)
+    rng = np.random.default_rng(1)
+    w = rng.standard_normal(Jn.shape[0])
+    mesh = _gpu.mesh_for(cfg)
+    S = np.arange(4)
+    D = np.arange(4, 13)
+    g = _gpu.jac_contract(mesh, torch.as_tensor(phi, device="cuda"), S, D, torch.as_tensor(w, device="cuda"))
+    np.testing.assert_allclose(g.cpu().numpy(), Jn.T @ w, rtol=1e-11, atol=1e-14 * np.abs(Jn).max())
