@@ -132,6 +132,12 @@ def main():
     nbytes = _spmm_bytes(mesh.nn, mesh.nnz, C, 8)  # complex64 inner solve
     achieved = nbytes / spmm_avg_s / 1e9
 
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        rec = json.load(open(tpath)).get("k_cg_spmm_w64<cplx<float>,1>")
+        if rec and a.h == 1.0:
+            traffic = rec["hbm_bytes_per_launch"]
     out = {
         "metric": "RHS-solves/s + GN iters/s @115k nodes: FP32+refine vs FP64, 1 GPU, <=1e-6",
         "value": k / dt_m,
@@ -161,8 +167,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": "k_cg_spmm<cplx<float>,64,1> (q = A p, p.q reduction)",
+            "traffic": traffic,
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated (profiles/krylov_fd115k_c64_pmc.md)",
+            "kernel": "k_cg_spmm_w64<cplx<float>,1> (q = A p with the p.q reduction)",
             "bytes_per_launch": nbytes,
             "avg_launch_us": spmm_avg_s * 1e6,
         },

@@ -43,10 +43,17 @@ struct ColState {
   int* iters;
   int* flag;
   int* nactive;
-  A* part;  // [nblocks][MAXND][C]
+  A* part;  // [MAXND][C][nblocks]
   int nb;   // producer blocks whose partials are valid
   double tol2;
   int maxiter;
+  // mixed precision (reliable updates): rho before the last beta, squared
+  // true residual at the last replacement, and the replacement request flag
+  A* rho_prev;
+  double* rmax2;
+  int* ru_req;
+  double delta2;  // request a replacement when rr <= delta2 * rmax2 (0: plain CG)
+  double tol2ru;  // ... or when rr <= tol2ru * bb
 };
 
 struct Csr {
@@ -56,7 +63,7 @@ struct Csr {
   int64_t n, nnz;
 };
 
-// per-thread partials pv[ch][d] (column lane + ch*G) -> part[blockIdx][d][c]
+// per-thread partials pv[ch][d] (column lane + ch*G) -> part[d][c][blockIdx]
 template <typename A, int G, int NCH>
 __device__ void block_partials(const ColState<A>& st, A (&pv)[NCH][MAXND], int nd, int C) {
   __shared__ A lds[KB * MAXND];
@@ -73,7 +80,7 @@ __device__ void block_partials(const ColState<A>& st, A (&pv)[NCH][MAXND], int n
       if (c < C) {
         A acc = lds[d * G + l];
         for (int r = 1; r < RPB; ++r) acc = acc_add(acc, lds[(r * MAXND + d) * G + l]);
-        st.part[(int64_t)blockIdx.x * MAXND * C + d * C + c] = acc;
+        st.part[(int64_t)(d * C + c) * st.nb + blockIdx.x] = acc;
       }
     }
     __syncthreads();
@@ -97,7 +104,8 @@ __device__ void finish_reduce(const ColState<A>& st, int nd, int C, int c, A* v)
   __shared__ A red[FB];
   for (int d = 0; d < nd; ++d) {
     A acc = acc_zero(A());
-    for (int b = threadIdx.x; b < st.nb; b += FB) acc = acc_add(acc, st.part[(int64_t)b * MAXND * C + d * C + c]);
+    const A* pp = st.part + (int64_t)(d * C + c) * st.nb;
+    for (int b = threadIdx.x; b < st.nb; b += FB) acc = acc_add(acc, pp[b]);
     red[threadIdx.x] = acc;
     __syncthreads();
     for (int o = FB / 2; o > 0; o >>= 1) {
@@ -148,11 +156,73 @@ __device__ inline void row_gather(const int32_t* __restrict__ col_idx, const T* 
   }
 }
 
+__device__ inline float rl_bcast(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+__device__ inline double rl_bcast(double v, int l) {
+  int2 b = __builtin_bit_cast(int2, v);
+  b.x = __builtin_amdgcn_readlane(b.x, l);
+  b.y = __builtin_amdgcn_readlane(b.y, l);
+  return __builtin_bit_cast(double, b);
+}
+template <typename R>
+__device__ inline cplx<R> rl_bcast(cplx<R> v, int l) {
+  return mk<R>(rl_bcast(v.x, l), rl_bcast(v.y, l));
+}
+
+#define RG 16  // gathers in flight per row step
+
+// one row per wavefront: lanes load up to 64 (col, val) entries of the row
+// at once, then readlane broadcasts feed RG masked gathers per step (a padded
+// slot re-reads the row's own entry with weight 0)
+template <typename T, int NCH>
+__device__ inline void row_gather_w64(const int32_t* __restrict__ col_idx, const T* __restrict__ vals, int32_t rs,
+                                      int32_t re, int64_t row, const T* __restrict__ X, int64_t ld, int lane,
+                                      int C, T (&y)[NCH]) {
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) y[ch] = sc<T>::zero();
+  for (int32_t base = rs; base < re; base += 64) {
+    const int len = min(64, re - base);
+    const int32_t cl = (lane < len) ? col_idx[base + lane] : (int32_t)row;
+    const T vl = (lane < len) ? vals[base + lane] : sc<T>::zero();
+    for (int j = 0; j < len; j += RG) {
+      int32_t cc[RG];
+      T a[RG];
+#pragma unroll
+      for (int u = 0; u < RG; ++u) {
+        const int jj = min(j + u, 63);
+        cc[u] = __builtin_amdgcn_readlane(cl, jj);
+        a[u] = (j + u < len) ? rl_bcast(vl, jj) : sc<T>::zero();
+      }
+#pragma unroll
+      for (int ch = 0; ch < NCH; ++ch) {
+        const int c = lane + ch * 64;
+        if (c < C) {
+          T x[RG];
+#pragma unroll
+          for (int u = 0; u < RG; ++u) x[u] = X[(int64_t)cc[u] * ld + c];
+#pragma unroll
+          for (int u = 0; u < RG; ++u) y[ch] = y[ch] + a[u] * x[u];
+        }
+      }
+    }
+  }
+}
+
+// XCD-aware row partition: workgroups are dispatched round-robin over the
+// NXCD accelerator dies (workgroup b runs on die b % NXCD), each with its own
+// L2.  Die x owns the contiguous row segment x, so the neighbours a CSR row
+// gathers (x/y/z lattice neighbours for a structured mesh) mostly sit in the
+// same die's L2 instead of being fetched by all eight.  The grid is a
+// multiple of NXCD.
+#define NXCD 8
 #define ROW_LOOP(nrows)                                                                    \
   const int lane = threadIdx.x % G;                                                        \
   const int rg = threadIdx.x / G;                                                          \
   constexpr int RPB = KB / G;                                                              \
-  for (int64_t row = (int64_t)blockIdx.x * RPB + rg; row < (nrows); row += (int64_t)gridDim.x * RPB)
+  const int64_t seg_ = ((nrows) + NXCD - 1) / NXCD;                                        \
+  const int64_t r0_ = (int64_t)(blockIdx.x % NXCD) * seg_;                                 \
+  const int64_t r1_ = min((int64_t)(nrows), r0_ + seg_);                                   \
+  const int64_t nbx_ = gridDim.x / NXCD;                                                   \
+  for (int64_t row = r0_ + (int64_t)(blockIdx.x / NXCD) * RPB + rg; row < r1_; row += nbx_ * RPB)
 
 // ---------------------------------------------------------------------------
 // producer kernels
@@ -217,14 +287,16 @@ __global__ void __launch_bounds__(KB) k_cg_spmm(Csr m, const T* __restrict__ val
   ROW_LOOP(m.n) {
     int32_t rs = m.row_ptr[row], re = m.row_ptr[row + 1];
     T q[NCH];
-    if (!MULTI) {
+    if (!MULTI && G == 64) {
+      row_gather_w64<T, NCH>(m.col_idx, vals, rs, re, row, P, C, lane, C, q);
+    } else if (!MULTI) {
       row_gather<T, G, NCH>(m.col_idx, vals, rs, re, P, C, lane, C, q);
-    } else if (k % G == 0) {  // every chunk of G columns belongs to one system
+    } else if (G == 64 && k % 64 == 0) {  // every chunk of 64 columns belongs to one system
 #pragma unroll
       for (int ch = 0; ch < NCH; ++ch) {
         T q1[1];
-        row_gather<T, G, 1>(m.col_idx, vals + (int64_t)((ch * G) / k) * m.nnz, rs, re, P + ch * G, C, lane,
-                            C - ch * G, q1);
+        row_gather_w64<T, 1>(m.col_idx, vals + (int64_t)((ch * G) / k) * m.nnz, rs, re, row, P + ch * G, C, lane,
+                             C - ch * G, q1);
         q[ch] = q1[0];
       }
     } else {
@@ -249,6 +321,101 @@ __global__ void __launch_bounds__(KB) k_cg_spmm(Csr m, const T* __restrict__ val
     }
   }
   block_partials<A, G, NCH>(st, pv, 1, C);
+}
+
+// q = A_s p ; partial p.q -- one wavefront per contiguous run of rows.
+// The run's row pointers come in with one lane-parallel load; rows are taken
+// two at a time, each with its (col, val) entries loaded by the lanes and
+// broadcast by readlane, so up to 2*RG gathers of 64-column rows are in
+// flight per wavefront.  Consecutive rows share most of their neighbours,
+// which the run order keeps in the CU's L1.  Requires C <= 64*NCH with every
+// 64-column chunk inside one system (nsys == 1 or k % 64 == 0).
+template <typename T, int NCH>
+__global__ void __launch_bounds__(KB) k_cg_spmm_w64(Csr m, const T* __restrict__ vals, const T* __restrict__ P,
+                                                    T* __restrict__ Q, int C, int k,
+                                                    ColState<typename sc<T>::acc> st) {
+  typedef typename sc<T>::acc A;
+  if (!block_any_active(st.active, C)) return;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int64_t seg = (m.n + NXCD - 1) / NXCD;
+  const int64_t s0 = (int64_t)(blockIdx.x % NXCD) * seg;
+  const int64_t s1 = min(m.n, s0 + seg);
+  const int64_t wpx = (int64_t)(gridDim.x / NXCD) * (KB / 64);  // wavefronts per die
+  const int64_t wid = (int64_t)(blockIdx.x / NXCD) * (KB / 64) + wave;
+  const int64_t rpw = (s1 - s0 + wpx - 1) / wpx;
+  const int64_t w0 = s0 + wid * rpw;
+  const int64_t w1 = min(s1, w0 + rpw);
+  A pv[NCH][MAXND];
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) pv[ch][0] = acc_zero(A());
+  for (int64_t b0 = w0; b0 < w1; b0 += 63) {
+    const int nr = (int)min((int64_t)63, w1 - b0);
+    const int32_t rp = (lane <= nr) ? m.row_ptr[b0 + lane] : 0;
+    for (int i = 0; i < nr; i += 2) {
+      const bool two = (i + 1 < nr);
+      const int32_t ra = __builtin_amdgcn_readlane(rp, i), rb = __builtin_amdgcn_readlane(rp, i + 1);
+      const int32_t rc = two ? __builtin_amdgcn_readlane(rp, i + 2) : rb;
+      const int la = min(64, rb - ra), lb = min(64, rc - rb);
+      const int32_t cla = (lane < la) ? m.col_idx[ra + lane] : (int32_t)(b0 + i);
+      const int32_t clb = (lane < lb) ? m.col_idx[rb + lane] : (int32_t)(b0 + i);
+#pragma unroll
+      for (int ch = 0; ch < NCH; ++ch) {
+        const int c = lane + ch * 64;
+        const T* v = vals + (int64_t)((ch * 64) / k) * m.nnz;
+        const T vla = (lane < la) ? v[ra + lane] : sc<T>::zero();
+        const T vlb = (lane < lb) ? v[rb + lane] : sc<T>::zero();
+        T ya = sc<T>::zero(), yb = sc<T>::zero();
+        if (c < C) {
+          const int lmax = max(la, lb);
+          for (int j = 0; j < lmax; j += RG) {
+            T xa[RG], xb[RG];
+#pragma unroll
+            for (int u = 0; u < RG; ++u) {
+              const int jj = min(j + u, 63);
+              xa[u] = P[(int64_t)__builtin_amdgcn_readlane(cla, jj) * C + c];
+              xb[u] = P[(int64_t)__builtin_amdgcn_readlane(clb, jj) * C + c];
+            }
+#pragma unroll
+            for (int u = 0; u < RG; ++u) {
+              const int jj = min(j + u, 63);
+              const T aa = (j + u < la) ? rl_bcast(vla, jj) : sc<T>::zero();
+              const T ab = (j + u < lb) ? rl_bcast(vlb, jj) : sc<T>::zero();
+              ya = ya + aa * xa[u];
+              yb = yb + ab * xb[u];
+            }
+          }
+          const int64_t oa = (b0 + i) * C + c;
+          Q[oa] = ya;
+          pv[ch][0] = acc_add(pv[ch][0], amul(P[oa], ya));
+          if (two) {
+            const int64_t ob = oa + C;
+            Q[ob] = yb;
+            pv[ch][0] = acc_add(pv[ch][0], amul(P[ob], yb));
+          }
+        }
+      }
+    }
+    // rows longer than 64 entries: finish the remainder one row at a time
+    for (int i = 0; i < nr; ++i) {
+      const int32_t ra = __builtin_amdgcn_readlane(rp, i), rb = __builtin_amdgcn_readlane(rp, i + 1);
+      if (rb - ra <= 64) continue;
+#pragma unroll
+      for (int ch = 0; ch < NCH; ++ch) {
+        const int c = lane + ch * 64;
+        if (c >= C) continue;
+        const T* v = vals + (int64_t)((ch * 64) / k) * m.nnz;
+        T y = sc<T>::zero();
+        for (int32_t j = ra + 64; j < rb; ++j) y = y + v[j] * P[(int64_t)m.col_idx[j] * C + c];
+        const int64_t oa = (b0 + i) * C + c;
+        const T q0 = Q[oa];
+        Q[oa] = q0 + y;
+        pv[ch][0] = acc_add(pv[ch][0], amul(P[oa], y));
+      }
+    }
+  }
+  // block reduction of the per-wave partials (same layout as block_partials with G = 64)
+  block_partials<A, 64, NCH>(st, pv, 1, C);
 }
 
 // x += alpha p ; r -= alpha q ; z = dinv r ; partials rz, rr
@@ -296,6 +463,9 @@ __global__ void __launch_bounds__(KB) k_cg_pupdate(int64_t n, const T* __restric
                                                    T* __restrict__ P, int C, int k,
                                                    ColState<typename sc<T>::acc> st) {
   typedef typename sc<T>::acc A;
+  // the replacement request of this iteration has been served (no other
+  // block of this kernel reads it)
+  if (st.ru_req && blockIdx.x == 0 && threadIdx.x == 0) *st.ru_req = 0;
   if (!block_any_active(st.active, C)) return;
   A be[NCH];
   int act[NCH];
@@ -319,20 +489,47 @@ __global__ void __launch_bounds__(KB) k_cg_pupdate(int64_t n, const T* __restric
   }
 }
 
-// outer residual of the mixed solve: r32 = (b - A64 x64); partials rr, bb
+// true residual of the mixed solve: r32 = (T32)(b - A64 x64) into the inner
+// residual buffer; partials rr64, bb64 and rz32 = r32 . (dinv32 r32)
 template <typename T64, typename T32, int G, int NCH>
 __global__ void __launch_bounds__(KB) k_outer_resid(Csr m, const T64* __restrict__ vals, const T64* __restrict__ B,
-                                                    const T64* __restrict__ X, int use_x, T32* __restrict__ R32,
-                                                    int C, int k, ColState<typename sc<T64>::acc> st) {
+                                                    const T64* __restrict__ X, int use_x,
+                                                    const T32* __restrict__ dinv32, T32* __restrict__ R32, int C,
+                                                    int k, const int* __restrict__ gate,
+                                                    ColState<typename sc<T64>::acc> st) {
   typedef typename sc<T64>::acc A;
+  if (gate && !*gate) return;
   A pv[NCH][MAXND];
 #pragma unroll
   for (int ch = 0; ch < NCH; ++ch) {
     pv[ch][0] = acc_zero(A());
     pv[ch][1] = acc_zero(A());
+    pv[ch][2] = acc_zero(A());
   }
   ROW_LOOP(m.n) {
     int32_t rs = m.row_ptr[row], re = m.row_ptr[row + 1];
+    T64 ax[NCH];
+    if (use_x) {
+      if (G == 64 && (k % 64 == 0 || k == C)) {  // each 64-column chunk within one system
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) {
+          T64 q1[1];
+          row_gather_w64<T64, 1>(m.col_idx, vals + (int64_t)((ch * G) / k) * m.nnz, rs, re, row, X + ch * G, C,
+                                 lane, C - ch * G, q1);
+          ax[ch] = q1[0];
+        }
+      } else {
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) {
+          int c = lane + ch * G;
+          ax[ch] = sc<T64>::zero();
+          if (c < C) {
+            const T64* v = vals + (int64_t)(c / k) * m.nnz;
+            for (int32_t j = rs; j < re; ++j) ax[ch] = ax[ch] + v[j] * X[(int64_t)m.col_idx[j] * C + c];
+          }
+        }
+      }
+    }
 #pragma unroll
     for (int ch = 0; ch < NCH; ++ch) {
       int c = lane + ch * G;
@@ -340,20 +537,17 @@ __global__ void __launch_bounds__(KB) k_outer_resid(Csr m, const T64* __restrict
         int s = c / k;
         int64_t o = row * C + c;
         T64 b = B[o];
-        T64 r = b;
-        if (use_x) {
-          T64 acc = sc<T64>::zero();
-          const T64* v = vals + (int64_t)s * m.nnz;
-          for (int32_t j = rs; j < re; ++j) acc = acc + v[j] * X[(int64_t)m.col_idx[j] * C + c];
-          r = b - acc;
-        }
-        R32[o] = narrow(r, T32());
+        T64 r = use_x ? b - ax[ch] : b;
+        T32 r32 = narrow(r, T32());
+        T32 z = dinv32[(int64_t)s * m.n + row] * r32;
+        R32[o] = r32;
         pv[ch][0] = acc_add(pv[ch][0], acc_from_real(sc<T64>::norm2(r), A()));
         pv[ch][1] = acc_add(pv[ch][1], acc_from_real(sc<T64>::norm2(b), A()));
+        pv[ch][2] = acc_add(pv[ch][2], amul(r32, z));
       }
     }
   }
-  block_partials<A, G, NCH>(st, pv, 2, C);
+  block_partials<A, G, NCH>(st, pv, 3, C);
 }
 
 // ---------------------------------------------------------------------------
@@ -414,7 +608,22 @@ __global__ void __launch_bounds__(FB) k_fin_beta(int C, ColState<A> st) {
     int it = st.iters[c] + 1;
     st.iters[c] = it;
     A rz = v[0];
-    if (rr <= st.tol2 * st.bb[c]) {
+    if (st.delta2 > 0.0) {  // mixed: the float64 true residual decides (k_fin_ru)
+      if (rr <= st.delta2 * st.rmax2[c] || rr <= st.tol2ru * st.bb[c]) *st.ru_req = 1;
+      if (it >= st.maxiter) {
+        st.active[c] = 0;
+        st.flag[c] = 1;
+        st.beta[c] = acc_zero(A());
+      } else if (acc_abs2(rz) == 0.0 || acc_abs2(st.rho[c]) == 0.0) {
+        st.active[c] = 0;
+        st.flag[c] = 4;
+        st.beta[c] = acc_zero(A());
+      } else {
+        st.rho_prev[c] = st.rho[c];
+        st.beta[c] = acc_div(rz, st.rho[c]);
+        st.rho[c] = rz;
+      }
+    } else if (rr <= st.tol2 * st.bb[c]) {
       st.active[c] = 0;
       st.beta[c] = acc_zero(A());
     } else if (it >= st.maxiter) {
@@ -432,21 +641,42 @@ __global__ void __launch_bounds__(FB) k_fin_beta(int C, ColState<A> st) {
   }
 }
 
+// reliable update: true residual norms decide convergence; the CG recurrence
+// continues from the replaced residual (beta = rz_true / rho before the
+// last beta).  Runs only when a column requested it (si.ru_req).
 template <typename A>
-__global__ void __launch_bounds__(FB) k_fin_outer(int C, int outer_it, int max_outer, ColState<A> st) {
+__global__ void __launch_bounds__(FB) k_fin_ru(int C, int first, double tol2, ColState<A> so, ColState<A> si) {
   int c = blockIdx.x;
-  A v[2];
-  finish_reduce(st, 2, C, c, v);
+  if (!first && (!*si.ru_req || !si.active[c])) return;
+  A v[3];
+  finish_reduce(so, 3, C, c, v);
   if (threadIdx.x == 0) {
     double rr = acc_re(v[0]), bb = acc_re(v[1]);
-    st.rr[c] = rr;
-    st.bb[c] = bb;
-    int act = (bb > 0.0) && (rr > st.tol2 * bb);
-    if (act && outer_it >= max_outer) {
-      act = 0;
-      st.flag[c] = 1;
+    A rz = v[2];
+    so.rr[c] = rr;
+    so.bb[c] = bb;
+    si.bb[c] = bb;
+    si.rmax2[c] = rr;
+    int conv = !(bb > 0.0) || rr <= tol2 * bb;
+    if (first) {
+      si.rho[c] = rz;
+      si.rho_prev[c] = rz;
+      si.beta[c] = acc_zero(A());
+      si.alpha[c] = acc_zero(A());
+      si.iters[c] = 0;
+      si.flag[c] = 0;
+      si.active[c] = !conv;
+    } else if (conv) {
+      si.active[c] = 0;
+      si.beta[c] = acc_zero(A());
+    } else if (acc_abs2(rz) == 0.0 || acc_abs2(si.rho_prev[c]) == 0.0) {
+      si.active[c] = 0;
+      si.flag[c] = 4;
+      si.beta[c] = acc_zero(A());
+    } else {
+      si.beta[c] = acc_div(rz, si.rho_prev[c]);
+      si.rho[c] = rz;
     }
-    st.active[c] = act;
   }
 }
 
@@ -472,11 +702,16 @@ __global__ void k_mask_cols(int64_t total, int C, const int* __restrict__ active
   }
 }
 
+// x64 += d32 ; d32 = 0   (skipped unless *gate when gate is given)
 template <typename T64, typename T32>
-__global__ void k_accum(int64_t total, const T32* __restrict__ D, T64* __restrict__ X) {
+__global__ void k_accum(int64_t total, T32* __restrict__ D, T64* __restrict__ X, const int* __restrict__ gate) {
+  if (gate && !*gate) return;
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t stp = (int64_t)gridDim.x * blockDim.x;
-  for (; i < total; i += stp) X[i] = X[i] + widen(D[i], T64());
+  for (; i < total; i += stp) {
+    X[i] = X[i] + widen(D[i], T64());
+    D[i] = sc<T32>::zero();
+  }
 }
 
 template <typename T>
@@ -517,9 +752,9 @@ static int pick_NCH(int C) {
 }
 static int64_t grid_for(int64_t n, int C) {
   int64_t rpb = KB / pick_G(C);
-  int64_t g = ceil_div(n, rpb);
+  int64_t g = ceil_div(ceil_div(n, NXCD), rpb) * NXCD;  // whole row segments per die
   if (g > NBMAX) g = NBMAX;
-  if (g < 1) g = 1;
+  if (g < NXCD) g = NXCD;
   return g;
 }
 
@@ -563,6 +798,8 @@ static void state_layout(Layout& L, int64_t C, int64_t nblocks, size_t* offs) {
   offs[7] = L.take(C * sizeof(int));                  // flag
   offs[8] = L.take(64);                               // nactive
   offs[9] = L.take(nblocks * MAXND * C * sizeof(A));  // part
+  offs[10] = L.take(C * sizeof(A));                   // rho_prev
+  offs[11] = L.take(C * sizeof(double));              // rmax2
 }
 
 template <typename A>
@@ -578,9 +815,14 @@ static ColState<A> state_at(char* base, const size_t* o, int64_t nblocks) {
   s.flag = (int*)(base + o[7]);
   s.nactive = (int*)(base + o[8]);
   s.part = (A*)(base + o[9]);
+  s.rho_prev = (A*)(base + o[10]);
+  s.rmax2 = (double*)(base + o[11]);
+  s.ru_req = nullptr;
   s.nb = (int)nblocks;
   s.tol2 = 0;
   s.maxiter = 0;
+  s.delta2 = 0;
+  s.tol2ru = 0;
   return s;
 }
 
@@ -613,6 +855,7 @@ static size_t ws_layout(int64_t n, int64_t C, int nsys, int mixed, size_t* o) {
     o[4] = L.take(n * C * sizeof(T32));  // D (inner solution)
     state_layout<A>(L, C, nb, o + 8);
     state_layout<A>(L, C, nb, o + 20);
+    o[6] = L.take(64);  // replacement request flag
   }
   return L.off;
 }
@@ -713,8 +956,20 @@ extern "C" int rb_prof_read(int id, double* total_ms, int64_t* count) {
 }
 
 #define L_INIT(G_, N_, ...) hipLaunchKernelGGL((k_cg_init<T, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
-#define L_SPMM1(G_, N_, ...) hipLaunchKernelGGL((k_cg_spmm<T, G_, N_, false>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
-#define L_SPMMN(G_, N_, ...) hipLaunchKernelGGL((k_cg_spmm<T, G_, N_, true>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
+#define L_SPMM1(G_, N_, ...)                                                                            \
+  do {                                                                                                \
+    if (G_ == 64)                                                                                     \
+      hipLaunchKernelGGL((k_cg_spmm_w64<T, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__);              \
+    else                                                                                              \
+      hipLaunchKernelGGL((k_cg_spmm<T, G_, N_, false>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__);       \
+  } while (0)
+#define L_SPMMN(G_, N_, ...)                                                                            \
+  do {                                                                                                \
+    if (G_ == 64 && k % 64 == 0)                                                                      \
+      hipLaunchKernelGGL((k_cg_spmm_w64<T, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__);              \
+    else                                                                                              \
+      hipLaunchKernelGGL((k_cg_spmm<T, G_, N_, true>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__);        \
+  } while (0)
 #define L_UPD(G_, N_, ...) hipLaunchKernelGGL((k_cg_update<T, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
 #define L_PUP(G_, N_, ...) hipLaunchKernelGGL((k_cg_pupdate<T, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
 
@@ -817,72 +1072,112 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
     return RB_OK;
   }
 
-  // mixed precision: inner CG/COCG in T32 on vals32, outer FP64 refinement
+  // mixed precision: CG/COCG in T32 on vals32 with reliable updates
+  // (Sleijpen & van der Vorst): when a column's recursive residual has
+  // dropped by RU_DELTA since its last replacement (or reaches the
+  // tolerance), x64 += d32 and every residual is replaced by the float64
+  // true residual b - A64 x64, which alone decides convergence.  The search
+  // direction is kept, so the iteration count stays that of one float64 solve.
   if (!vals32) {
     rb_set_error("rb_krylov_solve: mixed precision needs the 32-bit matrix values");
     return RB_ERR_ARG;
   }
+  const double RU_DELTA = 0.02;
   SolveBufs<T32> w{(T32*)(base + o[0]), (T32*)(base + o[1]), (T32*)(base + o[2]), (T32*)(base + o[3])};
   T32* D = (T32*)(base + o[4]);
-  ColState<A> sin = state_at<A>(base, o + 8, nb);
-  ColState<A> sout = state_at<A>(base, o + 20, nb);
-  const double inner_tol = std::max(tol, 1e-5);
-  sin.tol2 = inner_tol * inner_tol;
-  sin.maxiter = maxiter;
-  sout.tol2 = tol2;
-  sout.maxiter = maxiter;
+  int* ru_req = (int*)(base + o[6]);
+  ColState<A> si = state_at<A>(base, o + 8, nb);
+  ColState<A> so = state_at<A>(base, o + 20, nb);
+  si.tol2 = 0.0;
+  si.maxiter = maxiter;
+  si.delta2 = RU_DELTA * RU_DELTA;
+  si.tol2ru = tol2;
+  si.ru_req = ru_req;
+  so.tol2 = tol2;
+  so.maxiter = maxiter;
   const T64* A64 = (const T64*)vals64;
   const T32* A32 = (const T32*)vals32;
   hipLaunchKernelGGL(k_dinv<T32>, dim3((unsigned)ceil_div(n * nsys, 256)), dim3(256), 0, s, m, A32, nsys, w.dinv);
   RB_CHECK_LAUNCH();
-  const int max_outer = 12;
-  int total_it = 0;
-  rc = rb_fill_i32(sout.flag, 0, C, s);
+  rc = rb_fill_bytes0(w.P, (size_t)n * C * sizeof(T32), s);
   if (rc) return rc;
-  int inner_flag_max = 0;
-  for (int outer = 0;; ++outer) {
+  rc = rb_fill_bytes0(D, (size_t)n * C * sizeof(T32), s);
+  if (rc) return rc;
+  rc = rb_fill_i32(ru_req, 0, 1, s);
+  if (rc) return rc;
+  if (!use_x0) {
+    rc = rb_fill_bytes0(X, (size_t)n * C * sizeof(T64), s);
+    if (rc) return rc;
+  }
 #define L_ORES(G_, N_, ...) \
   hipLaunchKernelGGL((k_outer_resid<T64, T32, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
-    {
-      ProfScope ps(RB_PROF_RESID, s);
-      DISPATCH_GN(C, L_ORES, m, A64, B, X, (outer > 0 || use_x0) ? 1 : 0, w.Q, C, k, sout);
-      hipLaunchKernelGGL(k_fin_outer<A>, dim3(C), dim3(FB), 0, s, C, outer, max_outer, sout);
-    }
-#undef L_ORES
-    RB_CHECK_LAUNCH();
-    if (outer == 0 && !use_x0) {
-      rc = rb_fill_bytes0(X, (size_t)n * C * sizeof(T64), s);
-      if (rc) return rc;
-    }
-    int nact = 0;
-    rc = poll_nactive(sout.active, C, sout.nactive, s, &nact);
-    if (rc) return rc;
-    if (nact == 0) break;
-    // inner rhs = outer residual (in Q), zeroed on columns that are done; run_cg
-    // reads it only in its init kernel, before Q becomes the SpMM output
-    hipLaunchKernelGGL(k_mask_cols<T32>, dim3(ew_grid(n * C)), dim3(256), 0, s, (int64_t)n * C, C, sout.active, w.Q);
-    RB_CHECK_LAUNCH();
-    int its = 0;
-    rc = run_cg<T32>(m, nsys, A32, w.dinv, w.Q, D, 0, w, C, k, sin, check_every, s, &its);
-    if (rc) return rc;
-    total_it += its;
-    RB_HIP(hipMemcpyAsync(hflag.data(), sin.flag, C * sizeof(int), hipMemcpyDeviceToHost, s));
-    RB_HIP(hipStreamSynchronize(s));
-    for (int c = 0; c < C; ++c) inner_flag_max = std::max(inner_flag_max, hflag[c] == 4 ? 4 : 0);
-    hipLaunchKernelGGL((k_accum<T64, T32>), dim3(ew_grid(n * C)), dim3(256), 0, s, (int64_t)n * C, D, X);
-    RB_CHECK_LAUNCH();
+  // initial true residual (x0 or 0)
+  {
+    ProfScope ps(RB_PROF_RESID, s);
+    DISPATCH_GN(C, L_ORES, m, A64, B, X, use_x0 ? 1 : 0, w.dinv, w.R, C, k, (const int*)nullptr, so);
+    hipLaunchKernelGGL(k_fin_ru<A>, dim3(C), dim3(FB), 0, s, C, 1, tol2, so, si);
   }
-  RB_HIP(hipMemcpyAsync(hrr.data(), sout.rr, C * sizeof(double), hipMemcpyDeviceToHost, s));
-  RB_HIP(hipMemcpyAsync(hbb.data(), sout.bb, C * sizeof(double), hipMemcpyDeviceToHost, s));
-  RB_HIP(hipMemcpyAsync(hflag.data(), sout.flag, C * sizeof(int), hipMemcpyDeviceToHost, s));
+  RB_CHECK_LAUNCH();
+#define T T32
+  DISPATCH_GN(C, L_PUP, m.n, w.dinv, w.R, w.P, C, k, si);  // p = z (P is zero)
+  RB_CHECK_LAUNCH();
+  int nact = 0;
+  rc = poll_nactive(si.active, C, si.nactive, s, &nact);
+  if (rc) return rc;
+  int it = 0;
+  while (nact > 0 && it < maxiter) {
+    int chunk = std::min(check_every, maxiter - it);
+    for (int j = 0; j < chunk; ++j) {
+      {
+        ProfScope ps(RB_PROF_SPMM, s);
+        if (nsys == 1)
+          DISPATCH_GN(C, L_SPMM1, m, A32, w.P, w.Q, C, k, si);
+        else
+          DISPATCH_GN(C, L_SPMMN, m, A32, w.P, w.Q, C, k, si);
+      }
+      hipLaunchKernelGGL(k_fin_alpha<A>, dim3(C), dim3(FB), 0, s, C, si);
+      {
+        ProfScope ps(RB_PROF_UPDATE, s);
+        DISPATCH_GN(C, L_UPD, m.n, w.dinv, D, w.R, w.P, w.Q, C, k, si);
+      }
+      hipLaunchKernelGGL(k_fin_beta<A>, dim3(C), dim3(FB), 0, s, C, si);
+      {
+        // gated replacement: no-ops unless fin_beta raised ru_req
+        ProfScope ps(RB_PROF_RESID, s);
+        hipLaunchKernelGGL((k_accum<T64, T32>), dim3(1024), dim3(256), 0, s, (int64_t)n * C, D, X,
+                           (const int*)ru_req);
+        DISPATCH_GN(C, L_ORES, m, A64, B, X, 1, w.dinv, w.R, C, k, (const int*)ru_req, so);
+        hipLaunchKernelGGL(k_fin_ru<A>, dim3(C), dim3(FB), 0, s, C, 0, tol2, so, si);
+      }
+      {
+        ProfScope ps(RB_PROF_PUPDATE, s);
+        DISPATCH_GN(C, L_PUP, m.n, w.dinv, w.R, w.P, C, k, si);
+      }
+    }
+    RB_CHECK_LAUNCH();
+    it += chunk;
+    rc = poll_nactive(si.active, C, si.nactive, s, &nact);
+    if (rc) return rc;
+  }
+#undef T
+#undef L_ORES
+  // fold in the correction since the last replacement (columns stopped by maxiter)
+  hipLaunchKernelGGL((k_accum<T64, T32>), dim3(ew_grid(n * C)), dim3(256), 0, s, (int64_t)n * C, D, X,
+                     (const int*)nullptr);
+  RB_CHECK_LAUNCH();
+  RB_HIP(hipMemcpyAsync(hrr.data(), so.rr, C * sizeof(double), hipMemcpyDeviceToHost, s));
+  RB_HIP(hipMemcpyAsync(hbb.data(), so.bb, C * sizeof(double), hipMemcpyDeviceToHost, s));
+  RB_HIP(hipMemcpyAsync(hflag.data(), si.flag, C * sizeof(int), hipMemcpyDeviceToHost, s));
+  RB_HIP(hipMemcpyAsync(hit.data(), si.iters, C * sizeof(int), hipMemcpyDeviceToHost, s));
   RB_HIP(hipStreamSynchronize(s));
-  int fl = inner_flag_max;
+  int fl = 0, mx = 0;
   double rel = 0;
   for (int c = 0; c < C; ++c) {
     fl = std::max(fl, hflag[c]);
+    mx = std::max(mx, hit[c]);
     if (hbb[c] > 0) rel = std::max(rel, sqrt(hrr[c] / hbb[c]));
   }
-  if (iters_out) *iters_out = total_it;
+  if (iters_out) *iters_out = mx;
   if (relres_out) *relres_out = rel;
   if (flag_out) *flag_out = fl;
   return RB_OK;

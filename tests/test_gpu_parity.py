@@ -179,3 +179,30 @@ def test_multiwavelength_forward():
     for wv in ("690", "830"):
         assert np.linalg.norm(p1[wv] - p2[wv]) / np.linalg.norm(p2[wv]) < 1e-10
         assert np.max(np.abs(d1[wv] - d2[wv]) / np.abs(d2[wv])) < 1e-9
+
+
+@pytest.mark.parametrize("ncol", [3, 40, 64, 130])
+def test_femsolve_long_rows_and_widths(ncol):
+    """Rows longer than one wavefront, and every column-group width of the SpMM."""
+    import redbirdpy_amd as rb
+    from scipy.sparse.linalg import spsolve
+
+    rng = np.random.default_rng(ncol)
+    n = 3000
+    A = sparse.random(n, n, density=0.002, random_state=1, format="csr")
+    A = A + A.T
+    dense_rows = [5, 17, 2999]
+    for r in dense_rows:  # rows with ~150 entries
+        cols = rng.choice(n, 150, replace=False)
+        B = sparse.csr_matrix((rng.uniform(-0.1, 0.1, 150), ([r] * 150, cols)), shape=(n, n))
+        A = A + B + B.T
+    A = A + sparse.diags(np.asarray(abs(A).sum(axis=1)).ravel() + 1.0)  # SPD by diagonal dominance
+    b = rng.standard_normal((n, ncol))
+    x, flag = rb.femsolve(A, b, tol=1e-13)
+    assert flag == 0
+    xr = spsolve(A.tocsc(), b)
+    xr = xr.reshape(n, -1)
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-11
+    xm, flag = rb.femsolve(A, b, tol=1e-12, precision="mixed")
+    assert flag == 0
+    assert np.linalg.norm(xm - xr) / np.linalg.norm(xr) < 1e-10
