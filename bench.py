@@ -43,6 +43,8 @@ def _args():
     p.add_argument("--skip-cpu", action="store_true")
     p.add_argument("--skip-fp64", action="store_true")
     p.add_argument("--gn", action="store_true", help="also time Gauss-Newton iterations")
+    p.add_argument("--gn-iters", type=int, default=5)
+    p.add_argument("--gn-phases", action="store_true", help="add a synchronising per-phase timing run")
     p.add_argument("--cpu-h", type=float, default=2.0, help="mesh spacing of the bounded CPU sample")
     return p.parse_args()
 
@@ -76,6 +78,53 @@ def cpu_baseline(h):
         "seconds": dt,
         "nodes": int(cfg["node"].shape[0]),
     }
+
+
+def gn_bench(a):
+    """Gauss-Newton iterations/s: 4 wavelengths x 3 chromophores, 1024 pairs per wavelength.
+
+    Data: FP64 GPU forward of the true concentrations with 0.5% seeded
+    Gaussian noise.  Each leg starts from the background and runs
+    a.gn_iters iterations; iterations/s = iterations / wall time.
+    """
+    import copy
+    import torch
+    import redbirdpy_amd as rb
+    from redbirdpy_amd import workloads, _gpu
+    cfg0, bg, truth, extin, E = workloads.chrome_box(a.h)
+    ct, sd = rb.meshprep(copy.deepcopy(cfg0))
+    ct["param"] = {k: v.copy() for k, v in truth.items()}
+    ct["prop"] = rb.updateprop(ct, extin=extin)
+    det, _ = rb.runforward(ct)
+    rng = np.random.default_rng(40)
+    y0 = {wv: det[wv] * (1 + 0.005 * rng.standard_normal(det[wv].shape)) for wv in workloads.WAVELENGTHS}
+    m = sum(v.size for v in y0.values())
+    out = {"m": m, "nodes": int(ct["node"].shape[0]), "iters": a.gn_iters}
+    results = {}
+    for prec in ("mixed", "fp64"):
+        c, sd = rb.meshprep(copy.deepcopy(cfg0))
+        r = {"param": {k: v.copy() for k, v in bg.items()}}
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, resid, *_ = rb.runrecon(c, r, y0, sd, maxiter=a.gn_iters, lambda_=1e-3 / m, extin=extin,
+                                   precision=prec, report=False)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        # phase breakdown from a separate, synchronising run (not the timed one)
+        tm = {}
+        if a.gn_phases:
+            c2, sd2 = rb.meshprep(copy.deepcopy(cfg0))
+            r2 = {"param": {k: v.copy() for k, v in bg.items()}}
+            rb.runrecon(c2, r2, y0, sd2, maxiter=a.gn_iters, lambda_=1e-3 / m, extin=extin, precision=prec,
+                        report=False, timing=tm)
+        results[prec] = r
+        out[prec] = {"gn_iters_per_s": a.gn_iters / dt, "s_per_iter": dt / a.gn_iters,
+                     "resid": [float(x) for x in resid], "phases_s": tm}
+    errs = {k: float(np.max(np.abs(results["mixed"]["param"][k] - results["fp64"]["param"][k]))
+                     / np.max(np.abs(results["fp64"]["param"][k]))) for k in bg}
+    out["mixed"]["param_maxrel_err_vs_fp64"] = errs
+    out["speedup_mixed_vs_fp64"] = out["mixed"]["gn_iters_per_s"] / out["fp64"]["gn_iters_per_s"]
+    return out
 
 
 def main():
@@ -186,6 +235,8 @@ def main():
         out["mixed"]["err_phi_relL2_vs_fp64"] = err_phi
         out["mixed"]["err_detval_maxrel_vs_fp64"] = err_det
         out["speedup_mixed_vs_fp64"] = dt_d / dt_m
+    if a.gn:
+        out["gn"] = gn_bench(a)
     if not a.skip_cpu:
         out["cpu_baseline"] = cpu_baseline(a.cpu_h)
     print(json.dumps(out), flush=True)

@@ -203,6 +203,22 @@ int rb_gram_workspace(int64_t K, int64_t m, int mode, size_t* bytes /* host */);
 int rb_gram_mfma(int64_t K, int64_t m, const void* JT, int64_t ldj, int in_dtype, const double* d, int mode,
                  double* G, void* work, size_t work_bytes, void* stream);
 
+/*
+ * Jacobian matvecs with float64 accumulation over a node-major Jacobian
+ * JT (K x M, columns in nw blocks of np pairs; redbirdpy recon.py:501
+ * `Amat.T @ y` and the J x products of the refinement):
+ *   rb_jt_rows: t[k][w] = sum_p JT[k][w*np+p] v[w*np+p]    (t: K x nw)
+ *   rb_jt_cols: u[j] = sum_k JT[k][j] z[k][j/np] (mode 0, z: K x nw)
+ *               u[j] = sum_k JT[k][j]^2          (mode 1, the Gram diagonal)
+ * rb_jt_cols needs rb_jt_cols_workspace() bytes (per-block partials summed in
+ * a fixed order).
+ */
+int rb_jt_rows(int64_t K, int64_t M, const void* JT, int64_t ldj, int in_dtype, int nw, int64_t np,
+               const double* v, double* t, void* stream);
+int rb_jt_cols_workspace(int64_t K, int64_t M, size_t* bytes /* host */);
+int rb_jt_cols(int64_t K, int64_t M, const void* JT, int64_t ldj, int in_dtype, int nw, int64_t np,
+               const double* z, int mode, double* u, void* work, size_t work_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

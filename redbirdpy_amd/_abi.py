@@ -54,6 +54,9 @@ _SIGS = {
     "rb_gram": [_i64, _i64, _vp, _i64, _int, _vp, _int, _vp, _vp],
     "rb_gram_workspace": [_i64, _i64, _int, ctypes.POINTER(_sz)],
     "rb_gram_mfma": [_i64, _i64, _vp, _i64, _int, _vp, _int, _vp, _vp, _sz, _vp],
+    "rb_jt_rows": [_i64, _i64, _vp, _i64, _int, _int, _i64, _vp, _vp, _vp],
+    "rb_jt_cols_workspace": [_i64, _i64, ctypes.POINTER(_sz)],
+    "rb_jt_cols": [_i64, _i64, _vp, _i64, _int, _int, _i64, _vp, _int, _vp, _vp, _sz, _vp],
     "rb_prof_enable": [_int],
     "rb_prof_reset": [],
     "rb_prof_read": [_int, ctypes.POINTER(_dbl), ctypes.POINTER(_i64)],

@@ -395,8 +395,12 @@ def pair_grid(src_cols, det_cols):
     return S, D, pmap, dups
 
 
-def jac_node(mesh, phi, src_cols, det_cols, out_dtype=torch.float64, want_elem=False):
-    """JT (nn x npairs) node-major Jacobian (and optional element Jacobian, ne x npairs)."""
+def jac_node(mesh, phi, src_cols, det_cols, out_dtype=torch.float64, want_elem=False, out=None):
+    """JT (nn x npairs) node-major Jacobian (and optional element Jacobian, ne x npairs).
+
+    ``out`` may be a column slice of a wider node-major block (row stride
+    out.stride(0), unit column stride) that receives the Jacobian in place.
+    """
     npairs = len(src_cols)
     dev = phi.device
     is_complex = phi.is_complex()
@@ -404,7 +408,12 @@ def jac_node(mesh, phi, src_cols, det_cols, out_dtype=torch.float64, want_elem=F
     full = (len(S) * len(D) == npairs) and np.array_equal(pmap, np.arange(npairs))
     code = {torch.float64: _abi.RB_F64, torch.float32: _abi.RB_F32, torch.complex128: _abi.RB_C128,
             torch.complex64: _abi.RB_C64}[out_dtype]
-    JT = torch.empty((mesh.nn, npairs), dtype=out_dtype, device=dev)
+    if out is not None:
+        if out.shape != (mesh.nn, npairs) or out.dtype != out_dtype or out.stride(1) != 1:
+            raise ValueError("jac_node: out must be an (nn, npairs) view with unit column stride")
+        JT = out
+    else:
+        JT = torch.empty((mesh.nn, npairs), dtype=out_dtype, device=dev)
     JE = torch.empty((mesh.ne, npairs), dtype=out_dtype, device=dev) if want_elem else None
     if npairs == 0:
         return JT, JE
@@ -414,7 +423,7 @@ def jac_node(mesh, phi, src_cols, det_cols, out_dtype=torch.float64, want_elem=F
     phi = phi.contiguous()
     call("rb_jac_grid", mesh.nn, mesh.ne, ptr(mesh.el), ptr(mesh.evol), ptr(mesh.tinc_ptr), ptr(mesh.tinc),
          ptr(mesh.row_ptr), ptr(mesh.col_idx), mesh.maxnp, ptr(phi), phi.shape[1], int(is_complex), len(S), len(D),
-         ptr(s_d), ptr(d_d), ptr(pm), ptr(JT), npairs, code, ptr(JE), _abi.stream_ptr())
+         ptr(s_d), ptr(d_d), ptr(pm), ptr(JT), JT.stride(0), code, ptr(JE), _abi.stream_ptr())
     for r, f in dups:
         JT[:, r] = JT[:, f]
         if JE is not None:
@@ -459,3 +468,27 @@ def gram(JT, d=None, mode="fp64"):
     call("rb_gram_mfma", K, m, ptr(JT), JT.stride(0), code, ptr(d), md, ptr(G), ptr(work), nb.value,
          _abi.stream_ptr())
     return G
+
+
+def jt_rows(JT, nw, npairs, v):
+    """t[n, w] = sum_p JT[n, w*np + p] v[w*np + p] in float64 (J_w^T v_w for every wavelength)."""
+    K, M = JT.shape
+    t = torch.empty((K, nw), dtype=torch.float64, device=JT.device)
+    code = _abi.RB_F64 if JT.dtype == torch.float64 else _abi.RB_F32
+    v = v.to(torch.float64).contiguous()
+    call("rb_jt_rows", K, M, ptr(JT), JT.stride(0), code, nw, npairs, ptr(v), ptr(t), _abi.stream_ptr())
+    return t
+
+
+def jt_cols(JT, nw, npairs, z=None):
+    """u[j] = sum_n JT[n, j] z[n, j // np] (J_w z_w), or sum_n JT[n, j]^2 when z is None."""
+    K, M = JT.shape
+    u = torch.empty(M, dtype=torch.float64, device=JT.device)
+    code = _abi.RB_F64 if JT.dtype == torch.float64 else _abi.RB_F32
+    nb = ctypes.c_size_t(0)
+    call("rb_jt_cols_workspace", K, M, ctypes.byref(nb))
+    work = torch.empty(int(nb.value), dtype=torch.uint8, device=JT.device)
+    zz = None if z is None else z.to(torch.float64).contiguous()
+    call("rb_jt_cols", K, M, ptr(JT), JT.stride(0), code, nw, npairs, ptr(zz), 0 if z is not None else 1, ptr(u),
+         ptr(work), nb.value, _abi.stream_ptr())
+    return u

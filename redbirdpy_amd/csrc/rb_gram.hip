@@ -261,6 +261,100 @@ extern "C" int rb_gram_mfma(int64_t K, int64_t m, const void* JT, int64_t ldj, i
   return RB_OK;
 }
 
+// ---------------------------------------------------------------------------
+// K9: Jacobian matvecs with float64 accumulation (JT: K x M node-major, the
+// M columns grouped in nw blocks of np pairs).
+//   rows: t[k][w] = sum_p JT[k][w*np + p] v[w*np + p]           (J_w^T v_w)
+//   cols: u[j]    = sum_k JT[k][j] z[k][j / np]  (mode 0)        (J_w z_w)
+//         u[j]    = sum_k JT[k][j]^2             (mode 1)        (diag of J J^T)
+// ---------------------------------------------------------------------------
+template <typename TI>
+__global__ void __launch_bounds__(256) k_jt_rows(int64_t K, int64_t M, const TI* __restrict__ JT, int64_t ldj,
+                                                 int nw, int64_t np, const double* __restrict__ v,
+                                                 double* __restrict__ t) {
+  const int lane = threadIdx.x & 63;
+  const int64_t k = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (k >= K) return;
+  const TI* row = JT + k * ldj;
+  for (int w = 0; w < nw; ++w) {
+    double acc = 0.0;
+    for (int64_t p = lane; p < np; p += 64) acc += (double)row[w * np + p] * v[w * np + p];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+    if (lane == 0) t[k * nw + w] = acc;
+  }
+}
+
+#define JCR 256  // rows per block of the column kernel
+template <typename TI>
+__global__ void __launch_bounds__(256) k_jt_cols(int64_t K, int64_t M, const TI* __restrict__ JT, int64_t ldj,
+                                                 int nw, int64_t np, const double* __restrict__ z, int mode,
+                                                 double* __restrict__ part) {
+  const int64_t k0 = (int64_t)blockIdx.y * JCR;
+  const int64_t k1 = min(K, k0 + JCR);
+  for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < M; j += (int64_t)gridDim.x * 256) {
+    const int64_t w = j / np;
+    double acc = 0.0;
+    for (int64_t k = k0; k < k1; ++k) {
+      double a = (double)JT[k * ldj + j];
+      acc += mode ? a * a : a * z[k * nw + w];
+    }
+    part[(int64_t)blockIdx.y * M + j] = acc;
+  }
+}
+
+__global__ void k_sum_parts(int64_t M, int nparts, const double* __restrict__ part, double* __restrict__ u) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= M) return;
+  double acc = 0.0;
+  for (int b = 0; b < nparts; ++b) acc += part[(int64_t)b * M + j];
+  u[j] = acc;
+}
+
+extern "C" int rb_jt_rows(int64_t K, int64_t M, const void* JT, int64_t ldj, int in_dtype, int nw, int64_t np,
+                          const double* v, double* t, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (K <= 0 || nw * np > M) return RB_ERR_ARG;
+  unsigned g = (unsigned)ceil_div(K, 4);
+  if (in_dtype == RB_F64)
+    hipLaunchKernelGGL(k_jt_rows<double>, dim3(g), dim3(256), 0, s, K, M, (const double*)JT, ldj, nw, np, v, t);
+  else if (in_dtype == RB_F32)
+    hipLaunchKernelGGL(k_jt_rows<float>, dim3(g), dim3(256), 0, s, K, M, (const float*)JT, ldj, nw, np, v, t);
+  else
+    return RB_ERR_ARG;
+  RB_CHECK_LAUNCH();
+  return RB_OK;
+}
+
+extern "C" int rb_jt_cols_workspace(int64_t K, int64_t M, size_t* bytes) {
+  if (!bytes) return RB_ERR_ARG;
+  *bytes = (size_t)ceil_div(K, JCR) * M * sizeof(double);
+  return RB_OK;
+}
+
+extern "C" int rb_jt_cols(int64_t K, int64_t M, const void* JT, int64_t ldj, int in_dtype, int nw, int64_t np,
+                          const double* z, int mode, double* u, void* work, size_t work_bytes, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  int64_t nparts = ceil_div(K, JCR);
+  if (K <= 0 || nw * np > M || (size_t)nparts * M * sizeof(double) > work_bytes || !work) {
+    rb_set_error("rb_jt_cols: bad sizes or workspace");
+    return RB_ERR_ARG;
+  }
+  dim3 g((unsigned)ceil_div(M, 256), (unsigned)nparts);
+  if (in_dtype == RB_F64)
+    hipLaunchKernelGGL(k_jt_cols<double>, g, dim3(256), 0, s, K, M, (const double*)JT, ldj, nw, np, z, mode,
+                       (double*)work);
+  else if (in_dtype == RB_F32)
+    hipLaunchKernelGGL(k_jt_cols<float>, g, dim3(256), 0, s, K, M, (const float*)JT, ldj, nw, np, z, mode,
+                       (double*)work);
+  else
+    return RB_ERR_ARG;
+  RB_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_sum_parts, dim3((unsigned)ceil_div(M, 256)), dim3(256), 0, s, M, (int)nparts,
+                     (const double*)work, u);
+  RB_CHECK_LAUNCH();
+  return RB_OK;
+}
+
 extern "C" int rb_gram(int64_t K, int64_t m, const void* JT, int64_t ldj, int in_dtype, const double* d, int mode,
                        double* G, void* stream) {
   hipStream_t s = (hipStream_t)stream;

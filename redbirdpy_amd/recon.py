@@ -136,7 +136,7 @@ def reginvunder(Amat, rhs, lambda_, invR=None, blocks=None, **kwargs):
     H = _gpu.gram(AT)
     H.diagonal().add_(lambda_)
     y = _chol_solve_normalized(H, b)
-    return (AT @ y).cpu().numpy()
+    return _gpu.jt_rows(AT, 1, AT.shape[1], y)[:, 0].cpu().numpy()
 
 
 def reginvover(Amat, rhs, lambda_, LTL=None, blocks=None, **kwargs):
@@ -148,7 +148,7 @@ def reginvover(Amat, rhs, lambda_, LTL=None, blocks=None, **kwargs):
     b = torch.as_tensor(np.asarray(rhs, dtype=np.float64).ravel(), device=dev)
     H = _gpu.gram(A)
     H.diagonal().add_(lambda_)
-    g = A.t() @ b
+    g = _gpu.jt_cols(A, 1, A.shape[1], b.reshape(-1, 1))  # A^T b
     return _chol_solve_normalized(H, g).cpu().numpy()
 
 
@@ -164,37 +164,80 @@ def reginv(Amat, rhs, lambda_, Areg=None, blocks=None, **kwargs):
 # Gauss-Newton loop
 # ---------------------------------------------------------------------------
 class GNStep:
-    """One normal-equation solve in Kronecker form.
+    """One normal-equation solve in Kronecker form (recon.py:429-509 with blockscale, :235-240).
 
     Unknown blocks c (chromophores, or a single 'mua' block) share the node
     dimension; row blocks w (wavelengths) hold npairs rows each.  The
-    flattened Jacobian is J[(w,p), (c,n)] = E[w,c] Jw[p,n].
+    flattened Jacobian is J[(w,p), (c,n)] = E[w,c] Jw[p,n], so
+    J J^T = (E E^T)[w,w'] * (Jw Jw'^T): one Gram over the nodes.
+
+    fp64 : float64 Jacobian, float64 Gram, Cholesky of the diagonally
+           normalised s^2 J J^T + lambda I.
+    mixed: float32 Jacobian, bf16x3 MFMA Gram whose diagonal is replaced by
+           the float64 column norms, float64 Cholesky of it as a
+           preconditioner, then iterative refinement with float64-accumulated
+           products J (J^T v) until the residual is below ``reftol``.
+    Returns dmu (nn x nc) = s^2 J^T y.
     """
 
-    def __init__(self, JT, npairs, E, lam, precision="fp64"):
-        self.JT = JT  # (nn, nw*npairs) float64 (or float32 for mixed)
+    def __init__(self, JT, npairs, E, lam, precision="fp64", reftol=1e-10, maxref=20):
+        self.JT = JT
         self.np = npairs
-        self.E = torch.as_tensor(E, dtype=torch.float64, device=JT.device)  # (nw, nc)
-        self.lam = lam
+        self.E = torch.as_tensor(E, dtype=torch.float64, device=JT.device)
+        self.lam = float(lam)
         self.precision = precision
+        self.reftol = reftol
+        self.maxref = maxref
+        self.info = {}
+
+    def _hv(self, v, s2, W):
+        """H v = s^2 W o (J J^T) v + lambda v with float64 accumulation."""
+        nw = W.shape[0]
+        t = _gpu.jt_rows(self.JT, nw, self.np, v)  # (nn, nw)
+        z = t @ W  # W symmetric
+        u = _gpu.jt_cols(self.JT, nw, self.np, z)
+        return s2 * u + self.lam * v
 
     def solve(self, misfit):
         JT, npairs, E = self.JT, self.np, self.E
         nw, nc = E.shape
-        m = nw * npairs
-        G = _gpu.gram(JT)  # (m, m) float64
-        W = E @ E.t()  # (nw, nw)
+        W = E @ E.t()
         Wk = torch.repeat_interleave(torch.repeat_interleave(W, npairs, 0), npairs, 1)
-        H0 = G * Wk
-        s2 = 1.0 / torch.trace(H0)  # 1/||J_flat||_F^2
-        H = s2 * H0
+        mk = getattr(self, "mark", None) or (lambda name, t: t)
+        tq = time.perf_counter()
+        colsq = _gpu.jt_cols(JT, nw, npairs)  # exact ||J_w row||^2, float64
+        wdiag = torch.repeat_interleave(torch.diagonal(W), npairs)
+        s2 = 1.0 / torch.sum(wdiag * colsq)  # 1/||J_flat||_F^2
+        if self.precision == "fp64":
+            G = _gpu.gram(JT, None, "fp64")
+        else:
+            G = _gpu.gram(JT, None, "bf16x3")
+            G.diagonal().copy_(colsq)
+        tq = mk("gn_gram", tq)
+        H = s2 * (G * Wk)
         H.diagonal().add_(self.lam)
-        y = _chol_solve_normalized(H, misfit)
-        # J_flat^T y = sum_w E[w, c] JT_w y_w
-        T = torch.stack([JT[:, w * npairs : (w + 1) * npairs].to(torch.float64) @ y[w * npairs : (w + 1) * npairs]
-                         for w in range(nw)], dim=1)  # (nn, nw)
-        dmu = s2 * (T @ E)  # (nn, nc)
-        return dmu
+        Hn, di = _normalize_diag(H)
+        L = torch.linalg.cholesky(Hn)
+        tq = mk("gn_cholesky", tq)
+
+        def hsolve(r):
+            return di * torch.cholesky_solve((di * r).reshape(-1, 1), L).reshape(-1)
+
+        y = hsolve(misfit)
+        nref = 0
+        if self.precision != "fp64":
+            rn = torch.linalg.norm(misfit)
+            for nref in range(1, self.maxref + 1):
+                res = misfit - self._hv(y, s2, W)
+                rel = (torch.linalg.norm(res) / rn).item()
+                if rel <= self.reftol:
+                    break
+                y = y + hsolve(res)
+            self.info["refine_relres"] = rel
+        self.info["refinements"] = nref
+        tq = mk("gn_refine", tq)
+        T = _gpu.jt_rows(JT, nw, npairs, y)  # (nn, nw) = J_w^T y_w
+        return s2 * (T @ E)
 
 
 def _chromophores(cfg):
@@ -225,6 +268,8 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
     extin = kwargs.get("extin", None)
     stol = kwargs.get("solvertol", 1e-12 if precision == "fp64" else 1e-10)
     return_jac = kwargs.get("return_jac", False)
+    warm = kwargs.get("warmstart", precision != "fp64")  # start each forward solve from the last fields
+    optodes = kwargs.get("optodes", None)
     if isinstance(rfcw, int):
         rfcw = [rfcw]
     if rfcw != [1] or reform not in ("real", "complex"):
@@ -239,14 +284,29 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
     updates = []
     X0 = None
     detphi_out = phi_out = Jm_out = None
+    timing = kwargs.get("timing", None)  # dict: per-phase seconds (synchronising) when given
+
+    def mark(name, t_prev):
+        if timing is None:
+            return t_prev
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        timing[name] = timing.get(name, 0.0) + (t - t_prev)
+        return t
+
     for it in range(maxiter):
         t0 = time.time()
+        tp = time.perf_counter()
         if "param" in recon or "prop" in recon:
             cfg, recon = syncprop(cfg, recon)
         if "param" in cfg and isinstance(cfg.get("prop"), dict):
             cfg["prop"] = updateprop(cfg, extin=extin)
-        res = forward_device(cfg, rfcw, precision, stol, None, X0)
+        tp = mark("props", tp)
+        res = forward_device(cfg, rfcw, precision, stol, None, X0 if warm else None, optodes=optodes)
         X0 = res.X
+        tp = mark("forward", tp)
+        if timing is not None:
+            timing.setdefault("forward_iters", []).append(res.info.iters)
         wvs = list(cfg["prop"].keys()) if isinstance(cfg.get("prop"), dict) else [""]
         chs = _chromophores(cfg)
         mesh = _gpu.mesh_for(cfg)
@@ -263,8 +323,8 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
                 JT = torch.empty((mesh.nn, len(wvs) * npairs), dtype=jdt, device=mesh.dev)
             elif len(s) != npairs:
                 raise NotImplementedError("all wavelengths must have the same active pairs")
-            Jw, _ = _gpu.jac_node(mesh, res.phi(i), s, d, jdt)
-            JT[:, i * npairs : (i + 1) * npairs] = Jw
+            _gpu.jac_node(mesh, res.phi(i), s, d, jdt, out=JT[:, i * npairs : (i + 1) * npairs])
+        tp = mark("jacobian", tp)
         detphi = {wv: res.detvals[i].cpu().numpy() for i, wv in enumerate(wvs)}
         if wvs == [""]:
             detphi = detphi[""]
@@ -283,8 +343,13 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
                 raise NotImplementedError("multi-wavelength mua reconstruction needs chromophore params")
             E = np.ones((1, 1))
             keys = ["mua"]
-        step = GNStep(JT, npairs, E, lam, precision)
+        step = GNStep(JT, npairs, E, lam, precision, reftol=kwargs.get("reftol", 1e-10))
+        step.mark = (lambda name, t: mark(name, t)) if timing is not None else None
+        tp = mark("misfit", tp)
         dmu = step.solve(torch.as_tensor(misfit, dtype=torch.float64, device=mesh.dev)).cpu().numpy()
+        tp = mark("gn_solve", tp)
+        if timing is not None:
+            timing.setdefault("refinements", []).append(step.info.get("refinements", 0))
         update = {}
         for c, key in enumerate(keys):
             dx = dmu[:, c]

@@ -92,16 +92,22 @@ def chrome_extinction(seed=4):
     return {wv: {c: E[i, j] for j, c in enumerate(CHROMES)} for i, wv in enumerate(WAVELENGTHS)}, E
 
 
+CHROME_BASE = {"hbo": 20.0, "hbr": 10.0, "water": 0.5}
+
+
 def chrome_box(h=1.0, box=BOX, n_lattice=8):
-    """Multispectral truth: returns (cfg, true params, extin dict, E)."""
+    """Multispectral setup: returns (cfg, background params, true params, extin dict, E).
+
+    cfg carries label properties per wavelength (background optics) and no
+    'param', so meshprep does not run updateprop on it; the caller puts the
+    concentrations in after meshprep (or in recon['param']).
+    """
     cfg = jac_box(h, box, n_lattice)
-    node, elem = cfg["node"], cfg["elem"]
+    node = cfg["node"]
     nn = node.shape[0]
     inside = np.sum((node - np.array(SPHERE_C)) ** 2, axis=1) <= SPHERE_R ** 2
-    base = {"hbo": 20.0, "hbr": 10.0, "water": 0.5}
-    truth = {c: np.where(inside, 2 * v, v).astype(float) for c, v in base.items()}
+    truth = {c: np.where(inside, 2 * v, v).astype(float) for c, v in CHROME_BASE.items()}
+    background = {c: np.full(nn, v) for c, v in CHROME_BASE.items()}
     cfg["prop"] = {wv: np.array([[0, 0, 1, 1], [0.01, 1.0, 0, 1.37]]) for wv in WAVELENGTHS}
-    cfg["param"] = {c: np.full(nn, v) for c, v in base.items()}
-    cfg.pop("seg")
     extin, E = chrome_extinction()
-    return cfg, truth, extin, E
+    return cfg, background, truth, extin, E

@@ -1,0 +1,56 @@
+"""Gauss-Newton reconstruction on the GPU against the oracle's runrecon."""
+
+import copy
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(h=5.0, n_lattice=4):
+    from redbirdpy_amd import workloads
+    from oracle import rb_oracle as orc
+
+    cfg0, bg, truth, extin, E = workloads.chrome_box(h=h, n_lattice=n_lattice)
+    ct, _ = orc.meshprep(copy.deepcopy(cfg0))
+    ct["param"] = {k: v.copy() for k, v in truth.items()}
+    ct["prop"] = orc.updateprop(ct, extin=extin)
+    det, _ = orc.runforward(ct)
+    rng = np.random.default_rng(40)
+    y0 = {wv: det[wv] * (1 + 0.005 * rng.standard_normal(det[wv].shape)) for wv in workloads.WAVELENGTHS}
+    return cfg0, bg, extin, y0
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-7), ("mixed", 1e-6)])
+def test_runrecon_chromophores_match_oracle(precision, tol):
+    import redbirdpy_amd as rb
+    from oracle import rb_oracle as orc
+
+    cfg0, bg, extin, y0 = _problem()
+    c1, sd1 = rb.meshprep(copy.deepcopy(cfg0))
+    c2, sd2 = orc.meshprep(copy.deepcopy(cfg0))
+    m = sum(v.size for v in y0.values())
+    lam = 1e-3 / m
+    r1 = {"param": {k: v.copy() for k, v in bg.items()}}
+    r2 = {"param": {k: v.copy() for k, v in bg.items()}}
+    out1 = rb.runrecon(c1, r1, y0, sd1, maxiter=3, lambda_=lam, extin=extin, precision=precision, report=False)
+    out2 = orc.runrecon(c2, r2, y0, sd2, maxiter=3, lambda_=lam, extin=extin)
+    for k in bg:
+        a, b = r1["param"][k], r2["param"][k]
+        err = np.max(np.abs(a - b)) / np.max(np.abs(b))
+        assert err < tol, (k, err)
+    # the misfit sum is a difference of near-equal numbers at the noise floor
+    np.testing.assert_allclose(out1[1], out2[1], rtol=1e-9 if precision == "fp64" else 1e-5)
+    assert out1[1][-1] < out1[1][0]  # the misfit decreases
+
+
+def test_reginv_drop_in():
+    import redbirdpy_amd as rb
+    from oracle import rb_oracle as orc
+
+    rng = np.random.default_rng(2)
+    for m, n in ((40, 300), (300, 40)):
+        A = rng.standard_normal((m, n))
+        b = rng.standard_normal(m)
+        np.testing.assert_allclose(rb.reginv(A, b, 0.1), orc.reginv(A, b, 0.1), rtol=1e-9, atol=1e-12)
