@@ -44,6 +44,7 @@ def _args():
     p.add_argument("--skip-fp64", action="store_true")
     p.add_argument("--gn", action="store_true", help="also time Gauss-Newton iterations")
     p.add_argument("--gn-iters", type=int, default=5)
+    p.add_argument("--gram", action="store_true", help="time the Gram kernel modes at the GN shape")
     p.add_argument("--gn-phases", action="store_true", help="add a synchronising per-phase timing run")
     p.add_argument("--cpu-h", type=float, default=2.0, help="mesh spacing of the bounded CPU sample")
     return p.parse_args()
@@ -124,6 +125,44 @@ def gn_bench(a):
                      / np.max(np.abs(results["fp64"]["param"][k]))) for k in bg}
     out["mixed"]["param_maxrel_err_vs_fp64"] = errs
     out["speedup_mixed_vs_fp64"] = out["mixed"]["gn_iters_per_s"] / out["fp64"]["gn_iters_per_s"]
+    return out
+
+
+def gram_bench(nn, m=4096, reps=3):
+    """J J^T at the Gauss-Newton shape: K = nodes, m = 4 wavelengths x 1024 pairs.
+
+    SYRK flops m(m+1)K; bf16x3 does three MFMA products per term.  Peaks:
+    bf16 dense MFMA 2.5 PFLOP/s, fp64 78.6 TFLOP/s.
+    """
+    import torch
+    from redbirdpy_amd import _gpu
+
+    g = torch.Generator(device="cuda").manual_seed(0)
+    JT = torch.randn((nn, m), generator=g, device="cuda", dtype=torch.float32)
+    JT *= torch.exp(torch.linspace(-6, 0, m, device="cuda"))[None, :]
+    flops = m * (m + 1) * nn
+    out = {"K": nn, "m": m, "syrk_flop": flops}
+    ref = None
+    for mode, peak, terms in (("bf16x3", 2500.0, 3), ("bf16", 2500.0, 1), ("fp64", 78.6, 1)):
+        _gpu.gram(JT, None, mode)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            G = _gpu.gram(JT, None, mode)
+        e1.record()
+        torch.cuda.synchronize()
+        dt = e0.elapsed_time(e1) / 1e3 / reps
+        tf = terms * flops / dt / 1e12
+        rec = {"ms": dt * 1e3, "tflops": tf, "peak_tflops": peak, "frac": tf / peak}
+        if mode == "fp64":
+            ref = G
+        out[mode] = rec
+    for mode in ("bf16x3", "bf16"):
+        G = _gpu.gram(JT, None, mode)
+        d = torch.sqrt(torch.diagonal(ref))
+        out[mode]["max_abs_err_normalised"] = (torch.abs(G - ref) / (d[:, None] * d[None, :])).max().item()
     return out
 
 
@@ -235,6 +274,8 @@ def main():
         out["mixed"]["err_phi_relL2_vs_fp64"] = err_phi
         out["mixed"]["err_detval_maxrel_vs_fp64"] = err_det
         out["speedup_mixed_vs_fp64"] = dt_d / dt_m
+    if a.gram:
+        out["gram"] = gram_bench(mesh.nn)
     if a.gn:
         out["gn"] = gn_bench(a)
     if not a.skip_cpu:

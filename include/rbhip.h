@@ -194,10 +194,11 @@ int rb_gram(int64_t K, int64_t m, const void* JT, int64_t ldj, int in_dtype, con
 
 /*
  * Gram on the matrix cores: mode 1 = bf16 (hi*hi), mode 3 = bf16x3 split
- * (hi*hi + hi*lo + lo*hi, ~16 mantissa bits per product), fp32 MFMA
- * accumulation flushed to fp64 every 256 k, K split in 4 chunks summed in
- * a fixed order.  work: rb_gram_workspace() bytes (4 m^2 doubles).
- * Same result layout as rb_gram (redbirdpy recon.py:474).
+ * (hi*hi + hi*lo + lo*hi, ~16 mantissa bits per product).  JT is split once
+ * into transposed bf16 hi/lo arrays; fp32 MFMA accumulation within 16 K
+ * chunks whose partial tiles are summed in fp64 in a fixed order.
+ * work: rb_gram_workspace() bytes.  Same result as rb_gram
+ * (redbirdpy recon.py:474).
  */
 int rb_gram_workspace(int64_t K, int64_t m, int mode, size_t* bytes /* host */);
 int rb_gram_mfma(int64_t K, int64_t m, const void* JT, int64_t ldj, int in_dtype, const double* d, int mode,

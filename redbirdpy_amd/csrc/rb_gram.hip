@@ -72,29 +72,25 @@ __global__ void __launch_bounds__(256) k_gram_f64(int64_t K, int64_t m, const TI
 
 // ---------------------------------------------------------------------------
 // mode 1 (bf16 hi*hi) and mode 3 (bf16x3: hi*hi + hi*lo + lo*hi) on MFMA
-// v_mfma_f32_32x32x16_bf16.  Each float is split x = hi + lo with hi, lo
-// bf16 (round to nearest even), giving ~16 mantissa bits per product.
-// Block: 256 threads = 4 waves, 128x128 output tile, each wave 64x64 as
-// 2x2 MFMA tiles.  K is streamed in slabs of 32 through LDS stored [i][k]
-// (k contiguous, so a lane's 8 k-values are one 16-byte read); fp32 MFMA
-// accumulators are flushed into fp64 registers every 8 slabs; K is split
-// over S chunks whose fp64 partial tiles are summed in a fixed order by
-// k_gram_reduce.
+// v_mfma_f32_32x32x16_bf16.  Each value is split x = hi + lo with hi, lo
+// bf16 (round to nearest even), ~16 mantissa bits per product.
+//
+// 1. k_split_t: JT (K x m, fp32/fp64) -> Hi, Lo (m x Kp bf16, transposed,
+//    zero padded to Kp = K rounded up to 64), through 64x64 LDS tiles.
+// 2. k_gram_bf16: 256 threads = 4 waves per 128x128 upper-triangular output
+//    tile and K chunk (split-K over S chunks).  Slabs of 64 k are staged in
+//    LDS with 16-byte coalesced loads (each row's 64 k are 128 contiguous
+//    bytes of the transposed arrays); each wave owns 64x64 = 2x2 MFMA
+//    tiles.  Accumulation is fp32 inside a chunk; the S fp32 partial tiles
+//    are summed in fp64 in a fixed order by k_gram_reduce.
 // ---------------------------------------------------------------------------
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(16))) float f32x16;
 
 #define MT 128       // output tile
-#define MKS 32       // K slab
-#define MLD (MKS + 8)  // LDS row stride in bf16 (padding against bank conflicts)
-#define FLUSH 8      // slabs between fp32 -> fp64 flushes
-
-template <typename TI>
-__device__ inline void split_bf16(TI x, __bf16& hi, __bf16& lo) {
-  float xf = (float)x;
-  hi = (__bf16)xf;
-  lo = (__bf16)(float)((double)x - (double)(float)hi);
-}
+#define MKS 64       // K slab
+#define MLD (MKS + 8)  // LDS row stride in bf16 (16-byte pad against bank conflicts)
+#define GSPLIT 16    // K chunks
 
 // upper-triangular tile t -> (ti, tj), ti <= tj
 __device__ inline void tri_tile(int64_t t, int64_t nt, int64_t& ti, int64_t& tj) {
@@ -106,13 +102,38 @@ __device__ inline void tri_tile(int64_t t, int64_t nt, int64_t& ti, int64_t& tj)
   tj = ti + t;
 }
 
-template <typename TI, int NTERM>
-__global__ void __launch_bounds__(256) k_gram_mfma(int64_t K, int64_t m, const TI* __restrict__ JT, int64_t ldj,
-                                                   int nsplit, double* __restrict__ part) {
-  __shared__ __align__(16) __bf16 Ahi[MT * MLD];
-  __shared__ __align__(16) __bf16 Alo[MT * MLD];
-  __shared__ __align__(16) __bf16 Bhi[MT * MLD];
-  __shared__ __align__(16) __bf16 Blo[MT * MLD];
+template <typename TI>
+__global__ void __launch_bounds__(256) k_split_t(int64_t K, int64_t m, int64_t Kp, const TI* __restrict__ JT,
+                                                 int64_t ldj, __bf16* __restrict__ Hi, __bf16* __restrict__ Lo) {
+  __shared__ float th[64][65];
+  __shared__ float tl[64][65];
+  const int64_t k0 = (int64_t)blockIdx.x * 64, c0 = (int64_t)blockIdx.y * 64;
+  for (int q = threadIdx.x; q < 64 * 64; q += 256) {
+    int kk = q / 64, cc = q % 64;
+    int64_t k = k0 + kk, c = c0 + cc;
+    double x = (k < K && c < m) ? (double)JT[k * ldj + c] : 0.0;
+    __bf16 h = (__bf16)(float)x;
+    th[cc][kk] = (float)h;
+    tl[cc][kk] = (float)(x - (double)(float)h);
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < 64 * 64; q += 256) {
+    int cc = q / 64, kk = q % 64;
+    int64_t c = c0 + cc;
+    if (c < m) {
+      Hi[c * Kp + k0 + kk] = (__bf16)th[cc][kk];
+      Lo[c * Kp + k0 + kk] = (__bf16)tl[cc][kk];
+    }
+  }
+}
+
+template <int NTERM>
+__global__ void __launch_bounds__(256) k_gram_bf16(int64_t m, int64_t Kp, const __bf16* __restrict__ Hi,
+                                                   const __bf16* __restrict__ Lo, float* __restrict__ part) {
+  __shared__ __align__(16) __bf16 Ah[MT * MLD];
+  __shared__ __align__(16) __bf16 Al[MT * MLD];
+  __shared__ __align__(16) __bf16 Bh[MT * MLD];
+  __shared__ __align__(16) __bf16 Bl[MT * MLD];
   const int64_t nt = (m + MT - 1) / MT;
   const int64_t ntri = nt * (nt + 1) / 2;
   const int64_t tile = blockIdx.x % ntri;
@@ -120,38 +141,39 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int64_t K, int64_t m, const T
   int64_t ti, tj;
   tri_tile(tile, nt, ti, tj);
   const int64_t i0 = ti * MT, j0 = tj * MT;
-  const int64_t kchunk = ((K + nsplit - 1) / nsplit + MKS - 1) / MKS * MKS;
+  const int64_t kchunk = ((Kp + GSPLIT - 1) / GSPLIT + MKS - 1) / MKS * MKS;
   const int64_t kb = (int64_t)split * kchunk;
-  const int64_t ke = min(K, kb + kchunk);
+  const int64_t ke = min(Kp, kb + kchunk);
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
   f32x16 acc[2][2];
-  double acc64[2][2][16];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b)
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        acc[a][b][r] = 0.f;
-        acc64[a][b][r] = 0.0;
+      for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
+  for (int64_t k0 = kb; k0 < ke; k0 += MKS) {
+    // 128 rows x 64 k per operand = 1024 chunks of 8 bf16; 4 per thread
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int cidx = threadIdx.x + 256 * q;
+      const int row = cidx >> 3, c8 = (cidx & 7) * 8;
+      const int64_t gi = i0 + row, gj = j0 + row;
+      bf16x8 z;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) z[e] = (__bf16)0.f;
+      bf16x8 vah = (gi < m) ? *reinterpret_cast<const bf16x8*>(Hi + gi * Kp + k0 + c8) : z;
+      bf16x8 vbh = (gj < m) ? *reinterpret_cast<const bf16x8*>(Hi + gj * Kp + k0 + c8) : z;
+      *reinterpret_cast<bf16x8*>(&Ah[row * MLD + c8]) = vah;
+      *reinterpret_cast<bf16x8*>(&Bh[row * MLD + c8]) = vbh;
+      if (NTERM > 1) {
+        bf16x8 val = (gi < m) ? *reinterpret_cast<const bf16x8*>(Lo + gi * Kp + k0 + c8) : z;
+        bf16x8 vbl = (gj < m) ? *reinterpret_cast<const bf16x8*>(Lo + gj * Kp + k0 + c8) : z;
+        *reinterpret_cast<bf16x8*>(&Al[row * MLD + c8]) = val;
+        *reinterpret_cast<bf16x8*>(&Bl[row * MLD + c8]) = vbl;
       }
-  int slab = 0;
-  for (int64_t k0 = kb; k0 < ke; k0 += MKS, ++slab) {
-    // global -> bf16 split -> LDS [i][k]; 256 threads x 16 values per operand
-    for (int q = threadIdx.x; q < MKS * MT; q += 256) {
-      int kk = q / MT, c = q - kk * MT;
-      int64_t k = k0 + kk;
-      TI va = (k < ke && i0 + c < m) ? JT[k * ldj + i0 + c] : (TI)0;
-      TI vb = (k < ke && j0 + c < m) ? JT[k * ldj + j0 + c] : (TI)0;
-      __bf16 h, l;
-      split_bf16(va, h, l);
-      Ahi[c * MLD + kk] = h;
-      Alo[c * MLD + kk] = l;
-      split_bf16(vb, h, l);
-      Bhi[c * MLD + kk] = h;
-      Blo[c * MLD + kk] = l;
     }
     __syncthreads();
 #pragma unroll
@@ -160,13 +182,13 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int64_t K, int64_t m, const T
       bf16x8 ah[2], al[2], bh[2], bl[2];
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
-        int ra = wr + t * 32 + (lane & 31);
-        int rb = wc + t * 32 + (lane & 31);
-        ah[t] = *reinterpret_cast<const bf16x8*>(&Ahi[ra * MLD + kof]);
-        bh[t] = *reinterpret_cast<const bf16x8*>(&Bhi[rb * MLD + kof]);
+        const int ra = wr + t * 32 + (lane & 31);
+        const int rb = wc + t * 32 + (lane & 31);
+        ah[t] = *reinterpret_cast<const bf16x8*>(&Ah[ra * MLD + kof]);
+        bh[t] = *reinterpret_cast<const bf16x8*>(&Bh[rb * MLD + kof]);
         if (NTERM > 1) {
-          al[t] = *reinterpret_cast<const bf16x8*>(&Alo[ra * MLD + kof]);
-          bl[t] = *reinterpret_cast<const bf16x8*>(&Blo[rb * MLD + kof]);
+          al[t] = *reinterpret_cast<const bf16x8*>(&Al[ra * MLD + kof]);
+          bl[t] = *reinterpret_cast<const bf16x8*>(&Bl[rb * MLD + kof]);
         }
       }
 #pragma unroll
@@ -181,19 +203,8 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int64_t K, int64_t m, const T
         }
     }
     __syncthreads();
-    if ((slab + 1) % FLUSH == 0) {
-#pragma unroll
-      for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-          for (int r = 0; r < 16; ++r) {
-            acc64[a][b][r] += (double)acc[a][b][r];
-            acc[a][b][r] = 0.f;
-          }
-    }
   }
-  double* out = part + (int64_t)split * m * m;
+  float* out = part + (int64_t)split * m * m;
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -203,60 +214,70 @@ __global__ void __launch_bounds__(256) k_gram_mfma(int64_t K, int64_t m, const T
         // 32x32 MFMA output layout: col = lane % 32, row = 8*(r/4) + 4*(lane/32) + r%4
         int64_t i = i0 + wr + a * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
         int64_t j = j0 + wc + b * 32 + (lane & 31);
-        if (i < m && j < m) out[i * m + j] = acc64[a][b][r] + (double)acc[a][b][r];
+        if (i < m && j < m) out[i * m + j] = acc[a][b][r];
       }
 }
 
-// G = diag(d) (sum_s part_s) diag(d), upper tiles mirrored; fixed summation order
-__global__ void k_gram_reduce(int64_t m, int nsplit, const double* __restrict__ part, const double* __restrict__ d,
+// G = diag(d) (sum_s part_s) diag(d) in fp64; the upper triangle is read for
+// both halves (exact symmetry), the S partials are summed in a fixed order
+__global__ void k_gram_reduce(int64_t m, int nsplit, const float* __restrict__ part, const double* __restrict__ d,
                               double* __restrict__ G) {
   int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= m * m) return;
   int64_t i = idx / m, j = idx - i * m;
-  int64_t src = (i <= j) ? idx : j * m + i;  // upper triangle (upper tiles were written): exact symmetry
+  int64_t src = (i <= j) ? idx : j * m + i;
   double v = 0.0;
-  for (int s = 0; s < nsplit; ++s) v += part[(int64_t)s * m * m + src];
+  for (int s = 0; s < nsplit; ++s) v += (double)part[(int64_t)s * m * m + src];
   if (d) v *= d[i] * d[j];
   G[idx] = v;
 }
 
+static int64_t pad64(int64_t K) { return (K + 63) / 64 * 64; }
+
 extern "C" int rb_gram_workspace(int64_t K, int64_t m, int mode, size_t* bytes) {
   if (!bytes) return RB_ERR_ARG;
-  int nsplit = (mode == 0) ? 0 : 4;
-  *bytes = (size_t)nsplit * m * m * sizeof(double);
+  if (mode == 0) {
+    *bytes = 0;
+    return RB_OK;
+  }
+  size_t split_bytes = (size_t)2 * m * pad64(K) * sizeof(__bf16);
+  size_t part_bytes = (size_t)GSPLIT * m * m * sizeof(float);
+  *bytes = ((split_bytes + 255) & ~(size_t)255) + part_bytes;
   return RB_OK;
 }
 
 extern "C" int rb_gram_mfma(int64_t K, int64_t m, const void* JT, int64_t ldj, int in_dtype, const double* d,
                             int mode, double* G, void* work, size_t work_bytes, void* stream) {
   hipStream_t s = (hipStream_t)stream;
-  const int nsplit = 4;
-  if ((size_t)nsplit * m * m * sizeof(double) > work_bytes || !work) {
-    rb_set_error("rb_gram_mfma: workspace too small");
+  size_t need = 0;
+  rb_gram_workspace(K, m, mode, &need);
+  if (need > work_bytes || !work) {
+    rb_set_error("rb_gram_mfma: workspace %zu < %zu bytes", work_bytes, need);
     return RB_ERR_SIZE;
   }
-  int64_t nt = (m + MT - 1) / MT;
-  unsigned blocks = (unsigned)(nt * (nt + 1) / 2 * nsplit);
-  double* part = (double*)work;
-#define LG(TI, NT) \
-  hipLaunchKernelGGL((k_gram_mfma<TI, NT>), dim3(blocks), dim3(256), 0, s, K, m, (const TI*)JT, ldj, nsplit, part)
-  if (in_dtype == RB_F32) {
-    if (mode == 1)
-      LG(float, 1);
-    else
-      LG(float, 3);
-  } else if (in_dtype == RB_F64) {
-    if (mode == 1)
-      LG(double, 1);
-    else
-      LG(double, 3);
-  } else {
+  const int64_t Kp = pad64(K);
+  __bf16* Hi = (__bf16*)work;
+  __bf16* Lo = Hi + m * Kp;
+  size_t split_bytes = (size_t)2 * m * Kp * sizeof(__bf16);
+  float* part = (float*)((char*)work + ((split_bytes + 255) & ~(size_t)255));
+  dim3 gs((unsigned)(Kp / 64), (unsigned)((m + 63) / 64));
+  if (in_dtype == RB_F32)
+    hipLaunchKernelGGL(k_split_t<float>, gs, dim3(256), 0, s, K, m, Kp, (const float*)JT, ldj, Hi, Lo);
+  else if (in_dtype == RB_F64)
+    hipLaunchKernelGGL(k_split_t<double>, gs, dim3(256), 0, s, K, m, Kp, (const double*)JT, ldj, Hi, Lo);
+  else {
     rb_set_error("rb_gram_mfma: in_dtype must be RB_F64 or RB_F32");
     return RB_ERR_ARG;
   }
-#undef LG
   RB_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)ceil_div(m * m, 256)), dim3(256), 0, s, m, nsplit, part, d, G);
+  int64_t nt = (m + MT - 1) / MT;
+  unsigned blocks = (unsigned)(nt * (nt + 1) / 2 * GSPLIT);
+  if (mode == 1)
+    hipLaunchKernelGGL(k_gram_bf16<1>, dim3(blocks), dim3(256), 0, s, m, Kp, Hi, Lo, part);
+  else
+    hipLaunchKernelGGL(k_gram_bf16<3>, dim3(blocks), dim3(256), 0, s, m, Kp, Hi, Lo, part);
+  RB_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)ceil_div(m * m, 256)), dim3(256), 0, s, m, GSPLIT, part, d, G);
   RB_CHECK_LAUNCH();
   return RB_OK;
 }
