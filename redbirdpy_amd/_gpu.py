@@ -29,25 +29,63 @@ def _f64(a, dev):
     return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=dev)
 
 
+def _spread_bits(v):
+    """Insert two zero bits between the low 21 bits of v (3-D Morton interleave)."""
+    v = v & np.uint64(0x1FFFFF)
+    v = (v | (v << np.uint64(32))) & np.uint64(0x1F00000000FFFF)
+    v = (v | (v << np.uint64(16))) & np.uint64(0x1F0000FF0000FF)
+    v = (v | (v << np.uint64(8))) & np.uint64(0x100F00F00F00F00F)
+    v = (v | (v << np.uint64(4))) & np.uint64(0x10C30C30C30C30C3)
+    v = (v | (v << np.uint64(2))) & np.uint64(0x1249249249249249)
+    return v
+
+
+def morton_order(node):
+    """Node order along a 3-D Morton curve of the coordinates (new -> old).
+
+    Consecutive nodes of the new order form compact bricks, so a block of
+    consecutive CSR rows has a small union of neighbour rows (the SpMM
+    stages that union in LDS).  The order is a stable sort: deterministic.
+    """
+    node = np.asarray(node, dtype=np.float64)
+    lo = node.min(axis=0)
+    span = float(np.max(node.max(axis=0) - lo))
+    if span <= 0:
+        return np.arange(node.shape[0])
+    q = np.floor((node - lo) / span * (2 ** 21 - 1)).astype(np.uint64)
+    code = _spread_bits(q[:, 0]) | (_spread_bits(q[:, 1]) << np.uint64(1)) | (_spread_bits(q[:, 2]) << np.uint64(2))
+    return np.argsort(code, kind="stable")
+
+
 class GpuMesh:
     """Device copies of a tet mesh plus its CSR pattern and assembly slot map.
 
-    elem/face are taken 1-based (redbirdpy convention, __init__.py:13-15) and
-    stored 0-based int32.
+    elem/face are taken 1-based (redbirdpy convention, __init__.py:13-15).
+    On the device the nodes are renumbered along a Morton curve (``perm``:
+    new -> old, ``iperm``: old -> new) for SpMM locality; every device array
+    indexed by node (fields, RHS, Jacobian rows, node properties) uses the
+    new numbering, and ``user_rows``/``mesh_rows`` convert at the API.
     """
 
-    def __init__(self, node, elem, face=None, evol=None, area=None, dev=None):
+    def __init__(self, node, elem, face=None, evol=None, area=None, dev=None, reorder=True):
         dev = dev or device()
         self.dev = dev
         node = np.asarray(node, dtype=np.float64)
-        e0 = np.asarray(elem)[:, :4].astype(np.int64) - 1
+        e0u = np.asarray(elem)[:, :4].astype(np.int64) - 1
         self.nn = int(node.shape[0])
-        self.ne = int(e0.shape[0])
-        self.node = _f64(node, dev)
+        self.ne = int(e0u.shape[0])
+        self.perm = morton_order(node) if reorder else np.arange(self.nn)
+        self.iperm = np.empty_like(self.perm)
+        self.iperm[self.perm] = np.arange(self.nn)
+        self.reordered = bool(reorder)
+        self.perm_t = torch.as_tensor(self.perm, device=dev)
+        self.iperm_t = torch.as_tensor(self.iperm, device=dev)
+        e0 = self.iperm[e0u]
+        self.node = _f64(node[self.perm], dev)
         self.el = _i32(e0, dev)
         self.evol = _f64(evol, dev) if evol is not None else None
         if face is not None and np.size(face):
-            f0 = np.asarray(face)[:, :3].astype(np.int64) - 1
+            f0 = self.iperm[np.asarray(face)[:, :3].astype(np.int64) - 1]
             self.nf = int(f0.shape[0])
             self.fc = _i32(f0, dev)
             self.area = _f64(area, dev) if area is not None else None
@@ -98,8 +136,38 @@ class GpuMesh:
             self._area_src = area
 
     def pattern_host(self):
-        """(indptr int64, indices int64) of the CSR pattern on the host."""
-        return (self.row_ptr.cpu().numpy().astype(np.int64), self.col_idx[: self.nnz].cpu().numpy().astype(np.int64))
+        """(indptr, indices, slot order) of the CSR pattern in the user's node numbering.
+
+        data_user = data_device[order] maps device slot values to that pattern.
+        """
+        if getattr(self, "_user_pattern", None) is None:
+            rp = self.row_ptr.cpu().numpy().astype(np.int64)
+            ci = self.col_idx[: self.nnz].cpu().numpy().astype(np.int64)
+            rows = self.perm[np.repeat(np.arange(self.nn), np.diff(rp))]
+            cols = self.perm[ci]
+            order = np.lexsort((cols, rows))
+            indptr = np.zeros(self.nn + 1, dtype=np.int64)
+            np.cumsum(np.bincount(rows, minlength=self.nn), out=indptr[1:])
+            self._user_pattern = (indptr, cols[order], order)
+            self._user_order_t = torch.as_tensor(order, device=self.dev)
+        return self._user_pattern
+
+    def user_values(self, vals):
+        """Device slot values -> the user-numbered CSR data order."""
+        self.pattern_host()
+        return vals[..., self._user_order_t]
+
+    def user_rows(self, X):
+        """Rows of a node-indexed device array in the user's numbering."""
+        return X if not self.reordered else X[self.iperm_t]
+
+    def mesh_rows(self, X):
+        """User-numbered node rows -> the device numbering."""
+        return X if not self.reordered else X[self.perm_t]
+
+    def mesh_nodes(self, a):
+        """Host array indexed by user node -> device numbering."""
+        return a if not self.reordered else np.asarray(a)[self.perm]
 
 
 _MESH_CACHE = []  # [(key objects, GpuMesh)], small LRU
@@ -129,7 +197,7 @@ def mesh_from_elem(elem, nn, evol):
                 m._evol_src = evol
             return m
     node = np.zeros((nn, 3))
-    m = GpuMesh(node, elem, None, evol, None)
+    m = GpuMesh(node, elem, None, evol, None, reorder=False)
     m._evol_src = evol
     _MESH_CACHE.insert(0, (key, m))
     del _MESH_CACHE[4:]
@@ -234,10 +302,15 @@ def assemble(mesh, cfg, deldotdel, systems, want32=False):
             cols["musp"].append(rows[:, 1])
             cols["nref"].append(tab[:1, 2])  # n of the first element's label, as femlhs uses
         else:
-            cols["mua"].append(_f64(arr[1], dev))
-            cols["musp"].append(_f64(arr[2], dev))
-            cols["nref"].append(_f64(arr[3], dev))
-            nlen = max(nlen, len(arr[3]))
+            mua, musp, nref = arr[1], arr[2], arr[3]
+            if k == _abi.RB_PROP_NODE:  # node properties follow the device numbering
+                mua, musp = mesh.mesh_nodes(mua), mesh.mesh_nodes(musp)
+                if len(nref) == mesh.nn:
+                    nref = mesh.mesh_nodes(nref)
+            cols["mua"].append(_f64(mua, dev))
+            cols["musp"].append(_f64(musp, dev))
+            cols["nref"].append(_f64(nref, dev))
+            nlen = max(nlen, len(nref))
         oms.append(om)
         bcs.append((1 - reff) / (12.0 * (1 + reff)))
     nsys = len(systems)

@@ -29,8 +29,8 @@ def femlhs(cfg, deldotdel_mat, wavelength="", mode=1):
     """FEM system matrix as scipy CSR, assembled on the GPU (forward.py:90-256)."""
     mesh = _gpu.mesh_for(cfg)
     sysm = _gpu.assemble(mesh, cfg, deldotdel_mat, [(wavelength, mode)])
-    indptr, indices = mesh.pattern_host()
-    data = sysm.vals64[0].cpu().numpy()
+    indptr, indices, _ = mesh.pattern_host()
+    data = mesh.user_values(sysm.vals64[0]).cpu().numpy()
     return sparse.csr_matrix((data, indices, indptr), shape=(mesh.nn, mesh.nn))
 
 
@@ -95,16 +95,17 @@ class _Optodes:
         if self.ns:
             _gpu.rhs_block(mesh, B, c0, _loc0(self.sloc), self.sbary, is_complex)
         if self.ws:
-            B[:, c0 + self.ns : c0 + self.ns + self.ws] = torch.as_tensor(self.wsrc, device=B.device).to(B.dtype)
+            B[:, c0 + self.ns : c0 + self.ns + self.ws] = torch.as_tensor(mesh.mesh_nodes(self.wsrc),
+                                                                          device=B.device).to(B.dtype)
         if self.nd:
             _gpu.rhs_block(mesh, B, c0 + self.ns + self.ws, _loc0(self.dloc), self.dbary, is_complex)
         if self.wd:
             c = c0 + self.ns + self.ws + self.nd
-            B[:, c : c + self.wd] = torch.as_tensor(self.wdet, device=B.device).to(B.dtype)
+            B[:, c : c + self.wd] = torch.as_tensor(mesh.mesh_nodes(self.wdet), device=B.device).to(B.dtype)
 
     def det_columns(self, cfg, mesh):
-        """Detector columns as device CSC (ptr, row, val) for rb_getdet."""
-        e0 = cfg["elem"][:, :4].astype(np.int64) - 1
+        """Detector columns as device CSC (ptr, row, val) for rb_getdet, rows in device numbering."""
+        e0 = mesh.iperm[cfg["elem"][:, :4].astype(np.int64) - 1]
         ptrs, rows, vals = [0], [], []
         for i in range(self.nd):
             if not np.isnan(self.dloc[i]):
@@ -115,7 +116,7 @@ class _Optodes:
                 ptrs.append(ptrs[-1])
         for j in range(self.wd):
             r = np.flatnonzero(self.wdet[:, j])
-            rows.append(r)
+            rows.append(mesh.iperm[r])
             vals.append(self.wdet[r, j])
             ptrs.append(ptrs[-1] + len(r))
         dev = mesh.dev
@@ -178,9 +179,10 @@ def femgetdet(phi, cfg, rhs, loc=None, bary=None):
 
 
 class ForwardResult:
-    """Device-resident output of forward_device."""
+    """Device-resident output of forward_device (field rows in the mesh's device numbering)."""
 
-    def __init__(self, systems, k, X, optodes, detvals, info, is_complex):
+    def __init__(self, systems, k, X, optodes, detvals, info, is_complex, mesh=None):
+        self.mesh = mesh
         self.systems = systems  # [(wv, mode)]
         self.k = k  # columns per system
         self.X = X  # (nn, nsys*k) device
@@ -241,7 +243,7 @@ def forward_device(cfg, rfcw=(1,), precision="fp64", tol=1e-12, maxiter=None, X0
             op._detcols = op.det_columns(cfg, mesh)
         p, r, v = op._detcols
         detvals.append(_gpu.getdet(p, r, v, X[:, i * k : (i + 1) * k], 0, ts, sysm.is_complex))
-    return ForwardResult(systems, k, X, optodes, detvals, info, sysm.is_complex)
+    return ForwardResult(systems, k, X, optodes, detvals, info, sysm.is_complex, mesh)
 
 
 def runforward(cfg, **kwargs):
@@ -261,7 +263,7 @@ def runforward(cfg, **kwargs):
     wvs = list(cfg["prop"].keys()) if isinstance(cfg.get("prop"), dict) else [""]
     det = {md: {"detphi": {}} for md in rfcw}
     phi = {md: {"phi": {}} for md in rfcw}
-    Xh = res.X.cpu().numpy()
+    Xh = res.mesh.user_rows(res.X).cpu().numpy()
     for i, (wv, md) in enumerate(res.systems):
         phi[md]["phi"][wv] = np.ascontiguousarray(Xh[:, i * res.k : (i + 1) * res.k])
         dv = res.detvals[i]
@@ -297,8 +299,8 @@ def jac(sd, phi, deldotdel_mat, elem, evol, iselem=False, return_elem=True):
         phi_t = phi_t.to(torch.float64)
     mesh = _gpu.mesh_from_elem(elem, phi_t.shape[0], evol)
     out = torch.complex128 if phi_t.is_complex() else torch.float64
-    JT, JE = _gpu.jac_node(mesh, phi_t.contiguous(), s, d, out, want_elem=return_elem)
-    Jn = JT.cpu().numpy().T
+    JT, JE = _gpu.jac_node(mesh, mesh.mesh_rows(phi_t).contiguous(), s, d, out, want_elem=return_elem)
+    Jn = mesh.user_rows(JT).cpu().numpy().T
     Je = JE.cpu().numpy().T if JE is not None else None
     return Jn, Je
 

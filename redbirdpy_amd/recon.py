@@ -346,7 +346,8 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
         step = GNStep(JT, npairs, E, lam, precision, reftol=kwargs.get("reftol", 1e-10))
         step.mark = (lambda name, t: mark(name, t)) if timing is not None else None
         tp = mark("misfit", tp)
-        dmu = step.solve(torch.as_tensor(misfit, dtype=torch.float64, device=mesh.dev)).cpu().numpy()
+        dmu = mesh.user_rows(step.solve(torch.as_tensor(misfit, dtype=torch.float64, device=mesh.dev)))
+        dmu = dmu.cpu().numpy()
         tp = mark("gn_solve", tp)
         if timing is not None:
             timing.setdefault("refinements", []).append(step.info.get("refinements", 0))
@@ -388,8 +389,8 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
     if return_jac:
         Jm_out = {}
         for i, wv in enumerate(wvs):
-            Jm_out["mua" if wv == "" else wv] = JT[:, i * npairs : (i + 1) * npairs].t().cpu().numpy()
-    Xh = X0.cpu().numpy()
+            Jm_out["mua" if wv == "" else wv] = mesh.user_rows(JT[:, i * npairs : (i + 1) * npairs]).t().cpu().numpy()
+    Xh = mesh.user_rows(X0).cpu().numpy()
     k = res.k
     phi_out = {wv: Xh[:, i * k : (i + 1) * k] for i, wv in enumerate(wvs)}
     if wvs == [""]:

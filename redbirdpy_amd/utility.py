@@ -255,7 +255,7 @@ def _connectivity(cfg):
     from . import _gpu
 
     mesh = _gpu.mesh_for(cfg)
-    indptr, indices = mesh.pattern_host()
+    indptr, indices, _ = mesh.pattern_host()
     rows = np.repeat(np.arange(mesh.nn), np.diff(indptr))
     off = indices != rows
     return rows[off], indices[off], np.bincount(rows[off], minlength=mesh.nn)

@@ -45,5 +45,6 @@ def test_jac_contract_matches_dense():
     mesh = _gpu.mesh_for(cfg)
     S = np.arange(4)
     D = np.arange(4, 13)
-    g = _gpu.jac_contract(mesh, torch.as_tensor(phi, device="cuda"), S, D, torch.as_tensor(w, device="cuda"))
+    phid = mesh.mesh_rows(torch.as_tensor(phi, device="cuda"))
+    g = mesh.user_rows(_gpu.jac_contract(mesh, phid, S, D, torch.as_tensor(w, device="cuda")))
     np.testing.assert_allclose(g.cpu().numpy(), Jn.T @ w, rtol=1e-11, atol=1e-14 * np.abs(Jn).max())
