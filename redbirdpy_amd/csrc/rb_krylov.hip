@@ -23,6 +23,8 @@
 #include "rb_common.h"
 #include "../../include/rbhip.h"
 #include <vector>
+#include <mutex>
+#include <stdlib.h>
 #include <algorithm>
 #include <math.h>
 
@@ -280,6 +282,8 @@ template <typename T, int G, int NCH, bool MULTI>
 __global__ void __launch_bounds__(KB) k_cg_spmm(Csr m, const T* __restrict__ vals, const T* __restrict__ P,
                                                 T* __restrict__ Q, int C, int k, ColState<typename sc<T>::acc> st) {
   typedef typename sc<T>::acc A;
+  // the previous iteration's replacement request has been served
+  if (st.ru_req && blockIdx.x == 0 && threadIdx.x == 0) *st.ru_req = 0;
   if (!block_any_active(st.active, C)) return;
   A pv[NCH][MAXND];
 #pragma unroll
@@ -323,6 +327,187 @@ __global__ void __launch_bounds__(KB) k_cg_spmm(Csr m, const T* __restrict__ val
   block_partials<A, G, NCH>(st, pv, 1, C);
 }
 
+// q = A_s p ; partial p.q -- the die-wide sweep.  Workgroups run round-robin
+// on the NXCD dies (workgroup b on die b % NXCD) and die x owns the
+// contiguous row segment x.  The segment is cut into chunks of SRC rows and
+// the W wavefronts of a die take chunks w, w+W, w+2W, ...: at any moment a
+// die works on a narrow band of consecutive chunks, so a P row fetched for
+// one chunk is still in the die's L2 when the neighbouring chunks gather it
+// (with a locality-preserving node order each P row leaves HBM about once
+// per die; contiguous per-wavefront runs keep the whole segment live at once
+// and re-fetch it several times).  Lane c owns column c of a 64-column
+// chunk.  A chunk's CSR entries are taken in pieces of up to 64: the next
+// piece's (column, value) pairs are loaded while the current one is used and
+// parked in the wavefront's LDS double buffer (padding slots repeat the
+// chunk's first row with value 0); each entry becomes one coalesced
+// 64-column gather of P addressed by a scalar row offset through a buffer
+// descriptor.  Gathers are double-buffered across batch and piece
+// boundaries.  Row ends are scalar compares against the chunk's row
+// pointers (rows of any length, empty ones included); the row's own p (for
+// p.q) is taken from the gather of its diagonal entry.  A wavefront only
+// reads LDS it wrote itself and LDS operations of one wavefront complete in
+// order, so no barrier is needed.  Requires every 64-column chunk inside one
+// system (nsys == 1 or k % 64 == 0) and the P block < 2 GiB.
+#define FRG 8   // gathers per batch
+#define SRC 4   // rows per chunk
+#define SGC 12  // chunks whose row pointers one lane-parallel load covers
+static_assert(SGC * (SRC + 1) <= 64, "row pointers of a chunk group must fit one wavefront");
+#define BUF_WORD3 0x00020000  // raw (untyped) buffer descriptor for gfx9 parts
+
+// gathered rows are addressed through a buffer descriptor: the row's byte
+// offset is a scalar (soffset) and the lane's column offset one fixed VGPR
+template <typename T>
+__device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+  } else if constexpr (sizeof(T) == 8) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  } else {
+    static_assert(sizeof(T) == 16, "unsupported element size");
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+  }
+}
+
+template <typename T, int NCH>
+__global__ void __launch_bounds__(KB) k_cg_spmm_sweep(Csr m, const T* __restrict__ vals, const T* __restrict__ P,
+                                                      T* __restrict__ Q, int C, int k,
+                                                      ColState<typename sc<T>::acc> st) {
+  typedef typename sc<T>::acc A;
+  __shared__ int32_t s_col[KB / 64][2][64];
+  __shared__ T s_val[KB / 64][2][64];
+  // the previous iteration's replacement request has been served
+  if (st.ru_req && blockIdx.x == 0 && threadIdx.x == 0) *st.ru_req = 0;
+  if (!block_any_active(st.active, C)) return;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: bookkeeping stays scalar
+  const int64_t seg = (m.n + NXCD - 1) / NXCD;
+  const int64_t s0 = (int64_t)(blockIdx.x % NXCD) * seg;
+  const int64_t s1 = min(m.n, s0 + seg);
+  const int64_t W = (int64_t)(gridDim.x / NXCD) * (KB / 64);           // wavefronts per die
+  const int64_t wx = (int64_t)(blockIdx.x / NXCD) * (KB / 64) + wave;  // rank on the die
+  const int64_t nq = (s1 > s0) ? (s1 - s0 + SRC - 1) / SRC : 0;        // chunks of the segment
+  const int64_t nk = (nq > wx) ? (nq - wx + W - 1) / W : 0;            // chunks of this wavefront
+  const uint32_t rowb = (uint32_t)C * (uint32_t)sizeof(T);
+  const __amdgpu_buffer_rsrc_t rsrc =
+      __builtin_amdgcn_make_buffer_rsrc((void*)P, (short)0, (int)(m.n * rowb), BUF_WORD3);
+  A pv[NCH][MAXND];
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) pv[ch][0] = acc_zero(A());
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) {
+    if (ch * 64 >= C) break;
+    const int c = lane + ch * 64;
+    const bool live = c < C;
+    const uint32_t voff = (live ? c : 0) * (uint32_t)sizeof(T);
+    const T* v = vals + (int64_t)((ch * 64) / k) * m.nnz;
+    for (int64_t g0 = 0; g0 < nk; g0 += SGC) {
+      const int ng = (int)min((int64_t)SGC, nk - g0);
+      // lane l holds row pointer l % (SRC+1) of chunk l / (SRC+1) of the group
+      int32_t rp = 0;
+      {
+        const int kq = lane / (SRC + 1), o = lane % (SRC + 1);
+        if (kq < ng) rp = m.row_ptr[min(s0 + (wx + (g0 + kq) * W) * SRC + o, s1)];
+      }
+      // current chunk (scalar state)
+      int kc = 0;
+      int64_t cr0 = 0;
+      int cnr = 0, row = 0;
+      int32_t ce0 = 0, ce1 = 0, rend = 0;
+      T acc = sc<T>::zero(), pown = sc<T>::zero();
+      auto chunk_at = [&](int kq, int64_t& r0, int& nr, int32_t& e0, int32_t& e1) {
+        r0 = s0 + (wx + (g0 + kq) * W) * SRC;
+        nr = (int)min((int64_t)SRC, s1 - r0);
+        e0 = __builtin_amdgcn_readlane(rp, kq * (SRC + 1));
+        e1 = __builtin_amdgcn_readlane(rp, kq * (SRC + 1) + nr);
+      };
+      // rows of the current chunk that end at entry e (empty rows included)
+      auto drain = [&](int32_t e) {
+        while (row < cnr && rend == e) {
+          if (live) {
+            Q[(cr0 + row) * C + c] = acc;
+            pv[ch][0] = acc_add(pv[ch][0], amul(pown, acc));
+          }
+          acc = sc<T>::zero();
+          pown = sc<T>::zero();
+          ++row;
+          rend = (row < cnr) ? __builtin_amdgcn_readlane(rp, kc * (SRC + 1) + row + 1) : -1;
+        }
+      };
+      auto enter_chunk = [&](int kq) {
+        kc = kq;
+        chunk_at(kq, cr0, cnr, ce0, ce1);
+        row = 0;
+        rend = __builtin_amdgcn_readlane(rp, kq * (SRC + 1) + 1);
+        drain(ce0);
+      };
+      auto gather = [&](T(&x)[FRG], int sl, int jb) {
+#pragma unroll
+        for (int u = 0; u < FRG; ++u) {
+          const uint32_t cu = __builtin_amdgcn_readfirstlane(s_col[wave][sl][jb * FRG + u]);  // uniform
+          x[u] = buf_ld<T>(rsrc, voff, cu * rowb);
+        }
+      };
+      auto consume = [&](const T(&x)[FRG], int sl, int jb, int32_t pb) {
+#pragma unroll
+        for (int u = 0; u < FRG; ++u) {
+          acc = acc + s_val[wave][sl][jb * FRG + u] * x[u];
+          const int32_t cu = __builtin_amdgcn_readfirstlane(s_col[wave][sl][jb * FRG + u]);
+          if (cu == (int32_t)(cr0 + row)) pown = x[u];
+          drain(pb + jb * FRG + u + 1);
+        }
+      };
+      enter_chunk(0);
+      int32_t pb = ce0;
+      int sl = 0;
+      {
+        const int32_t e = pb + lane;
+        s_col[wave][0][lane] = (e < ce1) ? m.col_idx[e] : (int32_t)cr0;
+        s_val[wave][0][lane] = (e < ce1) ? v[e] : sc<T>::zero();
+      }
+      T xa[FRG], xb[FRG];
+      gather(xa, 0, 0);
+      for (;;) {
+        // the piece after this one: the rest of this chunk, or the next chunk
+        int kn = kc;
+        int64_t nr0 = cr0;
+        int nnr = cnr;
+        int32_t npb = pb + 64, ne0 = ce0, ne1 = ce1;
+        const bool more = npb < ce1;
+        const bool has_next = more || kc + 1 < ng;
+        if (!more && has_next) {
+          kn = kc + 1;
+          chunk_at(kn, nr0, nnr, ne0, ne1);
+          npb = ne0;
+        }
+        // its (column, value) pairs, parked in registers until the last pair
+        const int32_t en = min(npb + lane, max(ne1 - 1, 0));
+        const bool vn = has_next && npb + lane < ne1;
+        const int32_t tcol = m.col_idx[en];
+        const T tval = v[en];
+        // the piece's 64 slots as 4 pairs of batches, straight-line
+#pragma unroll
+        for (int pr = 0; pr < 64 / (2 * FRG); ++pr) {
+          gather(xb, sl, 2 * pr + 1);
+          consume(xa, sl, 2 * pr, pb);
+          if (pr + 1 < 64 / (2 * FRG)) {
+            gather(xa, sl, 2 * pr + 2);
+          } else {
+            s_col[wave][sl ^ 1][lane] = vn ? tcol : (int32_t)nr0;
+            s_val[wave][sl ^ 1][lane] = vn ? tval : sc<T>::zero();
+            gather(xa, sl ^ 1, 0);  // first batch of the next piece (padding when there is none)
+          }
+          consume(xb, sl, 2 * pr + 1, pb);
+        }
+        if (!has_next) break;
+        sl ^= 1;
+        pb = npb;
+        if (kn != kc) enter_chunk(kn);
+      }
+    }
+  }
+  block_partials<A, 64, NCH>(st, pv, 1, C);
+}
+
 // q = A_s p ; partial p.q -- one wavefront per contiguous run of rows.
 // The run's row pointers come in with one lane-parallel load; rows are taken
 // two at a time, each with its (col, val) entries loaded by the lanes and
@@ -337,7 +522,7 @@ __global__ void __launch_bounds__(KB) k_cg_spmm_w64(Csr m, const T* __restrict__
   typedef typename sc<T>::acc A;
   if (!block_any_active(st.active, C)) return;
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: row bookkeeping stays scalar
   const int64_t seg = (m.n + NXCD - 1) / NXCD;
   const int64_t s0 = (int64_t)(blockIdx.x % NXCD) * seg;
   const int64_t s1 = min(m.n, s0 + seg);
@@ -458,15 +643,26 @@ __global__ void __launch_bounds__(KB) k_cg_update(int64_t n, const T* __restrict
 }
 
 // p = dinv r + beta p on active columns
-template <typename T, int G, int NCH>
+template <typename T, typename TW, int G, int NCH>
 __global__ void __launch_bounds__(KB) k_cg_pupdate(int64_t n, const T* __restrict__ dinv, const T* __restrict__ R,
-                                                   T* __restrict__ P, int C, int k,
+                                                   T* __restrict__ P, int C, int k, TW* __restrict__ X64,
+                                                   T* __restrict__ D, const int* __restrict__ ru,
                                                    ColState<typename sc<T>::acc> st) {
   typedef typename sc<T>::acc A;
-  // the replacement request of this iteration has been served (no other
-  // block of this kernel reads it)
-  if (st.ru_req && blockIdx.x == 0 && threadIdx.x == 0) *st.ru_req = 0;
-  if (!block_any_active(st.active, C)) return;
+  // mixed solves: on a replacement iteration (*ru) the true residual was
+  // taken at x64 + d, which now becomes x64 (d restarts from 0).  The flag
+  // and the active columns are read together (one barrier).
+  __shared__ int s_any, s_fold;
+  if (threadIdx.x == 0) {
+    s_any = 0;
+    s_fold = ru ? *ru : 0;
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < C; c += blockDim.x)
+    if (st.active[c]) s_any = 1;
+  __syncthreads();
+  const bool fold = s_fold != 0;
+  if (!s_any && !fold) return;
   A be[NCH];
   int act[NCH];
 #pragma unroll
@@ -479,9 +675,14 @@ __global__ void __launch_bounds__(KB) k_cg_pupdate(int64_t n, const T* __restric
 #pragma unroll
     for (int ch = 0; ch < NCH; ++ch) {
       int c = lane + ch * G;
-      if (c < C && act[ch]) {
+      if (c >= C) continue;
+      const int64_t o = row * C + c;
+      if (fold) {
+        X64[o] = X64[o] + widen(D[o], TW());
+        D[o] = sc<T>::zero();
+      }
+      if (act[ch]) {
         int s = c / k;
-        int64_t o = row * C + c;
         T z = dinv[(int64_t)s * n + row] * R[o];
         P[o] = z + scale_to(P[o], be[ch]);
       }
@@ -489,11 +690,58 @@ __global__ void __launch_bounds__(KB) k_cg_pupdate(int64_t n, const T* __restric
   }
 }
 
-// true residual of the mixed solve: r32 = (T32)(b - A64 x64) into the inner
-// residual buffer; partials rr64, bb64 and rz32 = r32 . (dinv32 r32)
+// y = sum_j A_sj (x[col_j, c] + d[col_j, c]) for one row, one wavefront per
+// row (lane c of a 64-column chunk): the row's (col, val) entries come in
+// with one lane-parallel load and are broadcast by readlane, so RG gathers
+// of x and of the widened 32-bit correction d (d may be null) are in flight
+template <typename T64, typename T32>
+__device__ inline T64 row_gather_w64_xd(const int32_t* __restrict__ col_idx, const T64* __restrict__ vals,
+                                        int32_t rs, int32_t re, int64_t row, const T64* __restrict__ X,
+                                        const T32* __restrict__ D, int64_t ld, int lane, int c, bool live) {
+  T64 y = sc<T64>::zero();
+  for (int32_t base = rs; base < re; base += 64) {
+    const int len = min(64, re - base);
+    const int32_t cl = (lane < len) ? col_idx[base + lane] : (int32_t)row;
+    const T64 vl = (lane < len) ? vals[base + lane] : sc<T64>::zero();
+    for (int j = 0; j < len; j += RG) {
+      T64 x[RG];
+#pragma unroll
+      for (int u = 0; u < RG; ++u) {
+        const int64_t o = (int64_t)__builtin_amdgcn_readlane(cl, min(j + u, 63)) * ld + (live ? c : 0);
+        x[u] = D ? X[o] + widen(D[o], T64()) : X[o];
+      }
+#pragma unroll
+      for (int u = 0; u < RG; ++u) {
+        const T64 a = (j + u < len) ? rl_bcast(vl, min(j + u, 63)) : sc<T64>::zero();
+        y = y + a * x[u];
+      }
+    }
+  }
+  return y;
+}
+
+// y = sum_j A_sj (x[col_j] + d[col_j]) for one row and the lane's column
+// (any column group width; d may be null)
+template <typename T64, typename T32>
+__device__ inline T64 row_gather_xd(const int32_t* __restrict__ col_idx, const T64* __restrict__ vals, int32_t rs,
+                                    int32_t re, const T64* __restrict__ X, const T32* __restrict__ D, int64_t ld,
+                                    int c, bool live) {
+  T64 y = sc<T64>::zero();
+  if (!live) return y;
+  for (int32_t j = rs; j < re; ++j) {
+    const int64_t o = (int64_t)col_idx[j] * ld + c;
+    y = y + vals[j] * (D ? X[o] + widen(D[o], T64()) : X[o]);
+  }
+  return y;
+}
+
+// true residual of the mixed solve: r32 = (T32)(b - A64 (x64 + d32)) into
+// the inner residual buffer; partials rr64, bb64 and rz32 = r32 . (dinv32 r32).
+// Gated by *gate when given; launched on one resident round of workgroups so
+// a gated-off launch costs one flag read per workgroup.
 template <typename T64, typename T32, int G, int NCH>
 __global__ void __launch_bounds__(KB) k_outer_resid(Csr m, const T64* __restrict__ vals, const T64* __restrict__ B,
-                                                    const T64* __restrict__ X, int use_x,
+                                                    const T64* __restrict__ X, const T32* __restrict__ D, int use_x,
                                                     const T32* __restrict__ dinv32, T32* __restrict__ R32, int C,
                                                     int k, const int* __restrict__ gate,
                                                     ColState<typename sc<T64>::acc> st) {
@@ -510,24 +758,15 @@ __global__ void __launch_bounds__(KB) k_outer_resid(Csr m, const T64* __restrict
     int32_t rs = m.row_ptr[row], re = m.row_ptr[row + 1];
     T64 ax[NCH];
     if (use_x) {
-      if (G == 64 && (k % 64 == 0 || k == C)) {  // each 64-column chunk within one system
 #pragma unroll
-        for (int ch = 0; ch < NCH; ++ch) {
-          T64 q1[1];
-          row_gather_w64<T64, 1>(m.col_idx, vals + (int64_t)((ch * G) / k) * m.nnz, rs, re, row, X + ch * G, C,
-                                 lane, C - ch * G, q1);
-          ax[ch] = q1[0];
-        }
-      } else {
-#pragma unroll
-        for (int ch = 0; ch < NCH; ++ch) {
-          int c = lane + ch * G;
-          ax[ch] = sc<T64>::zero();
-          if (c < C) {
-            const T64* v = vals + (int64_t)(c / k) * m.nnz;
-            for (int32_t j = rs; j < re; ++j) ax[ch] = ax[ch] + v[j] * X[(int64_t)m.col_idx[j] * C + c];
-          }
-        }
+      for (int ch = 0; ch < NCH; ++ch) {
+        const int c = lane + ch * G;
+        const T64* v = vals + (int64_t)((c < C ? c : 0) / k) * m.nnz;
+        if (G == 64 && (k % 64 == 0 || k == C))  // each 64-column chunk within one system
+          ax[ch] = row_gather_w64_xd<T64, T32>(m.col_idx, vals + (int64_t)((ch * G) / k) * m.nnz, rs, re, row, X,
+                                               D, C, lane, c, c < C);
+        else
+          ax[ch] = row_gather_xd<T64, T32>(m.col_idx, v, rs, re, X, D, C, c, c < C);
       }
     }
 #pragma unroll
@@ -956,22 +1195,69 @@ extern "C" int rb_prof_read(int id, double* total_ms, int64_t* count) {
 }
 
 #define L_INIT(G_, N_, ...) hipLaunchKernelGGL((k_cg_init<T, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
-#define L_SPMM1(G_, N_, ...)                                                                            \
-  do {                                                                                                \
-    if (G_ == 64)                                                                                     \
-      hipLaunchKernelGGL((k_cg_spmm_w64<T, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__);              \
-    else                                                                                              \
-      hipLaunchKernelGGL((k_cg_spmm<T, G_, N_, false>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__);       \
+// q = A p launch over the solve's columns: the sweep kernel when every
+// 64-column chunk lies in one system and P fits its buffer descriptor
+// (< 2 GiB), on a grid of exactly one resident round of workgroups; the
+// row-per-group kernel otherwise.  `flat` and `nbs` come from spmm_plan.
+#define L_SPMM(G_, N_, ...)                                                                                 \
+  do {                                                                                                    \
+    if (G_ == 64 && flat)                                                                                 \
+      hipLaunchKernelGGL((k_cg_spmm_sweep<T, N_>), dim3(nbs), dim3(KB), 0, s, __VA_ARGS__);                 \
+    else if (nsys == 1)                                                                                   \
+      hipLaunchKernelGGL((k_cg_spmm<T, G_, N_, false>), dim3(nbs), dim3(KB), 0, s, __VA_ARGS__);           \
+    else                                                                                                  \
+      hipLaunchKernelGGL((k_cg_spmm<T, G_, N_, true>), dim3(nbs), dim3(KB), 0, s, __VA_ARGS__);            \
   } while (0)
-#define L_SPMMN(G_, N_, ...)                                                                            \
-  do {                                                                                                \
-    if (G_ == 64 && k % 64 == 0)                                                                      \
-      hipLaunchKernelGGL((k_cg_spmm_w64<T, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__);              \
-    else                                                                                              \
-      hipLaunchKernelGGL((k_cg_spmm<T, G_, N_, true>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__);        \
-  } while (0)
+
+// workgroups of kernel fn resident at once on the current device (one
+// dispatch round), rounded down to a multiple of NXCD and at most nb;
+// computed once per kernel and device
+static int64_t resident_grid(const void* fn, int64_t nb) {
+  struct Ent {
+    const void* fn;
+    int dev;
+    int64_t g;
+  };
+  static std::mutex mu;
+  static Ent cache[128];
+  static int ncache = 0;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+  std::lock_guard<std::mutex> lk(mu);
+  int64_t g = 0;
+  for (int i = 0; i < ncache; ++i)
+    if (cache[i].fn == fn && cache[i].dev == dev) g = cache[i].g;
+  if (!g) {
+    int per = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, KB, 0) != hipSuccess || per < 1) per = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = NXCD;
+    g = std::max<int64_t>(NXCD, (int64_t)per * cus / NXCD * NXCD);
+    if (ncache < 128) cache[ncache++] = {fn, dev, g};
+  }
+  return std::min(g, nb);
+}
+
+template <typename T>
+static void spmm_plan(const Csr& m, int nsys, int C, int k, int64_t nb, bool* flat, int64_t* nbs) {
+  *flat = pick_G(C) == 64 && (nsys == 1 || k % 64 == 0) && m.nnz > 0 &&
+          m.n * (int64_t)C * (int64_t)sizeof(T) < ((int64_t)1 << 31);
+  *nbs = nb;
+  if (!*flat) return;
+  const int nch = pick_NCH(C);
+  const void* fn = nch == 1   ? (const void*)k_cg_spmm_sweep<T, 1>
+                   : nch == 2 ? (const void*)k_cg_spmm_sweep<T, 2>
+                   : nch == 4 ? (const void*)k_cg_spmm_sweep<T, 4>
+                              : (const void*)k_cg_spmm_sweep<T, 8>;
+  *nbs = resident_grid(fn, nb);
+  // RB_SWEEP_MAX_GRID caps the grid (tests use it to give each wavefront
+  // many chunk groups on a small matrix)
+  if (const char* e = getenv("RB_SWEEP_MAX_GRID")) {
+    const int64_t cap = atoll(e) / NXCD * NXCD;
+    if (cap >= NXCD) *nbs = std::min(*nbs, cap);
+  }
+}
 #define L_UPD(G_, N_, ...) hipLaunchKernelGGL((k_cg_update<T, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
-#define L_PUP(G_, N_, ...) hipLaunchKernelGGL((k_cg_pupdate<T, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
+#define L_PUP(G_, N_, ...) hipLaunchKernelGGL((k_cg_pupdate<T, TW, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
 
 // one CG/COCG solve of all columns; X is the solution (and x0 if use_x0)
 template <typename T>
@@ -981,6 +1267,11 @@ static int run_cg(const Csr& m, int nsys, const T* vals, const T* dinv, const T*
   typedef typename sc<T>::acc A;
   int64_t nb = grid_for(m.n, C);
   st.nb = (int)nb;
+  bool flat;
+  int64_t nbs;
+  spmm_plan<T>(m, nsys, C, k, nb, &flat, &nbs);
+  ColState<A> ss = st;  // the SpMM's partials come from its own grid
+  ss.nb = (int)nbs;
   {
     ProfScope ps(RB_PROF_INIT, s);
     DISPATCH_GN(C, L_INIT, m, vals, dinv, B, X, use_x0, w.R, w.P, C, k, st);
@@ -996,12 +1287,9 @@ static int run_cg(const Csr& m, int nsys, const T* vals, const T* dinv, const T*
     for (int j = 0; j < chunk; ++j) {
       {
         ProfScope ps(RB_PROF_SPMM, s);
-        if (nsys == 1)
-          DISPATCH_GN(C, L_SPMM1, m, vals, w.P, w.Q, C, k, st);
-        else
-          DISPATCH_GN(C, L_SPMMN, m, vals, w.P, w.Q, C, k, st);
+        DISPATCH_GN(C, L_SPMM, m, vals, w.P, w.Q, C, k, ss);
       }
-      hipLaunchKernelGGL(k_fin_alpha<A>, dim3(C), dim3(FB), 0, s, C, st);
+      hipLaunchKernelGGL(k_fin_alpha<A>, dim3(C), dim3(FB), 0, s, C, ss);
       {
         ProfScope ps(RB_PROF_UPDATE, s);
         DISPATCH_GN(C, L_UPD, m.n, dinv, X, w.R, w.P, w.Q, C, k, st);
@@ -1009,7 +1297,8 @@ static int run_cg(const Csr& m, int nsys, const T* vals, const T* dinv, const T*
       hipLaunchKernelGGL(k_fin_beta<A>, dim3(C), dim3(FB), 0, s, C, st);
       {
         ProfScope ps(RB_PROF_PUPDATE, s);
-        DISPATCH_GN(C, L_PUP, m.n, dinv, w.R, w.P, C, k, st);
+        typedef T TW;
+        DISPATCH_GN(C, L_PUP, m.n, dinv, w.R, w.P, C, k, (TW*)nullptr, (T*)nullptr, (const int*)nullptr, st);
       }
     }
     RB_CHECK_LAUNCH();
@@ -1109,17 +1398,31 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
     rc = rb_fill_bytes0(X, (size_t)n * C * sizeof(T64), s);
     if (rc) return rc;
   }
+  // the outer residual (launched every iteration, usually gated off) runs
+  // on at most two resident rounds of workgroups
+  int64_t nbo = nb;
+#define G_ORES(G_, N_, unused) nbo = std::min(nb, 2 * resident_grid((const void*)k_outer_resid<T64, T32, G_, N_>, nb))
+  DISPATCH_GN(C, G_ORES, 0);
+#undef G_ORES
+  so.nb = (int)nbo;
 #define L_ORES(G_, N_, ...) \
-  hipLaunchKernelGGL((k_outer_resid<T64, T32, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
+  hipLaunchKernelGGL((k_outer_resid<T64, T32, G_, N_>), dim3(nbo), dim3(KB), 0, s, __VA_ARGS__)
   // initial true residual (x0 or 0)
   {
     ProfScope ps(RB_PROF_RESID, s);
-    DISPATCH_GN(C, L_ORES, m, A64, B, X, use_x0 ? 1 : 0, w.dinv, w.R, C, k, (const int*)nullptr, so);
+    DISPATCH_GN(C, L_ORES, m, A64, B, X, (const T32*)nullptr, use_x0 ? 1 : 0, w.dinv, w.R, C, k,
+                (const int*)nullptr, so);
     hipLaunchKernelGGL(k_fin_ru<A>, dim3(C), dim3(FB), 0, s, C, 1, tol2, so, si);
   }
   RB_CHECK_LAUNCH();
 #define T T32
-  DISPATCH_GN(C, L_PUP, m.n, w.dinv, w.R, w.P, C, k, si);  // p = z (P is zero)
+  bool flat;
+  int64_t nbs;
+  spmm_plan<T32>(m, nsys, C, k, nb, &flat, &nbs);
+  ColState<A> ss = si;  // the SpMM's partials come from its own grid
+  ss.nb = (int)nbs;
+  typedef T64 TW;
+  DISPATCH_GN(C, L_PUP, m.n, w.dinv, w.R, w.P, C, k, X, D, (const int*)nullptr, si);  // p = z (P is zero)
   RB_CHECK_LAUNCH();
   int nact = 0;
   rc = poll_nactive(si.active, C, si.nactive, s, &nact);
@@ -1130,12 +1433,9 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
     for (int j = 0; j < chunk; ++j) {
       {
         ProfScope ps(RB_PROF_SPMM, s);
-        if (nsys == 1)
-          DISPATCH_GN(C, L_SPMM1, m, A32, w.P, w.Q, C, k, si);
-        else
-          DISPATCH_GN(C, L_SPMMN, m, A32, w.P, w.Q, C, k, si);
+        DISPATCH_GN(C, L_SPMM, m, A32, w.P, w.Q, C, k, ss);
       }
-      hipLaunchKernelGGL(k_fin_alpha<A>, dim3(C), dim3(FB), 0, s, C, si);
+      hipLaunchKernelGGL(k_fin_alpha<A>, dim3(C), dim3(FB), 0, s, C, ss);
       {
         ProfScope ps(RB_PROF_UPDATE, s);
         DISPATCH_GN(C, L_UPD, m.n, w.dinv, D, w.R, w.P, w.Q, C, k, si);
@@ -1144,14 +1444,12 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
       {
         // gated replacement: no-ops unless fin_beta raised ru_req
         ProfScope ps(RB_PROF_RESID, s);
-        hipLaunchKernelGGL((k_accum<T64, T32>), dim3(1024), dim3(256), 0, s, (int64_t)n * C, D, X,
-                           (const int*)ru_req);
-        DISPATCH_GN(C, L_ORES, m, A64, B, X, 1, w.dinv, w.R, C, k, (const int*)ru_req, so);
+        DISPATCH_GN(C, L_ORES, m, A64, B, X, D, 1, w.dinv, w.R, C, k, (const int*)ru_req, so);
         hipLaunchKernelGGL(k_fin_ru<A>, dim3(C), dim3(FB), 0, s, C, 0, tol2, so, si);
       }
       {
         ProfScope ps(RB_PROF_PUPDATE, s);
-        DISPATCH_GN(C, L_PUP, m.n, w.dinv, w.R, w.P, C, k, si);
+        DISPATCH_GN(C, L_PUP, m.n, w.dinv, w.R, w.P, C, k, X, D, (const int*)ru_req, si);
       }
     }
     RB_CHECK_LAUNCH();

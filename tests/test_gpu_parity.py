@@ -191,10 +191,11 @@ def test_femsolve_long_rows_and_widths(ncol):
     n = 3000
     A = sparse.random(n, n, density=0.002, random_state=1, format="csr")
     A = A + A.T
-    dense_rows = [5, 17, 2999]
-    for r in dense_rows:  # rows with ~150 entries
-        cols = rng.choice(n, 150, replace=False)
-        B = sparse.csr_matrix((rng.uniform(-0.1, 0.1, 150), ([r] * 150, cols)), shape=(n, n))
+    # rows with ~150 entries, and one with ~900 (more than one staging window
+    # of the streaming SpMM)
+    for r, nz in ((5, 150), (17, 900), (2999, 150)):
+        cols = rng.choice(n, nz, replace=False)
+        B = sparse.csr_matrix((rng.uniform(-0.1, 0.1, nz), ([r] * nz, cols)), shape=(n, n))
         A = A + B + B.T
     A = A + sparse.diags(np.asarray(abs(A).sum(axis=1)).ravel() + 1.0)  # SPD by diagonal dominance
     b = rng.standard_normal((n, ncol))
@@ -206,3 +207,25 @@ def test_femsolve_long_rows_and_widths(ncol):
     xm, flag = rb.femsolve(A, b, tol=1e-12, precision="mixed")
     assert flag == 0
     assert np.linalg.norm(xm - xr) / np.linalg.norm(xr) < 1e-10
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_femsolve_sweep_many_chunk_groups(precision, monkeypatch):
+    """A capped SpMM grid gives each wavefront hundreds of row chunks (many
+    row-pointer groups per wavefront) on the FD box at 2 mm spacing."""
+    import redbirdpy_amd as rb
+    from scipy.sparse.linalg import spsolve
+    from redbirdpy_amd import workloads
+
+    cfg = workloads.fd_box(h=2.0)
+    cfg, _ = rb.meshprep(cfg)
+    A = rb.femlhs(cfg, cfg["deldotdel"])
+    n = A.shape[0]
+    rng = np.random.default_rng(7)
+    b = rng.standard_normal((n, 64)) + 1j * rng.standard_normal((n, 64))
+    xr = spsolve(A.tocsc(), b)
+    monkeypatch.setenv("RB_SWEEP_MAX_GRID", "8")
+    x, flag = rb.femsolve(A, b, tol=1e-12, precision=precision)
+    assert flag == 0
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-9
