@@ -30,6 +30,7 @@
 
 #define KB 256       // threads per block
 #define MAXND 3      // dot products per kernel
+#define MAXCHUNK 8   // 64-column chunks of one launch (C <= 512)
 #define NBMAX 2048   // blocks per launch (grid-stride over rows)
 #define FB 256       // threads of a finish block
 #define UNR 8        // CSR entries gathered per step
@@ -282,8 +283,8 @@ template <typename T, int G, int NCH, bool MULTI>
 __global__ void __launch_bounds__(KB) k_cg_spmm(Csr m, const T* __restrict__ vals, const T* __restrict__ P,
                                                 T* __restrict__ Q, int C, int k, ColState<typename sc<T>::acc> st) {
   typedef typename sc<T>::acc A;
-  // the previous iteration's replacement request has been served
-  if (st.ru_req && blockIdx.x == 0 && threadIdx.x == 0) *st.ru_req = 0;
+  // the previous iteration's replacement requests have been served
+  if (st.ru_req && blockIdx.x == 0 && threadIdx.x < MAXCHUNK) st.ru_req[threadIdx.x] = 0;
   if (!block_any_active(st.active, C)) return;
   A pv[NCH][MAXND];
 #pragma unroll
@@ -368,16 +369,28 @@ __device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uin
   }
 }
 
-template <typename T, int NCH>
-__global__ void __launch_bounds__(KB) k_cg_spmm_sweep(Csr m, const T* __restrict__ vals, const T* __restrict__ P,
-                                                      T* __restrict__ Q, int C, int k,
-                                                      ColState<typename sc<T>::acc> st) {
+// operands of one sweep: MODE 0 is q = A p (gathered vector p, output q,
+// partial p.q); MODE 1 is the float64 true residual of a mixed solve,
+// r32 = (TD)(b - A (x64 + d32)) with partials |r|^2, |b|^2 and r32.(dinv r32),
+// for the 64-column chunks whose gate flag is set (all when gate is null)
+template <typename T, typename TD>
+struct SweepOps {
+  const T* vals;
+  const T* P;      // gathered: p (MODE 0) or x64 (MODE 1)
+  const TD* D;     // MODE 1: 32-bit correction added to x64 (may be null)
+  T* Q;            // MODE 0: q
+  const T* B;      // MODE 1: right-hand sides
+  const TD* dinv;  // MODE 1: 32-bit inverse diagonal, n per system
+  TD* R32;         // MODE 1: replaced residual
+  const int* gate; // MODE 1: one flag per 64-column chunk
+};
+
+template <typename T, typename TD, int NCH, int MODE>
+__device__ __forceinline__ void sweep_body(const Csr& m, const SweepOps<T, TD>& op, int C, int k,
+                                           const ColState<typename sc<T>::acc>& st) {
   typedef typename sc<T>::acc A;
   __shared__ int32_t s_col[KB / 64][2][64];
   __shared__ T s_val[KB / 64][2][64];
-  // the previous iteration's replacement request has been served
-  if (st.ru_req && blockIdx.x == 0 && threadIdx.x == 0) *st.ru_req = 0;
-  if (!block_any_active(st.active, C)) return;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: bookkeeping stays scalar
   const int64_t seg = (m.n + NXCD - 1) / NXCD;
@@ -388,18 +401,27 @@ __global__ void __launch_bounds__(KB) k_cg_spmm_sweep(Csr m, const T* __restrict
   const int64_t nq = (s1 > s0) ? (s1 - s0 + SRC - 1) / SRC : 0;        // chunks of the segment
   const int64_t nk = (nq > wx) ? (nq - wx + W - 1) / W : 0;            // chunks of this wavefront
   const uint32_t rowb = (uint32_t)C * (uint32_t)sizeof(T);
+  const uint32_t rowd = (uint32_t)C * (uint32_t)sizeof(TD);
   const __amdgpu_buffer_rsrc_t rsrc =
-      __builtin_amdgcn_make_buffer_rsrc((void*)P, (short)0, (int)(m.n * rowb), BUF_WORD3);
+      __builtin_amdgcn_make_buffer_rsrc((void*)op.P, (short)0, (int)(m.n * rowb), BUF_WORD3);
+  const bool has_d = MODE == 1 && op.D != nullptr;
+  const __amdgpu_buffer_rsrc_t rsrd =
+      __builtin_amdgcn_make_buffer_rsrc((void*)(has_d ? op.D : (const TD*)op.P), (short)0,
+                                        has_d ? (int)(m.n * rowd) : 0, BUF_WORD3);
   A pv[NCH][MAXND];
 #pragma unroll
-  for (int ch = 0; ch < NCH; ++ch) pv[ch][0] = acc_zero(A());
+  for (int ch = 0; ch < NCH; ++ch)
+    for (int d = 0; d < MAXND; ++d) pv[ch][d] = acc_zero(A());
 #pragma unroll
   for (int ch = 0; ch < NCH; ++ch) {
     if (ch * 64 >= C) break;
+    if (MODE == 1 && op.gate && !op.gate[ch]) continue;  // chunk not replaced this time
     const int c = lane + ch * 64;
     const bool live = c < C;
     const uint32_t voff = (live ? c : 0) * (uint32_t)sizeof(T);
-    const T* v = vals + (int64_t)((ch * 64) / k) * m.nnz;
+    const uint32_t voffd = (live ? c : 0) * (uint32_t)sizeof(TD);
+    const int sys = (ch * 64) / k;
+    const T* v = op.vals + (int64_t)sys * m.nnz;
     for (int64_t g0 = 0; g0 < nk; g0 += SGC) {
       const int ng = (int)min((int64_t)SGC, nk - g0);
       // lane l holds row pointer l % (SRC+1) of chunk l / (SRC+1) of the group
@@ -424,8 +446,20 @@ __global__ void __launch_bounds__(KB) k_cg_spmm_sweep(Csr m, const T* __restrict
       auto drain = [&](int32_t e) {
         while (row < cnr && rend == e) {
           if (live) {
-            Q[(cr0 + row) * C + c] = acc;
-            pv[ch][0] = acc_add(pv[ch][0], amul(pown, acc));
+            const int64_t o = (cr0 + row) * C + c;
+            if constexpr (MODE == 0) {
+              op.Q[o] = acc;
+              pv[ch][0] = acc_add(pv[ch][0], amul(pown, acc));
+            } else {
+              const T b = op.B[o];
+              const T r = b - acc;
+              const TD r32 = narrow(r, TD());
+              const TD z = op.dinv[(int64_t)sys * m.n + cr0 + row] * r32;
+              op.R32[o] = r32;
+              pv[ch][0] = acc_add(pv[ch][0], acc_from_real(sc<T>::norm2(r), A()));
+              pv[ch][1] = acc_add(pv[ch][1], acc_from_real(sc<T>::norm2(b), A()));
+              pv[ch][2] = acc_add(pv[ch][2], amul(r32, z));
+            }
           }
           acc = sc<T>::zero();
           pown = sc<T>::zero();
@@ -440,19 +474,28 @@ __global__ void __launch_bounds__(KB) k_cg_spmm_sweep(Csr m, const T* __restrict
         rend = __builtin_amdgcn_readlane(rp, kq * (SRC + 1) + 1);
         drain(ce0);
       };
-      auto gather = [&](T(&x)[FRG], int sl, int jb) {
+      auto gather = [&](T(&x)[FRG], TD(&xd)[FRG], int sl, int jb) {
 #pragma unroll
         for (int u = 0; u < FRG; ++u) {
           const uint32_t cu = __builtin_amdgcn_readfirstlane(s_col[wave][sl][jb * FRG + u]);  // uniform
           x[u] = buf_ld<T>(rsrc, voff, cu * rowb);
+          if constexpr (MODE == 1) {
+            if (has_d) xd[u] = buf_ld<TD>(rsrd, voffd, cu * rowd);
+          }
         }
       };
-      auto consume = [&](const T(&x)[FRG], int sl, int jb, int32_t pb) {
+      auto consume = [&](const T(&x)[FRG], const TD(&xd)[FRG], int sl, int jb, int32_t pb) {
 #pragma unroll
         for (int u = 0; u < FRG; ++u) {
-          acc = acc + s_val[wave][sl][jb * FRG + u] * x[u];
-          const int32_t cu = __builtin_amdgcn_readfirstlane(s_col[wave][sl][jb * FRG + u]);
-          if (cu == (int32_t)(cr0 + row)) pown = x[u];
+          T xv = x[u];
+          if constexpr (MODE == 1) {
+            if (has_d) xv = xv + widen(xd[u], T());
+          }
+          acc = acc + s_val[wave][sl][jb * FRG + u] * xv;
+          if constexpr (MODE == 0) {
+            const int32_t cu = __builtin_amdgcn_readfirstlane(s_col[wave][sl][jb * FRG + u]);
+            if (cu == (int32_t)(cr0 + row)) pown = xv;
+          }
           drain(pb + jb * FRG + u + 1);
         }
       };
@@ -465,7 +508,8 @@ __global__ void __launch_bounds__(KB) k_cg_spmm_sweep(Csr m, const T* __restrict
         s_val[wave][0][lane] = (e < ce1) ? v[e] : sc<T>::zero();
       }
       T xa[FRG], xb[FRG];
-      gather(xa, 0, 0);
+      TD da[FRG], db[FRG];
+      gather(xa, da, 0, 0);
       for (;;) {
         // the piece after this one: the rest of this chunk, or the next chunk
         int kn = kc;
@@ -487,16 +531,16 @@ __global__ void __launch_bounds__(KB) k_cg_spmm_sweep(Csr m, const T* __restrict
         // the piece's 64 slots as 4 pairs of batches, straight-line
 #pragma unroll
         for (int pr = 0; pr < 64 / (2 * FRG); ++pr) {
-          gather(xb, sl, 2 * pr + 1);
-          consume(xa, sl, 2 * pr, pb);
+          gather(xb, db, sl, 2 * pr + 1);
+          consume(xa, da, sl, 2 * pr, pb);
           if (pr + 1 < 64 / (2 * FRG)) {
-            gather(xa, sl, 2 * pr + 2);
+            gather(xa, da, sl, 2 * pr + 2);
           } else {
             s_col[wave][sl ^ 1][lane] = vn ? tcol : (int32_t)nr0;
             s_val[wave][sl ^ 1][lane] = vn ? tval : sc<T>::zero();
-            gather(xa, sl ^ 1, 0);  // first batch of the next piece (padding when there is none)
+            gather(xa, da, sl ^ 1, 0);  // first batch of the next piece (padding when there is none)
           }
-          consume(xb, sl, 2 * pr + 1, pb);
+          consume(xb, db, sl, 2 * pr + 1, pb);
         }
         if (!has_next) break;
         sl ^= 1;
@@ -505,7 +549,35 @@ __global__ void __launch_bounds__(KB) k_cg_spmm_sweep(Csr m, const T* __restrict
       }
     }
   }
-  block_partials<A, 64, NCH>(st, pv, 1, C);
+  block_partials<A, 64, NCH>(st, pv, MODE == 0 ? 1 : 3, C);
+}
+
+template <typename T, int NCH>
+__global__ void __launch_bounds__(KB) k_cg_spmm_sweep(Csr m, const T* __restrict__ vals, const T* __restrict__ P,
+                                                      T* __restrict__ Q, int C, int k,
+                                                      ColState<typename sc<T>::acc> st) {
+  // the previous iteration's replacement requests have been served
+  if (st.ru_req && blockIdx.x == 0 && threadIdx.x < MAXCHUNK) st.ru_req[threadIdx.x] = 0;
+  if (!block_any_active(st.active, C)) return;
+  SweepOps<T, T> op{vals, P, nullptr, Q, nullptr, nullptr, nullptr, nullptr};
+  sweep_body<T, T, NCH, 0>(m, op, C, k, st);
+}
+
+// float64 true residual of a mixed solve on the sweep (gated per 64-column
+// chunk by gate; a launch with no flag set returns at once)
+template <typename T64, typename T32, int NCH>
+__global__ void __launch_bounds__(KB) k_ores_sweep(Csr m, const T64* __restrict__ vals, const T64* __restrict__ B,
+                                                   const T64* __restrict__ X, const T32* __restrict__ D,
+                                                   const T32* __restrict__ dinv32, T32* __restrict__ R32, int C,
+                                                   int k, const int* __restrict__ gate,
+                                                   ColState<typename sc<T64>::acc> st) {
+  if (gate) {
+    int any = 0;
+    for (int ch = 0; ch < NCH; ++ch) any |= gate[ch];
+    if (!any) return;
+  }
+  SweepOps<T64, T32> op{vals, X, D, nullptr, B, dinv32, R32, gate};
+  sweep_body<T64, T32, NCH, 1>(m, op, C, k, st);
 }
 
 // q = A_s p ; partial p.q -- one wavefront per contiguous run of rows.
@@ -649,20 +721,19 @@ __global__ void __launch_bounds__(KB) k_cg_pupdate(int64_t n, const T* __restric
                                                    T* __restrict__ D, const int* __restrict__ ru,
                                                    ColState<typename sc<T>::acc> st) {
   typedef typename sc<T>::acc A;
-  // mixed solves: on a replacement iteration (*ru) the true residual was
-  // taken at x64 + d, which now becomes x64 (d restarts from 0).  The flag
-  // and the active columns are read together (one barrier).
-  __shared__ int s_any, s_fold;
-  if (threadIdx.x == 0) {
-    s_any = 0;
-    s_fold = ru ? *ru : 0;
-  }
+  // mixed solves: for the 64-column chunks replaced this iteration (ru[chunk])
+  // the true residual was taken at x64 + d, which now becomes x64 (d
+  // restarts from 0).  The flags and the active columns are read together.
+  __shared__ int s_any, s_fold[MAXCHUNK];
+  if (threadIdx.x == 0) s_any = 0;
+  if (threadIdx.x < MAXCHUNK) s_fold[threadIdx.x] = ru ? ru[threadIdx.x] : 0;
   __syncthreads();
   for (int c = threadIdx.x; c < C; c += blockDim.x)
     if (st.active[c]) s_any = 1;
   __syncthreads();
-  const bool fold = s_fold != 0;
-  if (!s_any && !fold) return;
+  int fold_any = 0;
+  for (int i = 0; i < MAXCHUNK; ++i) fold_any |= s_fold[i];
+  if (!s_any && !fold_any) return;
   A be[NCH];
   int act[NCH];
 #pragma unroll
@@ -677,7 +748,7 @@ __global__ void __launch_bounds__(KB) k_cg_pupdate(int64_t n, const T* __restric
       int c = lane + ch * G;
       if (c >= C) continue;
       const int64_t o = row * C + c;
-      if (fold) {
+      if (fold_any && s_fold[c >> 6]) {
         X64[o] = X64[o] + widen(D[o], TW());
         D[o] = sc<T>::zero();
       }
@@ -746,7 +817,11 @@ __global__ void __launch_bounds__(KB) k_outer_resid(Csr m, const T64* __restrict
                                                     int k, const int* __restrict__ gate,
                                                     ColState<typename sc<T64>::acc> st) {
   typedef typename sc<T64>::acc A;
-  if (gate && !*gate) return;
+  if (gate) {
+    int any = 0;
+    for (int i = 0; i < MAXCHUNK; ++i) any |= gate[i];
+    if (!any) return;
+  }
   A pv[NCH][MAXND];
 #pragma unroll
   for (int ch = 0; ch < NCH; ++ch) {
@@ -772,7 +847,7 @@ __global__ void __launch_bounds__(KB) k_outer_resid(Csr m, const T64* __restrict
 #pragma unroll
     for (int ch = 0; ch < NCH; ++ch) {
       int c = lane + ch * G;
-      if (c < C) {
+      if (c < C && (!gate || gate[c >> 6])) {  // only the requested 64-column chunks
         int s = c / k;
         int64_t o = row * C + c;
         T64 b = B[o];
@@ -848,7 +923,7 @@ __global__ void __launch_bounds__(FB) k_fin_beta(int C, ColState<A> st) {
     st.iters[c] = it;
     A rz = v[0];
     if (st.delta2 > 0.0) {  // mixed: the float64 true residual decides (k_fin_ru)
-      if (rr <= st.delta2 * st.rmax2[c] || rr <= st.tol2ru * st.bb[c]) *st.ru_req = 1;
+      if (rr <= st.delta2 * st.rmax2[c] || rr <= st.tol2ru * st.bb[c]) st.ru_req[c >> 6] = 1;
       if (it >= st.maxiter) {
         st.active[c] = 0;
         st.flag[c] = 1;
@@ -882,11 +957,12 @@ __global__ void __launch_bounds__(FB) k_fin_beta(int C, ColState<A> st) {
 
 // reliable update: true residual norms decide convergence; the CG recurrence
 // continues from the replaced residual (beta = rz_true / rho before the
-// last beta).  Runs only when a column requested it (si.ru_req).
+// last beta).  Runs for the columns of the 64-column chunks whose
+// replacement was requested (si.ru_req, one flag per chunk).
 template <typename A>
 __global__ void __launch_bounds__(FB) k_fin_ru(int C, int first, double tol2, ColState<A> so, ColState<A> si) {
   int c = blockIdx.x;
-  if (!first && (!*si.ru_req || !si.active[c])) return;
+  if (!first && (!si.ru_req[c >> 6] || !si.active[c])) return;
   A v[3];
   finish_reduce(so, 3, C, c, v);
   if (threadIdx.x == 0) {
@@ -1392,21 +1468,34 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
   if (rc) return rc;
   rc = rb_fill_bytes0(D, (size_t)n * C * sizeof(T32), s);
   if (rc) return rc;
-  rc = rb_fill_i32(ru_req, 0, 1, s);
+  rc = rb_fill_i32(ru_req, 0, MAXCHUNK, s);
   if (rc) return rc;
   if (!use_x0) {
     rc = rb_fill_bytes0(X, (size_t)n * C * sizeof(T64), s);
     if (rc) return rc;
   }
-  // the outer residual (launched every iteration, usually gated off) runs
-  // on at most two resident rounds of workgroups
-  int64_t nbo = nb;
-#define G_ORES(G_, N_, unused) nbo = std::min(nb, 2 * resident_grid((const void*)k_outer_resid<T64, T32, G_, N_>, nb))
-  DISPATCH_GN(C, G_ORES, 0);
-#undef G_ORES
+  // the true residual (launched every iteration, gated off unless a chunk
+  // requested a replacement): on the sweep when its layout allows (one
+  // resident round), else the row-per-group kernel on the regular grid
+  bool oflat;
+  int64_t nbo;
+  spmm_plan<T64>(m, nsys, C, k, nb, &oflat, &nbo);
+  if (oflat) {
+    const int nch = pick_NCH(C);
+    const void* fn = nch == 1   ? (const void*)k_ores_sweep<T64, T32, 1>
+                     : nch == 2 ? (const void*)k_ores_sweep<T64, T32, 2>
+                     : nch == 4 ? (const void*)k_ores_sweep<T64, T32, 4>
+                                : (const void*)k_ores_sweep<T64, T32, 8>;
+    nbo = resident_grid(fn, nb);
+  }
   so.nb = (int)nbo;
 #define L_ORES(G_, N_, ...) \
   hipLaunchKernelGGL((k_outer_resid<T64, T32, G_, N_>), dim3(nbo), dim3(KB), 0, s, __VA_ARGS__)
+#define L_ORES_SWEEP(G_, N_, ...)                                                                 \
+  do {                                                                                          \
+    if (G_ == 64 && oflat)                                                                      \
+      hipLaunchKernelGGL((k_ores_sweep<T64, T32, N_>), dim3(nbo), dim3(KB), 0, s, __VA_ARGS__); \
+  } while (0)
   // initial true residual (x0 or 0)
   {
     ProfScope ps(RB_PROF_RESID, s);
@@ -1444,7 +1533,10 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
       {
         // gated replacement: no-ops unless fin_beta raised ru_req
         ProfScope ps(RB_PROF_RESID, s);
-        DISPATCH_GN(C, L_ORES, m, A64, B, X, D, 1, w.dinv, w.R, C, k, (const int*)ru_req, so);
+        if (oflat)
+          DISPATCH_GN(C, L_ORES_SWEEP, m, A64, B, X, D, w.dinv, w.R, C, k, (const int*)ru_req, so);
+        else
+          DISPATCH_GN(C, L_ORES, m, A64, B, X, D, 1, w.dinv, w.R, C, k, (const int*)ru_req, so);
         hipLaunchKernelGGL(k_fin_ru<A>, dim3(C), dim3(FB), 0, s, C, 0, tol2, so, si);
       }
       {
@@ -1459,6 +1551,7 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
   }
 #undef T
 #undef L_ORES
+#undef L_ORES_SWEEP
   // fold in the correction since the last replacement (columns stopped by maxiter)
   hipLaunchKernelGGL((k_accum<T64, T32>), dim3(ew_grid(n * C)), dim3(256), 0, s, (int64_t)n * C, D, X,
                      (const int*)nullptr);

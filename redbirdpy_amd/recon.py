@@ -11,8 +11,10 @@ device:
   built: its Gram is (E E^T)[w,w'] * (J_w J_w'^T), one Gram over nn;
 * the regularised system s^2 J J^T + lambda I is diagonally normalised
   (recon.py:784-789) and solved by a float64 Cholesky; with
-  ``precision='mixed'`` the Gram comes from the float32 Jacobian and the
-  solution is refined against float64 products J (J^T y).
+  ``precision='mixed'`` the Gram is a bf16x3 MFMA product of the float64
+  Jacobian and the solution is refined against float64 products J (J^T y),
+  so the step is that of the float64 Jacobian (a float32 Jacobian would
+  perturb the reconstructed parameters by ~1e-6 relative).
 """
 
 __all__ = ["runrecon", "reginv", "reginvover", "reginvunder", "matreform", "matflat", "syncprop"]
@@ -173,10 +175,11 @@ class GNStep:
 
     fp64 : float64 Jacobian, float64 Gram, Cholesky of the diagonally
            normalised s^2 J J^T + lambda I.
-    mixed: float32 Jacobian, bf16x3 MFMA Gram whose diagonal is replaced by
-           the float64 column norms, float64 Cholesky of it as a
-           preconditioner, then iterative refinement with float64-accumulated
-           products J (J^T v) until the residual is below ``reftol``.
+    mixed: float64 Jacobian, bf16x3 MFMA Gram (its hi/lo bf16 split) whose
+           diagonal is replaced by the float64 column norms, float64
+           Cholesky of it as a preconditioner, then iterative refinement
+           with float64 products J (J^T v) until the residual is below
+           ``reftol``.
     Returns dmu (nn x nc) = s^2 J^T y.
     """
 
@@ -312,7 +315,7 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
         mesh = _gpu.mesh_for(cfg)
         npairs = None
         JT = None
-        jdt = torch.float32 if precision == "mixed" else torch.float64
+        jdt = torch.float64  # both precisions: the refined step is that of this Jacobian
         for i, wv in enumerate(wvs):
             sdw = sd.get(wv, sd) if isinstance(sd, dict) else sd
             s, d = _active_pairs(sdw)
