@@ -49,6 +49,8 @@ def _args():
     p.add_argument("--gram", action="store_true", help="time the Gram kernel modes at the GN shape")
     p.add_argument("--gn-phases", action="store_true", help="add a synchronising per-phase timing run")
     p.add_argument("--cpu-h", type=float, default=2.0, help="mesh spacing of the bounded CPU sample")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="only copy 512 MiB three times (reference kernel for the FETCH_SIZE/WRITE_SIZE scale)")
     return p.parse_args()
 
 
@@ -184,6 +186,14 @@ def main():
     from redbirdpy_amd.forward import forward_device, locate_optodes
 
     torch.cuda.set_device(0)
+    if a.copy_calibration:
+        x = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")  # 512 MiB
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print("copied 512 MiB x3", flush=True)
+        return
     cfg = workloads.fd_box(a.h)
     cfg, sd = rb.meshprep(cfg)
     mesh = _gpu.mesh_for(cfg)
@@ -230,7 +240,7 @@ def main():
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
-        rec = json.load(open(tpath)).get("k_cg_spmm_flat<cplx<float>,1>")
+        rec = json.load(open(tpath)).get("k_cg_spmm_sweep<cplx<float>,1>")
         if rec and a.h == 1.0:
             traffic = rec["hbm_bytes_per_launch"]
     out = {
@@ -263,8 +273,8 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated (profiles/krylov_fd115k_c64_pmc.md)",
-            "kernel": "k_cg_spmm_flat<cplx<float>,1> (q = A p with the p.q reduction)",
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated (profiles/spmm_sweep_fd115k_c64_pmc.md)",
+            "kernel": "k_cg_spmm_sweep<cplx<float>,1> (q = A p with the p.q reduction)",
             "bytes_per_launch": nbytes,
             "avg_launch_us": spmm_avg_s * 1e6,
         },
