@@ -105,11 +105,15 @@ class _Optodes:
 
     def det_columns(self, cfg, mesh):
         """Detector columns as device CSC (ptr, row, val) for rb_getdet, rows in device numbering."""
-        e0 = mesh.iperm[cfg["elem"][:, :4].astype(np.int64) - 1]
         ptrs, rows, vals = [0], [], []
+        found = [i for i in range(self.nd) if not np.isnan(self.dloc[i])]
+        tets = {}
+        if found:  # device numbering of the located tets' nodes only
+            t = np.array([int(self.dloc[i]) - 1 for i in found], dtype=np.int64)
+            tets = dict(zip(found, mesh.iperm[np.asarray(cfg["elem"])[t, :4].astype(np.int64) - 1]))
         for i in range(self.nd):
-            if not np.isnan(self.dloc[i]):
-                rows.append(e0[int(self.dloc[i]) - 1])
+            if i in tets:
+                rows.append(tets[i])
                 vals.append(self.dbary[i])
                 ptrs.append(ptrs[-1] + 4)
             else:
@@ -210,8 +214,18 @@ def forward_device(cfg, rfcw=(1,), precision="fp64", tol=1e-12, maxiter=None, X0
     systems = [(wv, md) for wv in wvs for md in rfcw]
     mesh = _gpu.mesh_for(cfg)
     sysm = _gpu.assemble(mesh, cfg, cfg["deldotdel"], systems, want32=(precision == "mixed"))
-    if optodes is None:
-        optodes = {wv: _Optodes(cfg, mesh, wv) for wv in wvs}
+    # point optodes move inward by the bulk transport length, which changes
+    # with the optical properties: a previous set is reused only where the
+    # moved positions are unchanged
+    old = optodes or {}
+    optodes = {}
+    for wv in wvs:
+        src, det, _, _ = getoptodes(cfg, wv)
+        op = old.get(wv)
+        same = op is not None and all(
+            (a is None and b is None) or (a is not None and b is not None and np.array_equal(np.atleast_2d(a), b))
+            for a, b in ((src, op.src), (det, op.det)))
+        optodes[wv] = op if same else _Optodes(cfg, mesh, wv)
     k = optodes[wvs[0]].k
     nsys = len(systems)
     dt = torch.complex128 if sysm.is_complex else torch.float64

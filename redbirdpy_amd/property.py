@@ -12,6 +12,8 @@ matrix) in place of the table; with it, no default water/lipids are added.
 __all__ = ["extinction", "updateprop", "getbulk", "get_chromophore_table"]
 
 import os
+import weakref
+
 import numpy as np
 
 _HB_CSV = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "hemoglobin_prahl.csv")
@@ -132,10 +134,27 @@ def updateprop(cfg, wv=None, extin=None):
 _DEFAULT_BULK = (0.0, 0.0, 0.0, 1.37)
 
 
+_FIRST_ELEM_CACHE = {}  # (id(elem), id(face)) -> (weakref elem, weakref face, first element)
+
+
 def _first_face_elem(cfg):
-    e0 = cfg["elem"][:, :4].astype(np.int64) - 1
-    hit = np.flatnonzero(np.any(e0 == cfg["face"][0, 0] - 1, axis=1))
-    return int(hit[0]) if len(hit) else None
+    """First element holding the first face's first node (a scan of every
+    element, cached per (elem, face) array pair: getoptodes asks on every
+    forward call)."""
+    elem, face = cfg["elem"], cfg["face"]
+    key = (id(elem), id(face))
+    hit = _FIRST_ELEM_CACHE.get(key)
+    if hit is not None and hit[0]() is elem and hit[1]() is face:
+        return hit[2]
+    rows = np.flatnonzero(np.any(np.asarray(elem)[:, :4] == face[0, 0], axis=1))
+    first = int(rows[0]) if len(rows) else None
+    try:
+        if len(_FIRST_ELEM_CACHE) > 16:
+            _FIRST_ELEM_CACHE.clear()
+        _FIRST_ELEM_CACHE[key] = (weakref.ref(elem), weakref.ref(face), first)
+    except TypeError:  # not weak-referenceable (e.g. a list): no caching
+        pass
+    return first
 
 
 def _bulk_of(prop, cfg):

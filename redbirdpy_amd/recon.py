@@ -271,7 +271,7 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
     extin = kwargs.get("extin", None)
     stol = kwargs.get("solvertol", 1e-12 if precision == "fp64" else 1e-10)
     return_jac = kwargs.get("return_jac", False)
-    warm = kwargs.get("warmstart", precision != "fp64")  # start each forward solve from the last fields
+    warm = kwargs.get("warmstart", True)  # start each forward solve from the last iteration's fields
     optodes = kwargs.get("optodes", None)
     if isinstance(rfcw, int):
         rfcw = [rfcw]
@@ -306,6 +306,7 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
             cfg["prop"] = updateprop(cfg, extin=extin)
         tp = mark("props", tp)
         res = forward_device(cfg, rfcw, precision, stol, None, X0 if warm else None, optodes=optodes)
+        optodes = res.optodes  # reused next iteration where the moved optode positions are unchanged
         X0 = res.X
         tp = mark("forward", tp)
         if timing is not None:

@@ -41,7 +41,9 @@ def test_runrecon_chromophores_match_oracle(precision, tol):
         err = np.max(np.abs(a - b)) / np.max(np.abs(b))
         assert err < tol, (k, err)
     # the misfit sum is a difference of near-equal numbers at the noise floor
-    np.testing.assert_allclose(out1[1], out2[1], rtol=1e-9 if precision == "fp64" else 1e-5)
+    # (0.5% noise): a detector-value error e becomes ~e/0.005 in it, and the
+    # Krylov solves guarantee a 1e-12 relative residual, not a 1e-12 field
+    np.testing.assert_allclose(out1[1], out2[1], rtol=1e-7 if precision == "fp64" else 1e-5)
     assert out1[1][-1] < out1[1][0]  # the misfit decreases
 
 
