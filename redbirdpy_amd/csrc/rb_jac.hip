@@ -35,37 +35,58 @@ __device__ inline cplx<double> jmul(cplx<double> a, cplx<double> b) { return a *
 __device__ inline double jscale(double s, double v) { return s * v; }
 __device__ inline cplx<double> jscale(double s, cplx<double> v) { return mk<double>(s * v.x, s * v.y); }
 
-// Q_n (np x np, row-major in LDS) from the node's incident tets
+// Q_n (np x np, row-major in LDS) from the node's incident tets, 64 tets at
+// a time: each lane loads one tet (vertices, volume) and finds its vertices'
+// positions in the sorted patch (tl, tv in LDS), then one lane per patch row
+// adds the tets holding that row's node in ascending tet order (the same
+// order for every run, so Q is bit-reproducible).  One wavefront per block.
 __device__ inline void build_patch_q(int64_t n, const int32_t* el, const double* evol, const int32_t* inc_ptr,
-                                     const int32_t* inc_list, const int32_t* cols, int np, double* Q) {
-  for (int k = threadIdx.x; k < np; k += blockDim.x) {
-    double* qk = Q + k * np;
-    for (int l = 0; l < np; ++l) qk[l] = 0.0;
-    int32_t nk = cols[k];
-    for (int32_t t = inc_ptr[n]; t < inc_ptr[n + 1]; ++t) {
-      int32_t e = inc_list[t] >> 2;
+                                     const int32_t* inc_list, const int32_t* cols, int np, double* Q,
+                                     int32_t* tl /* 64 x 4 */, double* tv /* 64 */) {
+  const int lane = threadIdx.x;
+  for (int i = lane; i < np * np; i += JW) Q[i] = 0.0;
+  const int32_t t0 = inc_ptr[n], t1 = inc_ptr[n + 1];
+  for (int32_t c0 = t0; c0 < t1; c0 += JW) {
+    const int nt = min(JW, t1 - c0);
+    if (lane < nt) {
+      const int32_t e = inc_list[c0 + lane] >> 2;
       const int32_t* q = el + 4 * (int64_t)e;
-      bool has = (q[0] == nk) | (q[1] == nk) | (q[2] == nk) | (q[3] == nk);
-      if (!has) continue;
-      double V = evol[e];
-      for (int j = 0; j < 4; ++j) {
-        // local index of q[j] in the sorted patch
+      int32_t v[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v[j] = q[j];
+      tv[lane] = evol[e];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {  // position of v[j] in the sorted patch (every tet vertex is a neighbour)
         int lo = 0, hi = np - 1, pos = -1;
         while (lo <= hi) {
-          int mid = (lo + hi) >> 1;
-          int32_t v = cols[mid];
-          if (v == q[j]) {
+          const int mid = (lo + hi) >> 1;
+          const int32_t cv = cols[mid];
+          if (cv == v[j]) {
             pos = mid;
             break;
           }
-          if (v < q[j])
+          if (cv < v[j])
             lo = mid + 1;
           else
             hi = mid - 1;
         }
-        if (pos >= 0) qk[pos] += (q[j] == nk) ? 2.0 * V : V;
+        tl[lane * 4 + j] = pos;
       }
     }
+    __syncthreads();
+    for (int k = lane; k < np; k += JW) {
+      double* qk = Q + k * np;
+      for (int t = 0; t < nt; ++t) {
+        const int32_t l0 = tl[t * 4], l1 = tl[t * 4 + 1], l2 = tl[t * 4 + 2], l3 = tl[t * 4 + 3];
+        if (l0 != k && l1 != k && l2 != k && l3 != k) continue;
+        const double V = tv[t];
+        if (l0 >= 0) qk[l0] += (l0 == k) ? 2.0 * V : V;
+        if (l1 >= 0) qk[l1] += (l1 == k) ? 2.0 * V : V;
+        if (l2 >= 0) qk[l2] += (l2 == k) ? 2.0 * V : V;
+        if (l3 >= 0) qk[l3] += (l3 == k) ? 2.0 * V : V;
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -83,7 +104,9 @@ __global__ void __launch_bounds__(JW) k_jac_node(int64_t nn, const int32_t* __re
   T* PD = PS + maxnp * JC;                 // np x JC
   T* Y = PD + maxnp * JC;                  // np x JC
   double* Q = (double*)(Y + maxnp * JC);   // np x np
-  int32_t* cols = (int32_t*)(Q + maxnp * maxnp);
+  double* tv = Q + maxnp * maxnp;          // 64 staged tet volumes
+  int32_t* tl = (int32_t*)(tv + JW);       // their 64 x 4 patch positions
+  int32_t* cols = tl + 4 * JW;
   const int lane = threadIdx.x;
   // lane -> 4x4 tile of the 32x32 chunk: 8 tiles along s, 8 along d
   const int ts = (lane & 7) * 4;
@@ -93,7 +116,7 @@ __global__ void __launch_bounds__(JW) k_jac_node(int64_t nn, const int32_t* __re
     int np = row_ptr[n + 1] - r0;
     for (int k = lane; k < np; k += JW) cols[k] = col_idx[r0 + k];
     __syncthreads();
-    build_patch_q(n, el, evol, inc_ptr, inc_list, cols, np, Q);
+    build_patch_q(n, el, evol, inc_ptr, inc_list, cols, np, Q, tl, tv);
     __syncthreads();
     for (int s0 = 0; s0 < Ns; s0 += JC) {
       int ns = min(JC, Ns - s0);
@@ -183,7 +206,7 @@ __global__ void k_jac_elem(int64_t ne, const int32_t* __restrict__ el, const dou
 
 static size_t jac_lds(int maxnp, int is_complex) {
   size_t sv = is_complex ? 16 : 8;
-  return (size_t)3 * maxnp * JC * sv + (size_t)maxnp * maxnp * 8 + (size_t)maxnp * 4 + 16;
+  return (size_t)3 * maxnp * JC * sv + (size_t)maxnp * maxnp * 8 + (size_t)JW * (8 + 16) + (size_t)maxnp * 4 + 16;
 }
 
 extern "C" int rb_jac_grid(int64_t nn, int64_t ne, const int32_t* el, const double* evol, const int32_t* inc_ptr,
