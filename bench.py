@@ -46,9 +46,9 @@ def _args():
     p.add_argument("--no-gn", dest="gn", action="store_false", help="skip the Gauss-Newton iterations leg")
     p.add_argument("--gn-iters", type=int, default=5)
     p.add_argument("--gn-precisions", default="mixed,fp64", help="comma list of GN legs to run")
-    p.add_argument("--gram", action="store_true", help="time the Gram kernel modes at the GN shape")
+    p.add_argument("--no-kernels", action="store_true", help="skip the Jacobian and Gram kernel legs")
     p.add_argument("--gn-phases", action="store_true", help="add a synchronising per-phase timing run")
-    p.add_argument("--cpu-h", type=float, default=2.0, help="mesh spacing of the bounded CPU sample")
+    p.add_argument("--cpu-h", type=float, default=1.5, help="mesh spacing of the bounded CPU sample")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only copy 512 MiB three times (reference kernel for the FETCH_SIZE/WRITE_SIZE scale)")
     return p.parse_args()
@@ -147,8 +147,8 @@ def gram_bench(nn, m=4096, reps=3):
     from redbirdpy_amd import _gpu
 
     g = torch.Generator(device="cuda").manual_seed(0)
-    JT = torch.randn((nn, m), generator=g, device="cuda", dtype=torch.float32)
-    JT *= torch.exp(torch.linspace(-6, 0, m, device="cuda"))[None, :]
+    JT = torch.randn((nn, m), generator=g, device="cuda", dtype=torch.float64)  # float64, as in the GN step
+    JT *= torch.exp(torch.linspace(-6, 0, m, device="cuda", dtype=torch.float64))[None, :]
     flops = m * (m + 1) * nn
     out = {"K": nn, "m": m, "syrk_flop": flops}
     ref = None
@@ -173,6 +173,37 @@ def gram_bench(nn, m=4096, reps=3):
         d = torch.sqrt(torch.diagonal(ref))
         out[mode]["max_abs_err_normalised"] = (torch.abs(G - ref) / (d[:, None] * d[None, :])).max().item()
     return out
+
+
+def jac_bench(a, reps=5):
+    """Node Jacobian of 1024 pairs on the 1 mm CW box (115,351 nodes), float64 out.
+
+    Algorithmic bytes nsd*nn*8 (output) + nn*k*8 (fields) + ne*24 (tets, volumes).
+    """
+    import torch
+    import redbirdpy_amd as rb
+    from redbirdpy_amd import _gpu, workloads
+    from redbirdpy_amd.forward import forward_device, _active_pairs
+
+    cfg, sd = rb.meshprep(workloads.jac_box(a.h))
+    res = forward_device(cfg, [1], "fp64", 1e-12)
+    mesh = res.mesh
+    s, d = _active_pairs(sd)
+    phi = res.phi(0)
+    JT = torch.empty((mesh.nn, len(s)), dtype=torch.float64, device=mesh.dev)
+    _gpu.jac_node(mesh, phi, s, d, torch.float64, out=JT)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        _gpu.jac_node(mesh, phi, s, d, torch.float64, out=JT)
+    e1.record()
+    torch.cuda.synchronize()
+    dt = e0.elapsed_time(e1) / 1e3 / reps
+    nbytes = len(s) * mesh.nn * 8 + mesh.nn * res.k * 8 + mesh.ne * 24
+    return {"workload": "CW box 1 mm, %d pairs x %d nodes, float64" % (len(s), mesh.nn), "ms": dt * 1e3,
+            "bytes": nbytes, "achieved_gbs": nbytes / dt / 1e9, "frac_hbm": nbytes / dt / 1e9 / HBM_PEAK_GBS}
 
 
 def main():
@@ -291,10 +322,11 @@ def main():
         out["mixed"]["err_phi_relL2_vs_fp64"] = err_phi
         out["mixed"]["err_detval_maxrel_vs_fp64"] = err_det
         out["speedup_mixed_vs_fp64"] = dt_d / dt_m
-    if a.gram:
-        out["gram"] = gram_bench(mesh.nn)
     if a.gn:
         out["gn"] = gn_bench(a)
+    if not a.no_kernels:
+        out["jacobian"] = jac_bench(a)
+        out["gram"] = gram_bench(mesh.nn)
     if not a.skip_cpu:
         out["cpu_baseline"] = cpu_baseline(a.cpu_h)
     print(json.dumps(out), flush=True)
