@@ -54,10 +54,19 @@ struct ColState {
   // true residual at the last replacement, and the replacement request flag
   A* rho_prev;
   double* rmax2;
+  // replacement requests, one flag per 64-column chunk, double-buffered by
+  // iteration parity: iteration k's beta step raises ru_req[MAXCHUNK*par + ch]
+  // and iteration k+1 (parity par^1 on its launches) serves them
   int* ru_req;
+  int ru_par;
   double delta2;  // request a replacement when rr <= delta2 * rmax2 (0: plain CG)
   double tol2ru;  // ... or when rr <= tol2ru * bb
 };
+
+template <typename A>
+__device__ inline int* ru_cur(const ColState<A>& st) { return st.ru_req + MAXCHUNK * st.ru_par; }
+template <typename A>
+__device__ inline const int* ru_prev(const ColState<A>& st) { return st.ru_req + MAXCHUNK * (st.ru_par ^ 1); }
 
 struct Csr {
   const int32_t* row_ptr;
@@ -283,8 +292,8 @@ template <typename T, int G, int NCH, bool MULTI>
 __global__ void __launch_bounds__(KB) k_cg_spmm(Csr m, const T* __restrict__ vals, const T* __restrict__ P,
                                                 T* __restrict__ Q, int C, int k, ColState<typename sc<T>::acc> st) {
   typedef typename sc<T>::acc A;
-  // the previous iteration's replacement requests have been served
-  if (st.ru_req && blockIdx.x == 0 && threadIdx.x < MAXCHUNK) st.ru_req[threadIdx.x] = 0;
+  // this iteration's replacement requests start cleared
+  if (st.ru_req && blockIdx.x == 0 && threadIdx.x < MAXCHUNK) ru_cur(st)[threadIdx.x] = 0;
   if (!block_any_active(st.active, C)) return;
   A pv[NCH][MAXND];
 #pragma unroll
@@ -349,7 +358,6 @@ __global__ void __launch_bounds__(KB) k_cg_spmm(Csr m, const T* __restrict__ val
 // reads LDS it wrote itself and LDS operations of one wavefront complete in
 // order, so no barrier is needed.  Requires every 64-column chunk inside one
 // system (nsys == 1 or k % 64 == 0) and the P block < 2 GiB.
-#define FRG 8   // gathers per batch
 #define SRC 4   // rows per chunk
 #define SGC 12  // chunks whose row pointers one lane-parallel load covers
 static_assert(SGC * (SRC + 1) <= 64, "row pointers of a chunk group must fit one wavefront");
@@ -385,7 +393,8 @@ struct SweepOps {
   const int* gate; // MODE 1: one flag per 64-column chunk
 };
 
-template <typename T, typename TD, int NCH, int MODE>
+// FRG: gathers per batch (two batches in flight)
+template <typename T, typename TD, int NCH, int MODE, int FRG = 8>
 __device__ __forceinline__ void sweep_body(const Csr& m, const SweepOps<T, TD>& op, int C, int k,
                                            const ColState<typename sc<T>::acc>& st) {
   typedef typename sc<T>::acc A;
@@ -556,28 +565,35 @@ template <typename T, int NCH>
 __global__ void __launch_bounds__(KB) k_cg_spmm_sweep(Csr m, const T* __restrict__ vals, const T* __restrict__ P,
                                                       T* __restrict__ Q, int C, int k,
                                                       ColState<typename sc<T>::acc> st) {
-  // the previous iteration's replacement requests have been served
-  if (st.ru_req && blockIdx.x == 0 && threadIdx.x < MAXCHUNK) st.ru_req[threadIdx.x] = 0;
+  if (st.ru_req && blockIdx.x == 0 && threadIdx.x < MAXCHUNK) ru_cur(st)[threadIdx.x] = 0;
   if (!block_any_active(st.active, C)) return;
   SweepOps<T, T> op{vals, P, nullptr, Q, nullptr, nullptr, nullptr, nullptr};
   sweep_body<T, T, NCH, 0>(m, op, C, k, st);
 }
 
-// float64 true residual of a mixed solve on the sweep (gated per 64-column
-// chunk by gate; a launch with no flag set returns at once)
+// mixed solves: q = A32 p (partials in st) and, for the 64-column chunks
+// whose replacement the previous iteration requested, the float64 true
+// residual r32 = b - A64 (x64 + d32) (partials |r|^2, |b|^2, r.z in so) in
+// the same launch; the residual pass runs smaller batches so the launch
+// keeps the SpMM's occupancy.  Clears this iteration's request flags.
 template <typename T64, typename T32, int NCH>
-__global__ void __launch_bounds__(KB) k_ores_sweep(Csr m, const T64* __restrict__ vals, const T64* __restrict__ B,
-                                                   const T64* __restrict__ X, const T32* __restrict__ D,
-                                                   const T32* __restrict__ dinv32, T32* __restrict__ R32, int C,
-                                                   int k, const int* __restrict__ gate,
-                                                   ColState<typename sc<T64>::acc> st) {
-  if (gate) {
-    int any = 0;
-    for (int ch = 0; ch < NCH; ++ch) any |= gate[ch];
-    if (!any) return;
-  }
-  SweepOps<T64, T32> op{vals, X, D, nullptr, B, dinv32, R32, gate};
-  sweep_body<T64, T32, NCH, 1>(m, op, C, k, st);
+__global__ void __launch_bounds__(KB) __attribute__((amdgpu_waves_per_eu(5))) k_cg_spmm_sweep_ru(Csr m, const T32* __restrict__ vals32,
+                                                         const T32* __restrict__ P, T32* __restrict__ Q,
+                                                         const T64* __restrict__ vals64, const T64* __restrict__ B,
+                                                         const T64* __restrict__ X, const T32* __restrict__ D,
+                                                         const T32* __restrict__ dinv32, T32* __restrict__ R32, int C,
+                                                         int k, ColState<typename sc<T32>::acc> st,
+                                                         ColState<typename sc<T64>::acc> so) {
+  if (blockIdx.x == 0 && threadIdx.x < MAXCHUNK) ru_cur(st)[threadIdx.x] = 0;
+  if (!block_any_active(st.active, C)) return;
+  SweepOps<T32, T32> op{vals32, P, nullptr, Q, nullptr, nullptr, nullptr, nullptr};
+  sweep_body<T32, T32, NCH, 0>(m, op, C, k, st);
+  const int* gate = ru_prev(st);
+  int any = 0;
+  for (int ch = 0; ch < NCH; ++ch) any |= gate[ch];
+  if (!any) return;
+  SweepOps<T64, T32> oo{vals64, X, D, nullptr, B, dinv32, R32, gate};
+  sweep_body<T64, T32, NCH, 1, 4>(m, oo, C, k, so);
 }
 
 // q = A_s p ; partial p.q -- one wavefront per contiguous run of rows.
@@ -675,22 +691,28 @@ __global__ void __launch_bounds__(KB) k_cg_spmm_w64(Csr m, const T* __restrict__
   block_partials<A, 64, NCH>(st, pv, 1, C);
 }
 
-// x += alpha p ; r -= alpha q ; z = dinv r ; partials rz, rr
-template <typename T, int G, int NCH>
+// x += alpha p ; r -= alpha q ; z = dinv r ; partials rz, rr.  Mixed
+// solves (X64 given): X is the 32-bit correction d, and on the 64-column
+// chunks whose residual this iteration's SpMM launch replaced (ruprev) the
+// replacement was taken at x64 + d, so d moves into x64 and restarts at
+// alpha p.
+template <typename T, typename TW, int G, int NCH>
 __global__ void __launch_bounds__(KB) k_cg_update(int64_t n, const T* __restrict__ dinv, T* __restrict__ X,
                                                   T* __restrict__ R, const T* __restrict__ P,
-                                                  const T* __restrict__ Q, int C, int k,
+                                                  const T* __restrict__ Q, int C, int k, TW* __restrict__ X64,
                                                   ColState<typename sc<T>::acc> st) {
   typedef typename sc<T>::acc A;
   if (!block_any_active(st.active, C)) return;
   A pv[NCH][MAXND];
   A al[NCH];
+  bool fold[NCH];
 #pragma unroll
   for (int ch = 0; ch < NCH; ++ch) {
     pv[ch][0] = acc_zero(A());
     pv[ch][1] = acc_zero(A());
     int c = (int)(threadIdx.x % G) + ch * G;
     al[ch] = (c < C) ? st.alpha[c] : acc_zero(A());
+    fold[ch] = X64 && c < C && ru_prev(st)[c >> 6];
   }
   ROW_LOOP(n) {
 #pragma unroll
@@ -700,8 +722,14 @@ __global__ void __launch_bounds__(KB) k_cg_update(int64_t n, const T* __restrict
         int s = c / k;
         int64_t o = row * C + c;
         T r = R[o];
-        if (acc_abs2(al[ch]) != 0.0) {
+        const bool move = acc_abs2(al[ch]) != 0.0;
+        if (fold[ch]) {
+          X64[o] = X64[o] + widen(X[o], TW());
+          X[o] = move ? scale_to(P[o], al[ch]) : sc<T>::zero();
+        } else if (move) {
           X[o] = X[o] + scale_to(P[o], al[ch]);
+        }
+        if (move) {
           r = r - scale_to(Q[o], al[ch]);
           R[o] = r;
         }
@@ -715,25 +743,12 @@ __global__ void __launch_bounds__(KB) k_cg_update(int64_t n, const T* __restrict
 }
 
 // p = dinv r + beta p on active columns
-template <typename T, typename TW, int G, int NCH>
+template <typename T, int G, int NCH>
 __global__ void __launch_bounds__(KB) k_cg_pupdate(int64_t n, const T* __restrict__ dinv, const T* __restrict__ R,
-                                                   T* __restrict__ P, int C, int k, TW* __restrict__ X64,
-                                                   T* __restrict__ D, const int* __restrict__ ru,
+                                                   T* __restrict__ P, int C, int k,
                                                    ColState<typename sc<T>::acc> st) {
   typedef typename sc<T>::acc A;
-  // mixed solves: for the 64-column chunks replaced this iteration (ru[chunk])
-  // the true residual was taken at x64 + d, which now becomes x64 (d
-  // restarts from 0).  The flags and the active columns are read together.
-  __shared__ int s_any, s_fold[MAXCHUNK];
-  if (threadIdx.x == 0) s_any = 0;
-  if (threadIdx.x < MAXCHUNK) s_fold[threadIdx.x] = ru ? ru[threadIdx.x] : 0;
-  __syncthreads();
-  for (int c = threadIdx.x; c < C; c += blockDim.x)
-    if (st.active[c]) s_any = 1;
-  __syncthreads();
-  int fold_any = 0;
-  for (int i = 0; i < MAXCHUNK; ++i) fold_any |= s_fold[i];
-  if (!s_any && !fold_any) return;
+  if (!block_any_active(st.active, C)) return;
   A be[NCH];
   int act[NCH];
 #pragma unroll
@@ -746,14 +761,9 @@ __global__ void __launch_bounds__(KB) k_cg_pupdate(int64_t n, const T* __restric
 #pragma unroll
     for (int ch = 0; ch < NCH; ++ch) {
       int c = lane + ch * G;
-      if (c >= C) continue;
-      const int64_t o = row * C + c;
-      if (fold_any && s_fold[c >> 6]) {
-        X64[o] = X64[o] + widen(D[o], TW());
-        D[o] = sc<T>::zero();
-      }
-      if (act[ch]) {
+      if (c < C && act[ch]) {
         int s = c / k;
+        int64_t o = row * C + c;
         T z = dinv[(int64_t)s * n + row] * R[o];
         P[o] = z + scale_to(P[o], be[ch]);
       }
@@ -922,8 +932,8 @@ __global__ void __launch_bounds__(FB) k_fin_beta(int C, ColState<A> st) {
     int it = st.iters[c] + 1;
     st.iters[c] = it;
     A rz = v[0];
-    if (st.delta2 > 0.0) {  // mixed: the float64 true residual decides (k_fin_ru)
-      if (rr <= st.delta2 * st.rmax2[c] || rr <= st.tol2ru * st.bb[c]) st.ru_req[c >> 6] = 1;
+    if (st.delta2 > 0.0) {  // mixed: the float64 true residual decides (next iteration's k_fin_alpha_ru)
+      if (rr <= st.delta2 * st.rmax2[c] || rr <= st.tol2ru * st.bb[c]) ru_cur(st)[c >> 6] = 1;
       if (it >= st.maxiter) {
         st.active[c] = 0;
         st.flag[c] = 1;
@@ -955,42 +965,72 @@ __global__ void __launch_bounds__(FB) k_fin_beta(int C, ColState<A> st) {
   }
 }
 
-// reliable update: true residual norms decide convergence; the CG recurrence
-// continues from the replaced residual (beta = rz_true / rho before the
-// last beta).  Runs for the columns of the 64-column chunks whose
-// replacement was requested (si.ru_req, one flag per chunk).
+// mixed solves, before the first iteration: the float64 true residual of
+// x0 (partials in so) initialises the inner recurrence of every column
 template <typename A>
-__global__ void __launch_bounds__(FB) k_fin_ru(int C, int first, double tol2, ColState<A> so, ColState<A> si) {
+__global__ void __launch_bounds__(FB) k_fin_ru(int C, double tol2, ColState<A> so, ColState<A> si) {
   int c = blockIdx.x;
-  if (!first && (!si.ru_req[c >> 6] || !si.active[c])) return;
   A v[3];
   finish_reduce(so, 3, C, c, v);
   if (threadIdx.x == 0) {
     double rr = acc_re(v[0]), bb = acc_re(v[1]);
-    A rz = v[2];
     so.rr[c] = rr;
     so.bb[c] = bb;
     si.bb[c] = bb;
     si.rmax2[c] = rr;
-    int conv = !(bb > 0.0) || rr <= tol2 * bb;
-    if (first) {
-      si.rho[c] = rz;
-      si.rho_prev[c] = rz;
-      si.beta[c] = acc_zero(A());
+    si.rho[c] = v[2];
+    si.rho_prev[c] = v[2];
+    si.beta[c] = acc_zero(A());
+    si.alpha[c] = acc_zero(A());
+    si.iters[c] = 0;
+    si.flag[c] = 0;
+    si.active[c] = !(!(bb > 0.0) || rr <= tol2 * bb);
+  }
+}
+
+// mixed solves: alpha = rho / p.q, and for the columns whose chunk was
+// replaced in this iteration's SpMM launch (the previous iteration's
+// request) first the reliable update (Sleijpen & van der Vorst): the float64
+// true residual decides convergence and restarts rmax, and rho becomes
+// r32.(dinv r32) of the replaced residual.  The search direction is kept.
+template <typename A>
+__global__ void __launch_bounds__(FB) k_fin_alpha_ru(int C, double tol2, ColState<A> si, ColState<A> so) {
+  int c = blockIdx.x;
+  if (!si.active[c]) {
+    if (threadIdx.x == 0) si.alpha[c] = acc_zero(A());
+    return;
+  }
+  const bool ru = ru_prev(si)[c >> 6] != 0;
+  A v[3];
+  if (ru) finish_reduce(so, 3, C, c, v);
+  A pq[1];
+  finish_reduce(si, 1, C, c, pq);
+  if (threadIdx.x == 0) {
+    if (ru) {
+      double rr = acc_re(v[0]), bb = acc_re(v[1]);
+      so.rr[c] = rr;
+      so.bb[c] = bb;
+      si.bb[c] = bb;
+      si.rmax2[c] = rr;
+      if (!(bb > 0.0) || rr <= tol2 * bb) {  // converged
+        si.active[c] = 0;
+        si.alpha[c] = acc_zero(A());
+        return;
+      }
+      if (acc_abs2(v[2]) == 0.0) {
+        si.active[c] = 0;
+        si.flag[c] = 4;
+        si.alpha[c] = acc_zero(A());
+        return;
+      }
+      si.rho[c] = v[2];
+    }
+    if (acc_abs2(pq[0]) == 0.0) {
+      si.flag[c] = 4;  // breakdown
+      si.active[c] = 0;
       si.alpha[c] = acc_zero(A());
-      si.iters[c] = 0;
-      si.flag[c] = 0;
-      si.active[c] = !conv;
-    } else if (conv) {
-      si.active[c] = 0;
-      si.beta[c] = acc_zero(A());
-    } else if (acc_abs2(rz) == 0.0 || acc_abs2(si.rho_prev[c]) == 0.0) {
-      si.active[c] = 0;
-      si.flag[c] = 4;
-      si.beta[c] = acc_zero(A());
     } else {
-      si.beta[c] = acc_div(rz, si.rho_prev[c]);
-      si.rho[c] = rz;
+      si.alpha[c] = acc_div(si.rho[c], pq[0]);
     }
   }
 }
@@ -1332,8 +1372,8 @@ static void spmm_plan(const Csr& m, int nsys, int C, int k, int64_t nb, bool* fl
     if (cap >= NXCD) *nbs = std::min(*nbs, cap);
   }
 }
-#define L_UPD(G_, N_, ...) hipLaunchKernelGGL((k_cg_update<T, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
-#define L_PUP(G_, N_, ...) hipLaunchKernelGGL((k_cg_pupdate<T, TW, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
+#define L_UPD(G_, N_, ...) hipLaunchKernelGGL((k_cg_update<T, TW, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
+#define L_PUP(G_, N_, ...) hipLaunchKernelGGL((k_cg_pupdate<T, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
 
 // one CG/COCG solve of all columns; X is the solution (and x0 if use_x0)
 template <typename T>
@@ -1368,13 +1408,13 @@ static int run_cg(const Csr& m, int nsys, const T* vals, const T* dinv, const T*
       hipLaunchKernelGGL(k_fin_alpha<A>, dim3(C), dim3(FB), 0, s, C, ss);
       {
         ProfScope ps(RB_PROF_UPDATE, s);
-        DISPATCH_GN(C, L_UPD, m.n, dinv, X, w.R, w.P, w.Q, C, k, st);
+        typedef T TW;
+        DISPATCH_GN(C, L_UPD, m.n, dinv, X, w.R, w.P, w.Q, C, k, (TW*)nullptr, st);
       }
       hipLaunchKernelGGL(k_fin_beta<A>, dim3(C), dim3(FB), 0, s, C, st);
       {
         ProfScope ps(RB_PROF_PUPDATE, s);
-        typedef T TW;
-        DISPATCH_GN(C, L_PUP, m.n, dinv, w.R, w.P, C, k, (TW*)nullptr, (T*)nullptr, (const int*)nullptr, st);
+        DISPATCH_GN(C, L_PUP, m.n, dinv, w.R, w.P, C, k, st);
       }
     }
     RB_CHECK_LAUNCH();
@@ -1468,50 +1508,49 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
   if (rc) return rc;
   rc = rb_fill_bytes0(D, (size_t)n * C * sizeof(T32), s);
   if (rc) return rc;
-  rc = rb_fill_i32(ru_req, 0, MAXCHUNK, s);
+  rc = rb_fill_i32(ru_req, 0, 2 * MAXCHUNK, s);
   if (rc) return rc;
   if (!use_x0) {
     rc = rb_fill_bytes0(X, (size_t)n * C * sizeof(T64), s);
     if (rc) return rc;
   }
-  // the true residual (launched every iteration, gated off unless a chunk
-  // requested a replacement): on the sweep when its layout allows (one
-  // resident round), else the row-per-group kernel on the regular grid
-  bool oflat;
-  int64_t nbo;
-  spmm_plan<T64>(m, nsys, C, k, nb, &oflat, &nbo);
-  if (oflat) {
-    const int nch = pick_NCH(C);
-    const void* fn = nch == 1   ? (const void*)k_ores_sweep<T64, T32, 1>
-                     : nch == 2 ? (const void*)k_ores_sweep<T64, T32, 2>
-                     : nch == 4 ? (const void*)k_ores_sweep<T64, T32, 4>
-                                : (const void*)k_ores_sweep<T64, T32, 8>;
-    nbo = resident_grid(fn, nb);
-  }
+  // the float64 true residual: once of x0 here (row-per-group kernel on the
+  // regular grid), then in the loop for the chunks whose replacement the
+  // previous iteration requested -- inside the SpMM launch when the sweep
+  // applies, else a row-per-group launch gated by those flags
+  const int64_t nbo = nb;
   so.nb = (int)nbo;
 #define L_ORES(G_, N_, ...) \
   hipLaunchKernelGGL((k_outer_resid<T64, T32, G_, N_>), dim3(nbo), dim3(KB), 0, s, __VA_ARGS__)
-#define L_ORES_SWEEP(G_, N_, ...)                                                                 \
-  do {                                                                                          \
-    if (G_ == 64 && oflat)                                                                      \
-      hipLaunchKernelGGL((k_ores_sweep<T64, T32, N_>), dim3(nbo), dim3(KB), 0, s, __VA_ARGS__); \
-  } while (0)
-  // initial true residual (x0 or 0)
   {
     ProfScope ps(RB_PROF_RESID, s);
     DISPATCH_GN(C, L_ORES, m, A64, B, X, (const T32*)nullptr, use_x0 ? 1 : 0, w.dinv, w.R, C, k,
                 (const int*)nullptr, so);
-    hipLaunchKernelGGL(k_fin_ru<A>, dim3(C), dim3(FB), 0, s, C, 1, tol2, so, si);
+    hipLaunchKernelGGL(k_fin_ru<A>, dim3(C), dim3(FB), 0, s, C, tol2, so, si);
   }
   RB_CHECK_LAUNCH();
 #define T T32
   bool flat;
   int64_t nbs;
   spmm_plan<T32>(m, nsys, C, k, nb, &flat, &nbs);
+  if (flat) {  // the fused SpMM + residual launch has its own occupancy
+    const int nch = pick_NCH(C);
+    const void* fn = nch == 1   ? (const void*)k_cg_spmm_sweep_ru<T64, T32, 1>
+                     : nch == 2 ? (const void*)k_cg_spmm_sweep_ru<T64, T32, 2>
+                     : nch == 4 ? (const void*)k_cg_spmm_sweep_ru<T64, T32, 4>
+                                : (const void*)k_cg_spmm_sweep_ru<T64, T32, 8>;
+    nbs = std::min(nbs, resident_grid(fn, nb));
+  }
   ColState<A> ss = si;  // the SpMM's partials come from its own grid
   ss.nb = (int)nbs;
+  ColState<A> sol = so;  // the in-loop residual's partials: the SpMM grid, or the regular one
+  sol.nb = (int)(flat ? nbs : nbo);
   typedef T64 TW;
-  DISPATCH_GN(C, L_PUP, m.n, w.dinv, w.R, w.P, C, k, X, D, (const int*)nullptr, si);  // p = z (P is zero)
+#define L_SPMMRU(G_, N_, ...)                                                                                 \
+  do {                                                                                                      \
+    if (G_ == 64) hipLaunchKernelGGL((k_cg_spmm_sweep_ru<T64, T32, N_>), dim3(nbs), dim3(KB), 0, s, __VA_ARGS__); \
+  } while (0)
+  DISPATCH_GN(C, L_PUP, m.n, w.dinv, w.R, w.P, C, k, si);  // p = z (P is zero)
   RB_CHECK_LAUNCH();
   int nact = 0;
   rc = poll_nactive(si.active, C, si.nactive, s, &nact);
@@ -1520,28 +1559,27 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
   while (nact > 0 && it < maxiter) {
     int chunk = std::min(check_every, maxiter - it);
     for (int j = 0; j < chunk; ++j) {
+      // request flags: this iteration raises half (it+j)&1 and serves the other
+      si.ru_par = ss.ru_par = sol.ru_par = (it + j) & 1;
       {
         ProfScope ps(RB_PROF_SPMM, s);
-        DISPATCH_GN(C, L_SPMM, m, A32, w.P, w.Q, C, k, ss);
+        if (flat) {
+          DISPATCH_GN(C, L_SPMMRU, m, A32, w.P, w.Q, A64, B, X, D, w.dinv, w.R, C, k, ss, sol);
+        } else {
+          DISPATCH_GN(C, L_SPMM, m, A32, w.P, w.Q, C, k, ss);
+          DISPATCH_GN(C, L_ORES, m, A64, B, X, D, 1, w.dinv, w.R, C, k,
+                      (const int*)(ru_req + MAXCHUNK * (si.ru_par ^ 1)), sol);
+        }
       }
-      hipLaunchKernelGGL(k_fin_alpha<A>, dim3(C), dim3(FB), 0, s, C, ss);
+      hipLaunchKernelGGL(k_fin_alpha_ru<A>, dim3(C), dim3(FB), 0, s, C, tol2, ss, sol);
       {
         ProfScope ps(RB_PROF_UPDATE, s);
-        DISPATCH_GN(C, L_UPD, m.n, w.dinv, D, w.R, w.P, w.Q, C, k, si);
+        DISPATCH_GN(C, L_UPD, m.n, w.dinv, D, w.R, w.P, w.Q, C, k, X, si);
       }
       hipLaunchKernelGGL(k_fin_beta<A>, dim3(C), dim3(FB), 0, s, C, si);
       {
-        // gated replacement: no-ops unless fin_beta raised ru_req
-        ProfScope ps(RB_PROF_RESID, s);
-        if (oflat)
-          DISPATCH_GN(C, L_ORES_SWEEP, m, A64, B, X, D, w.dinv, w.R, C, k, (const int*)ru_req, so);
-        else
-          DISPATCH_GN(C, L_ORES, m, A64, B, X, D, 1, w.dinv, w.R, C, k, (const int*)ru_req, so);
-        hipLaunchKernelGGL(k_fin_ru<A>, dim3(C), dim3(FB), 0, s, C, 0, tol2, so, si);
-      }
-      {
         ProfScope ps(RB_PROF_PUPDATE, s);
-        DISPATCH_GN(C, L_PUP, m.n, w.dinv, w.R, w.P, C, k, X, D, (const int*)ru_req, si);
+        DISPATCH_GN(C, L_PUP, m.n, w.dinv, w.R, w.P, C, k, si);
       }
     }
     RB_CHECK_LAUNCH();
@@ -1551,7 +1589,7 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
   }
 #undef T
 #undef L_ORES
-#undef L_ORES_SWEEP
+#undef L_SPMMRU
   // fold in the correction since the last replacement (columns stopped by maxiter)
   hipLaunchKernelGGL((k_accum<T64, T32>), dim3(ew_grid(n * C)), dim3(256), 0, s, (int64_t)n * C, D, X,
                      (const int*)nullptr);
