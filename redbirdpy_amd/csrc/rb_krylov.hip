@@ -20,6 +20,7 @@
 // Precision modes: plain (everything in the 64-bit type) and mixed (inner
 // solves in the 32-bit type on a 32-bit copy of A; outer FP64 residual
 // r = b - A x and refinement x += d until ||r_c|| <= tol ||b_c||).
+#include <hip/hip_ext.h>
 #include "rb_common.h"
 #include "../../include/rbhip.h"
 #include <vector>
@@ -32,7 +33,7 @@
 #define MAXND 3      // dot products per kernel
 #define MAXCHUNK 8   // 64-column chunks of one launch (C <= 512)
 #define NBMAX 2048   // blocks per launch (grid-stride over rows)
-#define FB 256       // threads of a finish block
+#define FB 512       // threads of a finish block
 #define UNR 8        // CSR entries gathered per step
 
 template <typename A>
@@ -110,23 +111,57 @@ __device__ inline bool block_any_active(const int* active, int C) {
   return s_any != 0;
 }
 
-// finish-block helper: v[d] = sum over producer blocks of part[b][d][c]
+// sum over the 64 lanes of a wavefront; the xor butterfly adds the same two
+// operands on both lanes of every pair, so all lanes end with the same bits
+__device__ inline double wave_sum(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+__device__ inline cplx<double> wave_sum(cplx<double> x) { return mk<double>(wave_sum(x.x), wave_sum(x.y)); }
+
+// finish-block helper: v[d] = sum over producer blocks of part[b][d][c].
+// Every thread issues all of its partial loads (FB apart) before adding, the
+// wavefronts reduce by butterfly and wave 0 adds the FB/64 wave sums in
+// order: a fixed summation order for a given producer grid.
 template <typename A>
 __device__ void finish_reduce(const ColState<A>& st, int nd, int C, int c, A* v) {
-  __shared__ A red[FB];
-  for (int d = 0; d < nd; ++d) {
-    A acc = acc_zero(A());
-    const A* pp = st.part + (int64_t)(d * C + c) * st.nb;
-    for (int b = threadIdx.x; b < st.nb; b += FB) acc = acc_add(acc, pp[b]);
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int o = FB / 2; o > 0; o >>= 1) {
-      if (threadIdx.x < o) red[threadIdx.x] = acc_add(red[threadIdx.x], red[threadIdx.x + o]);
-      __syncthreads();
+  constexpr int NW = FB / 64;
+  constexpr int PER = 4;  // partials per thread and dot held in flight
+  __shared__ A red[MAXND][NW];
+  const int nb = st.nb;
+  const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+  A acc[MAXND];
+#pragma unroll
+  for (int d = 0; d < MAXND; ++d) acc[d] = acc_zero(A());
+  for (int b0 = 0; b0 < nb; b0 += PER * FB) {
+    A x[MAXND][PER];
+#pragma unroll
+    for (int d = 0; d < MAXND; ++d) {
+      const A* pp = st.part + (int64_t)(d * C + c) * nb;
+#pragma unroll
+      for (int u = 0; u < PER; ++u) {
+        const int b = b0 + u * FB + threadIdx.x;
+        x[d][u] = (d < nd && b < nb) ? pp[b] : acc_zero(A());
+      }
     }
-    v[d] = red[0];
-    __syncthreads();
+#pragma unroll
+    for (int d = 0; d < MAXND; ++d)
+#pragma unroll
+      for (int u = 0; u < PER; ++u) acc[d] = acc_add(acc[d], x[d][u]);
   }
+#pragma unroll
+  for (int d = 0; d < MAXND; ++d) {
+    A t = wave_sum(acc[d]);
+    if (lane == 0 && d < nd) red[d][w] = t;
+  }
+  __syncthreads();
+  for (int d = 0; d < nd; ++d) {
+    A t = red[d][0];
+    for (int i = 1; i < NW; ++i) t = acc_add(t, red[d][i]);
+    v[d] = t;
+  }
+  __syncthreads();
 }
 
 // y[ch] = sum_j A_sj x[col_j, lane + ch*G]; UNR column indices and values are
@@ -1255,24 +1290,30 @@ static hipEvent_t prof_event() {
   return e;
 }
 
+// id of the open ProfScope (-1: none).  Launches made through plaunch while
+// a scope is open carry their start/stop events in the dispatch packet
+// itself (hipExtLaunchKernel), so timing adds no packets to the stream.
+static int g_prof_open = -1;
+
 struct ProfScope {
-  int id;
-  hipStream_t s;
-  hipEvent_t a = nullptr, b = nullptr;
-  ProfScope(int id_, hipStream_t s_) : id(id_), s(s_) {
-    if (g_prof) {
-      a = prof_event();
-      b = prof_event();
-      if (a && b) (void)hipEventRecord(a, s);
-    }
+  int prev;
+  ProfScope(int id, hipStream_t) : prev(g_prof_open) {
+    if (g_prof) g_prof_open = id;
   }
-  ~ProfScope() {
-    if (g_prof && a && b) {
-      (void)hipEventRecord(b, s);
-      g_prof_pending[id].push_back({a, b});
-    }
-  }
+  ~ProfScope() { g_prof_open = prev; }
 };
+
+template <typename F, typename... Args>
+static inline void plaunch(F kernel, dim3 grid, dim3 block, uint32_t shm, hipStream_t s, Args... args) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (g_prof && g_prof_open >= 0) {
+    a = prof_event();
+    b = prof_event();
+    if (!a || !b) a = b = nullptr;
+  }
+  hipExtLaunchKernelGGL(kernel, grid, block, shm, s, a, b, 0, args...);
+  if (a) g_prof_pending[g_prof_open].push_back({a, b});
+}
 
 extern "C" int rb_prof_enable(int on) {
   g_prof = on ? 1 : 0;
@@ -1310,7 +1351,7 @@ extern "C" int rb_prof_read(int id, double* total_ms, int64_t* count) {
   return RB_OK;
 }
 
-#define L_INIT(G_, N_, ...) hipLaunchKernelGGL((k_cg_init<T, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
+#define L_INIT(G_, N_, ...) plaunch((k_cg_init<T, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
 // q = A p launch over the solve's columns: the sweep kernel when every
 // 64-column chunk lies in one system and P fits its buffer descriptor
 // (< 2 GiB), on a grid of exactly one resident round of workgroups; the
@@ -1318,11 +1359,11 @@ extern "C" int rb_prof_read(int id, double* total_ms, int64_t* count) {
 #define L_SPMM(G_, N_, ...)                                                                                 \
   do {                                                                                                    \
     if (G_ == 64 && flat)                                                                                 \
-      hipLaunchKernelGGL((k_cg_spmm_sweep<T, N_>), dim3(nbs), dim3(KB), 0, s, __VA_ARGS__);                 \
+      plaunch((k_cg_spmm_sweep<T, N_>), dim3(nbs), dim3(KB), 0, s, __VA_ARGS__);                            \
     else if (nsys == 1)                                                                                   \
-      hipLaunchKernelGGL((k_cg_spmm<T, G_, N_, false>), dim3(nbs), dim3(KB), 0, s, __VA_ARGS__);           \
+      plaunch((k_cg_spmm<T, G_, N_, false>), dim3(nbs), dim3(KB), 0, s, __VA_ARGS__);                      \
     else                                                                                                  \
-      hipLaunchKernelGGL((k_cg_spmm<T, G_, N_, true>), dim3(nbs), dim3(KB), 0, s, __VA_ARGS__);            \
+      plaunch((k_cg_spmm<T, G_, N_, true>), dim3(nbs), dim3(KB), 0, s, __VA_ARGS__);                       \
   } while (0)
 
 // workgroups of kernel fn resident at once on the current device (one
@@ -1372,8 +1413,8 @@ static void spmm_plan(const Csr& m, int nsys, int C, int k, int64_t nb, bool* fl
     if (cap >= NXCD) *nbs = std::min(*nbs, cap);
   }
 }
-#define L_UPD(G_, N_, ...) hipLaunchKernelGGL((k_cg_update<T, TW, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
-#define L_PUP(G_, N_, ...) hipLaunchKernelGGL((k_cg_pupdate<T, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
+#define L_UPD(G_, N_, ...) plaunch((k_cg_update<T, TW, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
+#define L_PUP(G_, N_, ...) plaunch((k_cg_pupdate<T, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
 
 // one CG/COCG solve of all columns; X is the solution (and x0 if use_x0)
 template <typename T>
@@ -1521,7 +1562,7 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
   const int64_t nbo = nb;
   so.nb = (int)nbo;
 #define L_ORES(G_, N_, ...) \
-  hipLaunchKernelGGL((k_outer_resid<T64, T32, G_, N_>), dim3(nbo), dim3(KB), 0, s, __VA_ARGS__)
+  plaunch((k_outer_resid<T64, T32, G_, N_>), dim3(nbo), dim3(KB), 0, s, __VA_ARGS__)
   {
     ProfScope ps(RB_PROF_RESID, s);
     DISPATCH_GN(C, L_ORES, m, A64, B, X, (const T32*)nullptr, use_x0 ? 1 : 0, w.dinv, w.R, C, k,
@@ -1548,7 +1589,7 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
   typedef T64 TW;
 #define L_SPMMRU(G_, N_, ...)                                                                                 \
   do {                                                                                                      \
-    if (G_ == 64) hipLaunchKernelGGL((k_cg_spmm_sweep_ru<T64, T32, N_>), dim3(nbs), dim3(KB), 0, s, __VA_ARGS__); \
+    if (G_ == 64) plaunch((k_cg_spmm_sweep_ru<T64, T32, N_>), dim3(nbs), dim3(KB), 0, s, __VA_ARGS__);      \
   } while (0)
   DISPATCH_GN(C, L_PUP, m.n, w.dinv, w.R, w.P, C, k, si);  // p = z (P is zero)
   RB_CHECK_LAUNCH();
