@@ -726,28 +726,21 @@ __global__ void __launch_bounds__(KB) k_cg_spmm_w64(Csr m, const T* __restrict__
   block_partials<A, 64, NCH>(st, pv, 1, C);
 }
 
-// x += alpha p ; r -= alpha q ; z = dinv r ; partials rz, rr.  Mixed
-// solves (X64 given): X is the 32-bit correction d, and on the 64-column
-// chunks whose residual this iteration's SpMM launch replaced (ruprev) the
-// replacement was taken at x64 + d, so d moves into x64 and restarts at
-// alpha p.
-template <typename T, typename TW, int G, int NCH>
-__global__ void __launch_bounds__(KB) k_cg_update(int64_t n, const T* __restrict__ dinv, T* __restrict__ X,
-                                                  T* __restrict__ R, const T* __restrict__ P,
-                                                  const T* __restrict__ Q, int C, int k, TW* __restrict__ X64,
+// r -= alpha q ; z = dinv r ; partials rz, rr
+template <typename T, int G, int NCH>
+__global__ void __launch_bounds__(KB) k_cg_update(int64_t n, const T* __restrict__ dinv, T* __restrict__ R,
+                                                  const T* __restrict__ Q, int C, int k,
                                                   ColState<typename sc<T>::acc> st) {
   typedef typename sc<T>::acc A;
   if (!block_any_active(st.active, C)) return;
   A pv[NCH][MAXND];
   A al[NCH];
-  bool fold[NCH];
 #pragma unroll
   for (int ch = 0; ch < NCH; ++ch) {
     pv[ch][0] = acc_zero(A());
     pv[ch][1] = acc_zero(A());
     int c = (int)(threadIdx.x % G) + ch * G;
     al[ch] = (c < C) ? st.alpha[c] : acc_zero(A());
-    fold[ch] = X64 && c < C && ru_prev(st)[c >> 6];
   }
   ROW_LOOP(n) {
 #pragma unroll
@@ -757,14 +750,7 @@ __global__ void __launch_bounds__(KB) k_cg_update(int64_t n, const T* __restrict
         int s = c / k;
         int64_t o = row * C + c;
         T r = R[o];
-        const bool move = acc_abs2(al[ch]) != 0.0;
-        if (fold[ch]) {
-          X64[o] = X64[o] + widen(X[o], TW());
-          X[o] = move ? scale_to(P[o], al[ch]) : sc<T>::zero();
-        } else if (move) {
-          X[o] = X[o] + scale_to(P[o], al[ch]);
-        }
-        if (move) {
+        if (acc_abs2(al[ch]) != 0.0) {
           r = r - scale_to(Q[o], al[ch]);
           R[o] = r;
         }
@@ -777,30 +763,57 @@ __global__ void __launch_bounds__(KB) k_cg_update(int64_t n, const T* __restrict
   block_partials<A, G, NCH>(st, pv, 2, C);
 }
 
-// p = dinv r + beta p on active columns
-template <typename T, int G, int NCH>
-__global__ void __launch_bounds__(KB) k_cg_pupdate(int64_t n, const T* __restrict__ dinv, const T* __restrict__ R,
-                                                   T* __restrict__ P, int C, int k,
-                                                   ColState<typename sc<T>::acc> st) {
+// true when some column is active or still moves by alpha p (a column that
+// stopped in this iteration's k_fin_beta takes its last step here)
+template <typename A>
+__device__ inline bool block_any_live(const ColState<A>& st, int C) {
+  __shared__ int s_any;
+  if (threadIdx.x == 0) s_any = 0;
+  __syncthreads();
+  for (int c = threadIdx.x; c < C; c += blockDim.x)
+    if (st.active[c] || acc_abs2(st.alpha[c]) != 0.0) s_any = 1;
+  __syncthreads();
+  return s_any != 0;
+}
+
+// x += alpha p (before p changes), then p = dinv r + beta p on active
+// columns.  Mixed solves (X64 given): X is the 32-bit correction d, and on
+// the 64-column chunks whose residual this iteration's SpMM launch replaced
+// (ru_prev) the replacement was taken at x64 + d, so d moves into x64 and
+// restarts at alpha p.
+template <typename T, typename TW, int G, int NCH>
+__global__ void __launch_bounds__(KB) k_cg_pupdate(int64_t n, const T* __restrict__ dinv, T* __restrict__ X,
+                                                   const T* __restrict__ R, T* __restrict__ P, int C, int k,
+                                                   TW* __restrict__ X64, ColState<typename sc<T>::acc> st) {
   typedef typename sc<T>::acc A;
-  if (!block_any_active(st.active, C)) return;
-  A be[NCH];
-  int act[NCH];
+  if (!block_any_live(st, C)) return;
+  A al[NCH], be[NCH];
+  bool act[NCH], fold[NCH];
 #pragma unroll
   for (int ch = 0; ch < NCH; ++ch) {
     int c = (int)(threadIdx.x % G) + ch * G;
+    al[ch] = (c < C) ? st.alpha[c] : acc_zero(A());
     be[ch] = (c < C) ? st.beta[c] : acc_zero(A());
-    act[ch] = (c < C) ? st.active[c] : 0;
+    act[ch] = c < C && st.active[c];
+    fold[ch] = X64 && st.ru_req && c < C && ru_prev(st)[c >> 6];
   }
   ROW_LOOP(n) {
 #pragma unroll
     for (int ch = 0; ch < NCH; ++ch) {
       int c = lane + ch * G;
-      if (c < C && act[ch]) {
+      if (c < C) {
+        const bool move = acc_abs2(al[ch]) != 0.0;
+        if (!(move || act[ch] || fold[ch])) continue;
         int s = c / k;
         int64_t o = row * C + c;
-        T z = dinv[(int64_t)s * n + row] * R[o];
-        P[o] = z + scale_to(P[o], be[ch]);
+        T p = (move || act[ch]) ? P[o] : sc<T>::zero();
+        if (fold[ch]) {
+          X64[o] = X64[o] + widen(X[o], TW());
+          X[o] = move ? scale_to(p, al[ch]) : sc<T>::zero();
+        } else if (move) {
+          X[o] = X[o] + scale_to(p, al[ch]);
+        }
+        if (act[ch]) P[o] = dinv[(int64_t)s * n + row] * R[o] + scale_to(p, be[ch]);
       }
     }
   }
@@ -1208,6 +1221,7 @@ static ColState<A> state_at(char* base, const size_t* o, int64_t nblocks) {
   s.rho_prev = (A*)(base + o[10]);
   s.rmax2 = (double*)(base + o[11]);
   s.ru_req = nullptr;
+  s.ru_par = 0;
   s.nb = (int)nblocks;
   s.tol2 = 0;
   s.maxiter = 0;
@@ -1413,8 +1427,8 @@ static void spmm_plan(const Csr& m, int nsys, int C, int k, int64_t nb, bool* fl
     if (cap >= NXCD) *nbs = std::min(*nbs, cap);
   }
 }
-#define L_UPD(G_, N_, ...) plaunch((k_cg_update<T, TW, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
-#define L_PUP(G_, N_, ...) plaunch((k_cg_pupdate<T, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
+#define L_UPD(G_, N_, ...) plaunch((k_cg_update<T, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
+#define L_PUP(G_, N_, ...) plaunch((k_cg_pupdate<T, TW, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
 
 // one CG/COCG solve of all columns; X is the solution (and x0 if use_x0)
 template <typename T>
@@ -1449,13 +1463,13 @@ static int run_cg(const Csr& m, int nsys, const T* vals, const T* dinv, const T*
       hipLaunchKernelGGL(k_fin_alpha<A>, dim3(C), dim3(FB), 0, s, C, ss);
       {
         ProfScope ps(RB_PROF_UPDATE, s);
-        typedef T TW;
-        DISPATCH_GN(C, L_UPD, m.n, dinv, X, w.R, w.P, w.Q, C, k, (TW*)nullptr, st);
+        DISPATCH_GN(C, L_UPD, m.n, dinv, w.R, w.Q, C, k, st);
       }
       hipLaunchKernelGGL(k_fin_beta<A>, dim3(C), dim3(FB), 0, s, C, st);
       {
         ProfScope ps(RB_PROF_PUPDATE, s);
-        DISPATCH_GN(C, L_PUP, m.n, dinv, w.R, w.P, C, k, st);
+        typedef T TW;
+        DISPATCH_GN(C, L_PUP, m.n, dinv, X, w.R, w.P, C, k, (TW*)nullptr, st);
       }
     }
     RB_CHECK_LAUNCH();
@@ -1591,7 +1605,7 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
   do {                                                                                                      \
     if (G_ == 64) plaunch((k_cg_spmm_sweep_ru<T64, T32, N_>), dim3(nbs), dim3(KB), 0, s, __VA_ARGS__);      \
   } while (0)
-  DISPATCH_GN(C, L_PUP, m.n, w.dinv, w.R, w.P, C, k, si);  // p = z (P is zero)
+  DISPATCH_GN(C, L_PUP, m.n, w.dinv, D, w.R, w.P, C, k, X, si);  // p = z (P is zero, alpha 0, no flags)
   RB_CHECK_LAUNCH();
   int nact = 0;
   rc = poll_nactive(si.active, C, si.nactive, s, &nact);
@@ -1615,12 +1629,12 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
       hipLaunchKernelGGL(k_fin_alpha_ru<A>, dim3(C), dim3(FB), 0, s, C, tol2, ss, sol);
       {
         ProfScope ps(RB_PROF_UPDATE, s);
-        DISPATCH_GN(C, L_UPD, m.n, w.dinv, D, w.R, w.P, w.Q, C, k, X, si);
+        DISPATCH_GN(C, L_UPD, m.n, w.dinv, w.R, w.Q, C, k, si);
       }
       hipLaunchKernelGGL(k_fin_beta<A>, dim3(C), dim3(FB), 0, s, C, si);
       {
         ProfScope ps(RB_PROF_PUPDATE, s);
-        DISPATCH_GN(C, L_PUP, m.n, w.dinv, w.R, w.P, C, k, si);
+        DISPATCH_GN(C, L_PUP, m.n, w.dinv, D, w.R, w.P, C, k, X, si);
       }
     }
     RB_CHECK_LAUNCH();
