@@ -262,6 +262,284 @@ def sdmap(cfg, maxdist=np.inf, **kw):
     return np.column_stack([pairs, dist_active()])
 
 
+# ---------------------------------------------------------------------------
+# wide-field sources and detectors (utility.py:252-669)
+# ---------------------------------------------------------------------------
+_TET_EDGES = ((0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3))
+
+
+def qmeshcut(elem, node, plane):
+    """Plane cut of a tet mesh: iso2mesh.qmeshcut(elem, node, value, plane), called at utility.py:454-479.
+
+    iso2mesh (>= 0.5.0) is not installed; the published algorithm is
+    restated.  Signed distance d = a x + b y + c z + e for plane [a b c e];
+    a node with d >= 0 counts as above.  Every mesh edge (n1 < n2) whose ends
+    lie on opposite sides gives one cut point w*p1 + (1-w)*p2 with
+    w = d2 / (d2 - d1).  Every crossing tet gives one polygon: a triangle
+    (one node alone on its side; corners in the local order of the other
+    three) or a quad (two and two: pos {a<b}, neg {c<d}, corners ac, ad, bd,
+    bc, i.e. around the polygon).  Polygons are in tet order.
+
+    Returns (cutpos (npt x 3), facedata (npoly x 4, 1-based; col 2 == col 3
+    for a triangle), elemid (npoly, 1-based), nodeid (npt x 3: n1, n2
+    1-based, w + 1)).  The weight column carries the +1 offset of the Python
+    iso2mesh port, which the reference removes at utility.py:610-612.
+    Exact zeros of d (nodes on the plane) follow the d >= 0 rule above:
+    parity unpinned for such cuts.
+    """
+    e = np.asarray(elem)[:, :4].astype(np.int64) - 1
+    node = np.asarray(node, dtype=float)
+    d = node @ np.asarray(plane[:3], dtype=float) + float(plane[3])
+    up = d[e] >= 0
+    nup = up.sum(axis=1)
+    tets = np.flatnonzero((nup > 0) & (nup < 4))
+    ecut = e[tets]
+    uc = up[tets]
+    # global cut edges of every crossing tet (local edge k of tet t)
+    pairs = []
+    for a, b in _TET_EDGES:
+        m = uc[:, a] != uc[:, b]
+        n1 = np.minimum(ecut[:, a], ecut[:, b])
+        n2 = np.maximum(ecut[:, a], ecut[:, b])
+        pairs.append((m, n1, n2))
+    allp = np.concatenate([np.stack([n1[m], n2[m]], 1) for m, n1, n2 in pairs]) if len(tets) else np.zeros((0, 2), int)
+    uniq, inv = np.unique(allp, axis=0, return_inverse=True)
+    inv = np.asarray(inv).reshape(-1)
+    d1, d2 = d[uniq[:, 0]], d[uniq[:, 1]]
+    w = d2 / (d2 - d1)
+    cutpos = node[uniq[:, 0]] * w[:, None] + node[uniq[:, 1]] * (1 - w)[:, None]
+    # point id of local edge k of each crossing tet (-1 when not cut)
+    pid = np.full((len(tets), 6), -1, dtype=np.int64)
+    o = 0
+    for k, (m, _, _) in enumerate(pairs):
+        cnt = int(m.sum())
+        pid[m, k] = inv[o : o + cnt]
+        o += cnt
+    eidx = {ab: k for k, ab in enumerate(_TET_EDGES)}
+
+    def edge(i, j):
+        return eidx[(min(i, j), max(i, j))]
+
+    face = np.zeros((len(tets), 4), dtype=np.int64)
+    for r in range(len(tets)):
+        u = uc[r]
+        if nup[tets[r]] in (1, 3):
+            lone = int(np.flatnonzero(u == (nup[tets[r]] == 1))[0])
+            others = [q for q in range(4) if q != lone]
+            c = [pid[r, edge(lone, q)] for q in others]
+            face[r] = [c[0], c[1], c[2], c[2]]
+        else:
+            a, b = np.flatnonzero(u)
+            c, dd = np.flatnonzero(~u)
+            face[r] = [pid[r, edge(a, c)], pid[r, edge(a, dd)], pid[r, edge(b, dd)], pid[r, edge(b, c)]]
+    nodeid = np.column_stack([uniq + 1, w + 1.0])
+    return cutpos, face + 1, tets + 1, nodeid
+
+
+def meshcentroid(node, face):
+    """iso2mesh.meshcentroid (utility.py:482): mean of each element's vertices (1-based face)."""
+    f = np.asarray(face).astype(np.int64) - 1
+    return np.asarray(node, dtype=float)[f].mean(axis=1)
+
+
+def rotatevec3d(pts, v):
+    """iso2mesh.rotatevec3d(pts, v) (utility.py:485-486): rotate so that direction v maps onto +z.
+
+    Rodrigues rotation about v x z by the angle between v and z (a half turn
+    about x when v points along -z).  The reference only uses the rotated
+    x, y coordinates for a point-in-polygon test and projections onto the
+    rotated polygon's edges, which every rotation taking v to +z leaves
+    unchanged.
+    """
+    v = np.asarray(v, dtype=float)[:3]
+    v = v / np.linalg.norm(v)
+    z = np.array([0.0, 0.0, 1.0])
+    ax = np.cross(v, z)
+    s, c = np.linalg.norm(ax), float(v @ z)
+    if s < 1e-15:
+        R = np.eye(3) if c > 0 else np.diag([1.0, -1.0, -1.0])
+    else:
+        k = ax / s
+        K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+        R = np.eye(3) + s * K + (1 - c) * (K @ K)
+    return np.asarray(pts, dtype=float) @ R.T
+
+
+def inpolygon(x, y, px, py):
+    """utility.py:639-669 -- even-odd ray casting (the reference's fallback branch)."""
+    x = np.asarray(x, dtype=float)
+    y = np.asarray(y, dtype=float)
+    inside = np.zeros(x.shape, dtype=bool)
+    for i in range(len(px) - 1):
+        x1, y1, x2, y2 = px[i], py[i], px[i + 1], py[i + 1]
+        hit = ((y1 <= y) & (y < y2)) | ((y2 <= y) & (y < y1))
+        if np.any(hit):
+            xc = (x2 - x1) * (y[hit] - y1) / (y2 - y1 + 1e-30) + x1
+            inside[hit] ^= x[hit] < xc
+    return inside
+
+
+def _planar_source(cfg, srcpos, srcdir, p1, p2, srctype, pattern, z0):
+    """utility.py:423-577 -- illuminated faces and per-pattern face weights."""
+    p1 = np.asarray(p1, dtype=float)
+    p2 = np.asarray(p2, dtype=float)
+    srcpos = np.asarray(srcpos, dtype=float)[:3]
+    srcdir = np.asarray(srcdir, dtype=float)[:3]
+    poly = np.array([srcpos, srcpos + p1[:3], srcpos + p1[:3] + p2[:3], srcpos + p2[:3], srcpos])
+    node = np.asarray(cfg["node"], dtype=float)
+    nodeid = None
+    sink = cfg.get("iscolimated", True)
+    if sink:
+        plane = np.concatenate([srcdir, [-(srcdir @ (srcpos + srcdir * z0))]])
+        pnode, fd, _, nodeid = qmeshcut(cfg["elem"], node, plane)
+        tri = fd[:, 2] == fd[:, 3]
+        q = np.flatnonzero(~tri)
+        pface = np.vstack([fd[tri, :3], fd[q][:, [0, 1, 2]], fd[q][:, [0, 2, 3]]]).astype(np.int64)
+        parea = elemvolume(pnode, pface)
+    else:
+        pnode, pface, parea = node, np.asarray(cfg["face"]).astype(np.int64), np.asarray(cfg["area"])
+    c0 = meshcentroid(pnode, pface)
+    rot = rotatevec3d(np.vstack([c0, poly]), srcdir)
+    spoly = rot[-5:, :2]
+    cen = rot[:-5, :2]
+    idx = np.flatnonzero(inpolygon(cen[:, 0], cen[:, 1], spoly[:, 0], spoly[:, 1]))
+    if len(idx) == 0:
+        raise ValueError("Source direction does not intersect with the domain")
+    f0 = pface - 1
+    nrm = np.cross(pnode[f0[idx, 1]] - pnode[f0[idx, 0]], pnode[f0[idx, 2]] - pnode[f0[idx, 0]])
+    dd = nrm @ srcdir
+    if sink:
+        dd = -np.abs(dd)
+    if np.all(dd >= 0):
+        raise ValueError("Please reorient the surface triangles")
+    good = idx[dd < 0]
+    dp = cen[good] - spoly[0]
+    dx, dy = spoly[1] - spoly[0], spoly[3] - spoly[0]
+    lx, ly = np.linalg.norm(dx), np.linalg.norm(dy)
+    bary = np.clip(np.column_stack([dp @ (dx / lx) / lx, dp @ (dy / ly) / ly]), 0, 1 - 1e-6)
+    npat = 1
+    bc = np.zeros((1, len(pface)))
+    bc[0, good] = 1.0
+    if pattern is not None and srctype == "pattern":
+        pat = np.asarray(pattern, dtype=float)
+        if pat.ndim == 2:
+            pat = pat[:, :, None]
+        npat = pat.shape[2]
+        bc = np.zeros((npat, len(pface)))
+        ix = np.clip((bary[:, 0] * pat.shape[1]).astype(int), 0, pat.shape[1] - 1)
+        iy = np.clip((bary[:, 1] * pat.shape[0]).astype(int), 0, pat.shape[0] - 1)
+        for i in range(npat):
+            bc[i, good] = pat[iy, ix, i]
+    elif srctype == "fourier":
+        kx, ky = int(p1[3]), int(p2[3])
+        phi0 = (p1[3] - kx) * 2 * np.pi
+        M = 1 - (p2[3] - ky)
+        npat = kx * ky
+        bc = np.zeros((npat, len(pface)))
+        for i in range(kx):
+            for j in range(ky):
+                bc[i * ky + j, good] = 0.5 * (1 + M * np.cos((i * bary[:, 0] + j * bary[:, 1]) * 2 * np.pi + phi0))
+    return bc, npat, pface, parea, pnode, nodeid, sink
+
+
+def _bc_weighting(cfg, bc, Reff, weight, pface, parea, nodeid, sink):
+    """utility.py:580-636 -- boundary flux to nodal values, L1-normalised per pattern; (npat, nn)."""
+    nn = np.asarray(cfg["node"]).shape[0]
+    npat = bc.shape[0]
+    wA = (parea * ((1 - Reff) / (18 * (1 + Reff))))[:, None] * bc.T
+    rhs = np.zeros((nn, npat))
+    f0 = pface - 1
+    if nodeid is not None and sink:
+        w1 = nodeid[:, 2] - 1.0
+        ends = nodeid[:, :2].astype(np.int64) - 1
+        for i in range(npat):
+            for j in range(3):
+                v = f0[:, j]
+                np.add.at(rhs[:, i], ends[v, 0], wA[:, i] * w1[v])
+                np.add.at(rhs[:, i], ends[v, 1], wA[:, i] * (1 - w1[v]))
+    else:
+        for i in range(npat):
+            for j in range(3):
+                np.add.at(rhs[:, i], f0[:, j], wA[:, i])
+    for i in range(npat):
+        ws = 1.0
+        if weight is not None:
+            if isinstance(weight, (int, float)):
+                ws = weight
+            elif len(weight) == npat:
+                ws = weight[i]
+        tot = np.sum(np.abs(rhs[:, i]))
+        if tot > 0:
+            rhs[:, i] *= ws / tot
+    return rhs.T
+
+
+def src2bc(cfg, isdet=False):
+    """utility.py:252-420 -- wide-field sources (or detectors) as nodal boundary-flux columns.
+
+    Adds cfg['widesrc'] (nn x npattern, patterns of every wavelength stacked),
+    cfg['wfsrcmapping'] and drops the wide-field entries from cfg['srcpos']
+    (or the det* counterparts).
+    """
+    if not isdet:
+        tk, pk, dk, a1, a2, patk, wk, idk, widk, outk, mapk = (
+            "srctype", "srcpos", "srcdir", "srcparam1", "srcparam2", "srcpattern", "srcweight", "srcid",
+            "widesrcid", "widesrc", "wfsrcmapping")
+    else:
+        tk, pk, dk, a1, a2, patk, wk, idk, widk, outk, mapk = (
+            "dettype", "detpos", "detdir", "detparam1", "detparam2", "detpattern", "detweight", "detid",
+            "widedetid", "widedet", "wfdetmapping")
+    stype = cfg.get(tk, "pencil")
+    if stype in ("pencil", "isotropic") and widk not in cfg:
+        return cfg
+    dirs = np.atleast_2d(cfg[dk])
+    pos = np.atleast_2d(cfg[pk])
+    if widk in cfg:
+        wid = cfg[widk] if isinstance(cfg[widk], dict) else {"": cfg[widk]}
+    else:
+        one = {"srctype": [stype], "srcid": [cfg.get(idk, 0)], "srcparam1": [np.atleast_1d(cfg[a1])],
+               "srcparam2": [np.atleast_1d(cfg[a2])]}
+        if patk in cfg:
+            one["srcpattern"] = [cfg[patk]]
+        if wk in cfg:
+            one["srcweight"] = [cfg[wk]]
+        wid = {"": one}
+    prop = cfg["prop"] if isinstance(cfg.get("prop"), dict) else {"": cfg["prop"]}
+    if set(wid) != set(prop) and "" in wid:
+        wid = {w: wid[""] for w in prop}
+    cols, mapping, ids = [], {}, []
+    for wv in prop:
+        par = wid[wv]
+        op = np.asarray(prop[wv])
+        z0 = 1.0 / (op[0] + op[1] * (1 - op[2])) if op.ndim == 1 else 1.0 / (op[1, 0] + op[1, 1] * (1 - op[1, 2]))
+        mp = []
+        for k in range(len(par["srcid"])):
+            sid = par["srcid"][k]
+            ids.append(sid)
+            st = par["srctype"][k]
+            pat = par["srcpattern"][k] if (st == "pattern" and "srcpattern" in par) else None
+            wgt = par["srcweight"][k] if "srcweight" in par else None
+            sdir = dirs[0, :3] if dirs.shape[0] == 1 else dirs[sid, :3]
+            if st not in ("planar", "pattern", "fourier"):
+                raise ValueError("Source type '%s' is not supported" % st)
+            bc, npat, pface, parea, pnode, nodeid, sink = _planar_source(
+                cfg, pos[sid, :3], sdir, par["srcparam1"][k], par["srcparam2"][k], st, pat, z0)
+            reff = cfg["reff"][wv] if isinstance(cfg["reff"], dict) else cfg["reff"]
+            rhs = _bc_weighting(cfg, bc, reff, wgt, pface, parea, nodeid, sink)
+            mp.append([sid, len(cols), len(cols) + npat - 1])
+            cols.append(rhs)
+        mapping[wv] = np.array(mp) if mp else np.array([])
+    wide = np.vstack(cols) if cols else np.array([])
+    keep = np.ones(pos.shape[0], dtype=bool)
+    keep[list(set(ids))] = False
+    if len(mapping) == 1:
+        mapping = mapping[list(prop)[0]]
+    cfg[outk] = wide.T if wide.size else np.array([])
+    cfg[mapk] = mapping
+    cfg[pk] = pos[keep]
+    return cfg
+
+
 def meshprep(cfg):
     """utility.py:101-246 -- derived mesh fields (fallback branch, no iso2mesh)."""
     cfg = forcearray(
@@ -302,6 +580,12 @@ def meshprep(cfg):
         else:
             cfg["reff"] = getreff(bk[3], 1.0)
             cfg["musp0"] = bk[1] * (1 - bk[2])
+    if (cfg.get("srctype", "pencil") not in ("pencil", "isotropic") or "widesrcid" in cfg) and "widesrc" not in cfg:
+        cfg["srcpos0"] = cfg["srcpos"].copy()
+        cfg = src2bc(cfg, isdet=False)
+    if (cfg.get("dettype", "pencil") not in ("pencil", "isotropic") or "widedetid" in cfg) and "widedet" not in cfg:
+        cfg["detpos0"] = cfg["detpos"].copy()
+        cfg = src2bc(cfg, isdet=True)
     if cfg.get("cols") is None:
         cfg["rows"], cfg["cols"], cfg["idxcount"] = femnz(cfg["elem"][:, :4], node.shape[0])
     if cfg.get("idxsum") is None:

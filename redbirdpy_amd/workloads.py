@@ -11,6 +11,9 @@
   sources, odd rows detectors.
 * ``jac_box``: the 1 mm box and optodes in CW with homogeneous
   [0.01, 1.0, 0, 1.37] (1024 pairs x 115,351 nodes).
+* ``widefield_box``: ``cw_box`` with its source turned into a wide-field
+  one: a 20 x 20 mm rectangle from (30,30,0) (planar, pattern or fourier
+  with kx, ky in the 4th parameter entries), collimated along +z.
 * ``chrome_box``: the 1 mm box at 4 wavelengths with 3 chromophores whose
   4x3 extinction matrix is uniform in [5e-4, 5e-3] /mm/uM (seeded), node
   concentrations 20/10/0.5 doubled inside the sphere.
@@ -56,6 +59,16 @@ def cw_box(h=2.0, box=BOX):
         "detdir": np.array([[0.0, 0.0, 1.0]]),
         "omega": 0,
     }
+
+
+def widefield_box(h=2.0, srctype="planar", kx=0, ky=0, pattern=None, box=BOX):
+    cfg = cw_box(h, box)
+    cfg["srctype"] = srctype
+    cfg["srcparam1"] = np.array([20.0, 0.0, 0.0, float(kx)])
+    cfg["srcparam2"] = np.array([0.0, 20.0, 0.0, float(ky)])
+    if pattern is not None:
+        cfg["srcpattern"] = np.asarray(pattern, dtype=float)
+    return cfg
 
 
 def fd_box(h=1.0, box=BOX, n_lattice=8, omega=OMEGA_100MHZ):
