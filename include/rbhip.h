@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define RB_ABI_VERSION 1
+#define RB_ABI_VERSION 2
 
 /* property layouts for rb_assemble */
 #define RB_PROP_ELEM 0   /* one row per element (label tables are expanded by the caller) */
@@ -91,6 +91,38 @@ int rb_assemble(const int32_t* slot_ptr, const int32_t* codes, int64_t nnz, cons
                 const double* area, int nsys, int kind, const double* mua, const double* musp, const double* nref,
                 int64_t plen, int64_t nlen, const double* omega, const double* bc, int is_complex, void* out64,
                 void* out32, void* stream);
+
+/* wide-field sources (rb_widefield) */
+#define RB_WF_SINK 0    /* collimated: the cut of the mesh by the sink plane */
+#define RB_WF_FACES 1   /* the boundary faces facing the source */
+#define RB_WF_UNIFORM 0 /* planar source: one uniform pattern */
+#define RB_WF_FOURIER 1 /* kx * ky Fourier patterns */
+#define RB_WF_IMAGE 2   /* patterns from a pdim0 x pdim1 x npat image */
+#define RB_WF_FRAME 34  /* doubles in the frame array */
+
+/*
+ * Wide-field source or detector: nodal boundary-flux columns of one planar,
+ * pattern or Fourier source (replaces redbirdpy utility.py:423-577
+ * `_process_planar_source`, with the iso2mesh plane cut `qmeshcut` at
+ * :454-479, and :580-634 `_apply_bc_weighting` up to its per-pattern L1
+ * normalisation, which the caller applies).  All arrays are device arrays
+ * except `frame` (host, RB_WF_FRAME doubles):
+ *   [0:3] source direction, [3:6] e1, [6:9] e2 (rows 1, 2 of a rotation
+ *   taking the direction to +z), [9:14] / [14:19] x / y of the closed source
+ *   rectangle in (e1, e2) coordinates, [19:21] its first corner, [21:23] /
+ *   [23:25] unit directions of its two edges, [25] / [26] their lengths,
+ *   [27:31] sink plane (a, b, c, e: a x + b y + c z + e = 0),
+ *   [31] (1 - Reff) / (18 (1 + Reff)), [32] Fourier depth M, [33] phase.
+ * node nn x 3; el ne x 4 with node-to-tet incidence tinc_ptr/tinc (RB_WF_SINK);
+ * fc nf x 3, area nf with node-to-face incidence finc_ptr/finc (RB_WF_FACES);
+ * img pdim0 x pdim1 x npat (RB_WF_IMAGE, else NULL); rhs nn x npat (written
+ * whole); counts[2] = triangles whose centroid lies in the rectangle, and
+ * those of them facing the source (the reference raises when either is 0).
+ */
+int rb_widefield(int64_t nn, const double* node, const int32_t* el, int64_t ne, const int32_t* tinc_ptr,
+                 const int32_t* tinc, const int32_t* fc, int64_t nf, const double* area, const int32_t* finc_ptr,
+                 const int32_t* finc, int mode, const double* frame, int kind, int npat, int kx, int ky,
+                 const double* img, int pdim0, int pdim1, double* rhs, unsigned long long* counts, void* stream);
 
 /*
  * Point location (iso2mesh.tsearchn as called by redbirdpy

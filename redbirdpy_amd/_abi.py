@@ -20,6 +20,9 @@ RB_PREC_FP64 = 0
 RB_PREC_MIXED = 1
 RB_F64, RB_C128, RB_F32, RB_C64 = 0, 1, 2, 3
 RB_PROF_SPMM, RB_PROF_UPDATE, RB_PROF_PUPDATE, RB_PROF_RESID, RB_PROF_INIT = 0, 1, 2, 3, 4
+RB_WF_SINK, RB_WF_FACES = 0, 1
+RB_WF_UNIFORM, RB_WF_FOURIER, RB_WF_IMAGE = 0, 1, 2
+RB_WF_FRAME = 34
 
 _i32 = ctypes.c_int32
 _i64 = ctypes.c_int64
@@ -41,6 +44,8 @@ _SIGS = {
                     _i64, _i64, _vp, _vp, _int, _vp, _vp, _vp],
     "rb_tsearch": [_vp, _vp, _i64, _vp, _int, _dbl, _vp, _vp, _vp],
     "rb_rhs_points": [_vp, _vp, _vp, _int, _vp, _i64, _int, _vp, _vp],
+    "rb_widefield": [_i64, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _int, _vp, _int, _int, _int, _int,
+                     _vp, _int, _int, _vp, _vp, _vp],
     "rb_krylov_workspace": [_i64, _int, _int, _int, _int, ctypes.POINTER(_sz)],
     "rb_krylov_solve": [_i64, _i64, _vp, _vp, _vp, _int, _int, _int, _vp, _vp, _vp, _vp, _int, _dbl, _int,
                         _int, _int, _vp, _sz, ctypes.POINTER(_int), ctypes.POINTER(_dbl), ctypes.POINTER(_int),

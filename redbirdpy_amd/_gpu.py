@@ -394,6 +394,26 @@ def solve_system(sysm, B, k, precision="fp64", tol=1e-12, maxiter=2000, X0=None)
 # ---------------------------------------------------------------------------
 # optodes and right-hand sides (redbirdpy forward.py:259-362)
 # ---------------------------------------------------------------------------
+def widefield(mesh, mode, frame, kind, npat, kx=0, ky=0, img=None, area=None):
+    """Un-normalised wide-field columns (nn x npat, device numbering) and the
+    (inside, inside and facing) triangle counts of one source (rb_widefield)."""
+    dev = mesh.dev
+    rhs = torch.empty((mesh.nn, npat), dtype=torch.float64, device=dev)
+    counts = torch.zeros(2, dtype=torch.int64, device=dev)
+    fr = (ctypes.c_double * _abi.RB_WF_FRAME)(*[float(v) for v in frame])
+    imgt, pd0, pd1 = None, 0, 0
+    if img is not None:
+        imgt = _f64(np.ascontiguousarray(img, dtype=np.float64), dev)
+        pd0, pd1 = int(img.shape[0]), int(img.shape[1])
+    ar = mesh.area
+    if mode == _abi.RB_WF_FACES and ar is None and area is not None:
+        ar = _f64(area, dev)
+    call("rb_widefield", mesh.nn, ptr(mesh.node), ptr(mesh.el), mesh.ne, ptr(mesh.tinc_ptr), ptr(mesh.tinc),
+         ptr(mesh.fc), mesh.nf, ptr(ar), ptr(mesh.finc_ptr), ptr(mesh.finc), int(mode), ctypes.addressof(fr),
+         int(kind), int(npat), int(kx), int(ky), ptr(imgt), pd0, pd1, ptr(rhs), ptr(counts), _abi.stream_ptr())
+    return rhs, counts.cpu().numpy()
+
+
 def tsearch(mesh, pts, tol=1e-10):
     """GPU point location; returns (loc0 int64 with -1 outside, bary (npts,4))."""
     pts = np.atleast_2d(np.asarray(pts, dtype=np.float64))[:, :3]

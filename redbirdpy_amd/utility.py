@@ -15,6 +15,7 @@ __all__ = [
     "getreff",
     "elem2node",
     "forcearray",
+    "src2bc",
 ]
 
 import numpy as np
@@ -261,6 +262,153 @@ def _connectivity(cfg):
     return rows[off], indices[off], np.bincount(rows[off], minlength=mesh.nn)
 
 
+# ---------------------------------------------------------------------------
+# wide-field sources and detectors (utility.py:252-669)
+# ---------------------------------------------------------------------------
+def _rotation_to_z(v):
+    """3x3 rotation taking direction v onto +z (about v x z; a half turn about x for v along -z)."""
+    v = np.asarray(v, dtype=float)[:3]
+    v = v / np.linalg.norm(v)
+    ax = np.array([v[1], -v[0], 0.0])  # v x z
+    s, c = np.linalg.norm(ax), v[2]
+    if s < 1e-15:
+        return np.eye(3) if c > 0 else np.diag([1.0, -1.0, -1.0])
+    k = ax / s
+    K = np.array([[0.0, -k[2], k[1]], [k[2], 0.0, -k[0]], [-k[1], k[0], 0.0]])
+    return np.eye(3) + s * K + (1 - c) * (K @ K)
+
+
+def _wide_columns(cfg, srcpos, srcdir, p1, p2, stype, pattern, z0, reff, weight):
+    """(npat, nn) normalised columns of one planar / pattern / Fourier source (utility.py:423-636).
+
+    The geometry (sink-plane cut or boundary faces, rectangle test, pattern
+    lookup, flux split over the cut edges) runs on the GPU (rb_widefield);
+    the per-pattern L1 normalisation follows (utility.py:622-634).
+    """
+    from . import _gpu, _abi
+
+    p1 = np.asarray(p1, dtype=float)
+    p2 = np.asarray(p2, dtype=float)
+    srcpos = np.asarray(srcpos, dtype=float)[:3]
+    srcdir = np.asarray(srcdir, dtype=float)[:3]
+    corners = np.array([srcpos, srcpos + p1[:3], srcpos + p1[:3] + p2[:3], srcpos + p2[:3], srcpos])
+    R = _rotation_to_z(srcdir)
+    poly = corners @ R[:2].T
+    ex, ey = poly[1] - poly[0], poly[3] - poly[0]
+    lx, ly = np.linalg.norm(ex), np.linalg.norm(ey)
+    sink = cfg.get("iscolimated", True)
+    plane = np.concatenate([srcdir, [-(srcdir @ (srcpos + srcdir * z0))]]) if sink else np.zeros(4)
+    kind, npat, kx, ky, M, phi0, img = _abi.RB_WF_UNIFORM, 1, 0, 0, 0.0, 0.0, None
+    if stype == "pattern" and pattern is not None:
+        img = np.asarray(pattern, dtype=float)
+        if img.ndim == 2:
+            img = img[:, :, None]
+        kind, npat = _abi.RB_WF_IMAGE, img.shape[2]
+    elif stype == "fourier":
+        kx, ky = int(p1[3]), int(p2[3])
+        phi0 = (p1[3] - kx) * 2 * np.pi
+        M = 1 - (p2[3] - ky)
+        kind, npat = _abi.RB_WF_FOURIER, kx * ky
+    nn = np.asarray(cfg["node"]).shape[0]
+    if npat == 0:
+        return np.zeros((0, nn))
+    frame = np.concatenate([srcdir, R[0], R[1], poly[:, 0], poly[:, 1], poly[0], ex / lx, ey / ly, [lx, ly],
+                            plane, [(1 - reff) / (18 * (1 + reff)), M, phi0]])
+    mesh = _gpu.mesh_for(cfg)
+    rhs, counts = _gpu.widefield(mesh, _abi.RB_WF_SINK if sink else _abi.RB_WF_FACES, frame, kind, npat, kx, ky,
+                                 img, area=cfg.get("area"))
+    if counts[0] == 0:
+        raise ValueError("Source direction does not intersect with the domain")
+    if counts[1] == 0:
+        raise ValueError("Please reorient the surface triangles")
+    ws = np.ones(npat)
+    if weight is not None:
+        if isinstance(weight, (int, float)):
+            ws[:] = weight
+        elif len(weight) == npat:
+            ws = np.asarray(weight, dtype=float)
+    import torch
+
+    tot = rhs.abs().sum(0)
+    one = torch.ones_like(tot)
+    wst = torch.as_tensor(ws, dtype=rhs.dtype, device=rhs.device)
+    rhs = rhs * torch.where(tot > 0, wst / torch.where(tot > 0, tot, one), one)
+    return mesh.user_rows(rhs).cpu().numpy().T
+
+
+def src2bc(cfg, isdet=False):
+    """Wide-field sources (or detectors) as nodal boundary-flux columns (utility.py:252-420).
+
+    Adds cfg['widesrc'] (nn x npattern; the patterns of every wavelength,
+    stacked), cfg['wfsrcmapping'] ([srcid, first, last] column per source,
+    per wavelength) and removes the wide-field entries from cfg['srcpos'];
+    isdet=True does the same for the det* fields.  Types: planar, pattern
+    (cfg['srcpattern'], ny x nx [x npattern]) and fourier (kx, ky in the 4th
+    entries of srcparam1/2, integer part; phase and depth in the fraction).
+    A collimated source (cfg['iscolimated'], default True) is sunk to 1/mus'
+    below the surface along its direction.
+    """
+    if not isdet:
+        tk, pk, dk, a1, a2, patk, wk, idk, widk, outk, mapk = (
+            "srctype", "srcpos", "srcdir", "srcparam1", "srcparam2", "srcpattern", "srcweight", "srcid",
+            "widesrcid", "widesrc", "wfsrcmapping")
+    else:
+        tk, pk, dk, a1, a2, patk, wk, idk, widk, outk, mapk = (
+            "dettype", "detpos", "detdir", "detparam1", "detparam2", "detpattern", "detweight", "detid",
+            "widedetid", "widedet", "wfdetmapping")
+    stype = cfg.get(tk, "pencil")
+    if stype in ("pencil", "isotropic") and widk not in cfg:
+        return cfg
+    dirs = np.atleast_2d(cfg[dk])
+    pos = np.atleast_2d(cfg[pk])
+    if widk in cfg:
+        wid = cfg[widk] if isinstance(cfg[widk], dict) else {"": cfg[widk]}
+    else:
+        one = {"srctype": [stype], "srcid": [cfg.get(idk, 0)], "srcparam1": [np.atleast_1d(cfg[a1])],
+               "srcparam2": [np.atleast_1d(cfg[a2])]}
+        if patk in cfg:
+            one["srcpattern"] = [cfg[patk]]
+        if wk in cfg:
+            one["srcweight"] = [cfg[wk]]
+        wid = {"": one}
+    prop = cfg["prop"] if isinstance(cfg.get("prop"), dict) else {"": cfg["prop"]}
+    if set(wid) != set(prop) and "" in wid:
+        wid = {w: wid[""] for w in prop}
+    blocks, mapping, used = [], {}, []
+    ncol = 0
+    for wv in prop:
+        par = wid[wv]
+        op = np.asarray(prop[wv])
+        # sink depth 1/mus' of the first label row (of the single row for 1-D props)
+        z0 = 1.0 / (op[0] + op[1] * (1 - op[2])) if op.ndim == 1 else 1.0 / (op[1, 0] + op[1, 1] * (1 - op[1, 2]))
+        rows = []
+        for k in range(len(par["srcid"])):
+            sid = par["srcid"][k]
+            used.append(sid)
+            st = par["srctype"][k]
+            if st not in ("planar", "pattern", "fourier"):
+                raise ValueError("Source type '%s' is not supported" % st)
+            pat = par["srcpattern"][k] if (st == "pattern" and "srcpattern" in par) else None
+            wgt = par["srcweight"][k] if "srcweight" in par else None
+            sdir = dirs[0, :3] if dirs.shape[0] == 1 else dirs[sid, :3]
+            reff = cfg["reff"][wv] if isinstance(cfg["reff"], dict) else cfg["reff"]
+            cols = _wide_columns(cfg, pos[sid, :3], sdir, par["srcparam1"][k], par["srcparam2"][k], st, pat, z0,
+                                 reff, wgt)
+            rows.append([sid, ncol, ncol + cols.shape[0] - 1])
+            ncol += cols.shape[0]
+            blocks.append(cols)
+        mapping[wv] = np.array(rows) if rows else np.array([])
+    wide = np.vstack(blocks) if blocks else np.array([])
+    keep = np.ones(pos.shape[0], dtype=bool)
+    keep[list(set(used))] = False
+    if len(mapping) == 1:
+        mapping = mapping[list(prop)[0]]
+    cfg[outk] = wide.T if wide.size else np.array([])
+    cfg[mapk] = mapping
+    cfg[pk] = pos[keep]
+    return cfg
+
+
 def meshprep(cfg):
     """Derived mesh fields and the sd map (utility.py:101-246).
 
@@ -305,6 +453,13 @@ def meshprep(cfg):
         else:
             cfg["reff"] = getreff(bk[3], 1.0)
             cfg["musp0"] = bk[1] * (1 - bk[2])
+    # wide-field sources / detectors become boundary-flux columns (utility.py:210-224)
+    if (cfg.get("srctype", "pencil") not in ("pencil", "isotropic") or "widesrcid" in cfg) and "widesrc" not in cfg:
+        cfg["srcpos0"] = cfg["srcpos"].copy()
+        cfg = src2bc(cfg, isdet=False)
+    if (cfg.get("dettype", "pencil") not in ("pencil", "isotropic") or "widedetid" in cfg) and "widedet" not in cfg:
+        cfg["detpos0"] = cfg["detpos"].copy()
+        cfg = src2bc(cfg, isdet=True)
     if cfg.get("cols") is None:
         cfg["rows"], cfg["cols"], cfg["idxcount"] = _connectivity(cfg)
     if cfg.get("idxsum") is None:
