@@ -237,6 +237,17 @@ int rb_gram_mfma(int64_t K, int64_t m, const void* JT, int64_t ldj, int in_dtype
                  double* G, void* work, size_t work_bytes, void* stream);
 
 /*
+ * Dual-mesh Jacobian (redbirdpy recon.py:818-876 `_remap_jacobian`, applied
+ * at :200-205): Y[r][:M] = sum_k w[k] X[idx[k]][:M], k in [ptr[r], ptr[r+1])
+ * taken in order.  X: node-major forward-mesh Jacobian (row stride ldx),
+ * Y: nrows recon-mesh rows (row stride ldy); ptr/idx/w: the transpose of the
+ * interpolation (mapid, mapweight) as CSR over recon nodes, forward nodes
+ * ascending within a row.  Float64.
+ */
+int rb_jt_remap(int64_t nrows, const int32_t* ptr, const int32_t* idx, const double* w, int64_t M, const double* X,
+                int64_t ldx, double* Y, int64_t ldy, void* stream);
+
+/*
  * Jacobian matvecs with float64 accumulation over a node-major Jacobian
  * JT (K x M, columns in nw blocks of np pairs; redbirdpy recon.py:501
  * `Amat.T @ y` and the J x products of the refinement):

@@ -1185,8 +1185,54 @@ def flatten_detphi(detphi, sd, wavelengths, rfcw):
     return np.array(out)
 
 
+def meshinterp(fromval, elemid, elembary, fromelem, toval=None):
+    """utility.py:973-999 -- nodal values of a source mesh at points located in it.
+
+    elemid: 1-based element of each point (NaN: outside, left at toval or 0);
+    elembary: (npts, 4) barycentric weights.
+    """
+    fv = np.asarray(fromval, dtype=float)
+    if fv.ndim == 1:
+        fv = fv[:, None]
+    e0 = np.asarray(fromelem)[:, :4].astype(np.int64) - 1
+    eid = np.asarray(elemid, dtype=float)
+    out = np.zeros((len(eid), fv.shape[1])) if toval is None else np.array(toval, dtype=float).reshape(len(eid), -1)
+    ok = ~np.isnan(eid)
+    nodes = e0[eid[ok].astype(np.int64) - 1]
+    out[ok] = np.einsum("pk,pkc->pc", np.asarray(elembary)[ok], fv[nodes])
+    return out if out.shape[1] > 1 else out[:, 0]
+
+
+def remap_jacobian(J, recon, cfg):
+    """recon.py:818-876 -- forward-node Jacobian columns moved onto recon-mesh nodes.
+
+    J_new[:, n] = sum over forward nodes i located in a recon element holding
+    n of J[:, i] * mapweight[i, j] (the transpose of meshinterp), forward nodes
+    taken in ascending order; NaN or out-of-range elements are skipped.
+    """
+    nrec = np.asarray(recon["node"]).shape[0]
+    e0 = np.asarray(recon["elem"])[:, :4].astype(np.int64) - 1
+    mid = np.asarray(recon["mapid"], dtype=float)
+    mw = np.asarray(recon["mapweight"], dtype=float)
+    out = np.zeros((J.shape[0], nrec), dtype=J.dtype)
+    for i in range(J.shape[1]):
+        if np.isnan(mid[i]):
+            continue
+        e = int(mid[i]) - 1
+        if e < 0 or e >= e0.shape[0] or np.any(e0[e] < 0) or np.any(e0[e] >= nrec):
+            continue
+        for j in range(4):
+            out[:, e0[e, j]] += J[:, i] * mw[i, j]
+    return out
+
+
+def _dual(recon):
+    return "mapid" in recon and "mapweight" in recon and "elem" in recon
+
+
 def syncprop(cfg, recon):
-    """recon.py:669-781 (single-mesh branches)."""
+    """recon.py:669-781 -- recon unknowns onto the forward mesh (label copy, same-mesh copy, or dual-mesh
+    interpolation with meshinterp over recon['mapid'] / ['mapweight'])."""
     if "param" in recon:
         keys = list(recon["param"].keys())
         first = recon["param"][keys[0]]
@@ -1196,17 +1242,26 @@ def syncprop(cfg, recon):
         else:
             cfg.setdefault("param", {})
             for k in keys:
-                cfg["param"][k] = recon["param"][k].copy()
+                cfg["param"][k] = (meshinterp(recon["param"][k], recon["mapid"], recon["mapweight"], recon["elem"])
+                                   if _dual(recon) else recon["param"][k].copy())
     elif "prop" in recon:
         if not isinstance(recon["prop"], dict):
-            cfg["prop"] = recon["prop"].copy()
+            if recon["prop"].shape[0] >= 50 and _dual(recon):
+                cfg["prop"] = meshinterp(recon["prop"], recon["mapid"], recon["mapweight"], recon["elem"])
+            else:
+                cfg["prop"] = recon["prop"].copy()
         else:
-            cfg["prop"] = {k: v.copy() for k, v in recon["prop"].items()}
+            first = next(iter(recon["prop"].values()))
+            if first.shape[0] >= 50 and _dual(recon):
+                cfg["prop"] = {k: meshinterp(v, recon["mapid"], recon["mapweight"], recon["elem"])
+                               for k, v in recon["prop"].items()}
+            else:
+                cfg["prop"] = {k: v.copy() for k, v in recon["prop"].items()}
     return cfg, recon
 
 
 def runrecon(cfg, recon, detphi0, sd=None, extin=None, **kw):
-    """recon.py:39-335 -- Gauss-Newton loop (single mesh, no prior, no labels).
+    """recon.py:39-335 -- Gauss-Newton loop (single or dual mesh, no prior, no label compression).
 
     ``extin`` (project extension): dict wavelength -> {chromophore: value},
     used by updateprop and jacchrome in place of the built-in table.
@@ -1252,6 +1307,10 @@ def runrecon(cfg, recon, detphi0, sd=None, extin=None, **kw):
             J, mis, _ = matreform(J, y0, y, reform)
         else:
             mis = y0 - y
+        if "node" in recon and "elem" in recon and "mapid" in recon:  # recon.py:200-205
+            J = remap_jacobian(J, recon, cfg)
+            nrec = np.asarray(recon["node"]).shape[0]
+            blocks = {k: (v[0], nrec) for k, v in blocks.items()}
         resid[it] = np.sum(np.abs(mis))
         s = 1.0 / np.sqrt(np.sum(J ** 2))
         J = J * s

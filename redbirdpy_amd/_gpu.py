@@ -577,6 +577,46 @@ def jt_rows(JT, nw, npairs, v):
     return t
 
 
+def remap_operator(mapid, mapweight, relem, nrec, perm_rows=None, dev=None):
+    """Device CSR (ptr, idx, w) over recon nodes of the interpolation's transpose (recon.py:848-874 loop order).
+
+    Forward node i (ascending) located in recon element mapid[i] (1-based,
+    NaN outside) gives weight mapweight[i, j] to recon node relem[e, j].
+    perm_rows maps user forward-node numbers to device rows (iperm).
+    """
+    dev = dev or device()
+    mid = np.asarray(mapid, dtype=float)
+    mw = np.asarray(mapweight, dtype=float)
+    e0 = np.asarray(relem)[:, :4].astype(np.int64) - 1
+    fwd = np.flatnonzero(~np.isnan(mid))
+    e = mid[fwd].astype(np.int64) - 1
+    ok = (e >= 0) & (e < e0.shape[0])
+    fwd, e = fwd[ok], e[ok]
+    nodes = e0[e]
+    ok = np.all((nodes >= 0) & (nodes < nrec), axis=1)
+    fwd, nodes = fwd[ok], nodes[ok]
+    rows = nodes.ravel()
+    cols = np.repeat(fwd, 4)
+    vals = mw[fwd].ravel()
+    order = np.lexsort((np.tile(np.arange(4), len(fwd)), cols, rows))  # per recon node: forward nodes ascending
+    rows, cols, vals = rows[order], cols[order], vals[order]
+    ptr_ = np.zeros(nrec + 1, dtype=np.int64)
+    np.cumsum(np.bincount(rows, minlength=nrec), out=ptr_[1:])
+    if perm_rows is not None:
+        cols = perm_rows[cols]
+    return _i32(ptr_, dev), _i32(cols, dev), _f64(vals, dev)
+
+
+def jt_remap(op, JT, nrec):
+    """(nrec x M) recon-mesh Jacobian rows from the node-major forward Jacobian JT (rb_jt_remap)."""
+    p, i, w = op
+    M = JT.shape[1]
+    Y = torch.empty((nrec, M), dtype=torch.float64, device=JT.device)
+    call("rb_jt_remap", nrec, ptr(p), ptr(i), ptr(w), M, ptr(JT), JT.stride(0), ptr(Y), Y.stride(0),
+         _abi.stream_ptr())
+    return Y
+
+
 def jt_cols(JT, nw, npairs, z=None):
     """u[j] = sum_n JT[n, j] z[n, j // np] (J_w z_w), or sum_n JT[n, j]^2 when z is None."""
     K, M = JT.shape

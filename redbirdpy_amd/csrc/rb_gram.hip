@@ -400,3 +400,36 @@ extern "C" int rb_gram(int64_t K, int64_t m, const void* JT, int64_t ldj, int in
   RB_CHECK_LAUNCH();
   return RB_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Jacobian columns onto a second (reconstruction) mesh: Y[r][:] =
+// sum_k w[k] X[idx[k]][:] for k in [ptr[r], ptr[r+1]), in k order.  One
+// workgroup per output row, lanes over the M contiguous columns (coalesced
+// row reads); deterministic.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_jt_remap(int64_t nrows, const int32_t* __restrict__ ptr,
+                                                  const int32_t* __restrict__ idx, const double* __restrict__ w,
+                                                  int64_t M, const double* __restrict__ X, int64_t ldx,
+                                                  double* __restrict__ Y, int64_t ldy) {
+  const int64_t r = blockIdx.x;
+  if (r >= nrows) return;
+  const int32_t k0 = ptr[r], k1 = ptr[r + 1];
+  for (int64_t c = threadIdx.x; c < M; c += blockDim.x) {
+    double acc = 0.0;
+    for (int32_t k = k0; k < k1; ++k) acc += X[(int64_t)idx[k] * ldx + c] * w[k];
+    Y[r * ldy + c] = acc;
+  }
+}
+
+extern "C" int rb_jt_remap(int64_t nrows, const int32_t* ptr, const int32_t* idx, const double* w, int64_t M,
+                           const double* X, int64_t ldx, double* Y, int64_t ldy, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (nrows < 0 || M < 0 || ldx < M || ldy < M || (nrows > 0 && (!ptr || !Y)) || nrows > 0x7fffffff) {
+    rb_set_error("rb_jt_remap: bad arguments");
+    return RB_ERR_ARG;
+  }
+  if (nrows == 0 || M == 0) return RB_OK;
+  hipLaunchKernelGGL(k_jt_remap, dim3((unsigned)nrows), dim3(256), 0, s, nrows, ptr, idx, w, M, X, ldx, Y, ldy);
+  RB_CHECK_LAUNCH();
+  return RB_OK;
+}

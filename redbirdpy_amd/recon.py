@@ -25,7 +25,7 @@ import torch
 
 from . import _gpu
 from .forward import forward_device, _active_pairs
-from .utility import sdmap
+from .utility import sdmap, meshinterp
 from .property import updateprop, extinction, CHROMOPHORES
 
 
@@ -94,8 +94,17 @@ def matflat(Amat, weight=None):
     return Amat
 
 
+def _dual(recon):
+    return "mapid" in recon and "mapweight" in recon and "elem" in recon
+
+
 def syncprop(cfg, recon):
-    """Copy recon unknowns to the forward mesh (recon.py:669-781, single-mesh branches)."""
+    """Recon unknowns onto the forward mesh (recon.py:669-781).
+
+    Label tables (< 50 rows) and same-mesh arrays are copied; on a dual mesh
+    (recon['mapid'] / ['mapweight'] / ['elem']) node arrays are interpolated
+    to the forward nodes with meshinterp.
+    """
     if "param" in recon:
         keys = list(recon["param"].keys())
         first = recon["param"][keys[0]]
@@ -105,14 +114,20 @@ def syncprop(cfg, recon):
         else:
             cfg.setdefault("param", {})
             for k in keys:
-                cfg["param"][k] = recon["param"][k].copy()
+                cfg["param"][k] = (meshinterp(recon["param"][k], recon["mapid"], recon["mapweight"], recon["elem"])
+                                   if _dual(recon) else recon["param"][k].copy())
     elif "prop" in recon:
         if isinstance(recon["prop"], dict):
-            cfg["prop"] = {k: v.copy() for k, v in recon["prop"].items()}
+            first = next(iter(recon["prop"].values()))
+            if first.shape[0] >= 50 and _dual(recon):
+                cfg["prop"] = {k: meshinterp(v, recon["mapid"], recon["mapweight"], recon["elem"])
+                               for k, v in recon["prop"].items()}
+            else:
+                cfg["prop"] = {k: v.copy() for k, v in recon["prop"].items()}
+        elif recon["prop"].shape[0] >= 50 and _dual(recon):
+            cfg["prop"] = meshinterp(recon["prop"], recon["mapid"], recon["mapweight"], recon["elem"])
         else:
             cfg["prop"] = recon["prop"].copy()
-    if "mapid" in recon and "node" in recon:
-        raise NotImplementedError("dual-mesh reconstruction is not supported")
     return cfg, recon
 
 
@@ -265,9 +280,18 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
 
     Supported on the GPU: a single-wavelength node-based mua unknown
     (recon['prop'] with nn rows), and multispectral chromophore unknowns
-    (recon['param'] node arrays).  Extra keywords: precision ('fp64' |
-    'mixed'), extin (dict wavelength -> {chromophore: value} replacing the
-    extinction table), solver tolerance ``solvertol``.
+    (recon['param'] node arrays); CW or frequency-domain data with reform
+    'real' ([Re; Im] rows for complex data, recon.py:532-546) or 'logphase'
+    (log-amplitude and phase rows, :548-565); a dual mesh (recon['node'],
+    ['elem'], ['mapid'], ['mapweight']) for a single unknown block, the
+    Jacobian moved onto the recon nodes as in recon.py:200-205.  Extra
+    keywords: precision ('fp64' | 'mixed'), extin (dict wavelength ->
+    {chromophore: value} replacing the extinction table), solver tolerance
+    ``solvertol`` (default 1e-12, mixed 1e-10).  Log-amplitude rows need
+    each detector value to relative accuracy: a value at r times the largest
+    one carries ~solvertol/r relative error (an iterative solve from zero
+    leaves values below ~solvertol of the peak at exactly 0, where
+    'logphase' is undefined).
     Returns (recon, resid, cfg, updates, Jmua, detphi, phi) like the reference;
     Jmua holds the node Jacobians per wavelength (numpy) only when
     ``return_jac=True`` (they can be several GB).
@@ -286,8 +310,8 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
     optodes = kwargs.get("optodes", None)
     if isinstance(rfcw, int):
         rfcw = [rfcw]
-    if rfcw != [1] or reform not in ("real", "complex"):
-        raise NotImplementedError("GPU runrecon supports rfcw=[1] with reform 'real'")
+    if rfcw != [1] or reform not in ("real", "complex", "logphase"):
+        raise NotImplementedError("GPU runrecon supports rfcw=[1] with reform 'real', 'logphase' (or 'complex' on CW)")
     if kwargs.get("prior") or "lmat" in kwargs or "ltl" in kwargs:
         raise NotImplementedError("structural priors are not supported")
     if sd is None:
@@ -299,6 +323,9 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
     X0 = None
     detphi_out = phi_out = Jm_out = None
     timing = kwargs.get("timing", None)  # dict: per-phase seconds (synchronising) when given
+    dual = "node" in recon and "elem" in recon and "mapid" in recon
+    jac_keep = {}  # unscaled (complex) Jacobians of the last iteration, for return_jac
+    remap = None  # device CSR of the dual-mesh Jacobian map, built once
 
     def mark(name, t_prev):
         if timing is None:
@@ -325,27 +352,55 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
         wvs = list(cfg["prop"].keys()) if isinstance(cfg.get("prop"), dict) else [""]
         chs = _chromophores(cfg)
         mesh = _gpu.mesh_for(cfg)
-        npairs = None
-        JT = None
-        jdt = torch.float64  # both precisions: the refined step is that of this Jacobian
-        for i, wv in enumerate(wvs):
-            sdw = sd.get(wv, sd) if isinstance(sd, dict) else sd
-            s, d = _active_pairs(sdw)
-            if res.is_complex:
-                raise NotImplementedError("GPU runrecon supports CW (omega = 0) data")
-            if JT is None:
-                npairs = len(s)
-                JT = torch.empty((mesh.nn, len(wvs) * npairs), dtype=jdt, device=mesh.dev)
-            elif len(s) != npairs:
-                raise NotImplementedError("all wavelengths must have the same active pairs")
-            _gpu.jac_node(mesh, res.phi(i), s, d, jdt, out=JT[:, i * npairs : (i + 1) * npairs])
-        tp = mark("jacobian", tp)
         detphi = {wv: res.detvals[i].cpu().numpy() for i, wv in enumerate(wvs)}
         if wvs == [""]:
             detphi = detphi[""]
         y0 = _flatten_detphi(detphi0, sd, wvs, rfcw)
         y = _flatten_detphi(detphi, sd, wvs, rfcw)
-        misfit = np.real(y0 - y)
+        cplx = res.is_complex
+        if cplx and reform == "complex":
+            raise NotImplementedError("GPU runrecon solves real normal equations: use reform 'real' or 'logphase'")
+        # rows of the real system: (part, wavelength, pair) with part Re [, Im]
+        # (recon.py:532-565); 'logphase' scales each complex row by
+        # conj(y)/|y|^2 before splitting
+        nparts = 2 if cplx else 1
+        rowscale = None
+        if reform == "logphase":
+            rowscale = np.conj(y) / np.abs(y * y)
+            misfit = np.log(np.abs(y0)) - np.log(np.abs(y))
+            if cplx:
+                misfit = np.concatenate([misfit, np.angle(y0) - np.angle(y)])
+        elif cplx:
+            misfit = np.concatenate([np.real(y0 - y), np.imag(y0 - y)])
+        else:
+            misfit = np.real(y0 - y)
+        npairs = None
+        JT = None
+        nw = len(wvs)
+        jdt = torch.float64  # both precisions: the refined step is that of this Jacobian
+        for i, wv in enumerate(wvs):
+            sdw = sd.get(wv, sd) if isinstance(sd, dict) else sd
+            s, d = _active_pairs(sdw)
+            if JT is None:
+                npairs = len(s)
+                JT = torch.empty((mesh.nn, nparts * nw * npairs), dtype=jdt, device=mesh.dev)
+            elif len(s) != npairs:
+                raise NotImplementedError("all wavelengths must have the same active pairs")
+            blk = slice(i * npairs, (i + 1) * npairs)
+            if not cplx and rowscale is None:
+                _gpu.jac_node(mesh, res.phi(i), s, d, jdt, out=JT[:, blk])
+                continue
+            Jw, _ = _gpu.jac_node(mesh, res.phi(i), s, d, torch.complex128 if cplx else jdt)
+            if return_jac:
+                jac_keep["mua" if wv == "" else wv] = mesh.user_rows(Jw).t().cpu().numpy()
+            if rowscale is not None:
+                sc = torch.as_tensor(rowscale[blk] if cplx else np.real(rowscale[blk]), device=mesh.dev)
+                Jw = Jw * sc[None, :]
+            JT[:, blk] = Jw.real
+            if cplx:
+                JT[:, nw * npairs + i * npairs : nw * npairs + (i + 1) * npairs] = Jw.imag
+            del Jw
+        tp = mark("jacobian", tp)
         resid[it] = np.sum(np.abs(misfit))
         if chs:
             if extin is None:
@@ -358,11 +413,24 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
                 raise NotImplementedError("multi-wavelength mua reconstruction needs chromophore params")
             E = np.ones((1, 1))
             keys = ["mua"]
-        step = GNStep(JT, npairs, E, lam, precision, reftol=kwargs.get("reftol", 1e-10))
+        if nparts == 2:
+            E = np.vstack([E, E])  # the Im row blocks carry the same extinction weights
+        JG = JT
+        if dual:
+            if len(keys) != 1:
+                # the reference maps one block of forward-node columns (recon.py:842-874)
+                raise NotImplementedError("dual-mesh reconstruction supports a single unknown block")
+            nrec = np.asarray(recon["node"]).shape[0]
+            if remap is None:
+                remap = _gpu.remap_operator(recon["mapid"], recon["mapweight"], recon["elem"], nrec,
+                                            perm_rows=mesh.iperm, dev=mesh.dev)
+            JG = _gpu.jt_remap(remap, JT, nrec)
+        step = GNStep(JG, npairs, E, lam, precision, reftol=kwargs.get("reftol", 1e-10))
         step.mark = (lambda name, t: mark(name, t)) if timing is not None else None
         tp = mark("misfit", tp)
-        dmu = mesh.user_rows(step.solve(torch.as_tensor(misfit, dtype=torch.float64, device=mesh.dev)))
-        dmu = dmu.cpu().numpy()
+        dmu = step.solve(torch.as_tensor(misfit, dtype=torch.float64, device=mesh.dev))
+        dmu = (dmu if dual else mesh.user_rows(dmu)).cpu().numpy()
+        del JG
         tp = mark("gn_solve", tp)
         if timing is not None:
             timing.setdefault("refinements", []).append(step.info.get("refinements", 0))
@@ -402,9 +470,9 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
             resid = resid[: it + 1]
             break
     if return_jac:
-        Jm_out = {}
-        for i, wv in enumerate(wvs):
-            Jm_out["mua" if wv == "" else wv] = mesh.user_rows(JT[:, i * npairs : (i + 1) * npairs]).t().cpu().numpy()
+        Jm_out = jac_keep or {
+            ("mua" if wv == "" else wv): mesh.user_rows(JT[:, i * npairs : (i + 1) * npairs]).t().cpu().numpy()
+            for i, wv in enumerate(wvs)}
     Xh = mesh.user_rows(X0).cpu().numpy()
     k = res.k
     phi_out = {wv: Xh[:, i * k : (i + 1) * k] for i, wv in enumerate(wvs)}

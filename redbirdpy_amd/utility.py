@@ -16,6 +16,7 @@ __all__ = [
     "elem2node",
     "forcearray",
     "src2bc",
+    "meshinterp",
 ]
 
 import numpy as np
@@ -407,6 +408,30 @@ def src2bc(cfg, isdet=False):
     cfg[mapk] = mapping
     cfg[pk] = pos[keep]
     return cfg
+
+
+def meshinterp(fromval, elemid, elembary, fromelem, toval=None):
+    """Nodal values of a source mesh at located points (utility.py:973-999).
+
+    elemid: 1-based source element per point (NaN: outside, keeps toval or
+    0); elembary: (npts, 4) barycentric weights; fromval: (nn_src[, ncol]).
+    """
+    fv = np.asarray(fromval, dtype=float)
+    fv = fv[:, None] if fv.ndim == 1 else fv
+    e0 = np.asarray(fromelem)[:, :4].astype(np.int64) - 1
+    eid = np.asarray(elemid, dtype=float)
+    if toval is None:
+        out = np.zeros((len(eid), fv.shape[1]))
+    else:
+        out = np.array(toval, dtype=float).reshape(len(eid), -1)
+    inside = np.flatnonzero(~np.isnan(eid))
+    corners = e0[eid[inside].astype(np.int64) - 1]  # (npts, 4) source nodes
+    w = np.asarray(elembary, dtype=float)[inside]
+    acc = np.zeros((len(inside), fv.shape[1]))
+    for j in range(4):
+        acc += w[:, j : j + 1] * fv[corners[:, j]]
+    out[inside] = acc
+    return out if out.shape[1] > 1 else out[:, 0]
 
 
 def meshprep(cfg):

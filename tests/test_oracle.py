@@ -378,3 +378,82 @@ def test_updateprop_and_jacchrome():
     e2, _ = orc.extinction(["690", "830"], ["hbo", "hbr"])
     np.testing.assert_allclose(Jc["hbr"][:2], e2[0, 1])
     np.testing.assert_allclose(Jc["hbr"][2:], 2 * e2[1, 1])
+
+
+# ---------------------------------------------------------------------------
+# dual mesh (meshinterp, _remap_jacobian) and the frequency-domain reforms
+# ---------------------------------------------------------------------------
+def _dual_meshes():
+    fine, _ = orc.meshprep(box_cfg((12, 12, 6), 1.0))
+    cnode, celem = kuhn_box([0, 0, 0], [12, 12, 6], 3.0)
+    loc, bary = orc.tsearchn(cnode, celem, fine["node"])
+    return fine, cnode, celem, loc, bary
+
+
+def test_meshinterp_reproduces_linear_fields():
+    fine, cnode, celem, loc, bary = _dual_meshes()
+    assert not np.any(np.isnan(loc))
+    coef = np.array([[0.3, -1.2], [0.7, 0.1], [-0.4, 2.0]])
+    v = cnode @ coef + np.array([1.5, -0.25])
+    got = orc.meshinterp(v, loc, bary, celem)
+    np.testing.assert_allclose(got, fine["node"] @ coef + np.array([1.5, -0.25]), rtol=1e-12, atol=1e-12)
+    # points outside the source mesh keep toval (or 0)
+    loc2 = loc.copy()
+    loc2[:5] = np.nan
+    got2 = orc.meshinterp(v[:, 0], loc2, bary, celem, toval=np.full(len(loc), 7.0))
+    assert np.all(got2[:5] == 7.0)
+
+
+def test_remap_jacobian_is_adjoint_of_meshinterp():
+    fine, cnode, celem, loc, bary = _dual_meshes()
+    rng = np.random.default_rng(11)
+    J = rng.standard_normal((5, fine["node"].shape[0]))
+    recon = {"node": cnode, "elem": celem, "mapid": loc, "mapweight": bary}
+    Jr = orc.remap_jacobian(J, recon, fine)
+    assert Jr.shape == (5, cnode.shape[0])
+    for _ in range(3):
+        v = rng.standard_normal(cnode.shape[0])
+        np.testing.assert_allclose(Jr @ v, J @ orc.meshinterp(v, loc, bary, celem), rtol=1e-12)
+    # a linear field's Jacobian sum: J_rec applied to ones equals J applied to ones
+    np.testing.assert_allclose(Jr.sum(1), J.sum(1), rtol=1e-12)
+
+
+def test_fd_jacobian_and_logphase_rows_match_finite_differences():
+    """Complex Jacobian (FD) and matreform('logphase') rows vs central differences of
+    detval and of (log|detval|, angle(detval)) under the jac derivative model."""
+    cfg, sd = _fd_cfg(omega=2 * np.pi * 1e8)
+    ne = cfg["elem"].shape[0]
+    base = np.tile(np.array([0.01, 1.0, 0.0, 1.37]), (ne, 1))
+    cfg["prop"] = base
+    A = orc.femlhs(cfg, cfg["deldotdel"])
+    rhs, loc, bary, _ = orc.femrhs(cfg, sd)
+    phi, _ = orc.femsolve(A, rhs)
+    y = orc.femgetdet(phi, cfg, rhs, loc, bary).ravel()
+    Jn, _ = orc.jac(sd, phi, cfg["deldotdel"], cfg["elem"], cfg["evol"])
+    assert np.iscomplexobj(Jn)
+    Alp, rlp, nb = orc.matreform(Jn, y * 1.01, y, "logphase")
+    assert nb == 2 and Alp.shape == (2 * len(y), Jn.shape[1])
+    np.testing.assert_allclose(rlp[: len(y)], np.log(1.01), rtol=1e-12)
+    np.testing.assert_allclose(rlp[len(y):], 0, atol=1e-12)
+    Are, _, nb2 = orc.matreform(Jn, y * 1.01, y, "real")
+    assert nb2 == 2
+    e0 = cfg["elem"][:, :4] - 1
+    delta = 1e-4 * 0.01
+    for n in np.random.default_rng(5).choice(cfg["node"].shape[0], 4, replace=False):
+        hold = np.any(e0 == n, axis=1)
+        vals = []
+        for sgn in (+1, -1):
+            p = base.copy()
+            p[hold, 0] += sgn * delta / 4
+            p[hold, 1] -= sgn * delta / 4
+            cfg["prop"] = p
+            xx, _ = orc.femsolve(orc.femlhs(cfg, cfg["deldotdel"]), rhs)
+            vals.append(orc.femgetdet(xx, cfg, rhs, loc, bary).ravel())
+        fd = (vals[0] - vals[1]) / (2 * delta)
+        np.testing.assert_allclose(Jn[:, n], fd, rtol=1e-6, atol=1e-7 * np.abs(Jn).max())
+        np.testing.assert_allclose(Are[:, n], np.concatenate([fd.real, fd.imag]), rtol=1e-6,
+                                   atol=1e-7 * np.abs(Jn).max())
+        fla = (np.log(np.abs(vals[0])) - np.log(np.abs(vals[1]))) / (2 * delta)
+        fph = (np.angle(vals[0]) - np.angle(vals[1])) / (2 * delta)
+        np.testing.assert_allclose(Alp[:, n], np.concatenate([fla, fph]), rtol=1e-5,
+                                   atol=1e-6 * np.abs(Alp).max())
