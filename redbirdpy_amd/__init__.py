@@ -8,11 +8,14 @@ elem/face indices stay 1-based as in redbirdpy.
     import redbirdpy_amd as rb
     cfg, sd = rb.meshprep(cfg)
     detval, phi = rb.run(cfg)
+
+rb.autograd.detval_mua(cfg, mua) gives the detector values as a torch
+function of nodal mua whose backward is the adjoint Jacobian contraction.
 """
 
 __version__ = "0.2.0"
 
-from . import forward, recon, utility, property, solver, mesh
+from . import forward, recon, utility, property, solver, mesh, autograd
 from .forward import *  # noqa: F401,F403
 from .recon import *  # noqa: F401,F403
 from .utility import *  # noqa: F401,F403
@@ -25,7 +28,7 @@ __all__ = (
     + recon.__all__
     + utility.__all__
     + property.__all__
-    + ["run", "forward", "recon", "utility", "property", "solver", "mesh"]
+    + ["run", "forward", "recon", "utility", "property", "solver", "mesh", "autograd"]
 )
 
 

@@ -118,9 +118,15 @@ def call(name, *args):
 
 
 def ptr(t):
-    """Device pointer of a tensor (None -> NULL)."""
+    """Device pointer of a tensor (None -> NULL).
+
+    A tensor with torch's lazy conjugate or negative bit set is refused: its
+    memory holds the unconjugated (un-negated) values the kernel would read.
+    """
     if t is None:
         return None
+    if t.is_conj() or t.is_neg():
+        raise ValueError("pass a tensor whose conjugation is resolved (resolve_conj() / resolve_neg())")
     return ctypes.c_void_p(t.data_ptr())
 
 

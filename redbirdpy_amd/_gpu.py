@@ -530,7 +530,7 @@ def jac_contract(mesh, phi, S, D, w):
     is_complex = phi.is_complex()
     Ns, Nd = len(S), len(D)
     dt = phi.dtype
-    w = w.to(dt).contiguous()
+    w = w.to(dt).resolve_conj().resolve_neg().contiguous()
     psi = torch.empty((mesh.nn, Ns), dtype=dt, device=dev)
     h = torch.empty(mesh.ne, dtype=dt, device=dev)
     g = torch.empty(mesh.nn, dtype=dt, device=dev)

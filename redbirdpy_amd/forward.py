@@ -206,8 +206,14 @@ def locate_optodes(cfg):
     return {wv: _Optodes(cfg, mesh, wv) for wv in wvs}
 
 
-def forward_device(cfg, rfcw=(1,), precision="fp64", tol=1e-12, maxiter=None, X0=None, optodes=None):
-    """Assemble, build RHS and solve all (wavelength, mode) systems on the GPU."""
+def forward_device(cfg, rfcw=(1,), precision="fp64", tol=1e-12, maxiter=None, X0=None, optodes=None,
+                   keep_optodes=False):
+    """Assemble, build RHS and solve all (wavelength, mode) systems on the GPU.
+
+    optodes: located optodes of an earlier call, reused where their moved
+    positions are unchanged (or as given, with keep_optodes=True: the
+    optodes then stay where they were when the optical properties change).
+    """
     if cfg.get("deldotdel") is None:
         cfg["deldotdel"], _ = deldotdel(cfg)
     wvs = list(cfg["prop"].keys()) if isinstance(cfg.get("prop"), dict) else [""]
@@ -220,6 +226,9 @@ def forward_device(cfg, rfcw=(1,), precision="fp64", tol=1e-12, maxiter=None, X0
     old = optodes or {}
     optodes = {}
     for wv in wvs:
+        if keep_optodes and wv in old:
+            optodes[wv] = old[wv]
+            continue
         src, det, _, _ = getoptodes(cfg, wv)
         op = old.get(wv)
         same = op is not None and all(
