@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define RB_ABI_VERSION 2
+#define RB_ABI_VERSION 3
 
 /* property layouts for rb_assemble */
 #define RB_PROP_ELEM 0   /* one row per element (label tables are expanded by the caller) */
@@ -246,6 +246,21 @@ int rb_gram_mfma(int64_t K, int64_t m, const void* JT, int64_t ldj, int in_dtype
  */
 int rb_jt_remap(int64_t nrows, const int32_t* ptr, const int32_t* idx, const double* w, int64_t M, const double* X,
                 int64_t ldx, double* Y, int64_t ldy, void* stream);
+
+/*
+ * Multi-RHS BiCGSTAB with right Jacobi preconditioning, float64 or
+ * complex128 (redbirdpy solver.py:791-824, method 'bicgstab'): for
+ * matrices that are not (complex) symmetric, and as the fallback when a
+ * CG/COCG column breaks down.  One system, ncol <= 512 columns of B/X
+ * (n x ncol row-major); x0 from X when use_x0.  Conjugated inner products;
+ * a column stops at |b - A x| <= tol |b|.  work: rb_bicgstab_workspace()
+ * bytes.  iters/relres/flag as rb_krylov_solve (flag 4: breakdown).
+ */
+int rb_bicgstab_workspace(int64_t n, int ncol, int is_complex, size_t* bytes /* host */);
+int rb_bicgstab_solve(int64_t n, int64_t nnz, const int32_t* row_ptr, const int32_t* col_idx, const int32_t* diag,
+                      int ncol, int is_complex, const void* vals, const void* B, void* X, double tol, int maxiter,
+                      int use_x0, int check_every, void* work, size_t work_bytes, int* iters /* host */,
+                      double* relres /* host */, int* flag /* host */, void* stream);
 
 /*
  * Jacobian matvecs with float64 accumulation over a node-major Jacobian

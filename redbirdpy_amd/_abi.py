@@ -51,6 +51,9 @@ _SIGS = {
                         _int, _int, _vp, _sz, ctypes.POINTER(_int), ctypes.POINTER(_dbl), ctypes.POINTER(_int),
                         _vp],
     "rb_spmm": [_i64, _i64, _vp, _vp, _int, _int, _int, _vp, _vp, _vp, _vp],
+    "rb_bicgstab_workspace": [_i64, _int, _int, ctypes.POINTER(_sz)],
+    "rb_bicgstab_solve": [_i64, _i64, _vp, _vp, _vp, _int, _int, _vp, _vp, _vp, _dbl, _int, _int, _int, _vp, _sz,
+                          ctypes.POINTER(_int), ctypes.POINTER(_dbl), ctypes.POINTER(_int), _vp],
     "rb_getdet": [_int, _vp, _vp, _vp, _vp, _i64, _int, _int, _int, _vp, _vp],
     "rb_jac_grid": [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _int, _vp, _i64, _int, _int, _int, _vp, _vp,
                     _vp, _vp, _i64, _int, _vp, _vp],

@@ -385,10 +385,39 @@ def krylov_solve(row_ptr, col_idx, diag, n, nnz, vals64, vals32, B, k, nsys, is_
     return X, SolveInfo(it.value, rel.value, fl.value)
 
 
+def bicgstab_solve(row_ptr, col_idx, diag, n, nnz, vals64, B, is_complex, tol=1e-12, maxiter=2000, X0=None,
+                   check_every=16):
+    """BiCGSTAB (rb_bicgstab_solve) for one system: B (n, ncol) on device, float64 / complex128."""
+    dev = B.device
+    cdt = torch.complex128 if is_complex else torch.float64
+    B = B.to(cdt).contiguous()
+    ncol = B.shape[1]
+    if ncol > 512:
+        raise ValueError("at most 512 columns per solve call")
+    X = X0.to(cdt).clone().contiguous() if X0 is not None else torch.empty_like(B)
+    nbytes = ctypes.c_size_t(0)
+    call("rb_bicgstab_workspace", n, ncol, int(is_complex), ctypes.byref(nbytes))
+    work = torch.empty(int(nbytes.value), dtype=torch.uint8, device=dev)
+    it = ctypes.c_int(0)
+    rel = ctypes.c_double(0)
+    fl = ctypes.c_int(0)
+    vals = vals64.reshape(-1)[:nnz].to(cdt).contiguous()
+    call("rb_bicgstab_solve", n, nnz, ptr(row_ptr), ptr(col_idx), ptr(diag), ncol, int(is_complex), ptr(vals),
+         ptr(B), ptr(X), float(tol), int(maxiter), int(X0 is not None), int(check_every), ptr(work), nbytes.value,
+         ctypes.byref(it), ctypes.byref(rel), ctypes.byref(fl), _abi.stream_ptr())
+    return X, SolveInfo(it.value, rel.value, fl.value)
+
+
 def solve_system(sysm, B, k, precision="fp64", tol=1e-12, maxiter=2000, X0=None):
+    """CG/COCG over every system; a breakdown (flag 4) of a single-system solve is
+    finished by BiCGSTAB from the CG iterate."""
     m = sysm.mesh
-    return krylov_solve(m.row_ptr, m.col_idx, m.diag, m.nn, m.nnz, sysm.vals64, sysm.vals32, B, k, sysm.nsys,
-                        sysm.is_complex, precision, tol, maxiter, X0)
+    X, info = krylov_solve(m.row_ptr, m.col_idx, m.diag, m.nn, m.nnz, sysm.vals64, sysm.vals32, B, k, sysm.nsys,
+                           sysm.is_complex, precision, tol, maxiter, X0)
+    if info.flag == 4 and sysm.nsys == 1:
+        X, info = bicgstab_solve(m.row_ptr, m.col_idx, m.diag, m.nn, m.nnz, sysm.vals64, B, sysm.is_complex, tol,
+                                 maxiter, X)
+    return X, info
 
 
 # ---------------------------------------------------------------------------

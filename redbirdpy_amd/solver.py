@@ -1,14 +1,16 @@
 """Linear solves (drop-in for redbirdpy.solver.femsolve on the hot path).
 
-Every ``method`` of the reference (solver.py:385-948) maps onto the GPU
-multi-RHS Krylov solver: Jacobi CG for real symmetric matrices and COCG
-(unconjugated) for complex symmetric ones, all columns in one launch
-sequence.  Direct methods ('auto', 'direct', 'superlu', 'pardiso',
-'umfpack', 'cholmod') are emulated with a tight relative-residual
-tolerance of 1e-12 (override with ``tol``); iterative methods use the
-reference default tol=1e-10 and maxiter=1000.  Extra keyword
-``precision='mixed'`` runs the inner iterations in float32/complex64 with
-float64 residual refinement.
+Every ``method`` of the reference (solver.py:385-948) maps onto a GPU
+multi-RHS Krylov solver, all columns in one launch sequence: Jacobi CG for
+real symmetric matrices and COCG (unconjugated) for complex symmetric ones,
+and Jacobi BiCGSTAB for 'bicgstab', 'gmres' and 'qmr' (the reference's
+general-matrix methods), for any matrix that is not symmetric, and to
+finish a CG/COCG column that breaks down.  Direct methods ('auto',
+'direct', 'superlu', 'pardiso', 'umfpack', 'cholmod') are emulated with a
+tight relative-residual tolerance of 1e-12 (override with ``tol``);
+iterative methods use the reference default tol=1e-10 and maxiter=1000.
+Extra keyword ``precision='mixed'`` runs the CG/COCG inner iterations in
+float32/complex64 with float64 residual refinement.
 """
 
 __all__ = ["femsolve", "solverinfo"]
@@ -21,6 +23,7 @@ from . import _gpu
 
 _DIRECT = {"auto", "direct", "superlu", "pardiso", "umfpack", "cholmod"}
 _ITER = {"cg", "cg+amg", "gmres", "bicgstab", "qmr", "minres", "minres+amg", "blqmr"}
+_GENERAL = {"bicgstab", "gmres", "qmr"}  # methods the reference applies to general matrices
 
 last_info = None  # SolveInfo of the most recent femsolve / runforward solve
 
@@ -54,8 +57,11 @@ def femsolve(Amat, rhs, method="auto", **kwargs):
     A.sort_indices()
     is_complex = np.iscomplexobj(A.data) or np.iscomplexobj(rhs)
     asym = abs(A - A.T)
-    if asym.nnz and asym.max() > 1e-12 * (abs(A).max() if A.nnz else 1.0):
-        raise NotImplementedError("femsolve on the GPU supports symmetric (or complex-symmetric) matrices only")
+    symmetric = not (asym.nnz and asym.max() > 1e-12 * (abs(A).max() if A.nnz else 1.0))
+    use_bicgstab = method in _GENERAL or not symmetric
+    diag_ok = np.all(np.diff(A.indptr) > 0) and np.all(_diag_slots(A.indptr, A.indices, n) >= 0)
+    if not diag_ok:
+        raise ValueError("femsolve: every row needs a stored diagonal entry (Jacobi preconditioning)")
     if method in _DIRECT:
         tol = kwargs.get("tol", 1e-12)
         tol = min(tol, 1e-12)
@@ -82,8 +88,14 @@ def femsolve(Amat, rhs, method="auto", **kwargs):
             x0a = np.asarray(x0).reshape(n, -1)
             X0 = torch.as_tensor(np.ascontiguousarray(x0a[:, c0:c1] if x0a.shape[1] > 1 else x0a), device=dev)
             X0 = X0.to(rdt).expand(n, c1 - c0).contiguous()
-        X, info = _gpu.krylov_solve(row_ptr, col_idx, diag, n, A.nnz, vals, None, B, c1 - c0, 1, is_complex,
-                                    precision, tol, maxiter, X0)
+        if use_bicgstab:
+            X, info = _gpu.bicgstab_solve(row_ptr, col_idx, diag, n, A.nnz, vals, B, is_complex, tol, maxiter, X0)
+        else:
+            X, info = _gpu.krylov_solve(row_ptr, col_idx, diag, n, A.nnz, vals, None, B, c1 - c0, 1, is_complex,
+                                        precision, tol, maxiter, X0)
+            if info.flag == 4:  # CG/COCG breakdown: BiCGSTAB from the last iterate
+                X, info = _gpu.bicgstab_solve(row_ptr, col_idx, diag, n, A.nnz, vals, B, is_complex, tol, maxiter,
+                                              X)
         x[:, c0:c1] = X.cpu().numpy()
         flag = max(flag, info.flag)
     last_info = info
@@ -100,7 +112,8 @@ def solverinfo():
         "has_amg": False,
         "has_blqmr": False,
         "complex_direct": "gpu-cocg",
-        "complex_iterative": ["cocg"],
+        "complex_iterative": ["cocg", "bicgstab"],
+        "general_iterative": ["bicgstab"],
         "symmetric_iterative": ["cg"],
         "precision": ["fp64", "mixed"],
         "cpu_count": 1,
