@@ -9,7 +9,7 @@ numpy/scipy objects like the reference; ``forward_device`` keeps results on
 the GPU for the reconstruction loop and for autograd.
 """
 
-__all__ = ["runforward", "femlhs", "femrhs", "femgetdet", "jac", "jacchrome", "C0"]
+__all__ = ["runforward", "femlhs", "femrhs", "femgetdet", "jac", "jacchrome", "C0", "forward_batch"]
 
 import numpy as np
 import torch
@@ -204,6 +204,142 @@ def locate_optodes(cfg):
     wvs = list(cfg["prop"].keys()) if isinstance(cfg.get("prop"), dict) else [""]
     mesh = _gpu.mesh_for(cfg)
     return {wv: _Optodes(cfg, mesh, wv) for wv in wvs}
+
+
+def _element_props(cfg):
+    """(ne x 4) element rows [mua, mus', 0, n] equivalent to cfg's label / element properties."""
+    props, _, _ = _gpu.props_for(cfg)
+    kind, plen, arrays = _gpu.expand_props(cfg, props)
+    ne = cfg["elem"].shape[0]
+    if arrays[0] == "label":
+        tab, seg = arrays[1], np.asarray(arrays[2]).astype(np.int64)
+        return np.column_stack([tab[seg, 0], tab[seg, 1], np.zeros(ne), tab[seg, 2]])
+    mua, musp, nref = arrays[1], arrays[2], np.asarray(arrays[3], dtype=float)
+    if len(mua) != ne:
+        raise NotImplementedError("forward_batch takes label or element properties (node properties: one by one)")
+    nref = nref if len(nref) == ne else np.full(ne, nref.ravel()[0])
+    return np.column_stack([mua, musp, np.zeros(ne), nref])
+
+
+class BatchResult:
+    """Per-mesh outputs of forward_batch: detval (ndet x nsrc), phi (nn x k, [src | det] columns),
+    and the node Jacobian when asked for."""
+
+    def __init__(self, detval, phi, jac=None):
+        self.detval = detval
+        self.phi = phi
+        self.jac = jac
+
+
+def forward_batch(cfgs, precision="fp64", tol=1e-12, maxiter=None, jacobian=False):
+    """Forward problems of several meshes in one multi-RHS solve (one launch sequence for all).
+
+    cfgs: meshprep'ed single-wavelength cfgs with label or element
+    properties, the same omega and the same Reff (the Robin coefficient is
+    one per assembly).  The meshes are translated apart (FEM matrices do not
+    depend on translation; the node order then keeps each mesh contiguous),
+    assembled as one block-diagonal system, and every mesh puts its sources
+    in the shared columns [0, TS) and its detectors in [TS, TS + TD) of its
+    own rows.  The solve runs to tol / sqrt(number of meshes) relative to a
+    column's norm over the batch, which bounds each mesh's own relative
+    residual by tol.  Returns a BatchResult per cfg.
+    """
+    if not cfgs:
+        return []
+    for c in cfgs:
+        if isinstance(c.get("prop"), dict):
+            raise NotImplementedError("forward_batch takes single-wavelength cfgs")
+    om = float(cfgs[0].get("omega", 0))
+    reff = float(cfgs[0]["reff"])
+    if any(float(c.get("omega", 0)) != om or float(c["reff"]) != reff for c in cfgs):
+        raise NotImplementedError("forward_batch needs one omega and one Reff for the whole batch")
+    nodes, elems, faces, evol, area, dd, props, shifts, noff = [], [], [], [], [], [], [], [], []
+    x0, nn = 0.0, 0
+    for c in cfgs:
+        nd = np.asarray(c["node"], dtype=float)
+        shift = np.array([x0 - nd[:, 0].min(), 0.0, 0.0])
+        x0 += (nd[:, 0].max() - nd[:, 0].min()) + 10.0
+        nodes.append(nd + shift)
+        elems.append(np.asarray(c["elem"])[:, :4].astype(np.int64) + nn)
+        faces.append(np.asarray(c["face"])[:, :3].astype(np.int64) + nn)
+        evol.append(np.asarray(c["evol"], dtype=float))
+        area.append(np.asarray(c["area"], dtype=float))
+        if c.get("deldotdel") is None:
+            c["deldotdel"], _ = deldotdel(c)
+        dd.append(np.asarray(c["deldotdel"], dtype=float))
+        props.append(_element_props(c))
+        shifts.append(shift)
+        noff.append(nn)
+        nn += nd.shape[0]
+    big = {"node": np.vstack(nodes), "elem": np.vstack(elems), "face": np.vstack(faces),
+           "evol": np.concatenate(evol), "area": np.concatenate(area), "prop": np.vstack(props),
+           "omega": om, "reff": reff, "isreoriented": True}
+    big["deldotdel"] = np.vstack(dd)
+    mesh = _gpu.mesh_for(big)
+    sysm = _gpu.assemble(mesh, big, big["deldotdel"], [("", 1)], want32=(precision == "mixed"))
+    ops = []
+    for m, c in enumerate(cfgs):
+        src, det, wsrc, wdet = getoptodes(c)
+        op = _Optodes.__new__(_Optodes)
+        op.ns, op.nd, op.ws, op.wd = optode_counts(c)
+        op.src = np.atleast_2d(src) + shifts[m] if op.ns else None
+        op.det = np.atleast_2d(det) + shifts[m] if op.nd else None
+        op.sloc, op.sbary = _locate(mesh, op.src)
+        op.dloc, op.dbary = _locate(mesh, op.det)
+        nm = np.asarray(c["node"]).shape[0]
+
+        def embed(w):  # this mesh's wide columns in the batch's node rows
+            if w is None:
+                return None
+            e = np.zeros((nn, w.shape[1]))
+            e[noff[m] : noff[m] + nm] = w
+            return e
+
+        op.wsrc = embed(np.asarray(wsrc)) if op.ws else None
+        op.wdet = embed(np.asarray(wdet)) if op.wd else None
+        op.k = op.ns + op.ws + op.nd + op.wd
+        ops.append(op)
+    TS = max(o.ns + o.ws for o in ops)
+    TD = max(o.nd + o.wd for o in ops)
+    C = TS + TD
+    if C > 512:
+        raise ValueError("forward_batch: at most 512 source + detector columns per mesh")
+    dt = torch.complex128 if sysm.is_complex else torch.float64
+    B = _gpu.zeros((mesh.nn, C), dt, mesh.dev)
+    for m, op in enumerate(ops):
+        rows = torch.as_tensor(mesh.iperm[noff[m] : noff[m] + np.asarray(cfgs[m]["node"]).shape[0]],
+                               device=mesh.dev)
+        if op.ns:
+            _gpu.rhs_block(mesh, B, 0, _loc0(op.sloc), op.sbary, sysm.is_complex)
+        if op.ws:
+            B[rows, op.ns : op.ns + op.ws] = torch.as_tensor(op.wsrc[noff[m] : noff[m] + len(rows)], device=mesh.dev,
+                                                             dtype=dt)
+        if op.nd:
+            _gpu.rhs_block(mesh, B, TS, _loc0(op.dloc), op.dbary, sysm.is_complex)
+        if op.wd:
+            B[rows, TS + op.nd : TS + op.nd + op.wd] = torch.as_tensor(op.wdet[noff[m] : noff[m] + len(rows)],
+                                                                       device=mesh.dev, dtype=dt)
+    if maxiter is None:
+        maxiter = max(4000, 4 * int(np.sqrt(mesh.nn)) + 1000)
+    X, info = _gpu.solve_system(sysm, B, C, precision, tol / np.sqrt(len(cfgs)), maxiter)
+    Xu = mesh.user_rows(X)
+    out = []
+    for m, op in enumerate(ops):
+        ts, td = op.ns + op.ws, op.nd + op.wd
+        det_val = None
+        if ts and td:
+            p, r, v = op.det_columns(big, mesh)
+            det_val = _gpu.getdet(p, r, v, X, 0, ts, sysm.is_complex).cpu().numpy()
+        nm = np.asarray(cfgs[m]["node"]).shape[0]
+        blk = Xu[noff[m] : noff[m] + nm]
+        phi = torch.cat([blk[:, :ts], blk[:, TS : TS + td]], dim=1).cpu().numpy()
+        J = None
+        if jacobian and ts and td:
+            sd = sdmap(cfgs[m])
+            J, _ = jac(sd, phi, cfgs[m]["deldotdel"], cfgs[m]["elem"], cfgs[m]["evol"], return_elem=False)
+        out.append(BatchResult(det_val, phi, J))
+    forward_batch.last_info = info
+    return out
 
 
 def forward_device(cfg, rfcw=(1,), precision="fp64", tol=1e-12, maxiter=None, X0=None, optodes=None,

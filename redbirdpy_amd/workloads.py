@@ -71,6 +71,46 @@ def widefield_box(h=2.0, srctype="planar", kx=0, ky=0, pattern=None, box=BOX):
     return cfg
 
 
+def batch_boxes(count=64, seed=5, sides=(20.0, 80.0), spacing=(0.8, 2.5), nsrc=(8, 128), nn_range=(2000, 200000),
+                kf=4, omega=0.0):
+    """Uneven batch: Kuhn boxes with seeded side lengths and spacing (resampled until the node
+    count lies in nn_range), a seeded spherical inclusion (mua in [0.005, 0.03], mus' in
+    [0.5, 1.5], radius 3-10 mm, at most 0.3 of the shortest side) on a [0.01, 1.0] background, nsrc point sources on a z=0
+    grid plus kf*kf Fourier patterns over the top face, and as many detectors."""
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < count:
+        box = tuple(rng.uniform(*sides, 3))
+        h = rng.uniform(*spacing)
+        dims = [int(round(b / h)) + 1 for b in box]
+        nn = dims[0] * dims[1] * dims[2]
+        if not (nn_range[0] <= nn <= nn_range[1]):
+            continue
+        box = tuple((d - 1) * h for d in dims)
+        node, elem = kuhn_box([0, 0, 0], box, h)
+        r = min(rng.uniform(3.0, 10.0), 0.3 * min(box))  # the sphere stays inside the box
+        cen = np.array([rng.uniform(r, b - r) for b in box])
+        inc = sphere_labels(node, elem, cen, r)
+        prop = np.array([[0, 0, 1, 1], [0.01, 1.0, 0, 1.37],
+                         [rng.uniform(0.005, 0.03), rng.uniform(0.5, 1.5), 0, 1.37]])
+        ns = int(rng.integers(nsrc[0], nsrc[1] + 1))
+        g = int(np.ceil(np.sqrt(ns)))
+        xs = np.linspace(0.15 * box[0], 0.85 * box[0], g)
+        ys = np.linspace(0.15 * box[1], 0.85 * box[1], g)
+        grid = np.array([[x, y, 0.0] for y in ys for x in xs])[:ns]
+        cfg = {"node": node, "elem": elem, "prop": prop, "seg": inc,
+               "srcpos": grid, "srcdir": np.array([[0.0, 0.0, 1.0]]),
+               "detpos": grid + np.array([0.5 * (xs[1] - xs[0]) if g > 1 else 1.0, 0.0, 0.0]),
+               "detdir": np.array([[0.0, 0.0, 1.0]]), "omega": omega}
+        if kf:
+            cfg["widesrcid"] = {"": {"srctype": ["fourier"], "srcid": [ns],
+                                     "srcparam1": [np.array([0.7 * box[0], 0, 0, kf])],
+                                     "srcparam2": [np.array([0, 0.7 * box[1], 0, kf])]}}
+            cfg["srcpos"] = np.vstack([grid, [[0.15 * box[0], 0.15 * box[1], 0.0]]])
+        out.append(cfg)
+    return out
+
+
 def fd_box(h=1.0, box=BOX, n_lattice=8, omega=OMEGA_100MHZ):
     node, elem = _base(h, box)
     lab = sphere_labels(node, elem, SPHERE_C, SPHERE_R, inside=2, outside=1)
