@@ -56,7 +56,7 @@ def _args():
     p.add_argument("--gn-precisions", default="mixed,fp64", help="comma list of GN legs to run")
     p.add_argument("--no-kernels", action="store_true", help="skip the Jacobian and Gram kernel legs")
     p.add_argument("--gn-phases", action="store_true", help="add a synchronising per-phase timing run")
-    p.add_argument("--cpu-h", type=float, default=1.5, help="mesh spacing of the bounded CPU sample")
+    p.add_argument("--cpu-h", type=float, default=1.3, help="mesh spacing of the bounded CPU sample")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only copy 512 MiB three times (reference kernel for the FETCH_SIZE/WRITE_SIZE scale)")
     return p.parse_args()
