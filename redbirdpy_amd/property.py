@@ -9,7 +9,7 @@ override (a dict wavelength -> {chromophore: value} or an Nwv x Nchrome
 matrix) in place of the table; with it, no default water/lipids are added.
 """
 
-__all__ = ["extinction", "updateprop", "getbulk", "get_chromophore_table"]
+__all__ = ["extinction", "updateprop", "getbulk", "get_chromophore_table", "musp2sasp", "setmesh"]
 
 import os
 import weakref
@@ -209,3 +209,44 @@ def getbulk(cfg):
     if isinstance(prop, dict):
         return {wv: _bulk_of(np.atleast_2d(p), cfg) for wv, p in prop.items()}
     return _bulk_of(np.atleast_2d(prop), cfg)
+
+
+def musp2sasp(musp, wavelength):
+    """Scattering amplitude and power from mus' at two wavelengths (property.py:335-366).
+
+    Model mus'(lambda) = sa (lambda / 500 nm)^(-sp): sp from the ratio of the
+    two values, sa the mean of the two back-solved amplitudes.
+    """
+    musp = np.asarray(musp, dtype=float)
+    wavelength = np.asarray(wavelength, dtype=float)
+    if len(musp) < 2 or len(wavelength) < 2:
+        raise ValueError("Need at least 2 wavelengths to fit scattering parameters")
+    x = wavelength[:2] / 500.0
+    sp = np.log(musp[0] / musp[1]) / np.log(x[1] / x[0])
+    sa = 0.5 * (musp[0] * x[0] ** sp + musp[1] * x[1] ** sp)
+    return sa, sp
+
+
+_MESH_DERIVED = ("face", "area", "evol", "deldotdel", "isreoriented", "nvol", "cols", "rows", "idxsum", "idxcount",
+                 "musp0", "reff")
+
+
+def setmesh(cfg0, node, elem, prop=None, propidx=None):
+    """A copy of cfg0 on a new mesh, its mesh-derived fields recomputed by meshprep (property.py:369-432).
+
+    elem is 1-based; a fifth column (or propidx) gives the labels.
+    """
+    from .utility import meshprep
+
+    elem = np.asarray(elem)
+    cfg = {k: v for k, v in cfg0.items() if k not in _MESH_DERIVED}
+    cfg["node"] = node
+    cfg["elem"] = elem[:, :4] if elem.shape[1] > 4 else elem
+    if prop is not None:
+        cfg["prop"] = prop
+    if propidx is not None:
+        cfg["seg"] = propidx
+    elif elem.shape[1] > 4:
+        cfg["seg"] = elem[:, 4].astype(int)
+    cfg, _ = meshprep(cfg)
+    return cfg

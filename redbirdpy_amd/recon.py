@@ -17,7 +17,7 @@ device:
   perturb the reconstructed parameters by ~1e-6 relative).
 """
 
-__all__ = ["runrecon", "reginv", "reginvover", "reginvunder", "matreform", "matflat", "syncprop"]
+__all__ = ["runrecon", "reginv", "reginvover", "reginvunder", "matreform", "matflat", "syncprop", "prior"]
 
 import time
 import numpy as np
@@ -92,6 +92,45 @@ def matflat(Amat, weight=None):
             return np.vstack([np.hstack([Amat[k][wv] * w[j] for j, k in enumerate(keys)]) for wv in inner])
         return np.hstack([Amat[k] * w[i] for i, k in enumerate(keys)])
     return Amat
+
+
+def prior(seg, priortype, params=None):
+    """Structure-prior regularisation matrix (recon.py:596-666).
+
+    1-D labels: 'laplace' gives L = I with -1/|g| between distinct members of
+    each label group g of more than one member; 'helmholtz' uses
+    -1/(|g| + beta).  A 2-D composition (n x nc) with 'comp' links members
+    whose compositions differ by dval = sum |s_i - s_j| < alpha nc with
+    -alpha - dval/nc, normalised by beta sqrt(|rowsum_i| |rowsum_j|), plus I
+    (sparse).  Anything else gives None.
+    """
+    if not priortype:
+        return None
+    params = params or {}
+    seg = np.asarray(seg)
+    if seg.ndim == 1:
+        if priortype not in ("laplace", "helmholtz"):
+            return None
+        _, inv = np.unique(seg, return_inverse=True)
+        c = np.bincount(inv)[inv].astype(float)  # size of each member's group
+        extra = 0.0 if priortype == "laplace" else params.get("beta", 1.0)
+        same = inv[:, None] == inv[None, :]
+        L = np.where(same & (c[:, None] > 1), -1.0 / (c[:, None] + extra), 0.0)
+        np.fill_diagonal(L, 1.0)
+        return L
+    if priortype == "comp" and seg.ndim == 2:
+        from scipy import sparse as sp
+
+        alpha = params.get("alpha", 0.1)
+        beta = params.get("beta", 1.0)
+        n, nc = seg.shape
+        dval = np.abs(seg[:, None, :] - seg[None, :, :]).sum(axis=2)
+        link = (dval < alpha * nc) & ~np.eye(n, dtype=bool)
+        W = np.where(link, -alpha - dval / nc, 0.0)
+        rs = np.abs(W.sum(axis=1))
+        W = np.where(link, W / (beta * np.sqrt(rs[:, None] * rs[None, :] + 1e-16)), 0.0)
+        return (sp.csr_matrix(W) + sp.eye(n)).tocsr()
+    return None
 
 
 def _dual(recon):

@@ -17,9 +17,17 @@ __all__ = [
     "forcearray",
     "src2bc",
     "meshinterp",
+    "addnoise",
+    "HAS_ISO2MESH",
 ]
 
+import warnings
+
 import numpy as np
+
+# iso2mesh is not used: plane cuts, point location and boundary faces are
+# computed here and on the GPU (the flag keeps the reference's name)
+HAS_ISO2MESH = False
 
 _ARRAY_KEYS = ["node", "elem", "face", "srcpos", "srcdir", "detpos", "detdir", "prop", "seg", "widesrc", "widedet"]
 
@@ -408,6 +416,34 @@ def src2bc(cfg, isdet=False):
     cfg[mapk] = mapping
     cfg[pk] = pos[keep]
     return cfg
+
+
+def addnoise(data, snrshot, snrthermal=np.inf, randseed=123456789):
+    """Shot and thermal noise at the given SNRs in dB (utility.py:932-970).
+
+    Shot noise scales with sqrt(|data|) (sigma 10^(-snr/20)), thermal noise
+    with the largest amplitude; complex data also get phase noise from the
+    imaginary parts of the SNRs.  The draws come from numpy's global
+    generator seeded with randseed, in the reference's order, so a seed
+    gives the reference's noise.
+    """
+    np.random.seed(randseed)
+    data = np.asarray(data)
+    if np.isinf(snrshot) and np.isinf(snrthermal):
+        warnings.warn("No noise added")
+        return data.copy()
+    s_shot = 10 ** (-np.real(snrshot) / 20)
+    s_therm = np.max(np.abs(data)) * 10 ** (-np.real(snrthermal) / 20)
+    if np.isreal(data).all():
+        out = data + np.sqrt(np.abs(data)) * np.random.randn(*data.shape) * s_shot
+        return out + np.random.randn(*data.shape) * s_therm
+    ph_shot = 10 ** (-np.imag(snrshot) / 20)
+    ph_therm = 10 ** (-np.imag(snrthermal) / 20)
+    a1 = np.random.randn(*data.shape) * s_shot
+    p1 = np.random.randn(*data.shape) * ph_shot * 2 * np.pi
+    a2 = np.random.randn(*data.shape) * s_therm
+    p2 = np.random.randn(*data.shape) * ph_therm * 2 * np.pi
+    return data + np.sqrt(np.abs(data)) * (a1 * np.exp(1j * p1)) + a2 * np.exp(1j * p2)
 
 
 def meshinterp(fromval, elemid, elembary, fromelem, toval=None):

@@ -229,3 +229,17 @@ def test_femsolve_sweep_many_chunk_groups(precision, monkeypatch):
     x, flag = rb.femsolve(A, b, tol=1e-12, precision=precision)
     assert flag == 0
     assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-9
+
+
+def test_setmesh_recomputes_mesh_fields():
+    import redbirdpy_amd as rb
+    from redbirdpy_amd.mesh import kuhn_box
+
+    cfg, _ = rb.meshprep(box_cfg((12, 10, 6), 2.0))
+    node, elem = kuhn_box([0, 0, 0], [16, 12, 8], 2.0)
+    c2 = rb.setmesh(cfg, node, np.column_stack([elem, np.ones(len(elem), dtype=int)]))
+    assert c2["node"] is node and c2["elem"].shape == elem.shape
+    assert c2["face"].shape[0] == 4 * (8 * 6 + 8 * 4 + 6 * 4)  # Kuhn box boundary triangles
+    assert c2["evol"].sum() == pytest.approx(16 * 12 * 8, rel=1e-12)
+    assert np.array_equal(c2["seg"], np.ones(len(elem), dtype=int))
+    assert cfg["face"].shape[0] != c2["face"].shape[0]  # the original is left alone
