@@ -14,7 +14,8 @@
 #include "rb_common.h"
 #include "../../include/rbhip.h"
 
-#define JW 64   // threads per block (one wavefront per node)
+#define JW 64   // tets staged at a time when building a node's patch matrix
+#define JB 256  // threads per node workgroup
 #define JC 32   // sources / detectors per staged chunk
 
 template <typename TO, typename T>
@@ -35,26 +36,26 @@ __device__ inline cplx<double> jmul(cplx<double> a, cplx<double> b) { return a *
 __device__ inline double jscale(double s, double v) { return s * v; }
 __device__ inline cplx<double> jscale(double s, cplx<double> v) { return mk<double>(s * v.x, s * v.y); }
 
-// Q_n (np x np, row-major in LDS) from the node's incident tets, 64 tets at
-// a time: each lane loads one tet (vertices, volume) and finds its vertices'
-// positions in the sorted patch (tl, tv in LDS), then one lane per patch row
-// adds the tets holding that row's node in ascending tet order (the same
-// order for every run, so Q is bit-reproducible).  One wavefront per block.
+// Q_n (np x np, row-major in LDS) from the node's incident tets, JW tets at
+// a time: a thread per staged tet loads it (vertices, volume) and finds its
+// vertices' positions in the sorted patch (tl, tv in LDS); then a thread per
+// entry (k, l) of Q adds V (2V on the diagonal) of every staged tet holding
+// both patch nodes, tets in ascending order (so Q is bit-reproducible).
 __device__ inline void build_patch_q(int64_t n, const int32_t* el, const double* evol, const int32_t* inc_ptr,
                                      const int32_t* inc_list, const int32_t* cols, int np, double* Q,
-                                     int32_t* tl /* 64 x 4 */, double* tv /* 64 */) {
-  const int lane = threadIdx.x;
-  for (int i = lane; i < np * np; i += JW) Q[i] = 0.0;
+                                     int32_t* tl /* JW x 4 */, double* tv /* JW */) {
+  const int t = threadIdx.x;
+  for (int i = t; i < np * np; i += JB) Q[i] = 0.0;
   const int32_t t0 = inc_ptr[n], t1 = inc_ptr[n + 1];
   for (int32_t c0 = t0; c0 < t1; c0 += JW) {
     const int nt = min(JW, t1 - c0);
-    if (lane < nt) {
-      const int32_t e = inc_list[c0 + lane] >> 2;
+    if (t < nt) {
+      const int32_t e = inc_list[c0 + t] >> 2;
       const int32_t* q = el + 4 * (int64_t)e;
       int32_t v[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) v[j] = q[j];
-      tv[lane] = evol[e];
+      tv[t] = evol[e];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {  // position of v[j] in the sorted patch (every tet vertex is a neighbour)
         int lo = 0, hi = np - 1, pos = -1;
@@ -70,28 +71,34 @@ __device__ inline void build_patch_q(int64_t n, const int32_t* el, const double*
           else
             hi = mid - 1;
         }
-        tl[lane * 4 + j] = pos;
+        tl[t * 4 + j] = pos;
       }
     }
     __syncthreads();
-    for (int k = lane; k < np; k += JW) {
-      double* qk = Q + k * np;
-      for (int t = 0; t < nt; ++t) {
-        const int32_t l0 = tl[t * 4], l1 = tl[t * 4 + 1], l2 = tl[t * 4 + 2], l3 = tl[t * 4 + 3];
-        if (l0 != k && l1 != k && l2 != k && l3 != k) continue;
-        const double V = tv[t];
-        if (l0 >= 0) qk[l0] += (l0 == k) ? 2.0 * V : V;
-        if (l1 >= 0) qk[l1] += (l1 == k) ? 2.0 * V : V;
-        if (l2 >= 0) qk[l2] += (l2 == k) ? 2.0 * V : V;
-        if (l3 >= 0) qk[l3] += (l3 == k) ? 2.0 * V : V;
+    for (int i = t; i < np * np; i += JB) {
+      const int k = i / np, l = i - (i / np) * np;
+      double acc = Q[i];
+      for (int u = 0; u < nt; ++u) {
+        const int32_t l0 = tl[u * 4], l1 = tl[u * 4 + 1], l2 = tl[u * 4 + 2], l3 = tl[u * 4 + 3];
+        const bool hk = l0 == k || l1 == k || l2 == k || l3 == k;
+        const bool hl = l0 == l || l1 == l || l2 == l || l3 == l;
+        if (hk && hl) acc += (k == l) ? 2.0 * tv[u] : tv[u];
       }
+      Q[i] = acc;
     }
     __syncthreads();
   }
 }
 
+// J_n = -0.0125 PS^T Q_n PD for every (s, d) chunk of JC x JC pairs, one
+// node per JB-thread workgroup: the node's patch rows of phi are staged in
+// LDS (PS, PD), Y = Q PD, then each thread forms JC*JC/JB consecutive
+// sources of one detector, so a wavefront's stores cover contiguous runs of
+// the node's JT row.
+#define JPT (JC * JC / JB)  // pairs per thread
+static_assert(JC % JPT == 0, "a thread's pairs must lie in one detector row");
 template <typename T, typename TO>
-__global__ void __launch_bounds__(JW) k_jac_node(int64_t nn, const int32_t* __restrict__ el,
+__global__ void __launch_bounds__(JB) k_jac_node(int64_t nn, const int32_t* __restrict__ el,
                                                  const double* __restrict__ evol, const int32_t* __restrict__ inc_ptr,
                                                  const int32_t* __restrict__ inc_list,
                                                  const int32_t* __restrict__ row_ptr,
@@ -104,70 +111,56 @@ __global__ void __launch_bounds__(JW) k_jac_node(int64_t nn, const int32_t* __re
   T* PD = PS + maxnp * JC;                 // np x JC
   T* Y = PD + maxnp * JC;                  // np x JC
   double* Q = (double*)(Y + maxnp * JC);   // np x np
-  double* tv = Q + maxnp * maxnp;          // 64 staged tet volumes
-  int32_t* tl = (int32_t*)(tv + JW);       // their 64 x 4 patch positions
+  double* tv = Q + maxnp * maxnp;          // JW staged tet volumes
+  int32_t* tl = (int32_t*)(tv + JW);       // their JW x 4 patch positions
   int32_t* cols = tl + 4 * JW;
-  const int lane = threadIdx.x;
-  // lane -> 4x4 tile of the 32x32 chunk: 8 tiles along s, 8 along d
-  const int ts = (lane & 7) * 4;
-  const int td = (lane >> 3) * 4;
+  const int t = threadIdx.x;
+  const int jd = (t * JPT) / JC;           // detector of this thread's pairs within the chunk
+  const int js = (t * JPT) - jd * JC;      // its first source
   for (int64_t n = blockIdx.x; n < nn; n += gridDim.x) {
-    int32_t r0 = row_ptr[n];
-    int np = row_ptr[n + 1] - r0;
-    for (int k = lane; k < np; k += JW) cols[k] = col_idx[r0 + k];
+    const int32_t r0 = row_ptr[n];
+    const int np = row_ptr[n + 1] - r0;
+    for (int k = t; k < np; k += JB) cols[k] = col_idx[r0 + k];
     __syncthreads();
     build_patch_q(n, el, evol, inc_ptr, inc_list, cols, np, Q, tl, tv);
-    __syncthreads();
     for (int s0 = 0; s0 < Ns; s0 += JC) {
-      int ns = min(JC, Ns - s0);
-      for (int i = lane; i < np * JC; i += JW) {
-        int k = i / JC, j = i - k * JC;
+      const int ns = min(JC, Ns - s0);
+      for (int i = t; i < np * JC; i += JB) {
+        const int k = i / JC, j = i - k * JC;
         PS[i] = (j < ns) ? phi[(int64_t)cols[k] * ldphi + scol[s0 + j]] : sc<T>::zero();
       }
       for (int d0 = 0; d0 < Nd; d0 += JC) {
-        int nd = min(JC, Nd - d0);
-        for (int i = lane; i < np * JC; i += JW) {
-          int k = i / JC, j = i - k * JC;
+        const int nd = min(JC, Nd - d0);
+        for (int i = t; i < np * JC; i += JB) {
+          const int k = i / JC, j = i - k * JC;
           PD[i] = (j < nd) ? phi[(int64_t)cols[k] * ldphi + dcol[d0 + j]] : sc<T>::zero();
         }
         __syncthreads();
         // Y = Q PD
-        for (int i = lane; i < np * JC; i += JW) {
-          int k = i / JC, j = i - k * JC;
+        for (int i = t; i < np * JC; i += JB) {
+          const int k = i / JC, j = i - k * JC;
           T acc = sc<T>::zero();
           const double* qk = Q + k * np;
           for (int l = 0; l < np; ++l) acc = acc + jscale(qk[l], PD[l * JC + j]);
           Y[i] = acc;
         }
         __syncthreads();
-        T acc[4][4];
+        // pairs (s0 + js + a, d0 + jd): PS^T Y
+        T acc[JPT];
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-          for (int b = 0; b < 4; ++b) acc[a][b] = sc<T>::zero();
+        for (int a = 0; a < JPT; ++a) acc[a] = sc<T>::zero();
         for (int k = 0; k < np; ++k) {
-          T ps[4], y[4];
+          const T y = Y[k * JC + jd];
 #pragma unroll
-          for (int a = 0; a < 4; ++a) {
-            ps[a] = PS[k * JC + ts + a];
-            y[a] = Y[k * JC + td + a];
-          }
-#pragma unroll
-          for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) acc[a][b] = acc[a][b] + jmul(ps[a], y[b]);
+          for (int a = 0; a < JPT; ++a) acc[a] = acc[a] + jmul(PS[k * JC + js + a], y);
         }
+        if (jd < nd) {
+          const int64_t base = (int64_t)(d0 + jd) * Ns + s0 + js;
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          int d = d0 + td + b;
-          if (td + b >= nd) continue;
-#pragma unroll
-          for (int a = 0; a < 4; ++a) {
-            int s = s0 + ts + a;
-            if (ts + a >= ns) continue;
-            int64_t cell = (int64_t)d * Ns + s;
-            int64_t p = pmap ? pmap[cell] : cell;
-            if (p >= 0) JT[n * ldj + p] = cast_out<TO, T>(jscale(-0.0125, acc[a][b]));
+          for (int a = 0; a < JPT; ++a) {
+            if (js + a >= ns) continue;
+            const int64_t p = pmap ? pmap[base + a] : base + a;
+            if (p >= 0) JT[n * ldj + p] = cast_out<TO, T>(jscale(-0.0125, acc[a]));
           }
         }
         __syncthreads();
@@ -229,7 +222,7 @@ extern "C" int rb_jac_grid(int64_t nn, int64_t ne, const int32_t* el, const doub
   if (!is_complex) {
     if (out_dtype == RB_F64) {
       if (JT)
-        hipLaunchKernelGGL((k_jac_node<double, double>), dim3(g), dim3(JW), lds, s, nn, el, evol, inc_ptr, inc_list,
+        hipLaunchKernelGGL((k_jac_node<double, double>), dim3(g), dim3(JB), lds, s, nn, el, evol, inc_ptr, inc_list,
                            row_ptr, col_idx, (const double*)phi, ldphi, Ns, Nd, scol, dcol, pmap, (double*)JT, ldj,
                            maxnp);
       if (Jelem && ne > 0)
@@ -237,7 +230,7 @@ extern "C" int rb_jac_grid(int64_t nn, int64_t ne, const int32_t* el, const doub
                            ldphi, Ns, Nd, scol, dcol, pmap, (double*)Jelem, ldj);
     } else if (out_dtype == RB_F32) {
       if (JT)
-        hipLaunchKernelGGL((k_jac_node<double, float>), dim3(g), dim3(JW), lds, s, nn, el, evol, inc_ptr, inc_list,
+        hipLaunchKernelGGL((k_jac_node<double, float>), dim3(g), dim3(JB), lds, s, nn, el, evol, inc_ptr, inc_list,
                            row_ptr, col_idx, (const double*)phi, ldphi, Ns, Nd, scol, dcol, pmap, (float*)JT, ldj,
                            maxnp);
       if (Jelem && ne > 0)
@@ -251,14 +244,14 @@ extern "C" int rb_jac_grid(int64_t nn, int64_t ne, const int32_t* el, const doub
     typedef cplx<double> C2;
     if (out_dtype == RB_C128) {
       if (JT)
-        hipLaunchKernelGGL((k_jac_node<C2, C2>), dim3(g), dim3(JW), lds, s, nn, el, evol, inc_ptr, inc_list, row_ptr,
+        hipLaunchKernelGGL((k_jac_node<C2, C2>), dim3(g), dim3(JB), lds, s, nn, el, evol, inc_ptr, inc_list, row_ptr,
                            col_idx, (const C2*)phi, ldphi, Ns, Nd, scol, dcol, pmap, (C2*)JT, ldj, maxnp);
       if (Jelem && ne > 0)
         hipLaunchKernelGGL((k_jac_elem<C2, C2>), dim3(ge), dim3(256), 0, s, ne, el, evol, (const C2*)phi, ldphi, Ns,
                            Nd, scol, dcol, pmap, (C2*)Jelem, ldj);
     } else if (out_dtype == RB_C64) {
       if (JT)
-        hipLaunchKernelGGL((k_jac_node<C2, cplx<float>>), dim3(g), dim3(JW), lds, s, nn, el, evol, inc_ptr, inc_list,
+        hipLaunchKernelGGL((k_jac_node<C2, cplx<float>>), dim3(g), dim3(JB), lds, s, nn, el, evol, inc_ptr, inc_list,
                            row_ptr, col_idx, (const C2*)phi, ldphi, Ns, Nd, scol, dcol, pmap, (cplx<float>*)JT, ldj,
                            maxnp);
       if (Jelem && ne > 0)
