@@ -278,6 +278,20 @@ int rb_jt_cols_workspace(int64_t K, int64_t M, size_t* bytes /* host */);
 int rb_jt_cols(int64_t K, int64_t M, const void* JT, int64_t ldj, int in_dtype, int nw, int64_t np,
                const double* z, int mode, double* u, void* work, size_t work_bytes, void* stream);
 
+/*
+ * The regularised normal-equation operator applied without forming it:
+ *   out = s2 * J W (J^T v) + lam * v,  (J^T v)[k][b] = sum_p JT[k][b*np+p] v[b*np+p]
+ * over a float64 node-major JT (K x M, M = nb * np), W an nb x nb float64
+ * block-mixing matrix (row-major).  redbirdpy recon.py:429-509 forms
+ * J J^T + lam I explicitly and solves it; this product drives iterative
+ * refinement of that solve.  One read of JT (rb_jt_rows then rb_jt_cols read
+ * it twice).  Needs np a multiple of 64, nb <= 16, M <= 8192 (RB_ERR_ARG
+ * otherwise) and rb_jt_hv_workspace() bytes; deterministic (fixed-order sums).
+ */
+int rb_jt_hv_workspace(int64_t K, int64_t M, size_t* bytes /* host */);
+int rb_jt_hv(int64_t K, int64_t M, const double* JT, int64_t ldj, int nb, int64_t np, const double* W,
+             const double* v, double s2, double lam, double* out, void* work, size_t work_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

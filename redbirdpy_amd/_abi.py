@@ -66,6 +66,8 @@ _SIGS = {
     "rb_jt_remap": [_i64, _vp, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp],
     "rb_jt_cols_workspace": [_i64, _i64, ctypes.POINTER(_sz)],
     "rb_jt_cols": [_i64, _i64, _vp, _i64, _int, _int, _i64, _vp, _int, _vp, _vp, _sz, _vp],
+    "rb_jt_hv_workspace": [_i64, _i64, ctypes.POINTER(_sz)],
+    "rb_jt_hv": [_i64, _i64, _vp, _i64, _int, _i64, _vp, _vp, _dbl, _dbl, _vp, _vp, _sz, _vp],
     "rb_prof_enable": [_int],
     "rb_prof_reset": [],
     "rb_prof_read": [_int, ctypes.POINTER(_dbl), ctypes.POINTER(_i64)],

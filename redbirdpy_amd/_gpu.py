@@ -480,6 +480,14 @@ def zeros(shape, dtype, dev):
     return t
 
 
+def to_host(t):
+    """Device tensor -> numpy array through page-locked memory (a full-rate DMA; torch's host
+    allocator caches the pinned block, so repeated copies of one size do not allocate again)."""
+    h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+    h.copy_(t)
+    return h.numpy()
+
+
 # ---------------------------------------------------------------------------
 # detector readout and Jacobian
 # ---------------------------------------------------------------------------
@@ -604,6 +612,28 @@ def jt_rows(JT, nw, npairs, v):
     v = v.to(torch.float64).contiguous()
     call("rb_jt_rows", K, M, ptr(JT), JT.stride(0), code, nw, npairs, ptr(v), ptr(t), _abi.stream_ptr())
     return t
+
+
+def jt_hv_ok(JT, nw, npairs):
+    """Whether rb_jt_hv takes this shape (float64, np a multiple of 64, nw <= 16, M <= 8192)."""
+    K, M = JT.shape
+    return (JT.dtype == torch.float64 and JT.stride(1) == 1 and npairs % 64 == 0 and 0 < nw <= 16
+            and nw * npairs == M and M <= 8192)
+
+
+def jt_hv(JT, W, npairs, v, s2, lam):
+    """s2 * J W (J^T v) + lam * v in one read of JT (float64 accumulation)."""
+    K, M = JT.shape
+    nw = W.shape[0]
+    W = W.to(torch.float64).contiguous()
+    v = v.to(torch.float64).contiguous()
+    out = torch.empty(M, dtype=torch.float64, device=JT.device)
+    nb = ctypes.c_size_t(0)
+    call("rb_jt_hv_workspace", K, M, ctypes.byref(nb))
+    work = torch.empty(int(nb.value), dtype=torch.uint8, device=JT.device)
+    call("rb_jt_hv", K, M, ptr(JT), JT.stride(0), nw, npairs, ptr(W), ptr(v), float(s2), float(lam), ptr(out),
+         ptr(work), nb.value, _abi.stream_ptr())
+    return out
 
 
 def remap_operator(mapid, mapweight, relem, nrec, perm_rows=None, dev=None):

@@ -376,6 +376,155 @@ extern "C" int rb_jt_cols(int64_t K, int64_t M, const void* JT, int64_t ldj, int
   return RB_OK;
 }
 
+// ---------------------------------------------------------------------------
+// K9b: the normal-equation product out = s2 * J W (J^T v) + lam * v in ONE read
+// of a float64 JT (K x M, M = nb blocks of np pairs, np a multiple of 64).
+// Per node k: t[b] = sum_p JT[k][b*np+p] v[b*np+p], z = t W, u[j] += JT[k][j] z[j/np].
+// A 256-thread workgroup holds NB node rows at a time, J columns per thread
+// (column 64*wave + 256*j + lane, so every 64-column segment lies in one
+// block); per-workgroup u goes to part[blockIdx][M] and a finish kernel sums
+// the parts in a fixed order.
+// ---------------------------------------------------------------------------
+#define HV_MAXB 16
+#define HV_MAXG 512
+
+__device__ __forceinline__ double hv_wave_sum(double a) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+  return a;
+}
+
+// does wave w own a 64-column segment of block b (segments s = w + 4j)
+__device__ __forceinline__ bool hv_wave_has(int w, int b, int64_t np) {
+  const int64_t q = np >> 6;
+  if (q >= 4) return true;
+  for (int64_t s = b * q; s < (b + 1) * q; ++s)
+    if ((s & 3) == w) return true;
+  return false;
+}
+
+template <int J, int NB>
+__global__ void __launch_bounds__(256) k_jt_hv(int64_t K, int64_t M, const double* __restrict__ JT, int64_t ldj,
+                                               int nb, int64_t np, const double* __restrict__ W,
+                                               const double* __restrict__ v, double* __restrict__ part) {
+  __shared__ double red[NB][4][HV_MAXB];
+  __shared__ double zs[NB][HV_MAXB];
+  const int t = threadIdx.x, lane = t & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+  double vv[J], u[J];
+  int bj[J];  // block of slot j (wave-uniform), -1 past M
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    const int64_t c0 = 64 * wv + 256 * j;
+    bj[j] = c0 < M ? (int)(c0 / np) : -1;
+    vv[j] = c0 < M ? v[c0 + lane] : 0.0;
+    u[j] = 0.0;
+  }
+  for (int64_t n0 = (int64_t)blockIdx.x * NB; n0 < K; n0 += (int64_t)gridDim.x * NB) {
+    double x[NB][J];
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+      for (int j = 0; j < J; ++j)
+        x[i][j] = (n0 + i < K && bj[j] >= 0) ? JT[(n0 + i) * ldj + 64 * wv + 256 * j + lane] : 0.0;
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      double acc = 0.0;
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        acc += x[i][j] * vv[j];
+        if (j == J - 1 || bj[j + 1] != bj[j]) {  // end of this wave's run in block bj[j]
+          if (bj[j] >= 0) {
+            const double r = hv_wave_sum(acc);
+            if (lane == 0) red[i][wv][bj[j]] = r;
+          }
+          acc = 0.0;
+        }
+      }
+    }
+    __syncthreads();
+    if (t < NB * nb) {
+      const int i = t / nb, b2 = t - (t / nb) * nb;
+      double z = 0.0;
+      for (int b = 0; b < nb; ++b) {
+        double tb = 0.0;
+        for (int w = 0; w < 4; ++w)
+          if (hv_wave_has(w, b, np)) tb += red[i][w][b];
+        z += tb * W[b * nb + b2];
+      }
+      zs[i][b2] = z;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+      for (int j = 0; j < J; ++j)
+        if (bj[j] >= 0) u[j] += x[i][j] * zs[i][bj[j]];
+  }
+#pragma unroll
+  for (int j = 0; j < J; ++j)
+    if (bj[j] >= 0) part[(int64_t)blockIdx.x * M + 64 * wv + 256 * j + lane] = u[j];
+}
+
+// out[p] = s2 * sum_g part[g][p] + lam * v[p]; 64 columns per workgroup, four
+// strided partial sums combined in a fixed order
+__global__ void __launch_bounds__(256) k_jt_hv_finish(int64_t M, int G, const double* __restrict__ part,
+                                                      const double* __restrict__ v, double s2, double lam,
+                                                      double* __restrict__ out) {
+  __shared__ double sh[4][64];
+  const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int64_t p = (int64_t)blockIdx.x * 64 + c;
+  double acc = 0.0;
+  if (p < M)
+    for (int b = g; b < G; b += 4) acc += part[(int64_t)b * M + p];
+  sh[g][c] = acc;
+  __syncthreads();
+  if (g == 0 && p < M) out[p] = s2 * (((sh[0][c] + sh[1][c]) + sh[2][c]) + sh[3][c]) + lam * v[p];
+}
+
+static int hv_cols(int64_t M) {  // columns per thread: the smallest J with 256 J >= M
+  return M <= 1024 ? 4 : M <= 2048 ? 8 : M <= 4096 ? 16 : M <= 8192 ? 32 : 0;
+}
+
+static unsigned hv_grid(int64_t K, int64_t M) {
+  const int64_t nbn = 64 / hv_cols(M);  // rows per workgroup pass
+  const int64_t g = ceil_div(K, nbn);
+  return (unsigned)(g < HV_MAXG ? g : HV_MAXG);
+}
+
+extern "C" int rb_jt_hv_workspace(int64_t K, int64_t M, size_t* bytes) {
+  if (!bytes || K <= 0 || M <= 0 || !hv_cols(M)) return RB_ERR_ARG;
+  *bytes = (size_t)hv_grid(K, M) * M * sizeof(double);
+  return RB_OK;
+}
+
+extern "C" int rb_jt_hv(int64_t K, int64_t M, const double* JT, int64_t ldj, int nb, int64_t np, const double* W,
+                        const double* v, double s2, double lam, double* out, void* work, size_t work_bytes,
+                        void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (K <= 0 || np <= 0 || np % 64 || nb < 1 || nb > HV_MAXB || (int64_t)nb * np != M || !hv_cols(M) || ldj < M) {
+    rb_set_error("rb_jt_hv: needs M = nb * np, np a multiple of 64, nb <= %d, M <= 8192", HV_MAXB);
+    return RB_ERR_ARG;
+  }
+  const unsigned g = hv_grid(K, M);
+  if (!work || work_bytes < (size_t)g * M * sizeof(double)) {
+    rb_set_error("rb_jt_hv: workspace too small");
+    return RB_ERR_ARG;
+  }
+  double* part = (double*)work;
+  switch (hv_cols(M)) {
+    case 4: hipLaunchKernelGGL((k_jt_hv<4, 16>), dim3(g), dim3(256), 0, s, K, M, JT, ldj, nb, np, W, v, part); break;
+    case 8: hipLaunchKernelGGL((k_jt_hv<8, 8>), dim3(g), dim3(256), 0, s, K, M, JT, ldj, nb, np, W, v, part); break;
+    case 16: hipLaunchKernelGGL((k_jt_hv<16, 4>), dim3(g), dim3(256), 0, s, K, M, JT, ldj, nb, np, W, v, part); break;
+    default: hipLaunchKernelGGL((k_jt_hv<32, 2>), dim3(g), dim3(256), 0, s, K, M, JT, ldj, nb, np, W, v, part); break;
+  }
+  RB_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_jt_hv_finish, dim3((unsigned)(M / 64)), dim3(256), 0, s, M, (int)g, (const double*)part, v,
+                     s2, lam, out);
+  RB_CHECK_LAUNCH();
+  return RB_OK;
+}
+
 extern "C" int rb_gram(int64_t K, int64_t m, const void* JT, int64_t ldj, int in_dtype, const double* d, int mode,
                        double* G, void* stream) {
   hipStream_t s = (hipStream_t)stream;

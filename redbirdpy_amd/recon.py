@@ -250,6 +250,8 @@ class GNStep:
     def _hv(self, v, s2, W):
         """H v = s^2 W o (J J^T) v + lambda v with float64 accumulation."""
         nw = W.shape[0]
+        if _gpu.jt_hv_ok(self.JT, nw, self.np):
+            return _gpu.jt_hv(self.JT, W, self.np, v, float(s2), self.lam)
         t = _gpu.jt_rows(self.JT, nw, self.np, v)  # (nn, nw)
         z = t @ W  # W symmetric
         u = _gpu.jt_cols(self.JT, nw, self.np, z)
@@ -264,7 +266,7 @@ class GNStep:
         tq = time.perf_counter()
         colsq = _gpu.jt_cols(JT, nw, npairs)  # exact ||J_w row||^2, float64
         wdiag = torch.repeat_interleave(torch.diagonal(W), npairs)
-        s2 = 1.0 / torch.sum(wdiag * colsq)  # 1/||J_flat||_F^2
+        s2 = 1.0 / torch.sum(wdiag * colsq).item()  # 1/||J_flat||_F^2
         if self.precision == "fp64":
             G = _gpu.gram(JT, None, "fp64")
         else:
@@ -512,7 +514,9 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
         Jm_out = jac_keep or {
             ("mua" if wv == "" else wv): mesh.user_rows(JT[:, i * npairs : (i + 1) * npairs]).t().cpu().numpy()
             for i, wv in enumerate(wvs)}
-    Xh = mesh.user_rows(X0).cpu().numpy()
+    tp = time.perf_counter()
+    Xh = _gpu.to_host(mesh.user_rows(X0))
+    mark("outputs", tp)
     k = res.k
     phi_out = {wv: Xh[:, i * k : (i + 1) * k] for i, wv in enumerate(wvs)}
     if wvs == [""]:

@@ -48,3 +48,50 @@ def test_jac_contract_matches_dense():
     phid = mesh.mesh_rows(torch.as_tensor(phi, device="cuda"))
     g = mesh.user_rows(_gpu.jac_contract(mesh, phid, S, D, torch.as_tensor(w, device="cuda")))
     np.testing.assert_allclose(g.cpu().numpy(), Jn.T @ w, rtol=1e-11, atol=1e-14 * np.abs(Jn).max())
+
+
+def _hv_ref(J, W, npairs, v, s2, lam):
+    K, M = J.shape
+    nb = M // npairs
+    t = (J * v[None, :]).reshape(K, nb, npairs).sum(-1)
+    z = t @ W
+    u = (J.reshape(K, nb, npairs) * z[:, :, None]).sum(0).reshape(-1)
+    return s2 * u + lam * v
+
+
+@pytest.mark.parametrize("K,nb,npairs", [(3001, 4, 1024), (2500, 3, 64), (777, 16, 64), (4099, 2, 1024),
+                                         (1500, 4, 128), (900, 8, 1024)])
+def test_jt_hv_matches_two_pass(K, nb, npairs):
+    """The one-pass s2 J W J^T v + lam v (refinement operator) against float64 numpy and the two-pass kernels."""
+    from redbirdpy_amd import _gpu
+
+    rng = np.random.default_rng(K + nb)
+    M = nb * npairs
+    J = rng.standard_normal((K, M)) * np.exp(rng.uniform(-4, 0, M))[None, :]
+    E = rng.uniform(0.1, 1.0, (nb, 3))
+    W = E @ E.T
+    v = rng.standard_normal(M)
+    s2, lam = 0.37, 1e-3
+    JT = torch.as_tensor(J, device="cuda")
+    Wd = torch.as_tensor(W, device="cuda")
+    vd = torch.as_tensor(v, device="cuda")
+    assert _gpu.jt_hv_ok(JT, nb, npairs)
+    out = _gpu.jt_hv(JT, Wd, npairs, vd, s2, lam)
+    ref = _hv_ref(J, W, npairs, v, s2, lam)
+    scale = s2 * np.abs(J).T @ (np.abs(J) @ np.abs(v)) * W.max() + lam * np.abs(v)
+    assert np.all(np.abs(out.cpu().numpy() - ref) <= 1e-12 * scale)
+    # the two-pass route (rows then columns)
+    t = _gpu.jt_rows(JT, nb, npairs, vd)
+    two = s2 * _gpu.jt_cols(JT, nb, npairs, t @ Wd) + lam * vd
+    assert np.all(np.abs(out.cpu().numpy() - two.cpu().numpy()) <= 1e-12 * scale)
+    # deterministic
+    assert torch.equal(out, _gpu.jt_hv(JT, Wd, npairs, vd, s2, lam))
+
+
+def test_jt_hv_rejects_unsupported_shapes():
+    from redbirdpy_amd import _gpu
+
+    JT = torch.zeros((10, 3 * 50), dtype=torch.float64, device="cuda")
+    assert not _gpu.jt_hv_ok(JT, 3, 50)
+    with pytest.raises(RuntimeError):
+        _gpu.jt_hv(JT, torch.eye(3, dtype=torch.float64, device="cuda"), 50, torch.zeros(150, device="cuda"), 1.0, 0.0)
