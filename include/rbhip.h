@@ -288,6 +288,19 @@ int rb_jt_cols(int64_t K, int64_t M, const void* JT, int64_t ldj, int in_dtype, 
  * it twice).  Needs np a multiple of 64, nb <= 16, M <= 8192 (RB_ERR_ARG
  * otherwise) and rb_jt_hv_workspace() bytes; deterministic (fixed-order sums).
  */
+/*
+ * In-place Cholesky factorisation of a dense float64 SPD matrix (n x n,
+ * row-major, lda >= n, n a multiple of 64): on return the lower triangle
+ * holds L (A = L L^T), the upper triangle of each diagonal 64 x 64 block is
+ * zero and the blocks above them keep A.  *info (device int) is 0, or the
+ * 1-based column at which a non-positive pivot appeared.  Replaces the dense
+ * solve of the Gauss-Newton normal equations in redbirdpy recon.py:429-509
+ * (reginvunder: `np.linalg.solve(Amat @ Amat.T + lambda I, rhs)`).
+ * Needs rb_potrf_workspace() bytes.
+ */
+int rb_potrf_workspace(int64_t n, size_t* bytes /* host */);
+int rb_potrf(int64_t n, double* A, int64_t lda, void* work, size_t work_bytes, int* info, void* stream);
+
 int rb_jt_hv_workspace(int64_t K, int64_t M, size_t* bytes /* host */);
 int rb_jt_hv(int64_t K, int64_t M, const double* JT, int64_t ldj, int nb, int64_t np, const double* W,
              const double* v, double s2, double lam, double* out, void* work, size_t work_bytes, void* stream);

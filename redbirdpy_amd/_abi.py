@@ -67,6 +67,8 @@ _SIGS = {
     "rb_jt_cols_workspace": [_i64, _i64, ctypes.POINTER(_sz)],
     "rb_jt_cols": [_i64, _i64, _vp, _i64, _int, _int, _i64, _vp, _int, _vp, _vp, _sz, _vp],
     "rb_jt_hv_workspace": [_i64, _i64, ctypes.POINTER(_sz)],
+    "rb_potrf_workspace": [_i64, ctypes.POINTER(_sz)],
+    "rb_potrf": [_i64, _vp, _i64, _vp, _sz, _vp, _vp],
     "rb_jt_hv": [_i64, _i64, _vp, _i64, _int, _i64, _vp, _vp, _dbl, _dbl, _vp, _vp, _sz, _vp],
     "rb_prof_enable": [_int],
     "rb_prof_reset": [],

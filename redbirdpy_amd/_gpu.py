@@ -636,6 +636,30 @@ def jt_hv(JT, W, npairs, v, s2, lam):
     return out
 
 
+def cholesky(H):
+    """Lower Cholesky factor of a float64 SPD device matrix (blocked 64 x 64, rb_potrf).
+
+    n is padded to a multiple of 64 with an identity block.  Raises
+    torch.linalg.LinAlgError at a non-positive pivot, like torch.linalg.cholesky.
+    """
+    n = H.shape[0]
+    npad = -(-n // 64) * 64
+    A = torch.zeros((npad, npad), dtype=torch.float64, device=H.device)
+    A[:n, :n] = H
+    if npad > n:
+        A.diagonal()[n:] = 1.0
+    nb = ctypes.c_size_t(0)
+    call("rb_potrf_workspace", npad, ctypes.byref(nb))
+    work = torch.empty(int(nb.value), dtype=torch.uint8, device=H.device)
+    info = torch.empty(1, dtype=torch.int32, device=H.device)
+    call("rb_potrf", npad, ptr(A), npad, ptr(work), nb.value, ptr(info), _abi.stream_ptr())
+    bad = int(info.item())
+    if bad:
+        raise torch.linalg.LinAlgError(f"cholesky: the leading minor of order {bad} is not positive-definite")
+    L = A[:n, :n].tril_()
+    return L if npad == n else L.contiguous()
+
+
 def remap_operator(mapid, mapweight, relem, nrec, perm_rows=None, dev=None):
     """Device CSR (ptr, idx, w) over recon nodes of the interpolation's transpose (recon.py:848-874 loop order).
 

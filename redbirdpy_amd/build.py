@@ -9,7 +9,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["rb_mesh.hip", "rb_assemble.hip", "rb_krylov.hip", "rb_jac.hip", "rb_gram.hip", "rb_widefield.hip", "rb_bicgstab.hip"]
+SOURCES = ["rb_mesh.hip", "rb_assemble.hip", "rb_krylov.hip", "rb_jac.hip", "rb_gram.hip", "rb_widefield.hip", "rb_bicgstab.hip", "rb_chol.hip"]
 OUT = os.path.join(HERE, "librbhip.so")
 FLAGS = [
     "--offload-arch=gfx950",

@@ -176,7 +176,7 @@ def syncprop(cfg, recon):
 def _chol_solve_normalized(H, rhs):
     """y = H^-1 rhs via the unit-diagonal normalisation and a float64 Cholesky."""
     Hn, di = _normalize_diag(H)
-    L = torch.linalg.cholesky(Hn)
+    L = _gpu.cholesky(Hn)
     z = torch.cholesky_solve((di * rhs).reshape(-1, 1), L).reshape(-1)
     return di * z
 
@@ -276,7 +276,7 @@ class GNStep:
         H = s2 * (G * Wk)
         H.diagonal().add_(self.lam)
         Hn, di = _normalize_diag(H)
-        L = torch.linalg.cholesky(Hn)
+        L = _gpu.cholesky(Hn)
         if self.precision == "fp64":
 
             def hsolve(r):

@@ -95,3 +95,38 @@ def test_jt_hv_rejects_unsupported_shapes():
     assert not _gpu.jt_hv_ok(JT, 3, 50)
     with pytest.raises(RuntimeError):
         _gpu.jt_hv(JT, torch.eye(3, dtype=torch.float64, device="cuda"), 50, torch.zeros(150, device="cuda"), 1.0, 0.0)
+
+
+@pytest.mark.parametrize("n", [64, 200, 1000, 4096])
+def test_cholesky_matches_lapack(n):
+    """Blocked HIP Cholesky (rb_potrf) against LAPACK on a normalised Gram-like SPD matrix."""
+    from redbirdpy_amd import _gpu
+
+    rng = np.random.default_rng(n)
+    X = rng.standard_normal((n, n + 50)) * np.exp(rng.uniform(-3, 0, n))[:, None]
+    H = X @ X.T + 1e-3 * np.eye(n)
+    d = 1 / np.sqrt(np.diag(H))
+    H = H * np.outer(d, d)
+    Hd = torch.as_tensor(H, device="cuda")
+    Hd_up = Hd.clone()
+    iu = torch.triu_indices(n, n, 1, device="cuda")
+    Hd_up[iu[0], iu[1]] = 7.0  # the factor reads only the lower triangle
+    L = _gpu.cholesky(Hd).cpu().numpy()
+    Lref = np.linalg.cholesky(H)
+    assert np.array_equal(L, np.tril(L))
+    assert np.max(np.abs(L - Lref)) < 1e-10 * np.max(np.abs(Lref))
+    assert np.max(np.abs(L @ L.T - H)) < 1e-13 * n
+    assert np.array_equal(_gpu.cholesky(Hd_up).cpu().numpy(), L)
+    # the solve the Gauss-Newton step makes with it
+    b = rng.standard_normal(n)
+    x = torch.cholesky_solve(torch.as_tensor(b, device="cuda").reshape(-1, 1), torch.as_tensor(L, device="cuda"))
+    np.testing.assert_allclose(x.cpu().numpy().ravel(), np.linalg.solve(H, b), rtol=1e-8, atol=1e-10)
+
+
+def test_cholesky_reports_first_bad_pivot():
+    from redbirdpy_amd import _gpu
+
+    H = np.eye(150)
+    H[70, 70] = -1.0
+    with pytest.raises(torch.linalg.LinAlgError, match="order 71"):
+        _gpu.cholesky(torch.as_tensor(H, device="cuda"))
