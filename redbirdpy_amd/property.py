@@ -120,11 +120,17 @@ def updateprop(cfg, wv=None, extin=None):
         elif musp is not None:
             nref = tmpl[1, 3] if tmpl.shape[0] > 1 else 1.37
             if hasattr(mua, "__len__"):
-                newp = np.column_stack([mua, musp, np.zeros_like(musp), np.full_like(musp, nref)])
+                newp = np.empty((len(mua), 4))
+                newp[:, 0] = mua
+                newp[:, 1] = musp
+                newp[:, 2] = 0.0
+                newp[:, 3] = nref
             else:
                 newp = np.array([[mua, musp, 0, nref]])
         elif hasattr(mua, "__len__"):
-            newp = np.column_stack([mua, np.tile(tmpl[1, 1:], (len(mua), 1))])
+            newp = np.empty((len(mua), tmpl.shape[1]))
+            newp[:, 0] = mua
+            newp[:, 1:] = tmpl[1, 1:]
         else:
             newp = np.array([[mua] + list(tmpl[1, 1:])])
         out[w] = newp
