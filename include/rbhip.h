@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define RB_ABI_VERSION 3
+#define RB_ABI_VERSION 4
 
 /* property layouts for rb_assemble */
 #define RB_PROP_ELEM 0   /* one row per element (label tables are expanded by the caller) */
@@ -227,14 +227,15 @@ int rb_gram(int64_t K, int64_t m, const void* JT, int64_t ldj, int in_dtype, con
 /*
  * Gram on the matrix cores: mode 1 = bf16 (hi*hi), mode 3 = bf16x3 split
  * (hi*hi + hi*lo + lo*hi, ~16 mantissa bits per product).  JT is split once
- * into transposed bf16 hi/lo arrays; fp32 MFMA accumulation within 16 K
- * chunks whose partial tiles are summed in fp64 in a fixed order.
- * work: rb_gram_workspace() bytes.  Same result as rb_gram
- * (redbirdpy recon.py:474).
+ * into transposed bf16 hi/lo arrays; fp32 MFMA accumulation within K chunks
+ * whose partial tiles are summed in fp64 in a fixed order.  colsq (m
+ * doubles, may be NULL) receives the exact float64 sum_k JT[k][c]^2 from the
+ * same read of JT (undivided by d).  work: rb_gram_workspace() bytes.  Same
+ * result as rb_gram (redbirdpy recon.py:474).
  */
 int rb_gram_workspace(int64_t K, int64_t m, int mode, size_t* bytes /* host */);
 int rb_gram_mfma(int64_t K, int64_t m, const void* JT, int64_t ldj, int in_dtype, const double* d, int mode,
-                 double* G, void* work, size_t work_bytes, void* stream);
+                 double* G, double* colsq, void* work, size_t work_bytes, void* stream);
 
 /*
  * Dual-mesh Jacobian (redbirdpy recon.py:818-876 `_remap_jacobian`, applied

@@ -61,7 +61,7 @@ _SIGS = {
                              _vp, _vp, _vp],
     "rb_gram": [_i64, _i64, _vp, _i64, _int, _vp, _int, _vp, _vp],
     "rb_gram_workspace": [_i64, _i64, _int, ctypes.POINTER(_sz)],
-    "rb_gram_mfma": [_i64, _i64, _vp, _i64, _int, _vp, _int, _vp, _vp, _sz, _vp],
+    "rb_gram_mfma": [_i64, _i64, _vp, _i64, _int, _vp, _int, _vp, _vp, _vp, _sz, _vp],
     "rb_jt_rows": [_i64, _i64, _vp, _i64, _int, _int, _i64, _vp, _vp, _vp],
     "rb_jt_remap": [_i64, _vp, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp],
     "rb_jt_cols_workspace": [_i64, _i64, ctypes.POINTER(_sz)],

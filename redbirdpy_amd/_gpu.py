@@ -583,8 +583,12 @@ def jac_contract(mesh, phi, S, D, w):
 GRAM_MODES = {"fp64": 0, "bf16": 1, "bf16x3": 3}
 
 
-def gram(JT, d=None, mode="fp64"):
-    """G = diag(d) JT^T JT diag(d) in float64 (m x m); mode fp64 | bf16 | bf16x3 (MFMA)."""
+def gram(JT, d=None, mode="fp64", want_colsq=False):
+    """G = diag(d) JT^T JT diag(d) in float64 (m x m); mode fp64 | bf16 | bf16x3 (MFMA).
+
+    want_colsq (MFMA modes): also return the exact float64 sum_k JT[k, c]^2,
+    taken from the same read of JT.
+    """
     K, m = JT.shape
     G = torch.empty((m, m), dtype=torch.float64, device=JT.device)
     code = _abi.RB_F64 if JT.dtype == torch.float64 else _abi.RB_F32
@@ -599,9 +603,10 @@ def gram(JT, d=None, mode="fp64"):
     nb = ctypes.c_size_t(0)
     call("rb_gram_workspace", K, m, md, ctypes.byref(nb))
     work = torch.empty(int(nb.value), dtype=torch.uint8, device=JT.device)
-    call("rb_gram_mfma", K, m, ptr(JT), JT.stride(0), code, ptr(d), md, ptr(G), ptr(work), nb.value,
+    colsq = torch.empty(m, dtype=torch.float64, device=JT.device) if want_colsq else None
+    call("rb_gram_mfma", K, m, ptr(JT), JT.stride(0), code, ptr(d), md, ptr(G), ptr(colsq), ptr(work), nb.value,
          _abi.stream_ptr())
-    return G
+    return (G, colsq) if want_colsq else G
 
 
 def jt_rows(JT, nw, npairs, v):

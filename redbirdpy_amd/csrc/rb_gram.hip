@@ -76,21 +76,16 @@ __global__ void __launch_bounds__(256) k_gram_f64(int64_t K, int64_t m, const TI
 // bf16 (round to nearest even), ~16 mantissa bits per product.
 //
 // 1. k_split_t: JT (K x m, fp32/fp64) -> Hi, Lo (m x Kp bf16, transposed,
-//    zero padded to Kp = K rounded up to 64), through 64x64 LDS tiles.
-// 2. k_gram_bf16: 256 threads = 4 waves per 128x128 upper-triangular output
-//    tile and K chunk (split-K over S chunks).  Slabs of 64 k are staged in
-//    LDS with 16-byte coalesced loads (each row's 64 k are 128 contiguous
-//    bytes of the transposed arrays); each wave owns 64x64 = 2x2 MFMA
-//    tiles.  Accumulation is fp32 inside a chunk; the S fp32 partial tiles
-//    are summed in fp64 in a fixed order by k_gram_reduce.
+//    zero padded to Kp = K rounded up to 64), through 64x64 LDS tiles; on
+//    request also the exact float64 column sums of squares (the Gram
+//    diagonal) from the same read.
+// 2. k_gram_bf16w: 512 threads = 8 waves per 256x256 upper-triangular
+//    output tile and K chunk (split-K, about 2048 workgroups in all).  fp32
+//    accumulation inside a chunk; the fp32 partial tiles are summed in fp64
+//    in a fixed order by k_gram_reduce.
 // ---------------------------------------------------------------------------
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(16))) float f32x16;
-
-#define MT 128       // output tile
-#define MKS 64       // K slab
-#define MLD (MKS + 8)  // LDS row stride in bf16 (16-byte pad against bank conflicts)
-#define GSPLIT 16    // K chunks
 
 // upper-triangular tile t -> (ti, tj), ti <= tj
 __device__ inline void tri_tile(int64_t t, int64_t nt, int64_t& ti, int64_t& tj) {
@@ -102,134 +97,233 @@ __device__ inline void tri_tile(int64_t t, int64_t nt, int64_t& ti, int64_t& tj)
   tj = ti + t;
 }
 
+#define SPLIT_GK 64  // K-strided workgroups per 64-column strip of the split
+
+// blockIdx.y: 64-column strip; blockIdx.x walks the 64-k tiles kt = x, x + GK, ...
+// colpart[x][c] (optional) = sum over those tiles of JT[k][c]^2 in float64,
+// each thread's k in ascending order, the four thread partials in a fixed order
 template <typename TI>
 __global__ void __launch_bounds__(256) k_split_t(int64_t K, int64_t m, int64_t Kp, const TI* __restrict__ JT,
-                                                 int64_t ldj, __bf16* __restrict__ Hi, __bf16* __restrict__ Lo) {
+                                                 int64_t ldj, __bf16* __restrict__ Hi, __bf16* __restrict__ Lo,
+                                                 double* __restrict__ colpart) {
   __shared__ float th[64][65];
   __shared__ float tl[64][65];
-  const int64_t k0 = (int64_t)blockIdx.x * 64, c0 = (int64_t)blockIdx.y * 64;
-  for (int q = threadIdx.x; q < 64 * 64; q += 256) {
-    int kk = q / 64, cc = q % 64;
-    int64_t k = k0 + kk, c = c0 + cc;
-    double x = (k < K && c < m) ? (double)JT[k * ldj + c] : 0.0;
-    __bf16 h = (__bf16)(float)x;
-    th[cc][kk] = (float)h;
-    tl[cc][kk] = (float)(x - (double)(float)h);
-  }
-  __syncthreads();
-  for (int q = threadIdx.x; q < 64 * 64; q += 256) {
-    int cc = q / 64, kk = q % 64;
-    int64_t c = c0 + cc;
-    if (c < m) {
-      Hi[c * Kp + k0 + kk] = (__bf16)th[cc][kk];
-      Lo[c * Kp + k0 + kk] = (__bf16)tl[cc][kk];
+  __shared__ double sq[4][64];
+  const int64_t c0 = (int64_t)blockIdx.y * 64;
+  const int cc0 = threadIdx.x & 63;  // this thread's column in the load loop (q % 64)
+  double acc = 0.0;
+  for (int64_t kt = blockIdx.x; kt * 64 < Kp; kt += gridDim.x) {
+    const int64_t k0 = kt * 64;
+    for (int q = threadIdx.x; q < 64 * 64; q += 256) {
+      int kk = q / 64, cc = q % 64;
+      int64_t k = k0 + kk, c = c0 + cc;
+      double x = (k < K && c < m) ? (double)JT[k * ldj + c] : 0.0;
+      acc += x * x;
+      __bf16 h = (__bf16)(float)x;
+      th[cc][kk] = (float)h;
+      tl[cc][kk] = (float)(x - (double)(float)h);
     }
+    __syncthreads();
+    for (int q = threadIdx.x; q < 64 * 64; q += 256) {
+      int cc = q / 64, kk = q % 64;
+      int64_t c = c0 + cc;
+      if (c < m) {
+        Hi[c * Kp + k0 + kk] = (__bf16)th[cc][kk];
+        Lo[c * Kp + k0 + kk] = (__bf16)tl[cc][kk];
+      }
+    }
+    __syncthreads();
+  }
+  if (colpart) {
+    sq[threadIdx.x >> 6][cc0] = acc;
+    __syncthreads();
+    if (threadIdx.x < 64 && c0 + cc0 < m)
+      colpart[(int64_t)blockIdx.x * m + c0 + cc0] = ((sq[0][cc0] + sq[1][cc0]) + sq[2][cc0]) + sq[3][cc0];
   }
 }
 
+__global__ void k_colsq_finish(int64_t m, int gk, const double* __restrict__ colpart, double* __restrict__ colsq) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m) return;
+  double v = 0.0;
+  for (int g = 0; g < gk; ++g) v += colpart[(int64_t)g * m + c];
+  colsq[c] = v;
+}
+
+// Wide variant: 512 threads = 8 waves per 256x256 upper-triangular output
+// tile and K chunk.  Slabs of 32 k are double-buffered in LDS (4 operand
+// arrays x 256 rows x 32 k, 64 KB per buffer): the next slab's global loads
+// are in flight in registers while the MFMAs run on the current one, one
+// barrier per slab.  Each row's four 16-byte chunks are stored XOR-swizzled
+// by (row >> 2) & 3, so the MFMA operand reads (32 rows, one chunk) hit
+// distinct banks.  Each wave owns a 128x64 block = 4x2 MFMA tiles.
+#define WT 256   // output tile
+#define WKS 32   // K slab
+#define WARR (WT * WKS)  // bf16 per operand array per buffer
+
+__device__ __forceinline__ int wswz(int row, int c) { return row * WKS + ((c ^ ((row >> 2) & 3)) << 3); }
+
 template <int NTERM>
-__global__ void __launch_bounds__(256) k_gram_bf16(int64_t m, int64_t Kp, const __bf16* __restrict__ Hi,
-                                                   const __bf16* __restrict__ Lo, float* __restrict__ part) {
-  __shared__ __align__(16) __bf16 Ah[MT * MLD];
-  __shared__ __align__(16) __bf16 Al[MT * MLD];
-  __shared__ __align__(16) __bf16 Bh[MT * MLD];
-  __shared__ __align__(16) __bf16 Bl[MT * MLD];
-  const int64_t nt = (m + MT - 1) / MT;
+__global__ void __launch_bounds__(512) k_gram_bf16w(int64_t m, int64_t Kp, int nsplit, const __bf16* __restrict__ Hi,
+                                                    const __bf16* __restrict__ Lo, float* __restrict__ part) {
+  __shared__ __align__(16) __bf16 lds[2 * 4 * WARR];
+  const int64_t nt = (m + WT - 1) / WT;
   const int64_t ntri = nt * (nt + 1) / 2;
   const int64_t tile = blockIdx.x % ntri;
   const int split = (int)(blockIdx.x / ntri);
   int64_t ti, tj;
   tri_tile(tile, nt, ti, tj);
-  const int64_t i0 = ti * MT, j0 = tj * MT;
-  const int64_t kchunk = ((Kp + GSPLIT - 1) / GSPLIT + MKS - 1) / MKS * MKS;
+  const int64_t i0 = ti * WT, j0 = tj * WT;
+  const int64_t kchunk = ((Kp + nsplit - 1) / nsplit + WKS - 1) / WKS * WKS;
   const int64_t kb = (int64_t)split * kchunk;
   const int64_t ke = min(Kp, kb + kchunk);
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
-  f32x16 acc[2][2];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 2) * 128, wc = (wave & 3) * 64;
+  // operand arrays: 0 A hi, 1 A lo, 2 B hi, 3 B lo; thread loads chunks q = tid and tid + 512
+  constexpr int NA = (NTERM > 1) ? 4 : 2;
+  const __bf16* src[4];
+  bool ok[4][2];
+  int lrow[2], lc[2];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int h = 0; h < 2; ++h) {
+    const int q = tid + 512 * h;
+    lrow[h] = q >> 2;
+    lc[h] = q & 3;
+  }
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+  for (int a = 0; a < 4; ++a) src[a] = ((a & 1) ? Lo : Hi);
+  int64_t grow[4][2];
 #pragma unroll
-      for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.f;
-  for (int64_t k0 = kb; k0 < ke; k0 += MKS) {
-    // 128 rows x 64 k per operand = 1024 chunks of 8 bf16; 4 per thread
+  for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int cidx = threadIdx.x + 256 * q;
-      const int row = cidx >> 3, c8 = (cidx & 7) * 8;
-      const int64_t gi = i0 + row, gj = j0 + row;
-      bf16x8 z;
+    for (int h = 0; h < 2; ++h) {
+      grow[a][h] = ((a < 2) ? i0 : j0) + lrow[h];
+      ok[a][h] = grow[a][h] < m;
+    }
+  bf16x8 pre[4][2];
+  auto load = [&](int64_t k0) {
 #pragma unroll
-      for (int e = 0; e < 8; ++e) z[e] = (__bf16)0.f;
-      bf16x8 vah = (gi < m) ? *reinterpret_cast<const bf16x8*>(Hi + gi * Kp + k0 + c8) : z;
-      bf16x8 vbh = (gj < m) ? *reinterpret_cast<const bf16x8*>(Hi + gj * Kp + k0 + c8) : z;
-      *reinterpret_cast<bf16x8*>(&Ah[row * MLD + c8]) = vah;
-      *reinterpret_cast<bf16x8*>(&Bh[row * MLD + c8]) = vbh;
-      if (NTERM > 1) {
-        bf16x8 val = (gi < m) ? *reinterpret_cast<const bf16x8*>(Lo + gi * Kp + k0 + c8) : z;
-        bf16x8 vbl = (gj < m) ? *reinterpret_cast<const bf16x8*>(Lo + gj * Kp + k0 + c8) : z;
-        *reinterpret_cast<bf16x8*>(&Al[row * MLD + c8]) = val;
-        *reinterpret_cast<bf16x8*>(&Bl[row * MLD + c8]) = vbl;
+    for (int a = 0; a < 4; ++a) {
+      if (NTERM == 1 && (a & 1)) continue;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        bf16x8 z;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) z[e] = (__bf16)0.f;
+        pre[a][h] = ok[a][h] ? *reinterpret_cast<const bf16x8*>(src[a] + grow[a][h] * Kp + k0 + lc[h] * 8) : z;
       }
     }
-    __syncthreads();
+  };
+  auto store = [&](int buf) {
 #pragma unroll
-    for (int ks = 0; ks < MKS; ks += 16) {
-      const int kof = ks + (lane >> 5) * 8;
-      bf16x8 ah[2], al[2], bh[2], bl[2];
+    for (int a = 0; a < 4; ++a) {
+      if (NTERM == 1 && (a & 1)) continue;
 #pragma unroll
-      for (int t = 0; t < 2; ++t) {
+      for (int h = 0; h < 2; ++h)
+        *reinterpret_cast<bf16x8*>(&lds[(buf * 4 + a) * WARR + wswz(lrow[h], lc[h])]) = pre[a][h];
+    }
+  };
+  f32x16 acc[4][2];
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[t][u][r] = 0.f;
+  if (kb < ke) {
+    load(kb);
+    store(0);
+  }
+  __syncthreads();
+  int cur = 0;
+  for (int64_t k0 = kb; k0 < ke; k0 += WKS) {
+    const bool more = k0 + WKS < ke;
+    if (more) load(k0 + WKS);
+    const __bf16* L = lds + cur * 4 * WARR;
+#pragma unroll
+    for (int ks = 0; ks < WKS; ks += 16) {
+      const int c = (ks >> 3) + (lane >> 5);
+      bf16x8 ah[4], al[4], bh[2], bl[2];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
         const int ra = wr + t * 32 + (lane & 31);
-        const int rb = wc + t * 32 + (lane & 31);
-        ah[t] = *reinterpret_cast<const bf16x8*>(&Ah[ra * MLD + kof]);
-        bh[t] = *reinterpret_cast<const bf16x8*>(&Bh[rb * MLD + kof]);
-        if (NTERM > 1) {
-          al[t] = *reinterpret_cast<const bf16x8*>(&Al[ra * MLD + kof]);
-          bl[t] = *reinterpret_cast<const bf16x8*>(&Bl[rb * MLD + kof]);
-        }
+        ah[t] = *reinterpret_cast<const bf16x8*>(&L[0 * WARR + wswz(ra, c)]);
+        if (NTERM > 1) al[t] = *reinterpret_cast<const bf16x8*>(&L[1 * WARR + wswz(ra, c)]);
       }
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+      for (int u = 0; u < 2; ++u) {
+        const int rb = wc + u * 32 + (lane & 31);
+        bh[u] = *reinterpret_cast<const bf16x8*>(&L[2 * WARR + wswz(rb, c)]);
+        if (NTERM > 1) bl[u] = *reinterpret_cast<const bf16x8*>(&L[3 * WARR + wswz(rb, c)]);
+      }
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-          acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[t], bh[u], acc[t][u], 0, 0, 0);
           if (NTERM > 1) {
-            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
-            acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
+            acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[t], bl[u], acc[t][u], 0, 0, 0);
+            acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[t], bh[u], acc[t][u], 0, 0, 0);
           }
         }
     }
+    if (more) store(cur ^ 1);
     __syncthreads();
+    cur ^= 1;
   }
+  (void)NA;
   float* out = part + (int64_t)split * m * m;
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int t = 0; t < 4; ++t)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int u = 0; u < 2; ++u)
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        // 32x32 MFMA output layout: col = lane % 32, row = 8*(r/4) + 4*(lane/32) + r%4
-        int64_t i = i0 + wr + a * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        int64_t j = j0 + wc + b * 32 + (lane & 31);
-        if (i < m && j < m) out[i * m + j] = acc[a][b][r];
+        const int64_t i = i0 + wr + t * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int64_t j = j0 + wc + u * 32 + (lane & 31);
+        if (i < m && j < m) out[i * m + j] = acc[t][u][r];
       }
 }
 
-// G = diag(d) (sum_s part_s) diag(d) in fp64; the upper triangle is read for
-// both halves (exact symmetry), the S partials are summed in a fixed order
-__global__ void k_gram_reduce(int64_t m, int nsplit, const float* __restrict__ part, const double* __restrict__ d,
-                              double* __restrict__ G) {
-  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= m * m) return;
-  int64_t i = idx / m, j = idx - i * m;
-  int64_t src = (i <= j) ? idx : j * m + i;
-  double v = 0.0;
-  for (int s = 0; s < nsplit; ++s) v += (double)part[(int64_t)s * m * m + src];
-  if (d) v *= d[i] * d[j];
-  G[idx] = v;
+// K chunks of the wide kernel: about 2048 workgroups in all (8 per CU)
+static int gram_nsplit(int64_t m) {
+  const int64_t nt = (m + WT - 1) / WT;
+  const int64_t ntri = nt * (nt + 1) / 2;
+  int64_t s = (2048 + ntri / 2) / ntri;
+  return (int)(s < 1 ? 1 : s > 32 ? 32 : s);
+}
+
+// G = diag(d) (sum_s part_s) diag(d) in fp64 over 64x64 tiles on or above
+// the diagonal: the partials are read once, coalesced, summed in a fixed
+// order, and each tile is written twice (as is and transposed through LDS);
+// in a diagonal tile the upper entries are mirrored, so G is exactly symmetric
+__global__ void __launch_bounds__(256) k_gram_reduce(int64_t m, int nsplit, const float* __restrict__ part,
+                                                     const double* __restrict__ d, double* __restrict__ G) {
+  __shared__ double t[64][65];
+  const int64_t nt = (m + 63) / 64;
+  int64_t ti, tj;
+  tri_tile(blockIdx.x, nt, ti, tj);
+  const int64_t i0 = ti * 64, j0 = tj * 64;
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    const int64_t i = i0 + r, j = j0 + c;
+    double v = 0.0;
+    if (i < m && j < m) {
+      for (int s = 0; s < nsplit; ++s) v += (double)part[(int64_t)s * m * m + i * m + j];
+      if (d) v *= d[i] * d[j];
+    }
+    t[r][c] = v;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    const int64_t i = i0 + r, j = j0 + c;
+    if (i < m && j < m) G[i * m + j] = (ti == tj && r > c) ? t[c][r] : t[r][c];
+    if (ti != tj) {
+      const int64_t jj = j0 + r, ii = i0 + c;
+      if (jj < m && ii < m) G[jj * m + ii] = t[c][r];
+    }
+  }
 }
 
 static int64_t pad64(int64_t K) { return (K + 63) / 64 * 64; }
@@ -241,13 +335,14 @@ extern "C" int rb_gram_workspace(int64_t K, int64_t m, int mode, size_t* bytes) 
     return RB_OK;
   }
   size_t split_bytes = (size_t)2 * m * pad64(K) * sizeof(__bf16);
-  size_t part_bytes = (size_t)GSPLIT * m * m * sizeof(float);
-  *bytes = ((split_bytes + 255) & ~(size_t)255) + part_bytes;
+  size_t part_bytes = (size_t)gram_nsplit(m) * m * m * sizeof(float);
+  size_t col_bytes = (size_t)SPLIT_GK * m * sizeof(double);
+  *bytes = ((split_bytes + 255) & ~(size_t)255) + ((part_bytes + 255) & ~(size_t)255) + col_bytes;
   return RB_OK;
 }
 
 extern "C" int rb_gram_mfma(int64_t K, int64_t m, const void* JT, int64_t ldj, int in_dtype, const double* d,
-                            int mode, double* G, void* work, size_t work_bytes, void* stream) {
+                            int mode, double* G, double* colsq, void* work, size_t work_bytes, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   size_t need = 0;
   rb_gram_workspace(K, m, mode, &need);
@@ -260,24 +355,36 @@ extern "C" int rb_gram_mfma(int64_t K, int64_t m, const void* JT, int64_t ldj, i
   __bf16* Lo = Hi + m * Kp;
   size_t split_bytes = (size_t)2 * m * Kp * sizeof(__bf16);
   float* part = (float*)((char*)work + ((split_bytes + 255) & ~(size_t)255));
-  dim3 gs((unsigned)(Kp / 64), (unsigned)((m + 63) / 64));
+  const int nsplit = gram_nsplit(m);
+  size_t part_bytes = (size_t)nsplit * m * m * sizeof(float);
+  double* colpart = colsq ? (double*)((char*)part + ((part_bytes + 255) & ~(size_t)255)) : nullptr;
+  const int64_t nkt = Kp / 64;
+  const int gk = (int)(nkt < SPLIT_GK ? nkt : SPLIT_GK);
+  dim3 gs((unsigned)gk, (unsigned)((m + 63) / 64));
   if (in_dtype == RB_F32)
-    hipLaunchKernelGGL(k_split_t<float>, gs, dim3(256), 0, s, K, m, Kp, (const float*)JT, ldj, Hi, Lo);
+    hipLaunchKernelGGL(k_split_t<float>, gs, dim3(256), 0, s, K, m, Kp, (const float*)JT, ldj, Hi, Lo, colpart);
   else if (in_dtype == RB_F64)
-    hipLaunchKernelGGL(k_split_t<double>, gs, dim3(256), 0, s, K, m, Kp, (const double*)JT, ldj, Hi, Lo);
+    hipLaunchKernelGGL(k_split_t<double>, gs, dim3(256), 0, s, K, m, Kp, (const double*)JT, ldj, Hi, Lo, colpart);
   else {
     rb_set_error("rb_gram_mfma: in_dtype must be RB_F64 or RB_F32");
     return RB_ERR_ARG;
   }
   RB_CHECK_LAUNCH();
-  int64_t nt = (m + MT - 1) / MT;
-  unsigned blocks = (unsigned)(nt * (nt + 1) / 2 * GSPLIT);
+  if (colsq) {
+    hipLaunchKernelGGL(k_colsq_finish, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, s, m, gk,
+                       (const double*)colpart, colsq);
+    RB_CHECK_LAUNCH();
+  }
+  const int64_t nt = (m + WT - 1) / WT;
+  unsigned blocks = (unsigned)(nt * (nt + 1) / 2 * nsplit);
   if (mode == 1)
-    hipLaunchKernelGGL(k_gram_bf16<1>, dim3(blocks), dim3(256), 0, s, m, Kp, Hi, Lo, part);
+    hipLaunchKernelGGL(k_gram_bf16w<1>, dim3(blocks), dim3(512), 0, s, m, Kp, nsplit, Hi, Lo, part);
   else
-    hipLaunchKernelGGL(k_gram_bf16<3>, dim3(blocks), dim3(256), 0, s, m, Kp, Hi, Lo, part);
+    hipLaunchKernelGGL(k_gram_bf16w<3>, dim3(blocks), dim3(512), 0, s, m, Kp, nsplit, Hi, Lo, part);
   RB_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)ceil_div(m * m, 256)), dim3(256), 0, s, m, GSPLIT, part, d, G);
+  const int64_t nt64 = (m + 63) / 64;
+  hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)(nt64 * (nt64 + 1) / 2)), dim3(256), 0, s, m, nsplit, part, d,
+                     G);
   RB_CHECK_LAUNCH();
   return RB_OK;
 }

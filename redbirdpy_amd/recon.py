@@ -264,14 +264,14 @@ class GNStep:
         Wk = torch.repeat_interleave(torch.repeat_interleave(W, npairs, 0), npairs, 1)
         mk = getattr(self, "mark", None) or (lambda name, t: t)
         tq = time.perf_counter()
-        colsq = _gpu.jt_cols(JT, nw, npairs)  # exact ||J_w row||^2, float64
-        wdiag = torch.repeat_interleave(torch.diagonal(W), npairs)
-        s2 = 1.0 / torch.sum(wdiag * colsq).item()  # 1/||J_flat||_F^2
         if self.precision == "fp64":
+            colsq = _gpu.jt_cols(JT, nw, npairs)  # exact ||J_w row||^2, float64
             G = _gpu.gram(JT, None, "fp64")
         else:
-            G = _gpu.gram(JT, None, "bf16x3")
+            G, colsq = _gpu.gram(JT, None, "bf16x3", want_colsq=True)
             G.diagonal().copy_(colsq)
+        wdiag = torch.repeat_interleave(torch.diagonal(W), npairs)
+        s2 = 1.0 / torch.sum(wdiag * colsq).item()  # 1/||J_flat||_F^2
         tq = mk("gn_gram", tq)
         H = s2 * (G * Wk)
         H.diagonal().add_(self.lam)

@@ -29,6 +29,10 @@ def test_gram_modes(K, m, dtype):
     G1 = _gpu.gram(JT, d, "bf16").cpu().numpy()
     assert np.max(np.abs(G1 - Gn_ref)) < 2e-2
     assert np.array_equal(G3, G3.T)
+    # the exact float64 column sums of squares taken from the split pass
+    G3b, colsq = _gpu.gram(JT, d, "bf16x3", want_colsq=True)
+    assert torch.equal(G3b, torch.as_tensor(G3, device="cuda"))
+    np.testing.assert_allclose(colsq.cpu().numpy(), np.diag(G_ref), rtol=1e-13)
 
 
 def test_jac_contract_matches_dense():
