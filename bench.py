@@ -53,6 +53,8 @@ def _args():
     p.add_argument("--skip-fp64", action="store_true")
     p.add_argument("--no-gn", dest="gn", action="store_false", help="skip the Gauss-Newton iterations leg")
     p.add_argument("--gn-iters", type=int, default=5)
+    p.add_argument("--gn-tol", type=float, default=1e-10,
+                   help="forward solver tolerance of both Gauss-Newton legs (redbirdpy femsolve's default)")
     p.add_argument("--gn-precisions", default="mixed,fp64", help="comma list of GN legs to run")
     p.add_argument("--no-kernels", action="store_true", help="skip the Jacobian and Gram kernel legs")
     p.add_argument("--gn-phases", action="store_true", help="add a synchronising per-phase timing run")
@@ -98,7 +100,9 @@ def gn_bench(a):
 
     Data: FP64 GPU forward of the true concentrations with 0.5% seeded
     Gaussian noise.  Each leg starts from the background and runs
-    a.gn_iters iterations; iterations/s = iterations / wall time.
+    a.gn_iters iterations; iterations/s = iterations / wall time.  Both legs
+    solve the forward systems to the same relative residual (a.gn_tol), so
+    their ratio measures precision alone.
     """
     import copy
     import torch
@@ -112,19 +116,20 @@ def gn_bench(a):
     rng = np.random.default_rng(40)
     y0 = {wv: det[wv] * (1 + 0.005 * rng.standard_normal(det[wv].shape)) for wv in workloads.WAVELENGTHS}
     m = sum(v.size for v in y0.values())
-    out = {"m": m, "nodes": int(ct["node"].shape[0]), "iters": a.gn_iters}
+    out = {"m": m, "nodes": int(ct["node"].shape[0]), "iters": a.gn_iters, "solver_tol": a.gn_tol}
     results = {}
     for prec in a.gn_precisions.split(","):
         # one untimed iteration first (workspace and mesh-structure allocation)
         c, sd = rb.meshprep(copy.deepcopy(cfg0))
         r = {"param": {k: v.copy() for k, v in bg.items()}}
-        rb.runrecon(c, r, y0, sd, maxiter=1, lambda_=1e-3 / m, extin=extin, precision=prec, report=False)
+        rb.runrecon(c, r, y0, sd, maxiter=1, lambda_=1e-3 / m, extin=extin, precision=prec, report=False,
+                    solvertol=a.gn_tol)
         c, sd = rb.meshprep(copy.deepcopy(cfg0))
         r = {"param": {k: v.copy() for k, v in bg.items()}}
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         _, resid, *_ = rb.runrecon(c, r, y0, sd, maxiter=a.gn_iters, lambda_=1e-3 / m, extin=extin,
-                                   precision=prec, report=False)
+                                   precision=prec, report=False, solvertol=a.gn_tol)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         # phase breakdown from a separate, synchronising run (not the timed one)
@@ -133,7 +138,7 @@ def gn_bench(a):
             c2, sd2 = rb.meshprep(copy.deepcopy(cfg0))
             r2 = {"param": {k: v.copy() for k, v in bg.items()}}
             rb.runrecon(c2, r2, y0, sd2, maxiter=a.gn_iters, lambda_=1e-3 / m, extin=extin, precision=prec,
-                        report=False, timing=tm)
+                        report=False, timing=tm, solvertol=a.gn_tol)
         results[prec] = r
         out[prec] = {"gn_iters_per_s": a.gn_iters / dt, "s_per_iter": dt / a.gn_iters,
                      "resid": [float(x) for x in resid], "phases_s": tm}
