@@ -5,7 +5,8 @@
 //                 each step's column broadcast through LDS) and forms
 //                 D = L_kk^-1 by forward substitution (lane j solves column j)
 //   k_potrf_panel L_ik = A_ik D^T for every block row i below (a workgroup each)
-//   k_potrf_syrk  A_ij -= L_ik L_jk^T for kb < j <= i (a workgroup per block)
+//   k_potrf_syrk  A_ij -= L_ik L_jk^T for kb < j <= i (a workgroup per block);
+//                 the workgroup of block (kb+1, kb+1) then factors it
 // The Gauss-Newton step of redbirdpy solves (J J^T + lambda I) x = r with a
 // dense direct solve (recon.py:429-509); this factor replaces that solve.
 #include <math.h>
@@ -21,18 +22,15 @@ __device__ __forceinline__ double rl64(double x, int lane) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// LDS traffic inside one wavefront needs no s_barrier (the wave's LDS
-// instructions complete in order); wave_barrier keeps the compiler from
+// Factor one diagonal block held in LDS (s: its 64 rows; cv: 64 doubles of
+// scratch) with one wavefront: lane i keeps row i in registers, each step's
+// column is broadcast through LDS.  Writes L_kk over the block (zero above
+// the diagonal), D = L_kk^-1 and the first bad pivot into *info.  Only this
+// wavefront runs it: LDS traffic inside one wavefront needs no s_barrier (its
+// LDS instructions complete in order); wave_barrier keeps the compiler from
 // moving memory operations across the exchange points.
-__global__ void __launch_bounds__(64) k_potrf_diag(double* __restrict__ A, int64_t lda, int64_t k0,
-                                                   double* __restrict__ D, int* __restrict__ info) {
-  __shared__ __align__(16) double s[CB][CB + 2];  // even row stride: 16-byte aligned rows
-  __shared__ __align__(16) double cv[CB];        // the current column, read back as broadcasts
-  const int l = threadIdx.x;
-  double* blk = A + k0 * lda + k0;
-#pragma unroll 16
-  for (int r = 0; r < CB; ++r) s[r][l] = blk[r * lda + l];
-  __syncthreads();
+__device__ __forceinline__ void diag_factor(double (*s)[CB + 2], double* cv, double* __restrict__ blk, int64_t lda,
+                                            int64_t k0, double* __restrict__ D, int* __restrict__ info, int l) {
   double a[CB];
 #pragma unroll
   for (int j = 0; j < CB; ++j) a[j] = s[l][j];
@@ -42,7 +40,11 @@ __global__ void __launch_bounds__(64) k_potrf_diag(double* __restrict__ A, int64
   for (int k = 0; k < CB; ++k) {
     const double dkk = rl64(a[k], k);
     if (bad < 0 && !(dkk > 0.0)) bad = k;
-    const double ri = 1.0 / sqrt(dkk);
+    // 1/sqrt: hardware estimate plus two Newton steps (a few ulp; the
+    // correctly rounded sqrt and divide are long dependent sequences)
+    double ri = __builtin_amdgcn_rsq(dkk);
+    ri = ri * (1.5 - 0.5 * dkk * ri * ri);
+    ri = ri * (1.5 - 0.5 * dkk * ri * ri);
     a[k] *= ri;  // lane k: L_kk; lanes i > k: L_ik (lanes i < k hold unused upper entries)
     cv[l] = a[k];
     if (l == k) myri = ri;
@@ -53,7 +55,7 @@ __global__ void __launch_bounds__(64) k_potrf_diag(double* __restrict__ A, int64
   }
 #pragma unroll
   for (int j = 0; j < CB; ++j) s[l][j] = (j <= l) ? a[j] : 0.0;  // row l of L_kk
-  __syncthreads();
+  __builtin_amdgcn_wave_barrier();
   // D = L_kk^-1: lane j solves column j (four partial sums shorten the chain)
   double x[CB];
 #pragma unroll
@@ -73,6 +75,20 @@ __global__ void __launch_bounds__(64) k_potrf_diag(double* __restrict__ A, int64
     if (cur == 0 && bad >= 0) cur = (int)(k0 + bad + 1);
     *info = cur;
   }
+}
+
+// the first diagonal block (the others are factored by the trailing update
+// that finishes them)
+__global__ void __launch_bounds__(64) k_potrf_diag(double* __restrict__ A, int64_t lda, int64_t k0,
+                                                   double* __restrict__ D, int* __restrict__ info) {
+  __shared__ __align__(16) double s[CB][CB + 2];  // even row stride: 16-byte aligned rows
+  __shared__ __align__(16) double cv[CB];
+  const int l = threadIdx.x;
+  double* blk = A + k0 * lda + k0;
+#pragma unroll 16
+  for (int r = 0; r < CB; ++r) s[r][l] = blk[r * lda + l];
+  __syncthreads();
+  diag_factor(s, cv, blk, lda, k0, D, info, l);
 }
 
 // L_ik = A_ik D^T, D = L_kk^-1; block row i = kb + 1 + blockIdx.x
@@ -109,10 +125,16 @@ __global__ void __launch_bounds__(256) k_potrf_panel(double* __restrict__ A, int
     for (int q = 0; q < 4; ++q) blk[(tr + p) * lda + tc + q] = acc[p][q];
 }
 
-// A_ij -= L_ik L_jk^T over the lower blocks kb < j <= i (triangular index)
-__global__ void __launch_bounds__(256) k_potrf_syrk(double* __restrict__ A, int64_t lda, int64_t k0) {
-  __shared__ double Ls[CB][CB + 1];  // Ls[m][r] = L_ik[r][m]
-  __shared__ double Rs[CB][CB + 1];  // Rs[m][c] = L_jk[c][m]
+// A_ij -= L_ik L_jk^T over the lower blocks kb < j <= i (triangular index).
+// Workgroup 0 holds block (kb+1, kb+1), which this update finishes: its
+// first wavefront then factors it (diag_factor) while the other workgroups
+// are still updating, so the next step's diagonal factor is off the
+// critical path whenever the update takes longer.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_potrf_syrk(double* __restrict__ A, int64_t lda, int64_t k0,
+                                                    double* __restrict__ D, int* __restrict__ info) {
+  __shared__ __align__(16) double sm[2 * CB * (CB + 1)];
+  double (*Ls)[CB + 1] = (double (*)[CB + 1])sm;              // Ls[m][r] = L_ik[r][m]
+  double (*Rs)[CB + 1] = (double (*)[CB + 1])(sm + CB * (CB + 1));  // Rs[m][c] = L_jk[c][m]
   const int64_t e = blockIdx.x;
   int64_t bi = (int64_t)((sqrt(8.0 * (double)e + 1.0) - 1.0) * 0.5);
   while (bi * (bi + 1) / 2 > e) --bi;
@@ -141,10 +163,22 @@ __global__ void __launch_bounds__(256) k_potrf_syrk(double* __restrict__ A, int6
       for (int q = 0; q < 4; ++q) acc[p][q] += lv[p] * rv[q];
   }
   double* c = A + i0 * lda + j0;
+  const bool fuse = (e == 0);  // block (kb+1, kb+1)
+  double (*S)[CB + 2] = (double (*)[CB + 2])sm;  // reuses Ls/Rs once they are read
+  double* cv = sm + CB * (CB + 2);
+  if (fuse) __syncthreads();
 #pragma unroll
   for (int p = 0; p < 4; ++p)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) c[(tr + p) * lda + tc + q] -= acc[p][q];
+    for (int q = 0; q < 4; ++q) {
+      const double v = c[(tr + p) * lda + tc + q] - acc[p][q];
+      c[(tr + p) * lda + tc + q] = v;
+      if (fuse) S[tr + p][tc + q] = v;
+    }
+  if (fuse) {
+    __syncthreads();
+    if (t < 64) diag_factor(S, cv, c, lda, k0 + CB, D, info, t);
+  }
 }
 
 extern "C" int rb_potrf_workspace(int64_t n, size_t* bytes) {
@@ -166,14 +200,15 @@ extern "C" int rb_potrf(int64_t n, double* A, int64_t lda, void* work, size_t wo
   }
   double* D = (double*)work;
   const int64_t nbk = n / CB;
-  for (int64_t kb = 0; kb < nbk; ++kb) {
+  hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(64), 0, s, A, lda, (int64_t)0, D, info);
+  RB_CHECK_LAUNCH();
+  for (int64_t kb = 0; kb + 1 < nbk; ++kb) {
     const int64_t k0 = kb * CB, rest = nbk - kb - 1;
-    hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(64), 0, s, A, lda, k0, D, info);
-    RB_CHECK_LAUNCH();
-    if (rest == 0) break;
     hipLaunchKernelGGL(k_potrf_panel, dim3((unsigned)rest), dim3(256), 0, s, A, lda, k0, (const double*)D);
     RB_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_potrf_syrk, dim3((unsigned)(rest * (rest + 1) / 2)), dim3(256), 0, s, A, lda, k0);
+    // also factors diagonal block kb + 1 (into D, for the next panel)
+    hipLaunchKernelGGL(k_potrf_syrk, dim3((unsigned)(rest * (rest + 1) / 2)), dim3(256), 0, s, A, lda, k0, D,
+                       info);
     RB_CHECK_LAUNCH();
   }
   return RB_OK;

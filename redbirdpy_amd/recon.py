@@ -231,7 +231,8 @@ class GNStep:
            normalised s^2 J J^T + lambda I.
     mixed: float64 Jacobian, bf16x3 MFMA Gram (its hi/lo bf16 split) whose
            diagonal is replaced by the float64 column norms, float64
-           Cholesky of it as a preconditioner, then iterative refinement
+           Cholesky of it and a float32 inverse of the factor as a
+           preconditioner, then iterative refinement
            with float64 products J (J^T v) until the residual is below
            ``reftol``.
     Returns dmu (nn x nc) = s^2 J^T y.
@@ -285,11 +286,16 @@ class GNStep:
         else:
             # refinement applies the factor ~5 times: an explicit L^-1 (one
             # triangular solve with m right-hand sides) turns each application
-            # into two dense products instead of two sequential triangular solves
-            Li = torch.linalg.solve_triangular(L, torch.eye(L.shape[0], dtype=L.dtype, device=L.device), upper=False)
+            # into two dense products instead of two sequential triangular solves.
+            # It is a preconditioner (the residuals are float64), so float32
+            # suffices: its ~1e-7 error is below the bf16x3 Gram's ~4e-6.
+            L32 = L.float()
+            Li = torch.linalg.solve_triangular(L32, torch.eye(L.shape[0], dtype=L32.dtype, device=L.device),
+                                               upper=False)
+            del L32
 
             def hsolve(r):
-                return di * (Li.t() @ (Li @ (di * r)))
+                return di * (Li.t() @ (Li @ (di * r).float())).double()
 
         tq = mk("gn_cholesky", tq)
 
