@@ -275,6 +275,14 @@ class GNStep:
         s2 = 1.0 / torch.sum(wdiag * colsq).item()  # 1/||J_flat||_F^2
         tq = mk("gn_gram", tq)
         H = s2 * (G * Wk)
+        del G
+        y = self._factor_refine(H, misfit, lambda v: self._hv(v, s2, W), mk, tq)
+        T = _gpu.jt_rows(JT, nw, npairs, y)  # (nn, nw) = J_w^T y_w
+        return s2 * (T @ E)
+
+    def _factor_refine(self, H, misfit, hv, mk, tq):
+        """y = (H + lambda I)^-1 misfit: Cholesky of the unit-diagonal form; the
+        mixed path refines with the float64 operator hv."""
         H.diagonal().add_(self.lam)
         Hn, di = _normalize_diag(H)
         L = _gpu.cholesky(Hn)
@@ -304,16 +312,62 @@ class GNStep:
         if self.precision != "fp64":
             rn = torch.linalg.norm(misfit)
             for nref in range(1, self.maxref + 1):
-                res = misfit - self._hv(y, s2, W)
+                res = misfit - hv(y)
                 rel = (torch.linalg.norm(res) / rn).item()
                 if rel <= self.reftol:
                     break
                 y = y + hsolve(res)
             self.info["refine_relres"] = rel
         self.info["refinements"] = nref
-        tq = mk("gn_refine", tq)
-        T = _gpu.jt_rows(JT, nw, npairs, y)  # (nn, nw) = J_w^T y_w
-        return s2 * (T @ E)
+        mk("gn_refine", tq)
+        return y
+
+
+class GNStepReIm(GNStep):
+    """'reim' rows for one complex unknown block (recon.py:537-541 with :243-251).
+
+    The real system is A = [[Jr, -Ji], [Ji, Jr]] on the unknowns [x; y] (the
+    real and imaginary parts of a complex update); the reference's update
+    parsing keeps the first N entries, x.  With the node-major [Re | Im]
+    Jacobian JT (K x 2m) and its Gram G = [[Grr, Gri], [Gir, Gii]]:
+    A A^T = [[Grr + Gii, Gri - Gir], [Gir - Gri, Grr + Gii]], ||A||_F^2 = 2 ||J||_F^2,
+    and x = s^2 (Jr^T y1 + Ji^T y2).
+    """
+
+    def __init__(self, JT, npairs, lam, precision="fp64", reftol=1e-10, maxref=20):
+        super().__init__(JT, npairs, np.ones((2, 1)), lam, precision, reftol, maxref)
+
+    def _hv(self, v, s2, W=None):
+        """(s^2 A A^T + lambda) v with float64 accumulation (four reads of J)."""
+        JT, m = self.JT, self.np
+        a = _gpu.jt_rows(JT, 1, 2 * m, v)[:, 0]  # Jr^T v1 + Ji^T v2
+        b = _gpu.jt_rows(JT, 1, 2 * m, torch.cat([v[m:], -v[:m]]))[:, 0]  # Jr^T v2 - Ji^T v1
+        u1 = _gpu.jt_cols(JT, 2, m, torch.stack([a, -b], 1))  # [Jr a; -Ji b]
+        u2 = _gpu.jt_cols(JT, 2, m, torch.stack([b, a], 1))  # [Jr b; Ji a]
+        return s2 * torch.cat([u1[:m] + u1[m:], u2[:m] + u2[m:]]) + self.lam * v
+
+    def solve(self, misfit):
+        JT, m = self.JT, self.np
+        mk = getattr(self, "mark", None) or (lambda name, t: t)
+        tq = time.perf_counter()
+        if self.precision == "fp64":
+            colsq = _gpu.jt_cols(JT, 2, m)
+            G = _gpu.gram(JT, None, "fp64")
+        else:
+            G, colsq = _gpu.gram(JT, None, "bf16x3", want_colsq=True)
+            G.diagonal().copy_(colsq)
+        s2 = 1.0 / (2.0 * torch.sum(colsq).item())
+        tq = mk("gn_gram", tq)
+        rr, ri, ir, ii = G[:m, :m], G[:m, m:], G[m:, :m], G[m:, m:]
+        H = torch.empty_like(G)
+        H[:m, :m] = rr + ii
+        H[m:, m:] = rr + ii
+        H[:m, m:] = ri - ir
+        H[m:, :m] = ir - ri
+        H.mul_(s2)
+        del G
+        y = self._factor_refine(H, misfit, lambda v: self._hv(v, s2), mk, tq)
+        return s2 * _gpu.jt_rows(JT, 1, 2 * m, y)  # (nn, 1) = x
 
 
 def _chromophores(cfg):
@@ -328,8 +382,9 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
     Supported on the GPU: a single-wavelength node-based mua unknown
     (recon['prop'] with nn rows), and multispectral chromophore unknowns
     (recon['param'] node arrays); CW or frequency-domain data with reform
-    'real' ([Re; Im] rows for complex data, recon.py:532-546) or 'logphase'
-    (log-amplitude and phase rows, :548-565); a dual mesh (recon['node'],
+    'real' ([Re; Im] rows for complex data, recon.py:532-546), 'reim'
+    ([[Re, -Im], [Im, Re]] rows, :537-541, one unknown block on one mesh) or
+    'logphase' (log-amplitude and phase rows, :548-565); a dual mesh (recon['node'],
     ['elem'], ['mapid'], ['mapweight']) for a single unknown block, the
     Jacobian moved onto the recon nodes as in recon.py:200-205.  Extra
     keywords: precision ('fp64' | 'mixed'), extin (dict wavelength ->
@@ -357,8 +412,9 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
     optodes = kwargs.get("optodes", None)
     if isinstance(rfcw, int):
         rfcw = [rfcw]
-    if rfcw != [1] or reform not in ("real", "complex", "logphase"):
-        raise NotImplementedError("GPU runrecon supports rfcw=[1] with reform 'real', 'logphase' (or 'complex' on CW)")
+    if rfcw != [1] or reform not in ("real", "reim", "complex", "logphase"):
+        raise NotImplementedError("GPU runrecon supports rfcw=[1] with reform 'real', 'reim', 'logphase' "
+                                  "(or 'complex' on CW)")
     if kwargs.get("prior") or "lmat" in kwargs or "ltl" in kwargs:
         raise NotImplementedError("structural priors are not supported")
     if sd is None:
@@ -472,7 +528,14 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
                 remap = _gpu.remap_operator(recon["mapid"], recon["mapweight"], recon["elem"], nrec,
                                             perm_rows=mesh.iperm, dev=mesh.dev)
             JG = _gpu.jt_remap(remap, JT, nrec)
-        step = GNStep(JG, npairs, E, lam, precision, reftol=kwargs.get("reftol", 1e-10))
+        if reform == "reim" and cplx:
+            # [[Jr, -Ji], [Ji, Jr]] rows; with several unknown blocks the reference's
+            # update parsing (recon.py:243-251) mixes real and imaginary parts
+            if len(keys) != 1 or nw != 1 or dual:
+                raise NotImplementedError("reform 'reim' is supported for one unknown block on one mesh")
+            step = GNStepReIm(JG, npairs, lam, precision, reftol=kwargs.get("reftol", 1e-10))
+        else:
+            step = GNStep(JG, npairs, E, lam, precision, reftol=kwargs.get("reftol", 1e-10))
         step.mark = (lambda name, t: mark(name, t)) if timing is not None else None
         tp = mark("misfit", tp)
         dmu = step.solve(torch.as_tensor(misfit, dtype=torch.float64, device=mesh.dev))

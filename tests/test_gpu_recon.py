@@ -167,12 +167,42 @@ def test_runrecon_mua_fd_and_dual_mesh_match_oracle(omega, dual, reform):
     assert out1[1][-1] < out1[1][0]
 
 
-def test_runrecon_unsupported_forms_raise():
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-7), ("mixed", 1e-6)])
+def test_runrecon_mua_reim_matches_oracle(precision, tol):
+    """FD data with [[Re, -Im], [Im, Re]] rows: the real part of the complex update."""
     import redbirdpy_amd as rb
+    from oracle import rb_oracle as orc
 
     cfg0, recon, y0 = _mua_problem(2 * np.pi * 1e8, False)
     c1, sd1 = rb.meshprep(copy.deepcopy(cfg0))
+    c2, sd2 = orc.meshprep(copy.deepcopy(cfg0))
+    r1, r2 = copy.deepcopy(recon), copy.deepcopy(recon)
+    lam = 1e-3 / y0.size
+    out1 = rb.runrecon(c1, r1, y0, sd1, maxiter=3, lambda_=lam, reform="reim", report=False,
+                       precision=precision, solvertol=1e-12)
+    out2 = orc.runrecon(c2, r2, y0, sd2, maxiter=3, lambda_=lam, reform="reim")
+    a, b = r1["prop"][:, 0], r2["prop"][:, 0]
+    assert np.max(np.abs(a - b)) / np.max(np.abs(b)) < tol
+    np.testing.assert_allclose(out1[1], out2[1], rtol=tol)
+    assert out1[1][-1] < out1[1][0]
+    # not the 'real' update: the imaginary rows couple to the second unknown block
+    r3 = copy.deepcopy(recon)
+    c3, sd3 = rb.meshprep(copy.deepcopy(cfg0))
+    rb.runrecon(c3, r3, y0, sd3, maxiter=1, lambda_=lam, reform="real", report=False)
+    r4 = copy.deepcopy(recon)
+    c4, sd4 = rb.meshprep(copy.deepcopy(cfg0))
+    rb.runrecon(c4, r4, y0, sd4, maxiter=1, lambda_=lam, reform="reim", report=False)
+    assert not np.allclose(r3["prop"][:, 0], r4["prop"][:, 0], rtol=1e-3)
+
+
+def test_runrecon_unsupported_forms_raise():
+    import redbirdpy_amd as rb
+
+    cfg0, recon, y0 = _mua_problem(2 * np.pi * 1e8, True)
+    c1, sd1 = rb.meshprep(copy.deepcopy(cfg0))
     with pytest.raises(NotImplementedError):
         rb.runrecon(c1, copy.deepcopy(recon), y0, sd1, maxiter=1, reform="reim", report=False)
+    cfg0, recon, y0 = _mua_problem(2 * np.pi * 1e8, False)
+    c1, sd1 = rb.meshprep(copy.deepcopy(cfg0))
     with pytest.raises(NotImplementedError):
         rb.runrecon(c1, copy.deepcopy(recon), y0, sd1, maxiter=1, reform="complex", report=False)

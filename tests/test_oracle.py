@@ -323,6 +323,28 @@ def test_reginv_matches_lstsq():
         np.testing.assert_allclose(x, xl, rtol=1e-10, atol=1e-12)
 
 
+def test_matreform_reim_is_the_complex_tikhonov_update():
+    """'reim' rows [[Re, -Im], [Im, Re]] are the real form of the complex map z -> J z, so the
+    underdetermined step x = A^T (A A^T + lam I)^-1 r is [Re z; Im z] with
+    z = J^H (J J^H + lam I)^-1 r; 'real' rows ([Re; Im], real unknown) give another step."""
+    rng = np.random.default_rng(12)
+    m, n = 7, 30
+    J = rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))
+    y0 = rng.standard_normal(m) + 1j * rng.standard_normal(m)
+    y = rng.standard_normal(m) + 1j * rng.standard_normal(m)
+    lam = 0.2
+    A, rhs, nb = orc.matreform(J, y0, y, "reim")
+    assert nb == 2 and A.shape == (2 * m, 2 * n)
+    x = orc.reginv(A, rhs, lam)
+    r = y0 - y
+    z = J.conj().T @ np.linalg.solve(J @ J.conj().T + lam * np.eye(m), r)
+    np.testing.assert_allclose(x[:n], z.real, rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(x[n:], z.imag, rtol=1e-10, atol=1e-12)
+    Ar, rr, _ = orc.matreform(J, y0, y, "real")
+    assert Ar.shape == (2 * m, n)
+    assert not np.allclose(orc.reginv(Ar, rr, lam), z.real, rtol=1e-3)
+
+
 def test_getreff_against_quadrature():
     def exact(n):
         oc = np.arcsin(1 / n)
