@@ -1066,7 +1066,8 @@ def normalize_diag(A):
 
 
 def reginvover(A, rhs, lam, LTL=None, blocks=None):
-    """recon.py:363-426 -- (A^T A + lam I) x = A^T rhs (no prior matrix)."""
+    """recon.py:363-426 -- (A^T A + lam (I | L^T L)) x = A^T rhs; an L^T L smaller than the
+    Hessian is added to each diagonal block of its size (recon.py:404-412)."""
     cs = np.sum(np.abs(A), axis=0)
     keep = np.where(cs != 0)[0]
     n0 = A.shape[1]
@@ -1084,8 +1085,13 @@ def reginvover(A, rhs, lam, LTL=None, blocks=None):
     H = A.T @ A
     if LTL is None:
         H[np.diag_indices_from(H)] += lam
-    else:
+    elif H.shape[0] == LTL.shape[0]:
         H = H + lam * LTL
+    else:
+        nx = LTL.shape[0]
+        for i in range(0, H.shape[0], nx):
+            e = min(i + nx, H.shape[0])
+            H[i:e, i:e] = H[i:e, i:e] + lam * LTL[: e - i, : e - i]
     Hn, di = normalize_diag(H)
     res = di * np.linalg.solve(Hn, di * g)
     if len(keep) < n0:
@@ -1096,7 +1102,9 @@ def reginvover(A, rhs, lam, LTL=None, blocks=None):
 
 
 def reginvunder(A, rhs, lam, invR=None, blocks=None):
-    """recon.py:429-509 -- x = A^T (A A^T + lam I)^-1 rhs (no prior transform)."""
+    """recon.py:429-509 -- x = R^-1 A'^T (A' A'^T + lam I)^-1 rhs with A' = A R^-1 (R^-1 = invR),
+    applied to the whole A when sizes match, else to each unknown block of invR's size
+    (recon.py:458-470, :487-499); no transform without invR."""
     n0 = A.shape[1]
     cs = np.sum(np.abs(A), axis=0)
     keep = np.where(cs != 0)[0]
@@ -1107,19 +1115,39 @@ def reginvunder(A, rhs, lam, invR=None, blocks=None):
     if good.sum() < A.shape[0]:
         A = A[good]
         rhs = rhs[good]
+    spans = []  # column ranges transformed by invR
     if invR is not None:
-        A = A @ invR
+        nx = invR.shape[0]
+        if nx == A.shape[1]:
+            spans = [(0, nx)]
+        elif blocks is not None:
+            cum = np.cumsum([0] + [v[1] for v in blocks.values()])
+            spans = [(cum[i], cum[i + 1]) for i in range(len(cum) - 1) if cum[i + 1] - cum[i] == nx]
+        A = A.copy()
+        for a, b in spans:
+            A[:, a:b] = A[:, a:b] @ invR
     rhs = rhs.ravel()
     H = A @ A.T
     H[np.diag_indices_from(H)] += lam
     Hn, di = normalize_diag(H)
     y = di * np.linalg.solve(Hn, di * rhs)
-    res = invR @ (A.T @ y) if invR is not None else A.T @ y
+    res = A.T @ y
+    for a, b in spans:
+        res[a:b] = invR @ res[a:b]
     if len(keep) < n0:
         full = np.zeros(n0, dtype=res.dtype)
         full[keep] = res
         res = full
     return res
+
+
+def masksum(J, mask):
+    """recon.py:879-893 -- sum the columns of J over each label of mask (labels ascending)."""
+    labels = np.unique(mask)
+    out = np.zeros((J.shape[0], len(labels)), dtype=J.dtype)
+    for i, lab in enumerate(labels):
+        out[:, i] = J[:, mask == lab].sum(axis=1)
+    return out
 
 
 def reginv(A, rhs, lam, Areg=None, blocks=None):
@@ -1261,7 +1289,8 @@ def syncprop(cfg, recon):
 
 
 def runrecon(cfg, recon, detphi0, sd=None, extin=None, **kw):
-    """recon.py:39-335 -- Gauss-Newton loop (single or dual mesh, no prior, no label compression).
+    """recon.py:39-335 -- Gauss-Newton loop (single or dual mesh; an explicit ``lmat`` / ``ltl``
+    prior; label compression of the Jacobian by a 1-D recon['seg'] of fewer than 50 labels).
 
     ``extin`` (project extension): dict wavelength -> {chromophore: value},
     used by updateprop and jacchrome in place of the built-in table.
@@ -1279,6 +1308,11 @@ def runrecon(cfg, recon, detphi0, sd=None, extin=None, **kw):
         recon["prop"] = recon["prop"].reshape(1, -1)
     resid = np.zeros(maxiter)
     updates = []
+    Aregu = {}  # recon.py:127-133 (an explicit lmat or ltl; the prior() builder is product code)
+    if "lmat" in kw:
+        Aregu["lmat"] = kw["lmat"]
+    if "ltl" in kw:
+        Aregu["ltl"] = kw["ltl"]
     for it in range(maxiter):
         if "param" in recon or "prop" in recon:
             cfg, recon = syncprop(cfg, recon)
@@ -1311,10 +1345,24 @@ def runrecon(cfg, recon, detphi0, sd=None, extin=None, **kw):
             J = remap_jacobian(J, recon, cfg)
             nrec = np.asarray(recon["node"]).shape[0]
             blocks = {k: (v[0], nrec) for k, v in blocks.items()}
+        if "seg" in recon and np.ndim(recon["seg"]) == 1:  # recon.py:210-218
+            seg = np.asarray(recon["seg"])
+            nlab = len(np.unique(seg))
+            if len(seg) == J.shape[1] and nlab < 50:
+                J = masksum(J, seg)
+                blocks = {k: (v[0], nlab) for k, v in blocks.items()}
         resid[it] = np.sum(np.abs(mis))
+        if it == 0 and Aregu and "lmat" in Aregu and "ltl" not in Aregu:  # recon.py:225-233
+            if J.shape[0] >= J.shape[1]:
+                Aregu["ltl"] = Aregu["lmat"].T @ Aregu["lmat"]
+            else:
+                from scipy.linalg import qr
+
+                _, R = qr(Aregu["lmat"])
+                Aregu["lir"] = np.linalg.inv(np.triu(R))
         s = 1.0 / np.sqrt(np.sum(J ** 2))
         J = J * s
-        dmu = reginv(J, mis, lam, {}, blocks) * s
+        dmu = reginv(J, mis, lam, Aregu, blocks) * s
         upd = {}
         idx = 0
         for key in blocks:

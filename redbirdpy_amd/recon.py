@@ -238,8 +238,9 @@ class GNStep:
     Returns dmu (nn x nc) = s^2 J^T y.
     """
 
-    def __init__(self, JT, npairs, E, lam, precision="fp64", reftol=1e-10, maxref=20):
+    def __init__(self, JT, npairs, E, lam, precision="fp64", reftol=1e-10, maxref=20, scale2=None):
         self.JT = JT
+        self.scale2 = scale2  # 1/||J||_F^2 taken before a prior transform of JT (recon.py:235)
         self.np = npairs
         self.E = torch.as_tensor(E, dtype=torch.float64, device=JT.device)
         self.lam = float(lam)
@@ -272,7 +273,7 @@ class GNStep:
             G, colsq = _gpu.gram(JT, None, "bf16x3", want_colsq=True)
             G.diagonal().copy_(colsq)
         wdiag = torch.repeat_interleave(torch.diagonal(W), npairs)
-        s2 = 1.0 / torch.sum(wdiag * colsq).item()  # 1/||J_flat||_F^2
+        s2 = 1.0 / torch.sum(wdiag * colsq).item() if self.scale2 is None else self.scale2  # 1/||J_flat||_F^2
         tq = mk("gn_gram", tq)
         H = s2 * (G * Wk)
         del G
@@ -370,6 +371,48 @@ class GNStepReIm(GNStep):
         return s2 * _gpu.jt_rows(JT, 1, 2 * m, y)  # (nn, 1) = x
 
 
+def _solve_over(JG, npairs, E, lam, misfit, LTL=None):
+    """reginvover (recon.py:363-426) on the flattened Jacobian, for few unknowns
+    (labels or a small mesh): J[(b,p),(c,n)] = E[b,c] JG[n, b*np+p], blockscale
+    s = 1/||J||_F, zero columns and rows dropped, (J^T J + lam (I | L^T L)) x = J^T r
+    in the unit-diagonal form; returns s * x as (n x nc)."""
+    N, M = JG.shape
+    nb, nc = E.shape
+    A = (JG.t().reshape(nb, npairs, 1, N) * E[:, None, :, None]).reshape(nb * npairs, nc * N)
+    s = 1.0 / torch.sqrt(torch.sum(A * A))
+    A = A * s
+    keep = torch.nonzero(torch.sum(torch.abs(A), 0) != 0).reshape(-1)
+    n0 = A.shape[1]
+    if keep.numel() < n0:
+        A = A[:, keep]
+        if LTL is not None and LTL.shape[0] > keep.numel():
+            li = keep.cpu().numpy()
+            li = li[li < LTL.shape[0]]
+            LTL = LTL[np.ix_(li, li)]
+    rows = torch.sum(torch.abs(A), 1) != 0
+    A, r = A[rows], misfit[rows]
+    g = A.t() @ r
+    H = A.t() @ A
+    if LTL is None:
+        H.diagonal().add_(lam)
+    else:
+        Lt = torch.as_tensor(LTL, dtype=torch.float64, device=A.device)
+        if Lt.shape[0] == H.shape[0]:
+            H = H + lam * Lt
+        else:
+            nx = Lt.shape[0]
+            for i in range(0, H.shape[0], nx):
+                e = min(i + nx, H.shape[0])
+                H[i:e, i:e] += lam * Lt[: e - i, : e - i]
+    Hn, di = _normalize_diag(H)
+    x = di * torch.linalg.solve(Hn, di * g)
+    if keep.numel() < n0:
+        full = torch.zeros(n0, dtype=x.dtype, device=x.device)
+        full[keep] = x
+        x = full
+    return (s * x).reshape(nc, N).t()
+
+
 def _chromophores(cfg):
     if isinstance(cfg.get("prop"), dict) and "param" in cfg:
         return [k for k in cfg["param"].keys() if k in CHROMOPHORES]
@@ -384,7 +427,10 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
     (recon['param'] node arrays); CW or frequency-domain data with reform
     'real' ([Re; Im] rows for complex data, recon.py:532-546), 'reim'
     ([[Re, -Im], [Im, Re]] rows, :537-541, one unknown block on one mesh) or
-    'logphase' (log-amplitude and phase rows, :548-565); a dual mesh (recon['node'],
+    'logphase' (log-amplitude and phase rows, :548-565); priors (lmat, ltl, or
+    prior= with recon['seg'], :127-133, :225-233) and label compression by a 1-D
+    recon['seg'] (:210-218), with the overdetermined step when the unknowns are
+    few (:363-426); a dual mesh (recon['node'],
     ['elem'], ['mapid'], ['mapweight']) for a single unknown block, the
     Jacobian moved onto the recon nodes as in recon.py:200-205.  Extra
     keywords: precision ('fp64' | 'mixed'), extin (dict wavelength ->
@@ -415,8 +461,15 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
     if rfcw != [1] or reform not in ("real", "reim", "complex", "logphase"):
         raise NotImplementedError("GPU runrecon supports rfcw=[1] with reform 'real', 'reim', 'logphase' "
                                   "(or 'complex' on CW)")
-    if kwargs.get("prior") or "lmat" in kwargs or "ltl" in kwargs:
-        raise NotImplementedError("structural priors are not supported")
+    Aregu = {}  # recon.py:127-133
+    if "lmat" in kwargs:
+        Aregu["lmat"] = np.asarray(kwargs["lmat"], dtype=float)
+    if "ltl" in kwargs:
+        Aregu["ltl"] = np.asarray(kwargs["ltl"], dtype=float)
+    elif "lmat" not in kwargs and kwargs.get("prior") and "seg" in recon:
+        lm = prior(recon["seg"], kwargs["prior"], kwargs)
+        if lm is not None:
+            Aregu["lmat"] = lm.toarray() if hasattr(lm, "toarray") else np.asarray(lm, dtype=float)
     if sd is None:
         sd = sdmap(cfg)
     if "prop" in recon and isinstance(recon["prop"], np.ndarray) and recon["prop"].ndim == 1:
@@ -528,31 +581,81 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
                 remap = _gpu.remap_operator(recon["mapid"], recon["mapweight"], recon["elem"], nrec,
                                             perm_rows=mesh.iperm, dev=mesh.dev)
             JG = _gpu.jt_remap(remap, JT, nrec)
+        # label compression (recon.py:210-218): node columns summed per label
+        labelled = False
+        if "seg" in recon and np.ndim(recon["seg"]) == 1:
+            seg = np.asarray(recon["seg"])
+            labs = np.unique(seg)
+            if len(seg) == JG.shape[0] and len(labs) < 50:
+                seg_rows = seg if dual else mesh.mesh_nodes(seg)  # JG rows are in device order
+                S = torch.as_tensor(seg_rows[None, :] == labs[:, None], dtype=torch.float64, device=mesh.dev)
+                JG = S @ JG  # (nlabels x M), one-hot product: a fixed summation order
+                labelled = True
+        ncol = JG.shape[0] * E.shape[1]
+        over = JG.shape[1] >= ncol  # reginv's branch by the flattened shape (recon.py:356)
+        if it == 0 and "lmat" in Aregu and "ltl" not in Aregu:  # recon.py:225-233
+            if over:
+                Aregu["ltl"] = Aregu["lmat"].T @ Aregu["lmat"]
+            else:
+                from scipy.linalg import qr
+
+                Aregu["lir"] = np.linalg.inv(np.triu(qr(Aregu["lmat"], mode="r")[0]))
         if reform == "reim" and cplx:
             # [[Jr, -Ji], [Ji, Jr]] rows; with several unknown blocks the reference's
             # update parsing (recon.py:243-251) mixes real and imaginary parts
-            if len(keys) != 1 or nw != 1 or dual:
-                raise NotImplementedError("reform 'reim' is supported for one unknown block on one mesh")
-            step = GNStepReIm(JG, npairs, lam, precision, reftol=kwargs.get("reftol", 1e-10))
+            if len(keys) != 1 or nw != 1 or dual or labelled or Aregu:
+                raise NotImplementedError("reform 'reim' is supported for one unknown block on one mesh, "
+                                          "without priors or label compression")
+        misfit_t = torch.as_tensor(misfit, dtype=torch.float64, device=mesh.dev)
+        step = None
+        if over:
+            tp = mark("misfit", tp)
+            dmu = _solve_over(JG, npairs, torch.as_tensor(E, dtype=torch.float64, device=mesh.dev), lam,
+                              misfit_t, Aregu.get("ltl"))
         else:
-            step = GNStep(JG, npairs, E, lam, precision, reftol=kwargs.get("reftol", 1e-10))
-        step.mark = (lambda name, t: mark(name, t)) if timing is not None else None
-        tp = mark("misfit", tp)
-        dmu = step.solve(torch.as_tensor(misfit, dtype=torch.float64, device=mesh.dev))
-        dmu = (dmu if dual else mesh.user_rows(dmu)).cpu().numpy()
+            invR = None
+            lir = Aregu.get("lir")
+            if lir is not None and lir.shape[0] == JG.shape[0]:
+                # recon.py:458-470 transforms the unknown blocks of the prior's size;
+                # every block here has JG.shape[0] columns (a prior of another size
+                # transforms nothing, as in the reference)
+                invR = torch.as_tensor(lir, dtype=torch.float64, device=mesh.dev)
+                if not (dual or labelled) and mesh.reordered:
+                    perm = torch.as_tensor(mesh.perm, device=mesh.dev)
+                    invR = invR[perm][:, perm]  # into the device node order of JG's rows
+                # the blockscale is that of the untransformed J (recon.py:235-236)
+                Et = torch.as_tensor(E, dtype=torch.float64, device=mesh.dev)
+                wdiag = torch.repeat_interleave(torch.diagonal(Et @ Et.t()), npairs)
+                s2pre = 1.0 / torch.sum(wdiag * _gpu.jt_cols(JG, E.shape[0], npairs)).item()
+                JG = invR.t() @ JG  # A' = A R^-1 per block: node-major J' = R^-T J
+            else:
+                s2pre = None
+            if reform == "reim" and cplx:
+                step = GNStepReIm(JG, npairs, lam, precision, reftol=kwargs.get("reftol", 1e-10))
+            else:
+                step = GNStep(JG, npairs, E, lam, precision, reftol=kwargs.get("reftol", 1e-10), scale2=s2pre)
+            step.mark = (lambda name, t: mark(name, t)) if timing is not None else None
+            tp = mark("misfit", tp)
+            dmu = step.solve(misfit_t)
+            if invR is not None:
+                dmu = invR @ dmu  # back to the unknowns (recon.py:487-499)
+        dmu = (dmu if (dual or labelled) else mesh.user_rows(dmu)).cpu().numpy()
         del JG
         tp = mark("gn_solve", tp)
-        if timing is not None:
+        if timing is not None and step is not None:
             timing.setdefault("refinements", []).append(step.info.get("refinements", 0))
         update = {}
         for c, key in enumerate(keys):
             dx = dmu[:, c]
             update[key] = dx
-            if key == "mua":
-                if "prop" in recon and isinstance(recon["prop"], np.ndarray) and recon["prop"].shape[0] >= 50:
-                    recon["prop"][:, 0] += dx
-                else:
-                    raise NotImplementedError("label-based mua reconstruction is not supported")
+            if key == "mua":  # recon.py:255-276
+                if "prop" in recon and isinstance(recon["prop"], np.ndarray):
+                    pr = recon["prop"]
+                    if pr.shape[0] < len(dx) and pr.shape[0] < 50:  # label table, node update
+                        for li in range(min(pr.shape[0], len(dx))):
+                            pr[li, 0] += dx[li]
+                    else:
+                        pr[:, 0] += dx
             else:
                 tgt = recon if ("param" in recon and key in recon["param"]) else cfg
                 if "param" not in tgt or key not in tgt["param"]:

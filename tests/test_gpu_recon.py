@@ -206,3 +206,99 @@ def test_runrecon_unsupported_forms_raise():
     c1, sd1 = rb.meshprep(copy.deepcopy(cfg0))
     with pytest.raises(NotImplementedError):
         rb.runrecon(c1, copy.deepcopy(recon), y0, sd1, maxiter=1, reform="complex", report=False)
+
+
+def _label_problem():
+    """Two tissue labels (0 background, 1 inclusion) on a 20x20x10 mm box; node labels
+    in recon['seg'] compress the Jacobian to two columns (an overdetermined step)."""
+    from conftest import box_cfg, grid_optodes
+    from redbirdpy_amd.mesh import kuhn_box
+    from oracle import rb_oracle as orc
+
+    p1 = (20, 20, 10)
+    node, elem = kuhn_box([0, 0, 0], p1, 2.0)
+    cen = node[elem[:, :4] - 1].mean(axis=1)
+    elab = (np.sum((cen - [10, 10, 4]) ** 2, axis=1) < 16).astype(int)
+    nlab = (np.sum((node - [10, 10, 4]) ** 2, axis=1) < 16).astype(int)
+    truth = np.array([[0.01, 1.0, 0.0, 1.37], [0.02, 1.0, 0.0, 1.37]])
+    cfg0 = box_cfg(p1, 2.0, prop=truth, labels=elab, src=grid_optodes(p1, 3), det=grid_optodes(p1, 3, margin=6.0))
+    ct, _ = orc.meshprep(copy.deepcopy(cfg0))
+    y0, _ = orc.runforward(ct)
+    cfg0["prop"] = np.array([[0.01, 1.0, 0.0, 1.37], [0.01, 1.0, 0.0, 1.37]])
+    recon = {"prop": cfg0["prop"].copy(), "seg": nlab}
+    return cfg0, recon, y0
+
+
+@pytest.mark.parametrize("reg", ["none", "ltl", "lmat"])
+def test_runrecon_label_compression_matches_oracle(reg):
+    """Label-based mua (recon.py:210-218 compression, reginvover with an optional L^T L)."""
+    import redbirdpy_amd as rb
+    from oracle import rb_oracle as orc
+
+    cfg0, recon, y0 = _label_problem()
+    kw = {}
+    if reg == "ltl":
+        kw["ltl"] = np.diag([1.0, 2.0])
+    elif reg == "lmat":
+        kw["lmat"] = np.array([[1.0, -0.5], [-0.5, 1.0]])
+    c1, sd1 = rb.meshprep(copy.deepcopy(cfg0))
+    c2, sd2 = orc.meshprep(copy.deepcopy(cfg0))
+    r1, r2 = copy.deepcopy(recon), copy.deepcopy(recon)
+    out1 = rb.runrecon(c1, r1, y0, sd1, maxiter=3, lambda_=0.05, report=False, **kw)
+    out2 = orc.runrecon(c2, r2, y0, sd2, maxiter=3, lambda_=0.05, **kw)
+    np.testing.assert_allclose(r1["prop"], r2["prop"], rtol=1e-8)
+    np.testing.assert_allclose(out1[1], out2[1], rtol=1e-8)
+    assert r1["prop"][1, 0] > 0.0125  # the inclusion label moves toward its true 0.02
+    assert out1[3][0]["mua"].shape == (2,)
+
+
+def test_runrecon_node_prior_matches_oracle():
+    """Underdetermined node-based mua with an explicit Laplace prior over two node groups:
+    J' = J R^-1 (R from QR of L), dmu = R^-1 J'^T y (recon.py:225-233, :458-499)."""
+    import redbirdpy_amd as rb
+    from oracle import rb_oracle as orc
+
+    cfg0, recon, y0 = _mua_problem(0.0, False)
+    ct, _ = orc.meshprep(copy.deepcopy(cfg0))
+    grp = (np.sum((ct["node"] - [10, 10, 4]) ** 2, axis=1) < 16).astype(int)
+    L = rb.prior(grp, "laplace")
+    c1, sd1 = rb.meshprep(copy.deepcopy(cfg0))
+    c2, sd2 = orc.meshprep(copy.deepcopy(cfg0))
+    r1, r2 = copy.deepcopy(recon), copy.deepcopy(recon)
+    lam = 1e-3 / y0.size
+    out1 = rb.runrecon(c1, r1, y0, sd1, maxiter=2, lambda_=lam, report=False, lmat=L)
+    out2 = orc.runrecon(c2, r2, y0, sd2, maxiter=2, lambda_=lam, lmat=L)
+    a, b = r1["prop"][:, 0], r2["prop"][:, 0]
+    assert np.max(np.abs(a - b)) / np.max(np.abs(b - 0.01)) < 1e-7
+    np.testing.assert_allclose(out1[1], out2[1], rtol=1e-7)
+    # the prior changes the step
+    r3 = copy.deepcopy(recon)
+    c3, sd3 = rb.meshprep(copy.deepcopy(cfg0))
+    rb.runrecon(c3, r3, y0, sd3, maxiter=1, lambda_=lam, report=False)
+    r4 = copy.deepcopy(recon)
+    c4, sd4 = rb.meshprep(copy.deepcopy(cfg0))
+    rb.runrecon(c4, r4, y0, sd4, maxiter=1, lambda_=lam, report=False, lmat=L)
+    assert not np.allclose(r3["prop"][:, 0], r4["prop"][:, 0], rtol=1e-4)
+
+
+@pytest.mark.parametrize("kw", [{"lmat": np.eye(1)}, {"ltl": np.eye(1)}])
+def test_runrecon_reference_prior_cases_match_oracle(kw):
+    """The reference's own prior cases (test/test_recon.py:723-767): a label table with a
+    1x1 lmat or ltl on a node Jacobian, where the prior's size matches no unknown block
+    and the label rows take the first node updates (recon.py:255-266)."""
+    import redbirdpy_amd as rb
+    from conftest import box_cfg
+    from oracle import rb_oracle as orc
+
+    cfg0 = box_cfg((20, 20, 10), 2.0)
+    ct, sdt = orc.meshprep(copy.deepcopy(cfg0))
+    y0, _ = orc.runforward(ct, sd=sdt)
+    y0 = y0 * 0.95
+    recon = {"prop": cfg0["prop"].copy(), "lambda": 0.1, "seg": np.arange(1)}
+    c1, sd1 = rb.meshprep(copy.deepcopy(cfg0))
+    c2, sd2 = orc.meshprep(copy.deepcopy(cfg0))
+    r1, r2 = copy.deepcopy(recon), copy.deepcopy(recon)
+    out1 = rb.runrecon(c1, r1, y0, sd1, maxiter=1, report=False, **kw)
+    out2 = orc.runrecon(c2, r2, y0, sd2, maxiter=1, **kw)
+    np.testing.assert_allclose(r1["prop"], r2["prop"], rtol=1e-8, atol=1e-14)
+    np.testing.assert_allclose(out1[1], out2[1], rtol=1e-8)

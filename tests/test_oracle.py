@@ -345,6 +345,58 @@ def test_matreform_reim_is_the_complex_tikhonov_update():
     assert not np.allclose(orc.reginv(Ar, rr, lam), z.real, rtol=1e-3)
 
 
+def test_reginv_with_priors_closed_forms():
+    """Overdetermined: (A^T A + lam L^T L) x = A^T b is the least-squares solution of
+    [A; sqrt(lam) L] x = [b; 0] (an L^T L of one block's size repeats down the diagonal).
+    Underdetermined with R^-1 from L = QR: x = P A^T (A P A^T + lam I)^-1 b, P = (L^T L)^-1,
+    applied per unknown block of R^-1's size."""
+    from scipy.linalg import qr
+
+    rng = np.random.default_rng(21)
+    m, n = 40, 6
+    A = rng.standard_normal((m, n))
+    b = rng.standard_normal(m)
+    L = np.eye(n) + 0.3 * rng.standard_normal((n, n))
+    lam = 0.5
+    x = orc.reginvover(A, b, lam, L.T @ L)
+    xl = np.linalg.lstsq(np.vstack([A, np.sqrt(lam) * L]), np.concatenate([b, np.zeros(n)]), rcond=None)[0]
+    np.testing.assert_allclose(x, xl, rtol=1e-10, atol=1e-12)
+    Lb = np.eye(3) + 0.3 * rng.standard_normal((3, 3))
+    x = orc.reginvover(A, b, lam, Lb.T @ Lb)
+    Lf = np.kron(np.eye(2), Lb)
+    xl = np.linalg.lstsq(np.vstack([A, np.sqrt(lam) * Lf]), np.concatenate([b, np.zeros(n)]), rcond=None)[0]
+    np.testing.assert_allclose(x, xl, rtol=1e-10, atol=1e-12)
+    # underdetermined
+    m, n = 5, 12
+    A = rng.standard_normal((m, n))
+    b = rng.standard_normal(m)
+    L = np.eye(n) + 0.2 * rng.standard_normal((n, n))
+    _, R = qr(L)
+    x = orc.reginvunder(A, b, lam, np.linalg.inv(np.triu(R)))
+    P = np.linalg.inv(L.T @ L)
+    np.testing.assert_allclose(x, P @ A.T @ np.linalg.solve(A @ P @ A.T + lam * np.eye(m), b), rtol=1e-9)
+    Lb = np.eye(4) + 0.2 * rng.standard_normal((4, 4))
+    _, R = qr(Lb)
+    blocks = {"a": (m, 4), "b": (m, 4), "c": (m, 4)}
+    x = orc.reginvunder(A, b, lam, np.linalg.inv(np.triu(R)), blocks)
+    P = np.kron(np.eye(3), np.linalg.inv(Lb.T @ Lb))
+    np.testing.assert_allclose(x, P @ A.T @ np.linalg.solve(A @ P @ A.T + lam * np.eye(m), b), rtol=1e-9)
+
+
+def test_masksum_is_the_adjoint_of_label_expansion():
+    """masksum(J, seg) v = J (v expanded to nodes by label) for every label vector v."""
+    rng = np.random.default_rng(22)
+    seg = rng.integers(3, 9, size=40)
+    J = rng.standard_normal((7, 40))
+    labs = np.unique(seg)
+    Jc = orc.masksum(J, seg)
+    assert Jc.shape == (7, len(labs))
+    for _ in range(3):
+        v = rng.standard_normal(len(labs))
+        np.testing.assert_allclose(Jc @ v, J @ v[np.searchsorted(labs, seg)], rtol=1e-12)
+    np.testing.assert_allclose(Jc.sum(axis=1), J.sum(axis=1), rtol=1e-12)
+
+
 def test_getreff_against_quadrature():
     def exact(n):
         oc = np.arcsin(1 / n)
