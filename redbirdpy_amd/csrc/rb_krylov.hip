@@ -460,6 +460,12 @@ __device__ __forceinline__ void sweep_body(const Csr& m, const SweepOps<T, TD>& 
   for (int ch = 0; ch < NCH; ++ch) {
     if (ch * 64 >= C) break;
     if (MODE == 1 && op.gate && !op.gate[ch]) continue;  // chunk not replaced this time
+    if constexpr (MODE == 0) {
+      // a chunk whose columns have all converged needs no q (its alpha is 0,
+      // its p.q partials stay 0 and its stale q is never read)
+      const int cc = lane + ch * 64;
+      if (!__any(cc < C && st.active[cc])) continue;
+    }
     const int c = lane + ch * 64;
     const bool live = c < C;
     const uint32_t voff = (live ? c : 0) * (uint32_t)sizeof(T);
@@ -735,18 +741,20 @@ __global__ void __launch_bounds__(KB) k_cg_update(int64_t n, const T* __restrict
   if (!block_any_active(st.active, C)) return;
   A pv[NCH][MAXND];
   A al[NCH];
+  bool act[NCH];  // converged columns (alpha 0) keep r and need no partials
 #pragma unroll
   for (int ch = 0; ch < NCH; ++ch) {
     pv[ch][0] = acc_zero(A());
     pv[ch][1] = acc_zero(A());
     int c = (int)(threadIdx.x % G) + ch * G;
     al[ch] = (c < C) ? st.alpha[c] : acc_zero(A());
+    act[ch] = c < C && (st.active[c] || acc_abs2(al[ch]) != 0.0);
   }
   ROW_LOOP(n) {
 #pragma unroll
     for (int ch = 0; ch < NCH; ++ch) {
       int c = lane + ch * G;
-      if (c < C) {
+      if (act[ch]) {
         int s = c / k;
         int64_t o = row * C + c;
         T r = R[o];
