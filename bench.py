@@ -191,7 +191,8 @@ def gram_bench(nn, m=4096, reps=3):
 def jac_bench(a, reps=5):
     """Node Jacobian of 1024 pairs on the 1 mm CW box (115,351 nodes), float64 out.
 
-    Algorithmic bytes nsd*nn*8 (output) + nn*k*8 (fields) + ne*24 (tets, volumes).
+    Algorithmic bytes nsd*nn*8 (output) + nn*k*8 (fields) + sum_n np_n^2 * 8 (the mesh's
+    patch matrices, built once per mesh and read on every call).
     """
     import torch
     import redbirdpy_amd as rb
@@ -214,7 +215,8 @@ def jac_bench(a, reps=5):
     e1.record()
     torch.cuda.synchronize()
     dt = e0.elapsed_time(e1) / 1e3 / reps
-    nbytes = len(s) * mesh.nn * 8 + mesh.nn * res.k * 8 + mesh.ne * 24
+    qptr, _ = _gpu.patch_matrices(mesh)
+    nbytes = len(s) * mesh.nn * 8 + mesh.nn * res.k * 8 + int(qptr[-1].item()) * 8
     return {"workload": "CW box 1 mm, %d pairs x %d nodes, float64" % (len(s), mesh.nn), "ms": dt * 1e3,
             "bytes": nbytes, "achieved_gbs": nbytes / dt / 1e9, "frac_hbm": nbytes / dt / 1e9 / HBM_PEAK_GBS}
 

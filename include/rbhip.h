@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define RB_ABI_VERSION 4
+#define RB_ABI_VERSION 5
 
 /* property layouts for rb_assemble */
 #define RB_PROP_ELEM 0   /* one row per element (label tables are expanded by the caller) */
@@ -185,6 +185,16 @@ int rb_getdet(int ndet, const int32_t* det_ptr, const int32_t* det_row, const do
               int64_t ldphi, int src0, int nsrc, int is_complex, void* out, void* stream);
 
 /*
+ * The node Jacobian's patch matrices, once per mesh (redbirdpy forward.py:562-567
+ * sums Jelem over each node's tets on every call; Q_n depends on the mesh
+ * only): Qc[qptr[n] + i*np + j] = Q_n[i][j] over the CSR row of n
+ * (np = row length, qptr = exclusive prefix sum of np^2).
+ */
+int rb_jac_patch_q(int64_t nn, const int32_t* el, const double* evol, const int32_t* inc_ptr,
+                   const int32_t* inc_list, const int32_t* row_ptr, const int32_t* col_idx, int maxnp,
+                   const int64_t* qptr, double* Qc, void* stream);
+
+/*
  * mua Jacobian on a source x detector grid (redbirdpy forward.py:509-569
  * `jac`, numpy branch :530-567).  For source column scol[s] and detector
  * column dcol[d] of phi (n x ldphi), cell = d*Ns + s, output row
@@ -195,12 +205,14 @@ int rb_getdet(int ndet, const int32_t* det_ptr, const int32_t* det_row, const do
  * Jelem = -0.05 V_e (sum_i ps_i pd_i + S_s S_d).  JT: nn x ldj node-major;
  * out_dtype RB_F64/RB_F32 (real phi) or RB_C128/RB_C64 (complex phi).
  * Optional Jelem: ne x ldj element-major.  maxnp = longest CSR row.
+ * qptr/Qc (both NULL or both given): the patch matrices Q_n from
+ * rb_jac_patch_q, read instead of rebuilt from the incident tets.
  */
 int rb_jac_grid(int64_t nn, int64_t ne, const int32_t* el, const double* evol, const int32_t* inc_ptr,
                 const int32_t* inc_list, const int32_t* row_ptr, const int32_t* col_idx, int maxnp,
                 const void* phi, int64_t ldphi, int is_complex, int Ns, int Nd, const int32_t* scol,
                 const int32_t* dcol, const int32_t* pmap, void* JT, int64_t ldj, int out_dtype, void* Jelem,
-                void* stream);
+                const int64_t* qptr, const double* Qc, void* stream);
 
 /*
  * Jacobian contraction g[n] = sum_{d,s} w[d*Ns+s] Jnode[n][(s,d)] without

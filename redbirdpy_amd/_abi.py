@@ -56,7 +56,8 @@ _SIGS = {
                           ctypes.POINTER(_int), ctypes.POINTER(_dbl), ctypes.POINTER(_int), _vp],
     "rb_getdet": [_int, _vp, _vp, _vp, _vp, _i64, _int, _int, _int, _vp, _vp],
     "rb_jac_grid": [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _int, _vp, _i64, _int, _int, _int, _vp, _vp,
-                    _vp, _vp, _i64, _int, _vp, _vp],
+                    _vp, _vp, _i64, _int, _vp, _vp, _vp, _vp],
+    "rb_jac_patch_q": [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _int, _vp, _vp, _vp],
     "rb_jac_contract_grid": [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _int, _int, _int, _vp, _vp, _vp, _vp,
                              _vp, _vp, _vp],
     "rb_gram": [_i64, _i64, _vp, _i64, _int, _vp, _int, _vp, _vp],

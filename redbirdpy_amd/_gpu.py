@@ -525,6 +525,23 @@ def pair_grid(src_cols, det_cols):
     return S, D, pmap, dups
 
 
+def patch_matrices(mesh):
+    """Per-node patch matrices Q_n of the node Jacobian (rb_jac_patch_q), built once per mesh
+    geometry and kept on the mesh (sum of np^2 doubles, ~0.2 GB at 115k nodes)."""
+    key = (id(mesh.evol), mesh.evol.data_ptr())
+    c = getattr(mesh, "_patch_q", None)
+    if c is not None and c[0] == key:
+        return c[1], c[2]
+    npr = (mesh.row_ptr[1:] - mesh.row_ptr[:-1]).to(torch.int64)
+    qptr = torch.zeros(mesh.nn + 1, dtype=torch.int64, device=mesh.dev)
+    torch.cumsum(npr * npr, 0, out=qptr[1:])
+    Qc = torch.empty(int(qptr[-1].item()), dtype=torch.float64, device=mesh.dev)
+    call("rb_jac_patch_q", mesh.nn, ptr(mesh.el), ptr(mesh.evol), ptr(mesh.tinc_ptr), ptr(mesh.tinc),
+         ptr(mesh.row_ptr), ptr(mesh.col_idx), mesh.maxnp, ptr(qptr), ptr(Qc), _abi.stream_ptr())
+    mesh._patch_q = (key, qptr, Qc)
+    return qptr, Qc
+
+
 def jac_node(mesh, phi, src_cols, det_cols, out_dtype=torch.float64, want_elem=False, out=None):
     """JT (nn x npairs) node-major Jacobian (and optional element Jacobian, ne x npairs).
 
@@ -551,9 +568,10 @@ def jac_node(mesh, phi, src_cols, det_cols, out_dtype=torch.float64, want_elem=F
     d_d = _i32(D, dev)
     pm = None if full else _i32(pmap, dev)
     phi = phi.contiguous()
+    qptr, Qc = patch_matrices(mesh)
     call("rb_jac_grid", mesh.nn, mesh.ne, ptr(mesh.el), ptr(mesh.evol), ptr(mesh.tinc_ptr), ptr(mesh.tinc),
          ptr(mesh.row_ptr), ptr(mesh.col_idx), mesh.maxnp, ptr(phi), phi.shape[1], int(is_complex), len(S), len(D),
-         ptr(s_d), ptr(d_d), ptr(pm), ptr(JT), JT.stride(0), code, ptr(JE), _abi.stream_ptr())
+         ptr(s_d), ptr(d_d), ptr(pm), ptr(JT), JT.stride(0), code, ptr(JE), ptr(qptr), ptr(Qc), _abi.stream_ptr())
     for r, f in dups:
         JT[:, r] = JT[:, f]
         if JE is not None:

@@ -105,7 +105,8 @@ __global__ void __launch_bounds__(JB) k_jac_node(int64_t nn, const int32_t* __re
                                                  const int32_t* __restrict__ col_idx, const T* __restrict__ phi,
                                                  int64_t ldphi, int Ns, int Nd, const int32_t* __restrict__ scol,
                                                  const int32_t* __restrict__ dcol, const int32_t* __restrict__ pmap,
-                                                 TO* __restrict__ JT, int64_t ldj, int maxnp) {
+                                                 TO* __restrict__ JT, int64_t ldj, int maxnp,
+                                                 const int64_t* __restrict__ qptr, const double* __restrict__ Qc) {
   extern __shared__ __align__(16) unsigned char smem[];
   T* PS = (T*)smem;                        // np x JC
   T* PD = PS + maxnp * JC;                 // np x JC
@@ -121,8 +122,12 @@ __global__ void __launch_bounds__(JB) k_jac_node(int64_t nn, const int32_t* __re
     const int32_t r0 = row_ptr[n];
     const int np = row_ptr[n + 1] - r0;
     for (int k = t; k < np; k += JB) cols[k] = col_idx[r0 + k];
+    if (Qc) {  // the mesh's patch matrices, built once (k_patch_q)
+      const double* q = Qc + qptr[n];
+      for (int i = t; i < np * np; i += JB) Q[i] = q[i];
+    }
     __syncthreads();
-    build_patch_q(n, el, evol, inc_ptr, inc_list, cols, np, Q, tl, tv);
+    if (!Qc) build_patch_q(n, el, evol, inc_ptr, inc_list, cols, np, Q, tl, tv);
     for (int s0 = 0; s0 < Ns; s0 += JC) {
       const int ns = min(JC, Ns - s0);
       for (int i = t; i < np * JC; i += JB) {
@@ -169,6 +174,31 @@ __global__ void __launch_bounds__(JB) k_jac_node(int64_t nn, const int32_t* __re
   }
 }
 
+// every node's patch matrix Q_n into Qc[qptr[n] ...] (row-major np x np)
+__global__ void __launch_bounds__(JB) k_patch_q(int64_t nn, const int32_t* __restrict__ el,
+                                                const double* __restrict__ evol, const int32_t* __restrict__ inc_ptr,
+                                                const int32_t* __restrict__ inc_list,
+                                                const int32_t* __restrict__ row_ptr,
+                                                const int32_t* __restrict__ col_idx, int maxnp,
+                                                const int64_t* __restrict__ qptr, double* __restrict__ Qc) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* Q = (double*)smem;  // np x np
+  double* tv = Q + maxnp * maxnp;
+  int32_t* tl = (int32_t*)(tv + JW);
+  int32_t* cols = tl + 4 * JW;
+  const int t = threadIdx.x;
+  for (int64_t n = blockIdx.x; n < nn; n += gridDim.x) {
+    const int32_t r0 = row_ptr[n];
+    const int np = row_ptr[n + 1] - r0;
+    for (int k = t; k < np; k += JB) cols[k] = col_idx[r0 + k];
+    __syncthreads();
+    build_patch_q(n, el, evol, inc_ptr, inc_list, cols, np, Q, tl, tv);
+    double* q = Qc + qptr[n];
+    for (int i = t; i < np * np; i += JB) q[i] = Q[i];
+    __syncthreads();
+  }
+}
+
 // Jelem[e][p] = -0.05 V_e (sum_i ps_i pd_i + S_s S_d), element-major
 template <typename T, typename TO>
 __global__ void k_jac_elem(int64_t ne, const int32_t* __restrict__ el, const double* __restrict__ evol,
@@ -202,13 +232,33 @@ static size_t jac_lds(int maxnp, int is_complex) {
   return (size_t)3 * maxnp * JC * sv + (size_t)maxnp * maxnp * 8 + (size_t)JW * (8 + 16) + (size_t)maxnp * 4 + 16;
 }
 
+extern "C" int rb_jac_patch_q(int64_t nn, const int32_t* el, const double* evol, const int32_t* inc_ptr,
+                              const int32_t* inc_list, const int32_t* row_ptr, const int32_t* col_idx, int maxnp,
+                              const int64_t* qptr, double* Qc, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (nn <= 0 || maxnp <= 0 || !qptr || !Qc) {
+    rb_set_error("rb_jac_patch_q: bad sizes");
+    return RB_ERR_ARG;
+  }
+  size_t lds = (size_t)maxnp * maxnp * 8 + (size_t)JW * (8 + 16) + (size_t)maxnp * 4 + 16;
+  if (lds > 64 * 1024) {
+    rb_set_error("rb_jac_patch_q: node patch of %d nodes needs %zu B of LDS (> 64 KiB)", maxnp, lds);
+    return RB_ERR_SIZE;
+  }
+  unsigned g = (unsigned)(nn < 262144 ? nn : 262144);
+  hipLaunchKernelGGL(k_patch_q, dim3(g), dim3(JB), lds, s, nn, el, evol, inc_ptr, inc_list, row_ptr, col_idx, maxnp,
+                     qptr, Qc);
+  RB_CHECK_LAUNCH();
+  return RB_OK;
+}
+
 extern "C" int rb_jac_grid(int64_t nn, int64_t ne, const int32_t* el, const double* evol, const int32_t* inc_ptr,
                            const int32_t* inc_list, const int32_t* row_ptr, const int32_t* col_idx, int maxnp,
                            const void* phi, int64_t ldphi, int is_complex, int Ns, int Nd, const int32_t* scol,
                            const int32_t* dcol, const int32_t* pmap, void* JT, int64_t ldj, int out_dtype,
-                           void* Jelem, void* stream) {
+                           void* Jelem, const int64_t* qptr, const double* Qc, void* stream) {
   hipStream_t s = (hipStream_t)stream;
-  if (nn <= 0 || Ns <= 0 || Nd <= 0 || maxnp <= 0) {
+  if (nn <= 0 || Ns <= 0 || Nd <= 0 || maxnp <= 0 || (!qptr) != (!Qc)) {
     rb_set_error("rb_jac_grid: bad sizes");
     return RB_ERR_ARG;
   }
@@ -224,7 +274,7 @@ extern "C" int rb_jac_grid(int64_t nn, int64_t ne, const int32_t* el, const doub
       if (JT)
         hipLaunchKernelGGL((k_jac_node<double, double>), dim3(g), dim3(JB), lds, s, nn, el, evol, inc_ptr, inc_list,
                            row_ptr, col_idx, (const double*)phi, ldphi, Ns, Nd, scol, dcol, pmap, (double*)JT, ldj,
-                           maxnp);
+                           maxnp, qptr, Qc);
       if (Jelem && ne > 0)
         hipLaunchKernelGGL((k_jac_elem<double, double>), dim3(ge), dim3(256), 0, s, ne, el, evol, (const double*)phi,
                            ldphi, Ns, Nd, scol, dcol, pmap, (double*)Jelem, ldj);
@@ -232,7 +282,7 @@ extern "C" int rb_jac_grid(int64_t nn, int64_t ne, const int32_t* el, const doub
       if (JT)
         hipLaunchKernelGGL((k_jac_node<double, float>), dim3(g), dim3(JB), lds, s, nn, el, evol, inc_ptr, inc_list,
                            row_ptr, col_idx, (const double*)phi, ldphi, Ns, Nd, scol, dcol, pmap, (float*)JT, ldj,
-                           maxnp);
+                           maxnp, qptr, Qc);
       if (Jelem && ne > 0)
         hipLaunchKernelGGL((k_jac_elem<double, float>), dim3(ge), dim3(256), 0, s, ne, el, evol, (const double*)phi,
                            ldphi, Ns, Nd, scol, dcol, pmap, (float*)Jelem, ldj);
@@ -245,7 +295,7 @@ extern "C" int rb_jac_grid(int64_t nn, int64_t ne, const int32_t* el, const doub
     if (out_dtype == RB_C128) {
       if (JT)
         hipLaunchKernelGGL((k_jac_node<C2, C2>), dim3(g), dim3(JB), lds, s, nn, el, evol, inc_ptr, inc_list, row_ptr,
-                           col_idx, (const C2*)phi, ldphi, Ns, Nd, scol, dcol, pmap, (C2*)JT, ldj, maxnp);
+                           col_idx, (const C2*)phi, ldphi, Ns, Nd, scol, dcol, pmap, (C2*)JT, ldj, maxnp, qptr, Qc);
       if (Jelem && ne > 0)
         hipLaunchKernelGGL((k_jac_elem<C2, C2>), dim3(ge), dim3(256), 0, s, ne, el, evol, (const C2*)phi, ldphi, Ns,
                            Nd, scol, dcol, pmap, (C2*)Jelem, ldj);
@@ -253,7 +303,7 @@ extern "C" int rb_jac_grid(int64_t nn, int64_t ne, const int32_t* el, const doub
       if (JT)
         hipLaunchKernelGGL((k_jac_node<C2, cplx<float>>), dim3(g), dim3(JB), lds, s, nn, el, evol, inc_ptr, inc_list,
                            row_ptr, col_idx, (const C2*)phi, ldphi, Ns, Nd, scol, dcol, pmap, (cplx<float>*)JT, ldj,
-                           maxnp);
+                           maxnp, qptr, Qc);
       if (Jelem && ne > 0)
         hipLaunchKernelGGL((k_jac_elem<C2, cplx<float>>), dim3(ge), dim3(256), 0, s, ne, el, evol, (const C2*)phi,
                            ldphi, Ns, Nd, scol, dcol, pmap, (cplx<float>*)Jelem, ldj);
