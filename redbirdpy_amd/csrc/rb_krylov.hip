@@ -27,6 +27,7 @@
 #include <mutex>
 #include <stdlib.h>
 #include <algorithm>
+#include <type_traits>
 #include <math.h>
 
 #define KB 256       // threads per block
@@ -602,6 +603,168 @@ __device__ __forceinline__ void sweep_body(const Csr& m, const SweepOps<T, TD>& 
   block_partials<A, 64, NCH>(st, pv, MODE == 0 ? 1 : 3, C);
 }
 
+// Real solves of 64-column systems (one system per chunk): one sweep covers
+// a pair of chunks, lane l gathering columns 2l and 2l+1 of the pair's 128
+// (8 bytes, so a 512-byte row per gather as for complex64); lanes 0-31
+// scale by the first system's matrix values and lanes 32-63 by the second's.
+// Schedule, LDS staging and row-end handling as in sweep_body (MODE 0).
+typedef __attribute__((ext_vector_type(2))) float fl2;
+
+template <int NCH>
+__device__ void block_partials_pair(const ColState<double>& st, double (&pv)[NCH / 2][2], int C) {
+  __shared__ double lds[KB / 64][128];
+  const int lane = threadIdx.x & 63, rg = threadIdx.x >> 6;
+  constexpr int RPB = KB / 64;
+#pragma unroll
+  for (int cp = 0; cp < NCH / 2; ++cp) {
+    lds[rg][2 * lane] = pv[cp][0];
+    lds[rg][2 * lane + 1] = pv[cp][1];
+    __syncthreads();
+    for (int i = threadIdx.x; i < 128; i += KB) {
+      const int c = cp * 128 + i;
+      if (c < C) {
+        double acc = lds[0][i];
+        for (int r = 1; r < RPB; ++r) acc = acc + lds[r][i];
+        st.part[(int64_t)c * st.nb + blockIdx.x] = acc;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int NCH, int FRG = 8>
+__device__ __forceinline__ void sweep_pair_body(const Csr& m, const float* __restrict__ vals,
+                                                const float* __restrict__ P, float* __restrict__ Q, int C,
+                                                const ColState<double>& st) {
+  __shared__ int32_t s_col[KB / 64][2][64];
+  __shared__ fl2 s_val[KB / 64][2][64];  // (first system, second system) values of an entry
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t seg = (m.n + NXCD - 1) / NXCD;
+  const int64_t s0 = (int64_t)(blockIdx.x % NXCD) * seg;
+  const int64_t s1 = min(m.n, s0 + seg);
+  const int64_t W = (int64_t)(gridDim.x / NXCD) * (KB / 64);
+  const int64_t wx = (int64_t)(blockIdx.x / NXCD) * (KB / 64) + wave;
+  const int64_t nq = (s1 > s0) ? (s1 - s0 + SRC - 1) / SRC : 0;
+  const int64_t nk = (nq > wx) ? (nq - wx + W - 1) / W : 0;
+  const uint32_t rowb = (uint32_t)C * (uint32_t)sizeof(float);
+  const __amdgpu_buffer_rsrc_t rsrc =
+      __builtin_amdgcn_make_buffer_rsrc((void*)P, (short)0, (int)(m.n * rowb), BUF_WORD3);
+  const bool hi = lane >= 32;
+  double pv[NCH / 2][2];
+#pragma unroll
+  for (int cp = 0; cp < NCH / 2; ++cp) pv[cp][0] = pv[cp][1] = 0.0;
+#pragma unroll
+  for (int cp = 0; cp < NCH / 2; ++cp) {
+    const int c = cp * 128 + 2 * lane;  // this lane's first column (C = NCH * 64)
+    if (!__any(st.active[c] || st.active[c + 1])) continue;  // both chunks converged
+    const uint32_t voff = (uint32_t)c * (uint32_t)sizeof(float);
+    const float* vA = vals + (int64_t)(2 * cp) * m.nnz;  // system = chunk
+    const float* vB = vA + m.nnz;
+    for (int64_t g0 = 0; g0 < nk; g0 += SGC) {
+      const int ng = (int)min((int64_t)SGC, nk - g0);
+      int32_t rp = 0;
+      {
+        const int kq = lane / (SRC + 1), o = lane % (SRC + 1);
+        if (kq < ng) rp = m.row_ptr[min(s0 + (wx + (g0 + kq) * W) * SRC + o, s1)];
+      }
+      int kc = 0;
+      int64_t cr0 = 0;
+      int cnr = 0, row = 0;
+      int32_t ce0 = 0, ce1 = 0, rend = 0;
+      fl2 acc = {0.f, 0.f}, pown = {0.f, 0.f};
+      auto chunk_at = [&](int kq, int64_t& r0, int& nr, int32_t& e0, int32_t& e1) {
+        r0 = s0 + (wx + (g0 + kq) * W) * SRC;
+        nr = (int)min((int64_t)SRC, s1 - r0);
+        e0 = __builtin_amdgcn_readlane(rp, kq * (SRC + 1));
+        e1 = __builtin_amdgcn_readlane(rp, kq * (SRC + 1) + nr);
+      };
+      auto drain = [&](int32_t e) {
+        while (row < cnr && rend == e) {
+          const int64_t o = (cr0 + row) * C + c;
+          *reinterpret_cast<fl2*>(Q + o) = acc;
+          pv[cp][0] += (double)pown.x * (double)acc.x;
+          pv[cp][1] += (double)pown.y * (double)acc.y;
+          acc = fl2{0.f, 0.f};
+          pown = fl2{0.f, 0.f};
+          ++row;
+          rend = (row < cnr) ? __builtin_amdgcn_readlane(rp, kc * (SRC + 1) + row + 1) : -1;
+        }
+      };
+      auto enter_chunk = [&](int kq) {
+        kc = kq;
+        chunk_at(kq, cr0, cnr, ce0, ce1);
+        row = 0;
+        rend = __builtin_amdgcn_readlane(rp, kq * (SRC + 1) + 1);
+        drain(ce0);
+      };
+      auto gather = [&](fl2(&x)[FRG], int sl, int jb) {
+#pragma unroll
+        for (int u = 0; u < FRG; ++u) {
+          const uint32_t cu = __builtin_amdgcn_readfirstlane(s_col[wave][sl][jb * FRG + u]);
+          x[u] = buf_ld<fl2>(rsrc, voff, cu * rowb);
+        }
+      };
+      auto consume = [&](const fl2(&x)[FRG], int sl, int jb, int32_t pb) {
+#pragma unroll
+        for (int u = 0; u < FRG; ++u) {
+          const fl2 sv = s_val[wave][sl][jb * FRG + u];
+          const float v = hi ? sv.y : sv.x;
+          acc = acc + v * x[u];
+          const int32_t cu = __builtin_amdgcn_readfirstlane(s_col[wave][sl][jb * FRG + u]);
+          if (cu == (int32_t)(cr0 + row)) pown = x[u];
+          drain(pb + jb * FRG + u + 1);
+        }
+      };
+      enter_chunk(0);
+      int32_t pb = ce0;
+      int sl = 0;
+      {
+        const int32_t e = pb + lane;
+        s_col[wave][0][lane] = (e < ce1) ? m.col_idx[e] : (int32_t)cr0;
+        s_val[wave][0][lane] = (e < ce1) ? fl2{vA[e], vB[e]} : fl2{0.f, 0.f};
+      }
+      fl2 xa[FRG], xb[FRG];
+      gather(xa, 0, 0);
+      for (;;) {
+        int kn = kc;
+        int64_t nr0 = cr0;
+        int nnr = cnr;
+        int32_t npb = pb + 64, ne0 = ce0, ne1 = ce1;
+        const bool more = npb < ce1;
+        const bool has_next = more || kc + 1 < ng;
+        if (!more && has_next) {
+          kn = kc + 1;
+          chunk_at(kn, nr0, nnr, ne0, ne1);
+          npb = ne0;
+        }
+        const int32_t en = min(npb + lane, max(ne1 - 1, 0));
+        const bool vn = has_next && npb + lane < ne1;
+        const int32_t tcol = m.col_idx[en];
+        const fl2 tval = fl2{vA[en], vB[en]};
+#pragma unroll
+        for (int pr = 0; pr < 64 / (2 * FRG); ++pr) {
+          gather(xb, sl, 2 * pr + 1);
+          consume(xa, sl, 2 * pr, pb);
+          if (pr + 1 < 64 / (2 * FRG)) {
+            gather(xa, sl, 2 * pr + 2);
+          } else {
+            s_col[wave][sl ^ 1][lane] = vn ? tcol : (int32_t)nr0;
+            s_val[wave][sl ^ 1][lane] = vn ? tval : fl2{0.f, 0.f};
+            gather(xa, sl ^ 1, 0);
+          }
+          consume(xb, sl, 2 * pr + 1, pb);
+        }
+        if (!has_next) break;
+        sl ^= 1;
+        pb = npb;
+        if (kn != kc) enter_chunk(kn);
+      }
+    }
+  }
+  block_partials_pair<NCH>(st, pv, C);
+}
+
 template <typename T, int NCH>
 __global__ void __launch_bounds__(KB) k_cg_spmm_sweep(Csr m, const T* __restrict__ vals, const T* __restrict__ P,
                                                       T* __restrict__ Q, int C, int k,
@@ -617,7 +780,7 @@ __global__ void __launch_bounds__(KB) k_cg_spmm_sweep(Csr m, const T* __restrict
 // residual r32 = b - A64 (x64 + d32) (partials |r|^2, |b|^2, r.z in so) in
 // the same launch; the residual pass runs smaller batches so the launch
 // keeps the SpMM's occupancy.  Clears this iteration's request flags.
-template <typename T64, typename T32, int NCH>
+template <typename T64, typename T32, int NCH, bool PAIR = false>
 __global__ void __launch_bounds__(KB) __attribute__((amdgpu_waves_per_eu(5))) k_cg_spmm_sweep_ru(Csr m, const T32* __restrict__ vals32,
                                                          const T32* __restrict__ P, T32* __restrict__ Q,
                                                          const T64* __restrict__ vals64, const T64* __restrict__ B,
@@ -627,8 +790,13 @@ __global__ void __launch_bounds__(KB) __attribute__((amdgpu_waves_per_eu(5))) k_
                                                          ColState<typename sc<T64>::acc> so) {
   if (blockIdx.x == 0 && threadIdx.x < MAXCHUNK) ru_cur(st)[threadIdx.x] = 0;
   if (!block_any_active(st.active, C)) return;
-  SweepOps<T32, T32> op{vals32, P, nullptr, Q, nullptr, nullptr, nullptr, nullptr};
-  sweep_body<T32, T32, NCH, 0>(m, op, C, k, st);
+  if constexpr (PAIR) {
+    static_assert(std::is_same<T32, float>::value && NCH % 2 == 0, "pair sweeps are for real 64-column systems");
+    sweep_pair_body<NCH>(m, vals32, P, Q, C, st);
+  } else {
+    SweepOps<T32, T32> op{vals32, P, nullptr, Q, nullptr, nullptr, nullptr, nullptr};
+    sweep_body<T32, T32, NCH, 0>(m, op, C, k, st);
+  }
   const int* gate = ru_prev(st);
   int any = 0;
   for (int ch = 0; ch < NCH; ++ch) any |= gate[ch];
@@ -1337,6 +1505,33 @@ static inline void plaunch(F kernel, dim3 grid, dim3 block, uint32_t shm, hipStr
   if (a) g_prof_pending[g_prof_open].push_back({a, b});
 }
 
+// the fused SpMM + residual launch, in its pair form for real 64-column systems
+template <typename T64, typename T32, int N, typename... Args>
+static inline void launch_spmm_ru(bool pair, dim3 grid, hipStream_t s, Args... args) {
+  if constexpr (std::is_same<T32, float>::value && N % 2 == 0) {
+    if (pair) {
+      plaunch((k_cg_spmm_sweep_ru<T64, T32, N, true>), grid, dim3(KB), 0, s, args...);
+      return;
+    }
+  }
+  plaunch((k_cg_spmm_sweep_ru<T64, T32, N>), grid, dim3(KB), 0, s, args...);
+}
+
+template <typename T64, typename T32>
+static const void* spmm_ru_fn(int nch, bool pair) {
+  if constexpr (std::is_same<T32, float>::value) {
+    if (pair) {
+      return nch == 2 ? (const void*)k_cg_spmm_sweep_ru<T64, T32, 2, true>
+                      : nch == 4 ? (const void*)k_cg_spmm_sweep_ru<T64, T32, 4, true>
+                                 : (const void*)k_cg_spmm_sweep_ru<T64, T32, 8, true>;
+    }
+  }
+  return nch == 1   ? (const void*)k_cg_spmm_sweep_ru<T64, T32, 1>
+         : nch == 2 ? (const void*)k_cg_spmm_sweep_ru<T64, T32, 2>
+         : nch == 4 ? (const void*)k_cg_spmm_sweep_ru<T64, T32, 4>
+                    : (const void*)k_cg_spmm_sweep_ru<T64, T32, 8>;
+}
+
 extern "C" int rb_prof_enable(int on) {
   g_prof = on ? 1 : 0;
   return RB_OK;
@@ -1596,22 +1791,20 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
   bool flat;
   int64_t nbs;
   spmm_plan<T32>(m, nsys, C, k, nb, &flat, &nbs);
+  // real systems of 64 columns each, whole chunk pairs: one sweep per pair of systems
+  const bool pair = std::is_same<T32, float>::value && k == 64 && C == nsys * 64 && C % 128 == 0 &&
+                    pick_G(C) == 64 && pick_NCH(C) == C / 64;
   if (flat) {  // the fused SpMM + residual launch has its own occupancy
-    const int nch = pick_NCH(C);
-    const void* fn = nch == 1   ? (const void*)k_cg_spmm_sweep_ru<T64, T32, 1>
-                     : nch == 2 ? (const void*)k_cg_spmm_sweep_ru<T64, T32, 2>
-                     : nch == 4 ? (const void*)k_cg_spmm_sweep_ru<T64, T32, 4>
-                                : (const void*)k_cg_spmm_sweep_ru<T64, T32, 8>;
-    nbs = std::min(nbs, resident_grid(fn, nb));
+    nbs = std::min(nbs, resident_grid(spmm_ru_fn<T64, T32>(pick_NCH(C), pair), nb));
   }
   ColState<A> ss = si;  // the SpMM's partials come from its own grid
   ss.nb = (int)nbs;
   ColState<A> sol = so;  // the in-loop residual's partials: the SpMM grid, or the regular one
   sol.nb = (int)(flat ? nbs : nbo);
   typedef T64 TW;
-#define L_SPMMRU(G_, N_, ...)                                                                                 \
-  do {                                                                                                      \
-    if (G_ == 64) plaunch((k_cg_spmm_sweep_ru<T64, T32, N_>), dim3(nbs), dim3(KB), 0, s, __VA_ARGS__);      \
+#define L_SPMMRU(G_, N_, ...)                                                    \
+  do {                                                                         \
+    if (G_ == 64) launch_spmm_ru<T64, T32, N_>(pair, dim3(nbs), s, __VA_ARGS__); \
   } while (0)
   DISPATCH_GN(C, L_PUP, m.n, w.dinv, D, w.R, w.P, C, k, X, si);  // p = z (P is zero, alpha 0, no flags)
   RB_CHECK_LAUNCH();

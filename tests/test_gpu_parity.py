@@ -243,3 +243,32 @@ def test_setmesh_recomputes_mesh_fields():
     assert c2["evol"].sum() == pytest.approx(16 * 12 * 8, rel=1e-12)
     assert np.array_equal(c2["seg"], np.ones(len(elem), dtype=int))
     assert cfg["face"].shape[0] != c2["face"].shape[0]  # the original is left alone
+
+
+@pytest.mark.parametrize("nw", [2, 4, 8])
+@pytest.mark.parametrize("cap", [None, "8"])
+def test_runforward_multiwavelength_64col_mixed(nw, cap, monkeypatch):
+    """Real multi-wavelength solves with 64 columns each (32 sources + 32 detectors): the mixed
+    path sweeps the wavelengths' systems in pairs (two columns per lane, 8-byte gathers, each half
+    of the wavefront scaled by its own system's values).  Detector values and fields against the
+    oracle; the capped grid gives each wavefront many row-chunk groups."""
+    import copy
+    from oracle import rb_oracle as orc
+    import redbirdpy_amd as rb
+
+    xs, ys = np.meshgrid(np.linspace(3, 17, 8), np.linspace(3, 17, 4))
+    src = np.column_stack([xs.ravel(), ys.ravel(), np.zeros(32)])
+    cfg = box_cfg((20, 20, 10), 2.0, src=src, det=src + [1.0, 1.0, 0.0])
+    cfg["prop"] = {str(650 + 25 * i): np.array([[0, 0, 1, 1], [0.004 + 0.003 * i, 1.0 - 0.05 * i, 0, 1.37]])
+                   for i in range(nw)}
+    if cap:
+        monkeypatch.setenv("RB_SWEEP_MAX_GRID", cap)
+    c1, _ = rb.meshprep(copy.deepcopy(cfg))
+    c2, _ = orc.meshprep(copy.deepcopy(cfg))
+    d1, f1 = rb.runforward(c1, precision="mixed", tol=1e-12)
+    d2, f2 = orc.runforward(c2)
+    assert set(d1) == set(d2) and len(d2) == nw
+    for wv in d2:
+        assert d1[wv].shape == (32, 32)
+        assert _relmax(d1[wv], d2[wv]) < 1e-8
+        assert np.linalg.norm(f1[wv] - f2[wv]) / np.linalg.norm(f2[wv]) < 1e-9
