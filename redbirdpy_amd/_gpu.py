@@ -302,15 +302,15 @@ def assemble(mesh, cfg, deldotdel, systems, want32=False):
             cols["musp"].append(rows[:, 1])
             cols["nref"].append(tab[:1, 2])  # n of the first element's label, as femlhs uses
         else:
-            mua, musp, nref = arr[1], arr[2], arr[3]
-            if k == _abi.RB_PROP_NODE:  # node properties follow the device numbering
-                mua, musp = mesh.mesh_nodes(mua), mesh.mesh_nodes(musp)
-                if len(nref) == mesh.nn:
-                    nref = mesh.mesh_nodes(nref)
-            cols["mua"].append(_f64(mua, dev))
-            cols["musp"].append(_f64(musp, dev))
-            cols["nref"].append(_f64(nref, dev))
-            nlen = max(nlen, len(nref))
+            mua, musp, nref = _f64(arr[1], dev), _f64(arr[2], dev), _f64(arr[3], dev)
+            if k == _abi.RB_PROP_NODE:  # node properties follow the device numbering (gathered on the device)
+                mua, musp = mesh.mesh_rows(mua), mesh.mesh_rows(musp)
+                if nref.numel() == mesh.nn:
+                    nref = mesh.mesh_rows(nref)
+            cols["mua"].append(mua)
+            cols["musp"].append(musp)
+            cols["nref"].append(nref)
+            nlen = max(nlen, nref.numel())
         oms.append(om)
         bcs.append((1 - reff) / (12.0 * (1 + reff)))
     nsys = len(systems)
