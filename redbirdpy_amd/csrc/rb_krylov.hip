@@ -609,6 +609,10 @@ __device__ __forceinline__ void sweep_body(const Csr& m, const SweepOps<T, TD>& 
 // scale by the first system's matrix values and lanes 32-63 by the second's.
 // Schedule, LDS staging and row-end handling as in sweep_body (MODE 0).
 typedef __attribute__((ext_vector_type(2))) float fl2;
+typedef __attribute__((ext_vector_type(2))) double dl2;
+template <typename TS> struct pair_of;
+template <> struct pair_of<float> { typedef fl2 type; };
+template <> struct pair_of<double> { typedef dl2 type; };
 
 template <int NCH>
 __device__ void block_partials_pair(const ColState<double>& st, double (&pv)[NCH / 2][2], int C) {
@@ -632,10 +636,11 @@ __device__ void block_partials_pair(const ColState<double>& st, double (&pv)[NCH
   }
 }
 
-template <int NCH, int FRG = 8>
-__device__ __forceinline__ void sweep_pair_body(const Csr& m, const float* __restrict__ vals,
-                                                const float* __restrict__ P, float* __restrict__ Q, int C,
+template <typename TS, int NCH, int FRG = 8>
+__device__ __forceinline__ void sweep_pair_body(const Csr& m, const TS* __restrict__ vals,
+                                                const TS* __restrict__ P, TS* __restrict__ Q, int C,
                                                 const ColState<double>& st) {
+  typedef typename pair_of<TS>::type fl2;
   __shared__ int32_t s_col[KB / 64][2][64];
   __shared__ fl2 s_val[KB / 64][2][64];  // (first system, second system) values of an entry
   const int lane = threadIdx.x & 63;
@@ -647,7 +652,7 @@ __device__ __forceinline__ void sweep_pair_body(const Csr& m, const float* __res
   const int64_t wx = (int64_t)(blockIdx.x / NXCD) * (KB / 64) + wave;
   const int64_t nq = (s1 > s0) ? (s1 - s0 + SRC - 1) / SRC : 0;
   const int64_t nk = (nq > wx) ? (nq - wx + W - 1) / W : 0;
-  const uint32_t rowb = (uint32_t)C * (uint32_t)sizeof(float);
+  const uint32_t rowb = (uint32_t)C * (uint32_t)sizeof(TS);
   const __amdgpu_buffer_rsrc_t rsrc =
       __builtin_amdgcn_make_buffer_rsrc((void*)P, (short)0, (int)(m.n * rowb), BUF_WORD3);
   const bool hi = lane >= 32;
@@ -658,9 +663,9 @@ __device__ __forceinline__ void sweep_pair_body(const Csr& m, const float* __res
   for (int cp = 0; cp < NCH / 2; ++cp) {
     const int c = cp * 128 + 2 * lane;  // this lane's first column (C = NCH * 64)
     if (!__any(st.active[c] || st.active[c + 1])) continue;  // both chunks converged
-    const uint32_t voff = (uint32_t)c * (uint32_t)sizeof(float);
-    const float* vA = vals + (int64_t)(2 * cp) * m.nnz;  // system = chunk
-    const float* vB = vA + m.nnz;
+    const uint32_t voff = (uint32_t)c * (uint32_t)sizeof(TS);
+    const TS* vA = vals + (int64_t)(2 * cp) * m.nnz;  // system = chunk
+    const TS* vB = vA + m.nnz;
     for (int64_t g0 = 0; g0 < nk; g0 += SGC) {
       const int ng = (int)min((int64_t)SGC, nk - g0);
       int32_t rp = 0;
@@ -672,7 +677,7 @@ __device__ __forceinline__ void sweep_pair_body(const Csr& m, const float* __res
       int64_t cr0 = 0;
       int cnr = 0, row = 0;
       int32_t ce0 = 0, ce1 = 0, rend = 0;
-      fl2 acc = {0.f, 0.f}, pown = {0.f, 0.f};
+      fl2 acc = {0, 0}, pown = {0, 0};
       auto chunk_at = [&](int kq, int64_t& r0, int& nr, int32_t& e0, int32_t& e1) {
         r0 = s0 + (wx + (g0 + kq) * W) * SRC;
         nr = (int)min((int64_t)SRC, s1 - r0);
@@ -685,8 +690,8 @@ __device__ __forceinline__ void sweep_pair_body(const Csr& m, const float* __res
           *reinterpret_cast<fl2*>(Q + o) = acc;
           pv[cp][0] += (double)pown.x * (double)acc.x;
           pv[cp][1] += (double)pown.y * (double)acc.y;
-          acc = fl2{0.f, 0.f};
-          pown = fl2{0.f, 0.f};
+          acc = fl2{0, 0};
+          pown = fl2{0, 0};
           ++row;
           rend = (row < cnr) ? __builtin_amdgcn_readlane(rp, kc * (SRC + 1) + row + 1) : -1;
         }
@@ -709,7 +714,7 @@ __device__ __forceinline__ void sweep_pair_body(const Csr& m, const float* __res
 #pragma unroll
         for (int u = 0; u < FRG; ++u) {
           const fl2 sv = s_val[wave][sl][jb * FRG + u];
-          const float v = hi ? sv.y : sv.x;
+          const TS v = hi ? sv.y : sv.x;
           acc = acc + v * x[u];
           const int32_t cu = __builtin_amdgcn_readfirstlane(s_col[wave][sl][jb * FRG + u]);
           if (cu == (int32_t)(cr0 + row)) pown = x[u];
@@ -722,7 +727,7 @@ __device__ __forceinline__ void sweep_pair_body(const Csr& m, const float* __res
       {
         const int32_t e = pb + lane;
         s_col[wave][0][lane] = (e < ce1) ? m.col_idx[e] : (int32_t)cr0;
-        s_val[wave][0][lane] = (e < ce1) ? fl2{vA[e], vB[e]} : fl2{0.f, 0.f};
+        s_val[wave][0][lane] = (e < ce1) ? fl2{vA[e], vB[e]} : fl2{0, 0};
       }
       fl2 xa[FRG], xb[FRG];
       gather(xa, 0, 0);
@@ -750,7 +755,7 @@ __device__ __forceinline__ void sweep_pair_body(const Csr& m, const float* __res
             gather(xa, sl, 2 * pr + 2);
           } else {
             s_col[wave][sl ^ 1][lane] = vn ? tcol : (int32_t)nr0;
-            s_val[wave][sl ^ 1][lane] = vn ? tval : fl2{0.f, 0.f};
+            s_val[wave][sl ^ 1][lane] = vn ? tval : fl2{0, 0};
             gather(xa, sl ^ 1, 0);
           }
           consume(xb, sl, 2 * pr + 1, pb);
@@ -765,14 +770,20 @@ __device__ __forceinline__ void sweep_pair_body(const Csr& m, const float* __res
   block_partials_pair<NCH>(st, pv, C);
 }
 
-template <typename T, int NCH>
+template <typename T, int NCH, bool PAIR = false>
 __global__ void __launch_bounds__(KB) k_cg_spmm_sweep(Csr m, const T* __restrict__ vals, const T* __restrict__ P,
                                                       T* __restrict__ Q, int C, int k,
                                                       ColState<typename sc<T>::acc> st) {
   if (st.ru_req && blockIdx.x == 0 && threadIdx.x < MAXCHUNK) ru_cur(st)[threadIdx.x] = 0;
   if (!block_any_active(st.active, C)) return;
-  SweepOps<T, T> op{vals, P, nullptr, Q, nullptr, nullptr, nullptr, nullptr};
-  sweep_body<T, T, NCH, 0>(m, op, C, k, st);
+  if constexpr (PAIR) {
+    static_assert((std::is_same<T, float>::value || std::is_same<T, double>::value) && NCH % 2 == 0,
+                  "pair sweeps are for real 64-column systems");
+    sweep_pair_body<T, NCH>(m, vals, P, Q, C, st);
+  } else {
+    SweepOps<T, T> op{vals, P, nullptr, Q, nullptr, nullptr, nullptr, nullptr};
+    sweep_body<T, T, NCH, 0>(m, op, C, k, st);
+  }
 }
 
 // mixed solves: q = A32 p (partials in st) and, for the 64-column chunks
@@ -792,7 +803,7 @@ __global__ void __launch_bounds__(KB) __attribute__((amdgpu_waves_per_eu(5))) k_
   if (!block_any_active(st.active, C)) return;
   if constexpr (PAIR) {
     static_assert(std::is_same<T32, float>::value && NCH % 2 == 0, "pair sweeps are for real 64-column systems");
-    sweep_pair_body<NCH>(m, vals32, P, Q, C, st);
+    sweep_pair_body<float, NCH>(m, vals32, P, Q, C, st);
   } else {
     SweepOps<T32, T32> op{vals32, P, nullptr, Q, nullptr, nullptr, nullptr, nullptr};
     sweep_body<T32, T32, NCH, 0>(m, op, C, k, st);
@@ -1517,6 +1528,25 @@ static inline void launch_spmm_ru(bool pair, dim3 grid, hipStream_t s, Args... a
   plaunch((k_cg_spmm_sweep_ru<T64, T32, N>), grid, dim3(KB), 0, s, args...);
 }
 
+// the plain sweep SpMM, in its pair form for real 64-column systems
+template <typename T, int N, typename... Args>
+static inline void launch_spmm_sweep(bool pair, dim3 grid, hipStream_t s, Args... args) {
+  if constexpr ((std::is_same<T, float>::value || std::is_same<T, double>::value) && N % 2 == 0) {
+    if (pair) {
+      plaunch((k_cg_spmm_sweep<T, N, true>), grid, dim3(KB), 0, s, args...);
+      return;
+    }
+  }
+  plaunch((k_cg_spmm_sweep<T, N>), grid, dim3(KB), 0, s, args...);
+}
+
+// real systems of 64 columns each in whole chunk pairs: one sweep per pair of systems
+template <typename T>
+static bool pair_sweep(int nsys, int C, int k) {
+  return (std::is_same<T, float>::value || std::is_same<T, double>::value) && k == 64 && C == nsys * 64 &&
+         (C == 128 || C == 256 || C == 512);  // C == 64 * NCH, NCH even
+}
+
 template <typename T64, typename T32>
 static const void* spmm_ru_fn(int nch, bool pair) {
   if constexpr (std::is_same<T32, float>::value) {
@@ -1576,7 +1606,7 @@ extern "C" int rb_prof_read(int id, double* total_ms, int64_t* count) {
 #define L_SPMM(G_, N_, ...)                                                                                 \
   do {                                                                                                    \
     if (G_ == 64 && flat)                                                                                 \
-      plaunch((k_cg_spmm_sweep<T, N_>), dim3(nbs), dim3(KB), 0, s, __VA_ARGS__);                            \
+      launch_spmm_sweep<T, N_>(pair_sweep<T>(nsys, C, k), dim3(nbs), s, __VA_ARGS__);                       \
     else if (nsys == 1)                                                                                   \
       plaunch((k_cg_spmm<T, G_, N_, false>), dim3(nbs), dim3(KB), 0, s, __VA_ARGS__);                      \
     else                                                                                                  \
@@ -1792,8 +1822,7 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
   int64_t nbs;
   spmm_plan<T32>(m, nsys, C, k, nb, &flat, &nbs);
   // real systems of 64 columns each, whole chunk pairs: one sweep per pair of systems
-  const bool pair = std::is_same<T32, float>::value && k == 64 && C == nsys * 64 && C % 128 == 0 &&
-                    pick_G(C) == 64 && pick_NCH(C) == C / 64;
+  const bool pair = std::is_same<T32, float>::value && pair_sweep<T32>(nsys, C, k);
   if (flat) {  // the fused SpMM + residual launch has its own occupancy
     nbs = std::min(nbs, resident_grid(spmm_ru_fn<T64, T32>(pick_NCH(C), pair), nb));
   }

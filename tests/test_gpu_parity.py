@@ -247,11 +247,13 @@ def test_setmesh_recomputes_mesh_fields():
 
 @pytest.mark.parametrize("nw", [2, 4, 8])
 @pytest.mark.parametrize("cap", [None, "8"])
-def test_runforward_multiwavelength_64col_mixed(nw, cap, monkeypatch):
-    """Real multi-wavelength solves with 64 columns each (32 sources + 32 detectors): the mixed
-    path sweeps the wavelengths' systems in pairs (two columns per lane, 8-byte gathers, each half
-    of the wavefront scaled by its own system's values).  Detector values and fields against the
-    oracle; the capped grid gives each wavefront many row-chunk groups."""
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_runforward_multiwavelength_64col(nw, cap, precision, monkeypatch):
+    """Real multi-wavelength solves with 64 columns each (32 sources + 32 detectors): the SpMM
+    sweeps the wavelengths' systems in pairs (two columns per lane: 8-byte gathers in float32,
+    16-byte in float64; each half of the wavefront scaled by its own system's values).  Detector
+    values and fields against the oracle; the capped grid gives each wavefront many row-chunk
+    groups."""
     import copy
     from oracle import rb_oracle as orc
     import redbirdpy_amd as rb
@@ -265,7 +267,7 @@ def test_runforward_multiwavelength_64col_mixed(nw, cap, monkeypatch):
         monkeypatch.setenv("RB_SWEEP_MAX_GRID", cap)
     c1, _ = rb.meshprep(copy.deepcopy(cfg))
     c2, _ = orc.meshprep(copy.deepcopy(cfg))
-    d1, f1 = rb.runforward(c1, precision="mixed", tol=1e-12)
+    d1, f1 = rb.runforward(c1, precision=precision, tol=1e-12)
     d2, f2 = orc.runforward(c2)
     assert set(d1) == set(d2) and len(d2) == nw
     for wv in d2:
