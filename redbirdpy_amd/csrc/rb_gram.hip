@@ -160,12 +160,18 @@ __global__ void k_colsq_finish(int64_t m, int gk, const double* __restrict__ col
 #define WT 256   // output tile
 #define WKS 32   // K slab
 #define WARR (WT * WKS)  // bf16 per operand array per buffer
+#define GNT 512          // threads per workgroup: 8 waves of 128x64 (256: 4 waves of 128x128, measured slower)
 
 __device__ __forceinline__ int wswz(int row, int c) { return row * WKS + ((c ^ ((row >> 2) & 3)) << 3); }
 
-template <int NTERM>
-__global__ void __launch_bounds__(512) k_gram_bf16w(int64_t m, int64_t Kp, int nsplit, const __bf16* __restrict__ Hi,
-                                                    const __bf16* __restrict__ Lo, float* __restrict__ part) {
+template <int NTERM, int NT>
+__global__ void __launch_bounds__(NT) k_gram_bf16w(int64_t m, int64_t Kp, int nsplit, const __bf16* __restrict__ Hi,
+                                                   const __bf16* __restrict__ Lo, float* __restrict__ part) {
+  constexpr int WV = NT / 64;            // waves: 2 rows x WV/2 columns of wave tiles
+  constexpr int TC = 16 / WV;            // 32-column MFMA tiles per wave (4 row tiles each)
+  static_assert(2 * 128 == WT && (WV / 2) * 32 * TC == WT, "the waves cover the WT x WT tile");
+  constexpr int HPT = WARR / 8 / NT;     // 16-byte chunks per operand array and thread
+  static_assert(HPT * NT * 8 == WARR && NT / 4 * HPT == WT, "one slab: WT rows x 4 chunks per array");
   __shared__ __align__(16) __bf16 lds[2 * 4 * WARR];
   const int64_t nt = (m + WT - 1) / WT;
   const int64_t ntri = nt * (nt + 1) / 2;
@@ -178,39 +184,24 @@ __global__ void __launch_bounds__(512) k_gram_bf16w(int64_t m, int64_t Kp, int n
   const int64_t kb = (int64_t)split * kchunk;
   const int64_t ke = min(Kp, kb + kchunk);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wr = (wave >> 2) * 128, wc = (wave & 3) * 64;
-  // operand arrays: 0 A hi, 1 A lo, 2 B hi, 3 B lo; thread loads chunks q = tid and tid + 512
-  constexpr int NA = (NTERM > 1) ? 4 : 2;
-  const __bf16* src[4];
-  bool ok[4][2];
-  int lrow[2], lc[2];
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int q = tid + 512 * h;
-    lrow[h] = q >> 2;
-    lc[h] = q & 3;
-  }
-#pragma unroll
-  for (int a = 0; a < 4; ++a) src[a] = ((a & 1) ? Lo : Hi);
-  int64_t grow[4][2];
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      grow[a][h] = ((a < 2) ? i0 : j0) + lrow[h];
-      ok[a][h] = grow[a][h] < m;
-    }
-  bf16x8 pre[4][2];
+  const int wr = (wave / (WV / 2)) * 128, wc = (wave % (WV / 2)) * (32 * TC);
+  // operand arrays: 0 A hi, 1 A lo, 2 B hi, 3 B lo; thread loads chunks q = tid + NT * h
+  // chunk q = tid + NT*h is row q >> 2 (tid >> 2 + NT/4 * h), 16-byte piece q & 3 (tid & 3)
+  const int lr0 = tid >> 2, lc = tid & 3;
+  bf16x8 pre[4][HPT];
   auto load = [&](int64_t k0) {
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
       if (NTERM == 1 && (a & 1)) continue;
+      const __bf16* base = ((a & 1) ? Lo : Hi) + k0 + lc * 8;
+      const int64_t r0 = ((a < 2) ? i0 : j0) + lr0;
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
+      for (int h = 0; h < HPT; ++h) {
+        const int64_t r = r0 + (NT / 4) * h;
         bf16x8 z;
 #pragma unroll
         for (int e = 0; e < 8; ++e) z[e] = (__bf16)0.f;
-        pre[a][h] = ok[a][h] ? *reinterpret_cast<const bf16x8*>(src[a] + grow[a][h] * Kp + k0 + lc[h] * 8) : z;
+        pre[a][h] = r < m ? *reinterpret_cast<const bf16x8*>(base + r * Kp) : z;
       }
     }
   };
@@ -219,15 +210,15 @@ __global__ void __launch_bounds__(512) k_gram_bf16w(int64_t m, int64_t Kp, int n
     for (int a = 0; a < 4; ++a) {
       if (NTERM == 1 && (a & 1)) continue;
 #pragma unroll
-      for (int h = 0; h < 2; ++h)
-        *reinterpret_cast<bf16x8*>(&lds[(buf * 4 + a) * WARR + wswz(lrow[h], lc[h])]) = pre[a][h];
+      for (int h = 0; h < HPT; ++h)
+        *reinterpret_cast<bf16x8*>(&lds[(buf * 4 + a) * WARR + wswz(lr0 + (NT / 4) * h, lc)]) = pre[a][h];
     }
   };
-  f32x16 acc[4][2];
+  f32x16 acc[4][TC];
 #pragma unroll
   for (int t = 0; t < 4; ++t)
 #pragma unroll
-    for (int u = 0; u < 2; ++u)
+    for (int u = 0; u < TC; ++u)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[t][u][r] = 0.f;
   if (kb < ke) {
@@ -243,7 +234,7 @@ __global__ void __launch_bounds__(512) k_gram_bf16w(int64_t m, int64_t Kp, int n
 #pragma unroll
     for (int ks = 0; ks < WKS; ks += 16) {
       const int c = (ks >> 3) + (lane >> 5);
-      bf16x8 ah[4], al[4], bh[2], bl[2];
+      bf16x8 ah[4], al[4];
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const int ra = wr + t * 32 + (lane & 31);
@@ -251,32 +242,30 @@ __global__ void __launch_bounds__(512) k_gram_bf16w(int64_t m, int64_t Kp, int n
         if (NTERM > 1) al[t] = *reinterpret_cast<const bf16x8*>(&L[1 * WARR + wswz(ra, c)]);
       }
 #pragma unroll
-      for (int u = 0; u < 2; ++u) {
+      for (int u = 0; u < TC; ++u) {  // B fragments one column tile at a time
         const int rb = wc + u * 32 + (lane & 31);
-        bh[u] = *reinterpret_cast<const bf16x8*>(&L[2 * WARR + wswz(rb, c)]);
-        if (NTERM > 1) bl[u] = *reinterpret_cast<const bf16x8*>(&L[3 * WARR + wswz(rb, c)]);
-      }
+        const bf16x8 bh = *reinterpret_cast<const bf16x8*>(&L[2 * WARR + wswz(rb, c)]);
+        bf16x8 bl;
+        if (NTERM > 1) bl = *reinterpret_cast<const bf16x8*>(&L[3 * WARR + wswz(rb, c)]);
 #pragma unroll
-      for (int t = 0; t < 4; ++t)
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-          acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[t], bh[u], acc[t][u], 0, 0, 0);
+        for (int t = 0; t < 4; ++t) {
+          acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[t], bh, acc[t][u], 0, 0, 0);
           if (NTERM > 1) {
-            acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[t], bl[u], acc[t][u], 0, 0, 0);
-            acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[t], bh[u], acc[t][u], 0, 0, 0);
+            acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[t], bl, acc[t][u], 0, 0, 0);
+            acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[t], bh, acc[t][u], 0, 0, 0);
           }
         }
+      }
     }
     if (more) store(cur ^ 1);
     __syncthreads();
     cur ^= 1;
   }
-  (void)NA;
   float* out = part + (int64_t)split * m * m;
 #pragma unroll
   for (int t = 0; t < 4; ++t)
 #pragma unroll
-    for (int u = 0; u < 2; ++u)
+    for (int u = 0; u < TC; ++u)
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int64_t i = i0 + wr + t * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
@@ -378,9 +367,9 @@ extern "C" int rb_gram_mfma(int64_t K, int64_t m, const void* JT, int64_t ldj, i
   const int64_t nt = (m + WT - 1) / WT;
   unsigned blocks = (unsigned)(nt * (nt + 1) / 2 * nsplit);
   if (mode == 1)
-    hipLaunchKernelGGL(k_gram_bf16w<1>, dim3(blocks), dim3(512), 0, s, m, Kp, nsplit, Hi, Lo, part);
+    hipLaunchKernelGGL((k_gram_bf16w<1, GNT>), dim3(blocks), dim3(GNT), 0, s, m, Kp, nsplit, Hi, Lo, part);
   else
-    hipLaunchKernelGGL(k_gram_bf16w<3>, dim3(blocks), dim3(512), 0, s, m, Kp, nsplit, Hi, Lo, part);
+    hipLaunchKernelGGL((k_gram_bf16w<3, GNT>), dim3(blocks), dim3(GNT), 0, s, m, Kp, nsplit, Hi, Lo, part);
   RB_CHECK_LAUNCH();
   const int64_t nt64 = (m + 63) / 64;
   hipLaunchKernelGGL(k_gram_reduce, dim3((unsigned)(nt64 * (nt64 + 1) / 2)), dim3(256), 0, s, m, nsplit, part, d,
