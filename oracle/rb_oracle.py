@@ -27,7 +27,19 @@ Pinning (what checks each function, see tests/test_oracle_*.py):
 * jac: central finite differences of femgetdet under the jac derivative
   model, and the mass-matrix identity.
 * getreff: independent adaptive quadrature of the Haskell integrals.
-* reginvunder/reginvover: numpy lstsq on the augmented Tikhonov system.
+* reginvunder/reginvover: numpy lstsq on the augmented Tikhonov system;
+  with priors, lstsq on [A; sqrt(lam) L] (also block-wise L^T L) and the
+  closed form P A^T (A P A^T + lam I)^-1 b, P = (L^T L)^-1 (also per block).
+* masksum: the adjoint of label expansion (masksum(J) v = J v[labels]).
+* matreform: 'real'/'logphase' rows by central differences of detval,
+  log|detval| and arg(detval); 'reim' rows give [Re z; Im z] of the complex
+  Tikhonov step z = J^H (J J^H + lam I)^-1 r.
+* meshinterp / remap_jacobian: linear fields reproduced exactly; the
+  adjoint identity J_rec v = J (P v); column sums preserved.
+* src2bc / qmeshcut / inpolygon: the reference's inpolygon known answers,
+  cut points on the plane and on their edges, cross-section areas of
+  horizontal and tilted planes, the depth moment of the P1 weights, the
+  Fourier closed form per face, L1 norms.
 * extinction: literal table entries and linear interpolation identities.
 """
 
