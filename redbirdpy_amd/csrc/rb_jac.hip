@@ -227,6 +227,23 @@ __global__ void k_jac_elem(int64_t ne, const int32_t* __restrict__ el, const dou
   }
 }
 
+// dynamic LDS beyond 64 KiB (a node patch of many neighbours) up to the
+// device's per-workgroup limit (160 KiB on gfx950), opted in per kernel
+static size_t lds_limit() {
+  int dev = 0, v = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+    v = 64 * 1024;
+  return (size_t)v;
+}
+
+template <typename K>
+static bool allow_lds(K kernel, size_t lds) {
+  if (lds <= 64 * 1024) return true;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)lds) == hipSuccess;
+}
+
 static size_t jac_lds(int maxnp, int is_complex) {
   size_t sv = is_complex ? 16 : 8;
   return (size_t)3 * maxnp * JC * sv + (size_t)maxnp * maxnp * 8 + (size_t)JW * (8 + 16) + (size_t)maxnp * 4 + 16;
@@ -241,8 +258,8 @@ extern "C" int rb_jac_patch_q(int64_t nn, const int32_t* el, const double* evol,
     return RB_ERR_ARG;
   }
   size_t lds = (size_t)maxnp * maxnp * 8 + (size_t)JW * (8 + 16) + (size_t)maxnp * 4 + 16;
-  if (lds > 64 * 1024) {
-    rb_set_error("rb_jac_patch_q: node patch of %d nodes needs %zu B of LDS (> 64 KiB)", maxnp, lds);
+  if (lds > lds_limit() || !allow_lds(k_patch_q, lds)) {
+    rb_set_error("rb_jac_patch_q: node patch of %d nodes needs %zu B of LDS (limit %zu)", maxnp, lds, lds_limit());
     return RB_ERR_SIZE;
   }
   unsigned g = (unsigned)(nn < 262144 ? nn : 262144);
@@ -263,8 +280,18 @@ extern "C" int rb_jac_grid(int64_t nn, int64_t ne, const int32_t* el, const doub
     return RB_ERR_ARG;
   }
   size_t lds = jac_lds(maxnp, is_complex);
-  if (lds > 64 * 1024) {
-    rb_set_error("rb_jac_grid: node patch of %d nodes needs %zu B of LDS (> 64 KiB)", maxnp, lds);
+  if (lds > lds_limit()) {
+    rb_set_error("rb_jac_grid: node patch of %d nodes needs %zu B of LDS (limit %zu)", maxnp, lds, lds_limit());
+    return RB_ERR_SIZE;
+  }
+  bool ok = true;
+  if (!is_complex)
+    ok = out_dtype == RB_F64 ? allow_lds(k_jac_node<double, double>, lds) : allow_lds(k_jac_node<double, float>, lds);
+  else
+    ok = out_dtype == RB_C128 ? allow_lds(k_jac_node<cplx<double>, cplx<double>>, lds)
+                              : allow_lds(k_jac_node<cplx<double>, cplx<float>>, lds);
+  if (!ok) {
+    rb_set_error("rb_jac_grid: %zu B of dynamic LDS refused", lds);
     return RB_ERR_SIZE;
   }
   unsigned g = (unsigned)(nn < 262144 ? nn : 262144);

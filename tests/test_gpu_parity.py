@@ -274,3 +274,45 @@ def test_runforward_multiwavelength_64col(nw, cap, precision, monkeypatch):
         assert d1[wv].shape == (32, 32)
         assert _relmax(d1[wv], d2[wv]) < 1e-8
         assert np.linalg.norm(f1[wv] - f2[wv]) / np.linalg.norm(f2[wv]) < 1e-9
+
+
+def _star_split_cfg(omega=0.0):
+    """A Kuhn box whose tets around the centre are split 1 -> 4 at their centroids:
+    irregular connectivity (rows from 5 to ~40 entries) for pattern, assembly, solve
+    and the node Jacobian's patch matrices."""
+    from redbirdpy_amd.mesh import kuhn_box
+
+    p1 = (16, 16, 8)
+    node, elem = kuhn_box([0, 0, 0], p1, 2.0)
+    cen = node[elem[:, :4] - 1].mean(axis=1)
+    split = np.flatnonzero(np.sum((cen - [8.0, 8.0, 4.0]) ** 2, axis=1) < 12.0)
+    keep = np.setdiff1d(np.arange(len(elem)), split)
+    new_nodes = cen[split]
+    ids = node.shape[0] + 1 + np.arange(len(split))
+    quads = []
+    for (a, b, c, d), x in zip(elem[split, :4], ids):
+        quads += [(a, b, c, x), (a, b, x, d), (a, x, c, d), (x, b, c, d)]
+    elem2 = np.vstack([elem[keep, :4], np.array(quads)]).astype(int)
+    cfg = box_cfg(p1, 2.0, src=grid_optodes(p1, 2), det=grid_optodes(p1, 3, margin=3.0), omega=omega)
+    cfg["node"] = np.vstack([node, new_nodes])
+    cfg["elem"] = elem2
+    cfg["seg"] = np.ones(len(elem2), dtype=int)
+    return cfg
+
+
+@pytest.mark.parametrize("omega", [0.0, 2 * np.pi * 1e8])
+def test_irregular_mesh_forward_and_jacobian(omega):
+    import copy
+
+    rb, orc, c1, sd1, c2, sd2 = _both(lambda: copy.deepcopy(_star_split_cfg(omega)))
+    assert np.array_equal(c1["face"], c2["face"]) and np.array_equal(c1["idxcount"], c2["idxcount"])
+    counts = c1["idxcount"]
+    assert counts.max() >= 3 * np.median(counts)  # the split nodes have long rows
+    d1, p1 = rb.runforward(c1)
+    d2, p2 = orc.runforward(c2)
+    assert _relmax(d1, d2) < 1e-9
+    assert np.linalg.norm(p1 - p2) / np.linalg.norm(p2) < 1e-10
+    Jn1, Je1 = rb.jac(sd2, p2, c1["deldotdel"], c1["elem"], c1["evol"])
+    Jn2, Je2 = orc.jac(sd2, p2, c2["deldotdel"], c2["elem"], c2["evol"])
+    assert np.linalg.norm(Jn1 - Jn2) / np.linalg.norm(Jn2) < 1e-12
+    assert np.linalg.norm(Je1 - Je2) / np.linalg.norm(Je2) < 1e-12
