@@ -160,6 +160,7 @@ __global__ void k_colsq_finish(int64_t m, int gk, const double* __restrict__ col
 #define WT 256   // output tile
 #define WKS 32   // K slab
 #define WARR (WT * WKS)  // bf16 per operand array per buffer
+#define GXR 32           // consecutive work items per XCD (its CUs' resident workgroups)
 #define GNT 512          // threads per workgroup: 8 waves of 128x64 (256: 4 waves of 128x128, measured slower)
 
 __device__ __forceinline__ int wswz(int row, int c) { return row * WKS + ((c ^ ((row >> 2) & 3)) << 3); }
@@ -175,8 +176,14 @@ __global__ void __launch_bounds__(NT) k_gram_bf16w(int64_t m, int64_t Kp, int ns
   __shared__ __align__(16) __bf16 lds[2 * 4 * WARR];
   const int64_t nt = (m + WT - 1) / WT;
   const int64_t ntri = nt * (nt + 1) / 2;
-  const int64_t tile = blockIdx.x % ntri;
-  const int split = (int)(blockIdx.x / ntri);
+  // workgroups go to the 8 XCDs round-robin: give each XCD runs of GXR
+  // consecutive (split, tile) items, so the tiles resident on one XCD at a
+  // time share operand rows in its L2 (the grid is padded to whole rounds)
+  const int64_t slot = blockIdx.x / 8;
+  const int64_t g = (slot / GXR) * (8 * GXR) + (int64_t)(blockIdx.x % 8) * GXR + slot % GXR;
+  if (g >= ntri * nsplit) return;
+  const int64_t tile = g % ntri;
+  const int split = (int)(g / ntri);
   int64_t ti, tj;
   tri_tile(tile, nt, ti, tj);
   const int64_t i0 = ti * WT, j0 = tj * WT;
@@ -365,7 +372,7 @@ extern "C" int rb_gram_mfma(int64_t K, int64_t m, const void* JT, int64_t ldj, i
     RB_CHECK_LAUNCH();
   }
   const int64_t nt = (m + WT - 1) / WT;
-  unsigned blocks = (unsigned)(nt * (nt + 1) / 2 * nsplit);
+  unsigned blocks = (unsigned)ceil_div(nt * (nt + 1) / 2 * nsplit, 8 * GXR) * (8 * GXR);
   if (mode == 1)
     hipLaunchKernelGGL((k_gram_bf16w<1, GNT>), dim3(blocks), dim3(GNT), 0, s, m, Kp, nsplit, Hi, Lo, part);
   else
