@@ -1006,6 +1006,137 @@ __global__ void __launch_bounds__(KB) k_cg_pupdate(int64_t n, const T* __restric
   }
 }
 
+// Float32 systems of 64 columns each in chunk pairs (the pair sweep's
+// layout): lane l handles columns 2l and 2l+1 of each 128-column pair with
+// 8-byte accesses; per column the same arithmetic as k_cg_update /
+// k_cg_pupdate (lanes 0-31 are the pair's first system, 32-63 its second).
+template <int NCH>
+__device__ void block_partials_pair2(const ColState<double>& st, double (&pv)[NCH / 2][2][2], int C) {
+  __shared__ double lds[KB / 64][2][128];
+  const int lane = threadIdx.x & 63, rg = threadIdx.x >> 6;
+  constexpr int RPB = KB / 64;
+#pragma unroll
+  for (int cp = 0; cp < NCH / 2; ++cp) {
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+      lds[rg][d][2 * lane] = pv[cp][0][d];
+      lds[rg][d][2 * lane + 1] = pv[cp][1][d];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 2 * 128; i += KB) {
+      const int d = i >> 7, j = i & 127;
+      const int c = cp * 128 + j;
+      double acc = lds[0][d][j];
+      for (int r = 1; r < RPB; ++r) acc = acc + lds[r][d][j];
+      st.part[(int64_t)(d * C + c) * st.nb + blockIdx.x] = acc;
+    }
+    __syncthreads();
+  }
+}
+
+template <int NCH>
+__global__ void __launch_bounds__(KB) k_cg_update_pair(int64_t n, const float* __restrict__ dinv,
+                                                       float* __restrict__ R, const float* __restrict__ Q, int C,
+                                                       ColState<double> st) {
+  constexpr int G = 64;
+  constexpr int NP = NCH / 2;
+  if (!block_any_active(st.active, C)) return;
+  double pv[NP][2][2], al[NP][2];
+  bool act[NP][2];
+#pragma unroll
+  for (int cp = 0; cp < NP; ++cp)
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int c = cp * 128 + 2 * (int)(threadIdx.x & 63) + e;
+      al[cp][e] = st.alpha[c];
+      act[cp][e] = st.active[c] || al[cp][e] != 0.0;
+      pv[cp][e][0] = pv[cp][e][1] = 0.0;
+    }
+  ROW_LOOP(n) {
+#pragma unroll
+    for (int cp = 0; cp < NP; ++cp) {
+      if (!(act[cp][0] || act[cp][1])) continue;
+      const int sys = 2 * cp + (lane >> 5);
+      const int64_t o = row * C + cp * 128 + 2 * lane;
+      fl2 r = *reinterpret_cast<const fl2*>(R + o);
+      if (al[cp][0] != 0.0 || al[cp][1] != 0.0) {
+        const fl2 q = *reinterpret_cast<const fl2*>(Q + o);
+        if (al[cp][0] != 0.0) r.x = r.x - scale_to(q.x, al[cp][0]);
+        if (al[cp][1] != 0.0) r.y = r.y - scale_to(q.y, al[cp][1]);
+        *reinterpret_cast<fl2*>(R + o) = r;
+      }
+      const float d = dinv[(int64_t)sys * n + row];
+      if (act[cp][0]) {
+        pv[cp][0][0] += amul(r.x, d * r.x);
+        pv[cp][0][1] += sc<float>::norm2(r.x);
+      }
+      if (act[cp][1]) {
+        pv[cp][1][0] += amul(r.y, d * r.y);
+        pv[cp][1][1] += sc<float>::norm2(r.y);
+      }
+    }
+  }
+  block_partials_pair2<NCH>(st, pv, C);
+}
+
+template <int NCH>
+__global__ void __launch_bounds__(KB) k_cg_pupdate_pair(int64_t n, const float* __restrict__ dinv,
+                                                        float* __restrict__ X, const float* __restrict__ R,
+                                                        float* __restrict__ P, int C, double* __restrict__ X64,
+                                                        ColState<double> st) {
+  constexpr int G = 64;
+  constexpr int NP = NCH / 2;
+  if (!block_any_live(st, C)) return;
+  double al[NP][2], be[NP][2];
+  bool act[NP][2], mv[NP][2], fold[NP];
+#pragma unroll
+  for (int cp = 0; cp < NP; ++cp) {
+    const int c0 = cp * 128 + 2 * (int)(threadIdx.x & 63);
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      al[cp][e] = st.alpha[c0 + e];
+      be[cp][e] = st.beta[c0 + e];
+      act[cp][e] = st.active[c0 + e];
+      mv[cp][e] = al[cp][e] != 0.0;
+    }
+    fold[cp] = X64 && st.ru_req && ru_prev(st)[c0 >> 6];  // both columns lie in one 64-column chunk
+  }
+  ROW_LOOP(n) {
+#pragma unroll
+    for (int cp = 0; cp < NP; ++cp) {
+      const bool use0 = mv[cp][0] || act[cp][0], use1 = mv[cp][1] || act[cp][1];
+      if (!(use0 || use1 || fold[cp])) continue;
+      const int sys = 2 * cp + (lane >> 5);
+      const int64_t o = row * C + cp * 128 + 2 * lane;
+      const fl2 p = (use0 || use1) ? *reinterpret_cast<const fl2*>(P + o) : fl2{0.f, 0.f};
+      if (fold[cp]) {
+        fl2 x = *reinterpret_cast<const fl2*>(X + o);
+        dl2 x64 = *reinterpret_cast<const dl2*>(X64 + o);
+        x64.x = x64.x + (double)x.x;
+        x64.y = x64.y + (double)x.y;
+        *reinterpret_cast<dl2*>(X64 + o) = x64;
+        x.x = mv[cp][0] ? scale_to(p.x, al[cp][0]) : 0.f;
+        x.y = mv[cp][1] ? scale_to(p.y, al[cp][1]) : 0.f;
+        *reinterpret_cast<fl2*>(X + o) = x;
+      } else if (mv[cp][0] || mv[cp][1]) {
+        fl2 x = *reinterpret_cast<const fl2*>(X + o);
+        if (mv[cp][0]) x.x = x.x + scale_to(p.x, al[cp][0]);
+        if (mv[cp][1]) x.y = x.y + scale_to(p.y, al[cp][1]);
+        *reinterpret_cast<fl2*>(X + o) = x;
+      }
+      if (act[cp][0] || act[cp][1]) {
+        const fl2 r = *reinterpret_cast<const fl2*>(R + o);
+        const float d = dinv[(int64_t)sys * n + row];
+        fl2 pn = p;
+        if (act[cp][0]) pn.x = d * r.x + scale_to(p.x, be[cp][0]);
+        if (act[cp][1]) pn.y = d * r.y + scale_to(p.y, be[cp][1]);
+        *reinterpret_cast<fl2*>(P + o) = pn;
+      }
+    }
+  }
+}
+
+
 // y = sum_j A_sj (x[col_j, c] + d[col_j, c]) for one row, one wavefront per
 // row (lane c of a 64-column chunk): the row's (col, val) entries come in
 // with one lane-parallel load and are broadcast by readlane, so RG gathers
@@ -1547,6 +1678,33 @@ static bool pair_sweep(int nsys, int C, int k) {
          (C == 128 || C == 256 || C == 512);  // C == 64 * NCH, NCH even
 }
 
+// the vector updates in their pair form (float32 only; false: not taken)
+template <typename T32, typename A>
+static bool launch_update_pair(int C, int64_t nb, hipStream_t s, int64_t n, const T32* dinv, T32* R, const T32* Q,
+                               ColState<A> st) {
+  if constexpr (std::is_same<T32, float>::value) {
+    const int nch = C / 64;
+    if (nch == 2) plaunch(k_cg_update_pair<2>, dim3(nb), dim3(KB), 0, s, n, dinv, R, Q, C, st);
+    else if (nch == 4) plaunch(k_cg_update_pair<4>, dim3(nb), dim3(KB), 0, s, n, dinv, R, Q, C, st);
+    else plaunch(k_cg_update_pair<8>, dim3(nb), dim3(KB), 0, s, n, dinv, R, Q, C, st);
+    return true;
+  }
+  return false;
+}
+
+template <typename T32, typename A, typename TW>
+static bool launch_pupdate_pair(int C, int64_t nb, hipStream_t s, int64_t n, const T32* dinv, T32* X, const T32* R,
+                                T32* P, TW* X64, ColState<A> st) {
+  if constexpr (std::is_same<T32, float>::value) {
+    const int nch = C / 64;
+    if (nch == 2) plaunch(k_cg_pupdate_pair<2>, dim3(nb), dim3(KB), 0, s, n, dinv, X, R, P, C, X64, st);
+    else if (nch == 4) plaunch(k_cg_pupdate_pair<4>, dim3(nb), dim3(KB), 0, s, n, dinv, X, R, P, C, X64, st);
+    else plaunch(k_cg_pupdate_pair<8>, dim3(nb), dim3(KB), 0, s, n, dinv, X, R, P, C, X64, st);
+    return true;
+  }
+  return false;
+}
+
 template <typename T64, typename T32>
 static const void* spmm_ru_fn(int nch, bool pair) {
   if constexpr (std::is_same<T32, float>::value) {
@@ -1859,12 +2017,14 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
       hipLaunchKernelGGL(k_fin_alpha_ru<A>, dim3(C), dim3(FB), 0, s, C, tol2, ss, sol);
       {
         ProfScope ps(RB_PROF_UPDATE, s);
-        DISPATCH_GN(C, L_UPD, m.n, w.dinv, w.R, w.Q, C, k, si);
+        if (!pair || !launch_update_pair<T32>(C, nb, s, m.n, w.dinv, w.R, w.Q, si))
+          DISPATCH_GN(C, L_UPD, m.n, w.dinv, w.R, w.Q, C, k, si);
       }
       hipLaunchKernelGGL(k_fin_beta<A>, dim3(C), dim3(FB), 0, s, C, si);
       {
         ProfScope ps(RB_PROF_PUPDATE, s);
-        DISPATCH_GN(C, L_PUP, m.n, w.dinv, D, w.R, w.P, C, k, X, si);
+        if (!pair || !launch_pupdate_pair<T32>(C, nb, s, m.n, w.dinv, D, w.R, w.P, X, si))
+          DISPATCH_GN(C, L_PUP, m.n, w.dinv, D, w.R, w.P, C, k, X, si);
       }
     }
     RB_CHECK_LAUNCH();
