@@ -1034,10 +1034,11 @@ __device__ void block_partials_pair2(const ColState<double>& st, double (&pv)[NC
   }
 }
 
-template <int NCH>
-__global__ void __launch_bounds__(KB) k_cg_update_pair(int64_t n, const float* __restrict__ dinv,
-                                                       float* __restrict__ R, const float* __restrict__ Q, int C,
+template <typename TS, int NCH>
+__global__ void __launch_bounds__(KB) k_cg_update_pair(int64_t n, const TS* __restrict__ dinv,
+                                                       TS* __restrict__ R, const TS* __restrict__ Q, int C,
                                                        ColState<double> st) {
+  typedef typename pair_of<TS>::type fl2;
   constexpr int G = 64;
   constexpr int NP = NCH / 2;
   if (!block_any_active(st.active, C)) return;
@@ -1065,25 +1066,26 @@ __global__ void __launch_bounds__(KB) k_cg_update_pair(int64_t n, const float* _
         if (al[cp][1] != 0.0) r.y = r.y - scale_to(q.y, al[cp][1]);
         *reinterpret_cast<fl2*>(R + o) = r;
       }
-      const float d = dinv[(int64_t)sys * n + row];
+      const TS d = dinv[(int64_t)sys * n + row];
       if (act[cp][0]) {
-        pv[cp][0][0] += amul(r.x, d * r.x);
-        pv[cp][0][1] += sc<float>::norm2(r.x);
+        pv[cp][0][0] += amul(r.x, (TS)(d * r.x));
+        pv[cp][0][1] += sc<TS>::norm2(r.x);
       }
       if (act[cp][1]) {
-        pv[cp][1][0] += amul(r.y, d * r.y);
-        pv[cp][1][1] += sc<float>::norm2(r.y);
+        pv[cp][1][0] += amul(r.y, (TS)(d * r.y));
+        pv[cp][1][1] += sc<TS>::norm2(r.y);
       }
     }
   }
   block_partials_pair2<NCH>(st, pv, C);
 }
 
-template <int NCH>
-__global__ void __launch_bounds__(KB) k_cg_pupdate_pair(int64_t n, const float* __restrict__ dinv,
-                                                        float* __restrict__ X, const float* __restrict__ R,
-                                                        float* __restrict__ P, int C, double* __restrict__ X64,
+template <typename TS, int NCH>
+__global__ void __launch_bounds__(KB) k_cg_pupdate_pair(int64_t n, const TS* __restrict__ dinv,
+                                                        TS* __restrict__ X, const TS* __restrict__ R,
+                                                        TS* __restrict__ P, int C, double* __restrict__ X64,
                                                         ColState<double> st) {
+  typedef typename pair_of<TS>::type fl2;
   constexpr int G = 64;
   constexpr int NP = NCH / 2;
   if (!block_any_live(st, C)) return;
@@ -1108,15 +1110,15 @@ __global__ void __launch_bounds__(KB) k_cg_pupdate_pair(int64_t n, const float* 
       if (!(use0 || use1 || fold[cp])) continue;
       const int sys = 2 * cp + (lane >> 5);
       const int64_t o = row * C + cp * 128 + 2 * lane;
-      const fl2 p = (use0 || use1) ? *reinterpret_cast<const fl2*>(P + o) : fl2{0.f, 0.f};
+      const fl2 p = (use0 || use1) ? *reinterpret_cast<const fl2*>(P + o) : fl2{0, 0};
       if (fold[cp]) {
         fl2 x = *reinterpret_cast<const fl2*>(X + o);
         dl2 x64 = *reinterpret_cast<const dl2*>(X64 + o);
         x64.x = x64.x + (double)x.x;
         x64.y = x64.y + (double)x.y;
         *reinterpret_cast<dl2*>(X64 + o) = x64;
-        x.x = mv[cp][0] ? scale_to(p.x, al[cp][0]) : 0.f;
-        x.y = mv[cp][1] ? scale_to(p.y, al[cp][1]) : 0.f;
+        x.x = mv[cp][0] ? scale_to(p.x, al[cp][0]) : (TS)0;
+        x.y = mv[cp][1] ? scale_to(p.y, al[cp][1]) : (TS)0;
         *reinterpret_cast<fl2*>(X + o) = x;
       } else if (mv[cp][0] || mv[cp][1]) {
         fl2 x = *reinterpret_cast<const fl2*>(X + o);
@@ -1126,7 +1128,7 @@ __global__ void __launch_bounds__(KB) k_cg_pupdate_pair(int64_t n, const float* 
       }
       if (act[cp][0] || act[cp][1]) {
         const fl2 r = *reinterpret_cast<const fl2*>(R + o);
-        const float d = dinv[(int64_t)sys * n + row];
+        const TS d = dinv[(int64_t)sys * n + row];
         fl2 pn = p;
         if (act[cp][0]) pn.x = d * r.x + scale_to(p.x, be[cp][0]);
         if (act[cp][1]) pn.y = d * r.y + scale_to(p.y, be[cp][1]);
@@ -1678,15 +1680,15 @@ static bool pair_sweep(int nsys, int C, int k) {
          (C == 128 || C == 256 || C == 512);  // C == 64 * NCH, NCH even
 }
 
-// the vector updates in their pair form (float32 only; false: not taken)
+// the vector updates in their pair form (real types; false: not taken)
 template <typename T32, typename A>
 static bool launch_update_pair(int C, int64_t nb, hipStream_t s, int64_t n, const T32* dinv, T32* R, const T32* Q,
                                ColState<A> st) {
-  if constexpr (std::is_same<T32, float>::value) {
+  if constexpr (std::is_same<T32, float>::value || std::is_same<T32, double>::value) {
     const int nch = C / 64;
-    if (nch == 2) plaunch(k_cg_update_pair<2>, dim3(nb), dim3(KB), 0, s, n, dinv, R, Q, C, st);
-    else if (nch == 4) plaunch(k_cg_update_pair<4>, dim3(nb), dim3(KB), 0, s, n, dinv, R, Q, C, st);
-    else plaunch(k_cg_update_pair<8>, dim3(nb), dim3(KB), 0, s, n, dinv, R, Q, C, st);
+    if (nch == 2) plaunch(k_cg_update_pair<T32, 2>, dim3(nb), dim3(KB), 0, s, n, dinv, R, Q, C, st);
+    else if (nch == 4) plaunch(k_cg_update_pair<T32, 4>, dim3(nb), dim3(KB), 0, s, n, dinv, R, Q, C, st);
+    else plaunch(k_cg_update_pair<T32, 8>, dim3(nb), dim3(KB), 0, s, n, dinv, R, Q, C, st);
     return true;
   }
   return false;
@@ -1695,11 +1697,12 @@ static bool launch_update_pair(int C, int64_t nb, hipStream_t s, int64_t n, cons
 template <typename T32, typename A, typename TW>
 static bool launch_pupdate_pair(int C, int64_t nb, hipStream_t s, int64_t n, const T32* dinv, T32* X, const T32* R,
                                 T32* P, TW* X64, ColState<A> st) {
-  if constexpr (std::is_same<T32, float>::value) {
+  if constexpr (std::is_same<T32, float>::value || std::is_same<T32, double>::value) {
     const int nch = C / 64;
-    if (nch == 2) plaunch(k_cg_pupdate_pair<2>, dim3(nb), dim3(KB), 0, s, n, dinv, X, R, P, C, X64, st);
-    else if (nch == 4) plaunch(k_cg_pupdate_pair<4>, dim3(nb), dim3(KB), 0, s, n, dinv, X, R, P, C, X64, st);
-    else plaunch(k_cg_pupdate_pair<8>, dim3(nb), dim3(KB), 0, s, n, dinv, X, R, P, C, X64, st);
+    double* x64 = (double*)X64;  // the float64 solution (mixed) or null
+    if (nch == 2) plaunch(k_cg_pupdate_pair<T32, 2>, dim3(nb), dim3(KB), 0, s, n, dinv, X, R, P, C, x64, st);
+    else if (nch == 4) plaunch(k_cg_pupdate_pair<T32, 4>, dim3(nb), dim3(KB), 0, s, n, dinv, X, R, P, C, x64, st);
+    else plaunch(k_cg_pupdate_pair<T32, 8>, dim3(nb), dim3(KB), 0, s, n, dinv, X, R, P, C, x64, st);
     return true;
   }
   return false;
@@ -1854,13 +1857,16 @@ static int run_cg(const Csr& m, int nsys, const T* vals, const T* dinv, const T*
       hipLaunchKernelGGL(k_fin_alpha<A>, dim3(C), dim3(FB), 0, s, C, ss);
       {
         ProfScope ps(RB_PROF_UPDATE, s);
-        DISPATCH_GN(C, L_UPD, m.n, dinv, w.R, w.Q, C, k, st);
+        if (!pair_sweep<T>(nsys, C, k) || !launch_update_pair<T>(C, nb, s, m.n, dinv, w.R, w.Q, st))
+          DISPATCH_GN(C, L_UPD, m.n, dinv, w.R, w.Q, C, k, st);
       }
       hipLaunchKernelGGL(k_fin_beta<A>, dim3(C), dim3(FB), 0, s, C, st);
       {
         ProfScope ps(RB_PROF_PUPDATE, s);
         typedef T TW;
-        DISPATCH_GN(C, L_PUP, m.n, dinv, X, w.R, w.P, C, k, (TW*)nullptr, st);
+        if (!pair_sweep<T>(nsys, C, k) ||
+            !launch_pupdate_pair<T>(C, nb, s, m.n, dinv, X, w.R, w.P, (TW*)nullptr, st))
+          DISPATCH_GN(C, L_PUP, m.n, dinv, X, w.R, w.P, C, k, (TW*)nullptr, st);
       }
     }
     RB_CHECK_LAUNCH();
