@@ -1,9 +1,9 @@
 // K10: Cholesky factorisation A = L L^T of a dense float64 SPD matrix, blocked
 // right-looking in 64 x 64 blocks, in place in the lower triangle.  Per block
 // column kb:
-//   k_potrf_diag  one wavefront factors A_kk (lane i holds row i in registers,
-//                 each step's column broadcast through LDS) and forms
-//                 D = L_kk^-1 by forward substitution (lane j solves column j)
+//   k_potrf_diag  one wavefront factors A_kk (lane i: row i; 8-column panels
+//                 factored in registers, their rank-8 updates through LDS) and
+//                 forms D = L_kk^-1 by forward substitution (lane j: column j)
 //   k_potrf_panel L_ik = A_ik D^T for every block row i below (a workgroup each)
 //   k_potrf_syrk  A_ij -= L_ik L_jk^T for kb < j <= i (a workgroup per block);
 //                 the workgroup of block (kb+1, kb+1) then factors it
@@ -13,7 +13,18 @@
 #include "rb_common.h"
 #include "../../include/rbhip.h"
 
-#define CB 64  // block size
+#define CB 64   // block size
+#define TP 66   // column stride of the transposed block in LDS (even: 16-byte aligned column segments)
+#define PW 8    // panel width inside the diagonal factor
+#define LPS 10  // row stride of the panel broadcast buffer
+
+// LDS of one diagonal-block factorisation
+struct DiagLds {
+  double T[CB * TP];   // T[c * TP + r] = block[r][c]; ends as L (lower part)
+  double X[CB * CB];   // X[r * CB + j] = (L^-1)[r][j]
+  double Lp[CB * LPS]; // Lp[j * LPS + c]: the current panel's L[j][kb*PW + c]
+  double rv[CB];       // 1 / L[k][k]
+};
 
 __device__ __forceinline__ double rl64(double x, int lane) {
   const long long u = __double_as_longlong(x);
@@ -22,54 +33,101 @@ __device__ __forceinline__ double rl64(double x, int lane) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Factor one diagonal block held in LDS (s: its 64 rows; cv: 64 doubles of
-// scratch) with one wavefront: lane i keeps row i in registers, each step's
-// column is broadcast through LDS.  Writes L_kk over the block (zero above
-// the diagonal), D = L_kk^-1 and the first bad pivot into *info.  Only this
-// wavefront runs it: LDS traffic inside one wavefront needs no s_barrier (its
-// LDS instructions complete in order); wave_barrier keeps the compiler from
-// moving memory operations across the exchange points.
-__device__ __forceinline__ void diag_factor(double (*s)[CB + 2], double* cv, double* __restrict__ blk, int64_t lda,
-                                            int64_t k0, double* __restrict__ D, int* __restrict__ info, int l) {
-  double a[CB];
-#pragma unroll
-  for (int j = 0; j < CB; ++j) a[j] = s[l][j];
-  int bad = -1;
-  double myri = 0.0;  // lane l: 1 / L_ll
-#pragma unroll
-  for (int k = 0; k < CB; ++k) {
-    const double dkk = rl64(a[k], k);
-    if (bad < 0 && !(dkk > 0.0)) bad = k;
-    // 1/sqrt: hardware estimate plus two Newton steps (a few ulp; the
-    // correctly rounded sqrt and divide are long dependent sequences)
-    double ri = __builtin_amdgcn_rsq(dkk);
-    ri = ri * (1.5 - 0.5 * dkk * ri * ri);
-    ri = ri * (1.5 - 0.5 * dkk * ri * ri);
-    a[k] *= ri;  // lane k: L_kk; lanes i > k: L_ik (lanes i < k hold unused upper entries)
-    cv[l] = a[k];
-    if (l == k) myri = ri;
-    __builtin_amdgcn_wave_barrier();
-#pragma unroll
-    for (int j = k + 1; j < CB; ++j) a[j] -= a[k] * cv[j];
-    __builtin_amdgcn_wave_barrier();
-  }
-#pragma unroll
-  for (int j = 0; j < CB; ++j) s[l][j] = (j <= l) ? a[j] : 0.0;  // row l of L_kk
+// orders LDS accesses between the lanes of one wavefront
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
-  // D = L_kk^-1: lane j solves column j (four partial sums shorten the chain)
-  double x[CB];
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Factor one diagonal block held transposed in z.T with one wavefront (lane l:
+// row l of the block, column l of its inverse).  Blocked in panels of PW
+// columns: a panel's columns sit in registers and are factored with the pivot
+// rows read across lanes (readlane), then the columns to its right take the
+// panel's rank-PW update from LDS.  The inverse D = L^-1 is formed a row block
+// at a time by forward substitution.  Writes L over the block in global memory
+// (zero above the diagonal), D (row-major, CB x CB) and the first bad pivot
+// into *info.  The loops stay rolled: a fully unrolled 64-step factor is
+// larger than the instruction cache and runs at its miss rate.
+__device__ __forceinline__ void diag_factor(DiagLds& z, double* __restrict__ blk, int64_t lda, int64_t k0,
+                            double* __restrict__ D, int* __restrict__ info, int l) {
+  int bad = -1;
+  for (int kb = 0; kb < CB / PW; ++kb) {
+    const int c0 = kb * PW;
+    double p[PW];  // row l, columns c0 .. c0 + PW - 1
 #pragma unroll
-  for (int r = 0; r < CB; ++r) {
-    double acc[4] = {(r == l) ? 1.0 : 0.0, 0.0, 0.0, 0.0};
+    for (int c = 0; c < PW; ++c) p[c] = z.T[(c0 + c) * TP + l];
 #pragma unroll
-    for (int k = 0; k < r; ++k) acc[k & 3] -= s[r][k] * x[k];
-    x[r] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) * rl64(myri, r);
-    __builtin_amdgcn_sched_barrier(0);  // keep row r's loads from piling up ahead
+    for (int c = 0; c < PW; ++c) {
+      const double dkk = rl64(p[c], c0 + c);
+      if (bad < 0 && !(dkk > 0.0)) bad = c0 + c;
+      // 1/sqrt: hardware estimate plus two Newton steps (a few ulp; the
+      // correctly rounded sqrt and divide are long dependent sequences)
+      double ri = __builtin_amdgcn_rsq(dkk);
+      ri = ri * (1.5 - 0.5 * dkk * ri * ri);
+      ri = ri * (1.5 - 0.5 * dkk * ri * ri);
+      p[c] *= ri;  // rows l >= c0 + c: L[l][c0 + c] (rows above hold unused upper entries)
+      if (l == 0) z.rv[c0 + c] = ri;
+#pragma unroll
+      for (int d = c + 1; d < PW; ++d) p[d] -= p[c] * rl64(p[c], c0 + d);
+    }
+#pragma unroll
+    for (int c = 0; c < PW; ++c) {
+      z.T[(c0 + c) * TP + l] = p[c];
+      z.Lp[l * LPS + c] = p[c];
+    }
+    wave_sync();
+    // columns right of the panel: block[l][j] -= sum_c L[l][c0+c] L[j][c0+c],
+    // JU columns per pass with every load issued before the stores (a store
+    // ahead of a load that may alias it would serialise the LDS latency)
+    constexpr int JU = 4;  // divides CB - c0 - PW (a multiple of PW)
+    for (int j = c0 + PW; j < CB; j += JU) {
+      double a[JU], lp[JU][PW];
+#pragma unroll
+      for (int u = 0; u < JU; ++u) {
+        a[u] = z.T[(j + u) * TP + l];
+#pragma unroll
+        for (int c = 0; c < PW; ++c) lp[u][c] = z.Lp[(j + u) * LPS + c];
+      }
+#pragma unroll
+      for (int u = 0; u < JU; ++u)
+#pragma unroll
+        for (int c = 0; c < PW; ++c) a[u] -= p[c] * lp[u][c];
+#pragma unroll
+      for (int u = 0; u < JU; ++u) z.T[(j + u) * TP + l] = a[u];
+    }
+    wave_sync();
   }
+  // D = L^-1, lane l solves column l, PW rows at a time
+  for (int rb = 0; rb < CB / PW; ++rb) {
+    const int r0 = rb * PW;
+    double x[PW];
 #pragma unroll
-  for (int r = 0; r < CB; ++r) D[r * CB + l] = x[r];
-#pragma unroll 16
-  for (int r = 0; r < CB; ++r) blk[r * lda + l] = s[r][l];
+    for (int i = 0; i < PW; ++i) x[i] = (l == r0 + i) ? 1.0 : 0.0;
+#pragma unroll 2
+    for (int k = 0; k < r0; ++k) {
+      const double xk = z.X[k * CB + l];
+#pragma unroll
+      for (int i = 0; i < PW; ++i) x[i] -= z.T[k * TP + r0 + i] * xk;
+    }
+#pragma unroll
+    for (int i = 0; i < PW; ++i) {
+#pragma unroll
+      for (int c = 0; c < i; ++c) x[i] -= z.T[(r0 + c) * TP + r0 + i] * x[c];
+      x[i] *= z.rv[r0 + i];
+    }
+#pragma unroll
+    for (int i = 0; i < PW; ++i) {  // stores after the block's loads
+      z.X[(r0 + i) * CB + l] = x[i];
+      D[(r0 + i) * CB + l] = x[i];
+    }
+  }
+  wave_sync();
+#pragma unroll 8
+  for (int r = 0; r < CB; ++r) {
+    const double v = z.T[l * TP + r];  // read unconditionally: a guarded load would wait per row
+    blk[r * lda + l] = (l <= r) ? v : 0.0;
+  }
   if (l == 0) {
     int cur = (k0 == 0) ? 0 : *info;
     if (cur == 0 && bad >= 0) cur = (int)(k0 + bad + 1);
@@ -81,14 +139,12 @@ __device__ __forceinline__ void diag_factor(double (*s)[CB + 2], double* cv, dou
 // that finishes them)
 __global__ void __launch_bounds__(64) k_potrf_diag(double* __restrict__ A, int64_t lda, int64_t k0,
                                                    double* __restrict__ D, int* __restrict__ info) {
-  __shared__ __align__(16) double s[CB][CB + 2];  // even row stride: 16-byte aligned rows
-  __shared__ __align__(16) double cv[CB];
+  __shared__ __align__(16) DiagLds z;
   const int l = threadIdx.x;
   double* blk = A + k0 * lda + k0;
-#pragma unroll 16
-  for (int r = 0; r < CB; ++r) s[r][l] = blk[r * lda + l];
+  for (int r = 0; r < CB; ++r) z.T[l * TP + r] = blk[r * lda + l];
   __syncthreads();
-  diag_factor(s, cv, blk, lda, k0, D, info, l);
+  diag_factor(z, blk, lda, k0, D, info, l);
 }
 
 // L_ik = A_ik D^T, D = L_kk^-1; block row i = kb + 1 + blockIdx.x
@@ -132,7 +188,9 @@ __global__ void __launch_bounds__(256) k_potrf_panel(double* __restrict__ A, int
 // critical path whenever the update takes longer.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_potrf_syrk(double* __restrict__ A, int64_t lda, int64_t k0,
                                                     double* __restrict__ D, int* __restrict__ info) {
-  __shared__ __align__(16) double sm[2 * CB * (CB + 1)];
+  constexpr size_t kUpd = 2 * CB * (CB + 1) * sizeof(double);
+  __shared__ __align__(16) unsigned char smem[kUpd > sizeof(DiagLds) ? kUpd : sizeof(DiagLds)];
+  double* sm = (double*)smem;
   double (*Ls)[CB + 1] = (double (*)[CB + 1])sm;              // Ls[m][r] = L_ik[r][m]
   double (*Rs)[CB + 1] = (double (*)[CB + 1])(sm + CB * (CB + 1));  // Rs[m][c] = L_jk[c][m]
   const int64_t e = blockIdx.x;
@@ -164,8 +222,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   }
   double* c = A + i0 * lda + j0;
   const bool fuse = (e == 0);  // block (kb+1, kb+1)
-  double (*S)[CB + 2] = (double (*)[CB + 2])sm;  // reuses Ls/Rs once they are read
-  double* cv = sm + CB * (CB + 2);
+  DiagLds& z = *(DiagLds*)smem;  // reuses Ls/Rs once they are read
   if (fuse) __syncthreads();
 #pragma unroll
   for (int p = 0; p < 4; ++p)
@@ -173,11 +230,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     for (int q = 0; q < 4; ++q) {
       const double v = c[(tr + p) * lda + tc + q] - acc[p][q];
       c[(tr + p) * lda + tc + q] = v;
-      if (fuse) S[tr + p][tc + q] = v;
+      if (fuse) z.T[(tc + q) * TP + tr + p] = v;
     }
   if (fuse) {
     __syncthreads();
-    if (t < 64) diag_factor(S, cv, c, lda, k0 + CB, D, info, t);
+    if (t < 64) diag_factor(z, c, lda, k0 + CB, D, info, t);
   }
 }
 
