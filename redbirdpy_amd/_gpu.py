@@ -21,12 +21,31 @@ def device():
     return torch.device("cuda", torch.cuda.current_device())
 
 
+_NP2TORCH = {np.dtype(np.float64): torch.float64, np.dtype(np.float32): torch.float32,
+             np.dtype(np.int32): torch.int32, np.dtype(np.int64): torch.int64,
+             np.dtype(np.complex128): torch.complex128, np.dtype(np.complex64): torch.complex64}
+
+
+def h2d(a, dev, dtype=None):
+    """Host array -> device tensor through a pinned staging copy enqueued on the
+    current stream without waiting for it (a pageable copy first drains the
+    stream, which leaves the GPU idle while the host prepares the next launch).
+    The staging block is torch's cached pinned memory, held until the copy ends."""
+    a = np.asarray(a) if dtype is None else np.asarray(a, dtype=dtype)
+    tdt = _NP2TORCH.get(a.dtype)
+    if tdt is None or a.size == 0:
+        return torch.as_tensor(np.ascontiguousarray(a), device=dev)
+    stage = torch.empty(a.shape, dtype=tdt, pin_memory=True)
+    stage.numpy()[...] = a
+    return stage.to(dev, non_blocking=True)
+
+
 def _i32(a, dev):
-    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32), device=dev)
+    return h2d(a, dev, np.int32)
 
 
 def _f64(a, dev):
-    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=dev)
+    return h2d(a, dev, np.float64)
 
 
 def _spread_bits(v):

@@ -127,9 +127,9 @@ class _Optodes:
         row = np.concatenate(rows) if rows else np.zeros(0, np.int64)
         val = np.concatenate(vals) if vals else np.zeros(0)
         return (
-            torch.as_tensor(np.array(ptrs, np.int32), device=dev),
-            torch.as_tensor(row.astype(np.int32), device=dev),
-            torch.as_tensor(val.astype(np.float64), device=dev),
+            _gpu.h2d(np.array(ptrs), dev, np.int32),
+            _gpu.h2d(row, dev, np.int32),
+            _gpu.h2d(val, dev, np.float64),
         )
 
 

@@ -242,7 +242,7 @@ class GNStep:
         self.JT = JT
         self.scale2 = scale2  # 1/||J||_F^2 taken before a prior transform of JT (recon.py:235)
         self.np = npairs
-        self.E = torch.as_tensor(E, dtype=torch.float64, device=JT.device)
+        self.E = _gpu.h2d(E, JT.device, np.float64)
         self.lam = float(lam)
         self.precision = precision
         self.reftol = reftol
@@ -550,7 +550,7 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
             if return_jac:
                 jac_keep["mua" if wv == "" else wv] = mesh.user_rows(Jw).t().cpu().numpy()
             if rowscale is not None:
-                sc = torch.as_tensor(rowscale[blk] if cplx else np.real(rowscale[blk]), device=mesh.dev)
+                sc = _gpu.h2d(rowscale[blk] if cplx else np.real(rowscale[blk]), mesh.dev)
                 Jw = Jw * sc[None, :]
             JT[:, blk] = Jw.real
             if cplx:
@@ -606,11 +606,11 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
             if len(keys) != 1 or nw != 1 or dual or labelled or Aregu:
                 raise NotImplementedError("reform 'reim' is supported for one unknown block on one mesh, "
                                           "without priors or label compression")
-        misfit_t = torch.as_tensor(misfit, dtype=torch.float64, device=mesh.dev)
+        misfit_t = _gpu.h2d(misfit, mesh.dev, np.float64)
         step = None
         if over:
             tp = mark("misfit", tp)
-            dmu = _solve_over(JG, npairs, torch.as_tensor(E, dtype=torch.float64, device=mesh.dev), lam,
+            dmu = _solve_over(JG, npairs, _gpu.h2d(E, mesh.dev, np.float64), lam,
                               misfit_t, Aregu.get("ltl"))
         else:
             invR = None
@@ -624,7 +624,7 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
                     perm = torch.as_tensor(mesh.perm, device=mesh.dev)
                     invR = invR[perm][:, perm]  # into the device node order of JG's rows
                 # the blockscale is that of the untransformed J (recon.py:235-236)
-                Et = torch.as_tensor(E, dtype=torch.float64, device=mesh.dev)
+                Et = _gpu.h2d(E, mesh.dev, np.float64)
                 wdiag = torch.repeat_interleave(torch.diagonal(Et @ Et.t()), npairs)
                 s2pre = 1.0 / torch.sum(wdiag * _gpu.jt_cols(JG, E.shape[0], npairs)).item()
                 JG = invR.t() @ JG  # A' = A R^-1 per block: node-major J' = R^-T J
