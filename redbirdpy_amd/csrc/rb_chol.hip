@@ -1,9 +1,9 @@
 // K10: Cholesky factorisation A = L L^T of a dense float64 SPD matrix, blocked
 // right-looking in 64 x 64 blocks, in place in the lower triangle.  Per block
 // column kb:
-//   k_potrf_diag  one wavefront factors A_kk (lane i: row i; 8-column panels
+//   k_potrf_diag  four wavefronts factor A_kk (lane i: row i; 8-column panels
 //                 factored in registers, their rank-8 updates through LDS) and
-//                 forms D = L_kk^-1 by forward substitution (lane j: column j)
+//                 form D = L_kk^-1 by forward substitution (lane j: column j)
 //   k_potrf_panel L_ik = A_ik D^T for every block row i below (a workgroup each)
 //   k_potrf_syrk  A_ij -= L_ik L_jk^T for kb < j <= i (a workgroup per block);
 //                 the workgroup of block (kb+1, kb+1) then factors it
@@ -40,17 +40,24 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Factor one diagonal block held transposed in z.T with one wavefront (lane l:
-// row l of the block, column l of its inverse).  Blocked in panels of PW
-// columns: a panel's columns sit in registers and are factored with the pivot
-// rows read across lanes (readlane), then the columns to its right take the
-// panel's rank-PW update from LDS.  The inverse D = L^-1 is formed a row block
-// at a time by forward substitution.  Writes L over the block in global memory
-// (zero above the diagonal), D (row-major, CB x CB) and the first bad pivot
-// into *info.  The loops stay rolled: a fully unrolled 64-step factor is
-// larger than the instruction cache and runs at its miss rate.
+// Factor one diagonal block held transposed in z.T with the NW wavefronts of
+// the workgroup (t: thread index; lane l: row l of the block, column l of its
+// inverse).  Blocked in panels of PW columns: every wavefront factors the
+// panel's columns in registers (the same arithmetic in each, pivot rows read
+// across lanes with readlane), then the wavefronts share the rank-PW update
+// of the columns to its right (JU columns per pass, every load of a pass
+// issued before its stores).  The inverse D = L^-1 is formed PW rows at a
+// time: the wavefronts split the rows' sums over the earlier rows, the first
+// wavefront finishes the block by forward substitution.  Writes L over the
+// block in global memory (zero above the diagonal), D (row-major, CB x CB)
+// and the first bad pivot into *info.  The loops stay rolled: a fully
+// unrolled 64-step factor is larger than the instruction cache.  Every
+// thread of the workgroup must call it (it holds workgroup barriers).
+#define NW 4  // wavefronts per diagonal factor
 __device__ __forceinline__ void diag_factor(DiagLds& z, double* __restrict__ blk, int64_t lda, int64_t k0,
-                            double* __restrict__ D, int* __restrict__ info, int l) {
+                                            double* __restrict__ D, int* __restrict__ info, int t) {
+  const int l = t & 63;
+  const int w = __builtin_amdgcn_readfirstlane(t >> 6);
   int bad = -1;
   for (int kb = 0; kb < CB / PW; ++kb) {
     const int c0 = kb * PW;
@@ -67,21 +74,21 @@ __device__ __forceinline__ void diag_factor(DiagLds& z, double* __restrict__ blk
       ri = ri * (1.5 - 0.5 * dkk * ri * ri);
       ri = ri * (1.5 - 0.5 * dkk * ri * ri);
       p[c] *= ri;  // rows l >= c0 + c: L[l][c0 + c] (rows above hold unused upper entries)
-      if (l == 0) z.rv[c0 + c] = ri;
+      if (t == 0) z.rv[c0 + c] = ri;
 #pragma unroll
       for (int d = c + 1; d < PW; ++d) p[d] -= p[c] * rl64(p[c], c0 + d);
     }
+    if (w == 0) {
 #pragma unroll
-    for (int c = 0; c < PW; ++c) {
-      z.T[(c0 + c) * TP + l] = p[c];
-      z.Lp[l * LPS + c] = p[c];
+      for (int c = 0; c < PW; ++c) {
+        z.T[(c0 + c) * TP + l] = p[c];
+        z.Lp[l * LPS + c] = p[c];
+      }
     }
-    wave_sync();
-    // columns right of the panel: block[l][j] -= sum_c L[l][c0+c] L[j][c0+c],
-    // JU columns per pass with every load issued before the stores (a store
-    // ahead of a load that may alias it would serialise the LDS latency)
+    __syncthreads();
+    // columns right of the panel: block[l][j] -= sum_c L[l][c0+c] L[j][c0+c]
     constexpr int JU = 4;  // divides CB - c0 - PW (a multiple of PW)
-    for (int j = c0 + PW; j < CB; j += JU) {
+    for (int j = c0 + PW + JU * w; j < CB; j += JU * NW) {
       double a[JU], lp[JU][PW];
 #pragma unroll
       for (int u = 0; u < JU; ++u) {
@@ -96,39 +103,50 @@ __device__ __forceinline__ void diag_factor(DiagLds& z, double* __restrict__ blk
 #pragma unroll
       for (int u = 0; u < JU; ++u) z.T[(j + u) * TP + l] = a[u];
     }
-    wave_sync();
+    __syncthreads();
   }
-  // D = L^-1, lane l solves column l, PW rows at a time
+  // D = L^-1, lane l solves column l, PW rows at a time; wavefront w sums
+  // rows r0 + RW*w .. r0 + RW*w + RW - 1 over the earlier rows into z.Lp
+  constexpr int RW = PW / NW;
+  static_assert(RW * NW == PW && PW * CB <= CB * LPS, "inverse row split");
   for (int rb = 0; rb < CB / PW; ++rb) {
     const int r0 = rb * PW;
-    double x[PW];
+    double sm[RW];
 #pragma unroll
-    for (int i = 0; i < PW; ++i) x[i] = (l == r0 + i) ? 1.0 : 0.0;
-#pragma unroll 2
+    for (int i = 0; i < RW; ++i) sm[i] = 0.0;
+#pragma unroll 4
     for (int k = 0; k < r0; ++k) {
       const double xk = z.X[k * CB + l];
 #pragma unroll
-      for (int i = 0; i < PW; ++i) x[i] -= z.T[k * TP + r0 + i] * xk;
+      for (int i = 0; i < RW; ++i) sm[i] += z.T[k * TP + r0 + RW * w + i] * xk;
     }
 #pragma unroll
-    for (int i = 0; i < PW; ++i) {
+    for (int i = 0; i < RW; ++i) z.Lp[(RW * w + i) * CB + l] = sm[i];
+    __syncthreads();
+    if (w == 0) {
+      double x[PW];
 #pragma unroll
-      for (int c = 0; c < i; ++c) x[i] -= z.T[(r0 + c) * TP + r0 + i] * x[c];
-      x[i] *= z.rv[r0 + i];
-    }
+      for (int i = 0; i < PW; ++i) x[i] = ((l == r0 + i) ? 1.0 : 0.0) - z.Lp[i * CB + l];
 #pragma unroll
-    for (int i = 0; i < PW; ++i) {  // stores after the block's loads
-      z.X[(r0 + i) * CB + l] = x[i];
-      D[(r0 + i) * CB + l] = x[i];
+      for (int i = 0; i < PW; ++i) {
+#pragma unroll
+        for (int c = 0; c < i; ++c) x[i] -= z.T[(r0 + c) * TP + r0 + i] * x[c];
+        x[i] *= z.rv[r0 + i];
+      }
+#pragma unroll
+      for (int i = 0; i < PW; ++i) {  // stores after the block's loads
+        z.X[(r0 + i) * CB + l] = x[i];
+        D[(r0 + i) * CB + l] = x[i];
+      }
     }
+    __syncthreads();
   }
-  wave_sync();
-#pragma unroll 8
-  for (int r = 0; r < CB; ++r) {
+#pragma unroll 4
+  for (int r = w; r < CB; r += NW) {
     const double v = z.T[l * TP + r];  // read unconditionally: a guarded load would wait per row
     blk[r * lda + l] = (l <= r) ? v : 0.0;
   }
-  if (l == 0) {
+  if (t == 0) {
     int cur = (k0 == 0) ? 0 : *info;
     if (cur == 0 && bad >= 0) cur = (int)(k0 + bad + 1);
     *info = cur;
@@ -137,14 +155,14 @@ __device__ __forceinline__ void diag_factor(DiagLds& z, double* __restrict__ blk
 
 // the first diagonal block (the others are factored by the trailing update
 // that finishes them)
-__global__ void __launch_bounds__(64) k_potrf_diag(double* __restrict__ A, int64_t lda, int64_t k0,
-                                                   double* __restrict__ D, int* __restrict__ info) {
+__global__ void __launch_bounds__(64 * NW) k_potrf_diag(double* __restrict__ A, int64_t lda, int64_t k0,
+                                                        double* __restrict__ D, int* __restrict__ info) {
   __shared__ __align__(16) DiagLds z;
-  const int l = threadIdx.x;
+  const int t = threadIdx.x, l = t & 63;
   double* blk = A + k0 * lda + k0;
-  for (int r = 0; r < CB; ++r) z.T[l * TP + r] = blk[r * lda + l];
+  for (int r = t >> 6; r < CB; r += NW) z.T[l * TP + r] = blk[r * lda + l];
   __syncthreads();
-  diag_factor(z, blk, lda, k0, D, info, l);
+  diag_factor(z, blk, lda, k0, D, info, t);
 }
 
 // L_ik = A_ik D^T, D = L_kk^-1; block row i = kb + 1 + blockIdx.x
@@ -234,7 +252,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     }
   if (fuse) {
     __syncthreads();
-    if (t < 64) diag_factor(z, c, lda, k0 + CB, D, info, t);
+    diag_factor(z, c, lda, k0 + CB, D, info, t);  // all 256 threads
   }
 }
 
@@ -257,7 +275,7 @@ extern "C" int rb_potrf(int64_t n, double* A, int64_t lda, void* work, size_t wo
   }
   double* D = (double*)work;
   const int64_t nbk = n / CB;
-  hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(64), 0, s, A, lda, (int64_t)0, D, info);
+  hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(64 * NW), 0, s, A, lda, (int64_t)0, D, info);
   RB_CHECK_LAUNCH();
   for (int64_t kb = 0; kb + 1 < nbk; ++kb) {
     const int64_t k0 = kb * CB, rest = nbk - kb - 1;
