@@ -179,14 +179,14 @@ __global__ void __launch_bounds__(256) k_potrf_panel(double* __restrict__ A, int
     Ds[m][r] = D[e];
   }
   __syncthreads();
-  const int tr = (t >> 4) * 4, tc = (t & 15) * 4;
+  const int tr = t >> 4, tc = t & 15;  // rows tr + 16 p, columns tc + 16 q (interleaved)
   double acc[4][4] = {};
   for (int m = 0; m < CB; ++m) {
     double av[4], dv[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      av[q] = As[m][tr + q];
-      dv[q] = Ds[m][tc + q];
+      av[q] = As[m][tr + 16 * q];
+      dv[q] = Ds[m][tc + 16 * q];
     }
 #pragma unroll
     for (int p = 0; p < 4; ++p)
@@ -196,7 +196,7 @@ __global__ void __launch_bounds__(256) k_potrf_panel(double* __restrict__ A, int
 #pragma unroll
   for (int p = 0; p < 4; ++p)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) blk[(tr + p) * lda + tc + q] = acc[p][q];
+    for (int q = 0; q < 4; ++q) blk[(tr + 16 * p) * lda + tc + 16 * q] = acc[p][q];
 }
 
 // A_ij -= L_ik L_jk^T over the lower blocks kb < j <= i (triangular index).
@@ -218,27 +218,36 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   const int64_t bj = e - bi * (bi + 1) / 2;
   const int64_t i0 = k0 + CB * (1 + bi), j0 = k0 + CB * (1 + bj);
   const int t = threadIdx.x;
+  // thread t: rows (t >> 4) + 16 p, columns (t & 15) + 16 q of the block
+  // (interleaved: a wavefront's LDS reads hit consecutive banks, its global
+  // accesses consecutive addresses)
+  const int tr = t >> 4, tc = t & 15;
+  double* c = A + i0 * lda + j0;
+  double cv[4][4];  // this thread's entries of block (i, j), loaded with the L blocks (one latency)
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) cv[p][q] = c[(tr + 16 * p) * lda + tc + 16 * q];
   for (int q = t; q < CB * CB; q += 256) {
     const int r = q >> 6, m = q & 63;
     Ls[m][r] = A[(i0 + r) * lda + k0 + m];
     Rs[m][r] = A[(j0 + r) * lda + k0 + m];
   }
   __syncthreads();
-  const int tr = (t >> 4) * 4, tc = (t & 15) * 4;
   double acc[4][4] = {};
+#pragma unroll 4
   for (int m = 0; m < CB; ++m) {
     double lv[4], rv[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      lv[q] = Ls[m][tr + q];
-      rv[q] = Rs[m][tc + q];
+      lv[q] = Ls[m][tr + 16 * q];
+      rv[q] = Rs[m][tc + 16 * q];
     }
 #pragma unroll
     for (int p = 0; p < 4; ++p)
 #pragma unroll
       for (int q = 0; q < 4; ++q) acc[p][q] += lv[p] * rv[q];
   }
-  double* c = A + i0 * lda + j0;
   const bool fuse = (e == 0);  // block (kb+1, kb+1)
   DiagLds& z = *(DiagLds*)smem;  // reuses Ls/Rs once they are read
   if (fuse) __syncthreads();
@@ -246,9 +255,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   for (int p = 0; p < 4; ++p)
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const double v = c[(tr + p) * lda + tc + q] - acc[p][q];
-      c[(tr + p) * lda + tc + q] = v;
-      if (fuse) z.T[(tc + q) * TP + tr + p] = v;
+      const double v = cv[p][q] - acc[p][q];
+      c[(tr + 16 * p) * lda + tc + 16 * q] = v;
+      if (fuse) z.T[(tc + 16 * q) * TP + tr + 16 * p] = v;
     }
   if (fuse) {
     __syncthreads();
