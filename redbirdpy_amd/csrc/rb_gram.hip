@@ -25,6 +25,8 @@ __global__ void __launch_bounds__(256) k_gram_f64(int64_t K, int64_t m, const TI
   }
   int64_t tj = ti + t;
   int64_t i0 = ti * GT, j0 = tj * GT;
+  // thread: rows ty + 16 p, columns tx + 16 q of the tile (interleaved: a
+  // wavefront's LDS reads of Bs fall on consecutive banks)
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
   double acc[4][4];
 #pragma unroll
@@ -44,8 +46,8 @@ __global__ void __launch_bounds__(256) k_gram_f64(int64_t K, int64_t m, const TI
       double a[4], b[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        a[q] = As[kk][ty * 4 + q];
-        b[q] = Bs[kk][tx * 4 + q];
+        a[q] = As[kk][ty + 16 * q];
+        b[q] = Bs[kk][tx + 16 * q];
       }
 #pragma unroll
       for (int p = 0; p < 4; ++p)
@@ -56,11 +58,11 @@ __global__ void __launch_bounds__(256) k_gram_f64(int64_t K, int64_t m, const TI
   }
 #pragma unroll
   for (int p = 0; p < 4; ++p) {
-    int64_t i = i0 + ty * 4 + p;
+    int64_t i = i0 + ty + 16 * p;
     if (i >= m) continue;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      int64_t j = j0 + tx * 4 + q;
+      int64_t j = j0 + tx + 16 * q;
       if (j >= m) continue;
       double v = acc[p][q];
       if (d) v *= d[i] * d[j];
