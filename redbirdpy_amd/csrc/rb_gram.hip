@@ -116,7 +116,9 @@ __global__ void __launch_bounds__(256) k_split_t(int64_t K, int64_t m, int64_t K
   double acc = 0.0;
   for (int64_t kt = blockIdx.x; kt * 64 < Kp; kt += gridDim.x) {
     const int64_t k0 = kt * 64;
-    for (int q = threadIdx.x; q < 64 * 64; q += 256) {
+#pragma unroll
+    for (int it = 0; it < 64 * 64 / 256; ++it) {  // all 16 loads in flight together
+      const int q = threadIdx.x + 256 * it;
       int kk = q / 64, cc = q % 64;
       int64_t k = k0 + kk, c = c0 + cc;
       double x = (k < K && c < m) ? (double)JT[k * ldj + c] : 0.0;
@@ -126,12 +128,22 @@ __global__ void __launch_bounds__(256) k_split_t(int64_t K, int64_t m, int64_t K
       tl[cc][kk] = (float)(x - (double)(float)h);
     }
     __syncthreads();
-    for (int q = threadIdx.x; q < 64 * 64; q += 256) {
-      int cc = q / 64, kk = q % 64;
-      int64_t c = c0 + cc;
+    // eight consecutive k of one column per thread, one 16-byte store each
+    // (Kp and k0 are multiples of 64: every store is aligned and in the row)
+    for (int q = threadIdx.x; q < 64 * 8; q += 256) {
+      const int cc = q >> 3, kk = (q & 7) * 8;
+      const int64_t c = c0 + cc;
       if (c < m) {
-        Hi[c * Kp + k0 + kk] = (__bf16)th[cc][kk];
-        Lo[c * Kp + k0 + kk] = (__bf16)tl[cc][kk];
+        unsigned int hw[4], lw[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          hw[i] = (unsigned int)__builtin_bit_cast(unsigned short, (__bf16)th[cc][kk + 2 * i]) |
+                  ((unsigned int)__builtin_bit_cast(unsigned short, (__bf16)th[cc][kk + 2 * i + 1]) << 16);
+          lw[i] = (unsigned int)__builtin_bit_cast(unsigned short, (__bf16)tl[cc][kk + 2 * i]) |
+                  ((unsigned int)__builtin_bit_cast(unsigned short, (__bf16)tl[cc][kk + 2 * i + 1]) << 16);
+        }
+        *reinterpret_cast<uint4*>(Hi + c * Kp + k0 + kk) = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+        *reinterpret_cast<uint4*>(Lo + c * Kp + k0 + kk) = make_uint4(lw[0], lw[1], lw[2], lw[3]);
       }
     }
     __syncthreads();
