@@ -200,10 +200,11 @@ __global__ void __launch_bounds__(256) k_potrf_panel(double* __restrict__ A, int
 }
 
 // A_ij -= L_ik L_jk^T over the lower blocks kb < j <= i (triangular index).
-// Workgroup 0 holds block (kb+1, kb+1), which this update finishes: its
-// first wavefront then factors it (diag_factor) while the other workgroups
-// are still updating, so the next step's diagonal factor is off the
-// critical path whenever the update takes longer.
+// Workgroup 0 holds block (kb+1, kb+1), which this update finishes: its four
+// wavefronts then factor it (diag_factor) while the other workgroups are
+// still updating, so the next step's diagonal factor is off the critical
+// path whenever the update takes longer.
+static_assert(64 * NW == 256, "k_potrf_syrk's workgroup runs diag_factor");
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_potrf_syrk(double* __restrict__ A, int64_t lda, int64_t k0,
                                                     double* __restrict__ D, int* __restrict__ info) {
   constexpr size_t kUpd = 2 * CB * (CB + 1) * sizeof(double);
