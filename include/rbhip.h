@@ -158,6 +158,27 @@ int rb_krylov_solve(int64_t n, int64_t nnz, const int32_t* row_ptr, const int32_
                     double* relres_out /* host */, int* flag_out /* host */, void* stream);
 
 /*
+ * rb_krylov_solve without a workspace argument and without any allocation
+ * (no hipMalloc/hipFree, no pinned or pageable host buffers, no events).
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Its workspace is a fixed per-device array of
+ * RB_NOALLOC_SCRATCH_BYTES bytes inside the library (held by every device
+ * the library runs on until the process ends); a solve whose
+ * rb_krylov_workspace() size exceeds it fails with RB_ERR_SIZE (-3).  All
+ * calls on one device share that array: calls that overlap on one device
+ * race.  `stream` must belong to the current device.  Arguments, results and
+ * flags are those of rb_krylov_solve (same kernels, same bits).
+ */
+#define RB_NOALLOC_SCRATCH_BYTES (64u << 20)
+int rb_krylov_solve_noalloc(int64_t n, int64_t nnz, const int32_t* row_ptr, const int32_t* col_idx,
+                            const int32_t* diag_slot, int nsys, int k, int is_complex, const void* vals64,
+                            const void* vals32, const void* B, void* X, int precision, double tol, int maxiter,
+                            int use_x0, int check_every, int* iters_out /* host */, double* relres_out /* host */,
+                            int* flag_out /* host */, void* stream);
+
+/*
+ * Per-kernel timing of the Krylov loop/*
  * Per-kernel timing of the Krylov loop: when enabled, hipEvents are
  * recorded on the launch stream around every launch of the kernels below;
  * rb_prof_read returns the summed milliseconds and the launch count.

@@ -50,6 +50,8 @@ _SIGS = {
     "rb_krylov_solve": [_i64, _i64, _vp, _vp, _vp, _int, _int, _int, _vp, _vp, _vp, _vp, _int, _dbl, _int,
                         _int, _int, _vp, _sz, ctypes.POINTER(_int), ctypes.POINTER(_dbl), ctypes.POINTER(_int),
                         _vp],
+    "rb_krylov_solve_noalloc": [_i64, _i64, _vp, _vp, _vp, _int, _int, _int, _vp, _vp, _vp, _vp, _int, _dbl, _int,
+                                _int, _int, ctypes.POINTER(_int), ctypes.POINTER(_dbl), ctypes.POINTER(_int), _vp],
     "rb_spmm": [_i64, _i64, _vp, _vp, _int, _int, _int, _vp, _vp, _vp, _vp],
     "rb_bicgstab_workspace": [_i64, _int, _int, ctypes.POINTER(_sz)],
     "rb_bicgstab_solve": [_i64, _i64, _vp, _vp, _vp, _int, _int, _vp, _vp, _vp, _dbl, _int, _int, _int, _vp, _sz,

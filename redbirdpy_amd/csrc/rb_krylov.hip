@@ -1593,13 +1593,24 @@ static unsigned ew_grid(int64_t total) {
 
 static int* g_pinned = nullptr;  // host flag for convergence polls
 
+// set while rb_krylov_solve_noalloc runs on this thread: no host or device
+// allocation (no pinned poll buffer, no timing events)
+static thread_local bool t_noalloc = false;
+
 static int poll_nactive(const int* active, int C, int* d_nactive, hipStream_t s, int* out) {
-  if (!g_pinned) RB_HIP(hipHostMalloc((void**)&g_pinned, 64, hipHostMallocDefault));
   hipLaunchKernelGGL(k_count_active, dim3(1), dim3(256), 0, s, active, C, d_nactive);
   RB_CHECK_LAUNCH();
-  RB_HIP(hipMemcpyAsync(g_pinned, d_nactive, sizeof(int), hipMemcpyDeviceToHost, s));
-  RB_HIP(hipStreamSynchronize(s));
-  *out = *g_pinned;
+  if (!g_pinned && !t_noalloc) RB_HIP(hipHostMalloc((void**)&g_pinned, 64, hipHostMallocDefault));
+  if (g_pinned) {
+    RB_HIP(hipMemcpyAsync(g_pinned, d_nactive, sizeof(int), hipMemcpyDeviceToHost, s));
+    RB_HIP(hipStreamSynchronize(s));
+    *out = *g_pinned;
+  } else {  // pageable destination: the copy completes before it returns
+    int v = 0;
+    RB_HIP(hipMemcpyAsync(&v, d_nactive, sizeof(int), hipMemcpyDeviceToHost, s));
+    RB_HIP(hipStreamSynchronize(s));
+    *out = v;
+  }
   return RB_OK;
 }
 
@@ -1632,7 +1643,7 @@ static int g_prof_open = -1;
 struct ProfScope {
   int prev;
   ProfScope(int id, hipStream_t) : prev(g_prof_open) {
-    if (g_prof) g_prof_open = id;
+    if (g_prof && !t_noalloc) g_prof_open = id;
   }
   ~ProfScope() { g_prof_open = prev; }
 };
@@ -1640,7 +1651,7 @@ struct ProfScope {
 template <typename F, typename... Args>
 static inline void plaunch(F kernel, dim3 grid, dim3 block, uint32_t shm, hipStream_t s, Args... args) {
   hipEvent_t a = nullptr, b = nullptr;
-  if (g_prof && g_prof_open >= 0) {
+  if (g_prof && g_prof_open >= 0 && !t_noalloc) {
     a = prof_event();
     b = prof_event();
     if (!a || !b) a = b = nullptr;
@@ -1897,8 +1908,9 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
   T64* X = (T64*)Xv;
   const double tol2 = tol * tol;
   int rc;
-  std::vector<double> hrr(C), hbb(C);
-  std::vector<int> hflag(C), hit(C);
+  // per-column results (C <= 512, checked by the entry points); stack arrays, no allocation
+  double hrr[512], hbb[512];
+  int hflag[512], hit[512];
 
   if (!precision) {
     SolveBufs<T64> w{(T64*)(base + o[0]), (T64*)(base + o[1]), (T64*)(base + o[2]), (T64*)(base + o[3])};
@@ -1911,10 +1923,10 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
     int its = 0;
     rc = run_cg<T64>(m, nsys, A64, w.dinv, B, X, use_x0, w, C, k, st, check_every, s, &its);
     if (rc) return rc;
-    RB_HIP(hipMemcpyAsync(hrr.data(), st.rr, C * sizeof(double), hipMemcpyDeviceToHost, s));
-    RB_HIP(hipMemcpyAsync(hbb.data(), st.bb, C * sizeof(double), hipMemcpyDeviceToHost, s));
-    RB_HIP(hipMemcpyAsync(hflag.data(), st.flag, C * sizeof(int), hipMemcpyDeviceToHost, s));
-    RB_HIP(hipMemcpyAsync(hit.data(), st.iters, C * sizeof(int), hipMemcpyDeviceToHost, s));
+    RB_HIP(hipMemcpyAsync(hrr, st.rr, C * sizeof(double), hipMemcpyDeviceToHost, s));
+    RB_HIP(hipMemcpyAsync(hbb, st.bb, C * sizeof(double), hipMemcpyDeviceToHost, s));
+    RB_HIP(hipMemcpyAsync(hflag, st.flag, C * sizeof(int), hipMemcpyDeviceToHost, s));
+    RB_HIP(hipMemcpyAsync(hit, st.iters, C * sizeof(int), hipMemcpyDeviceToHost, s));
     RB_HIP(hipStreamSynchronize(s));
     int mx = 0, fl = 0;
     double rel = 0;
@@ -2045,10 +2057,10 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
   hipLaunchKernelGGL((k_accum<T64, T32>), dim3(ew_grid(n * C)), dim3(256), 0, s, (int64_t)n * C, D, X,
                      (const int*)nullptr);
   RB_CHECK_LAUNCH();
-  RB_HIP(hipMemcpyAsync(hrr.data(), so.rr, C * sizeof(double), hipMemcpyDeviceToHost, s));
-  RB_HIP(hipMemcpyAsync(hbb.data(), so.bb, C * sizeof(double), hipMemcpyDeviceToHost, s));
-  RB_HIP(hipMemcpyAsync(hflag.data(), si.flag, C * sizeof(int), hipMemcpyDeviceToHost, s));
-  RB_HIP(hipMemcpyAsync(hit.data(), si.iters, C * sizeof(int), hipMemcpyDeviceToHost, s));
+  RB_HIP(hipMemcpyAsync(hrr, so.rr, C * sizeof(double), hipMemcpyDeviceToHost, s));
+  RB_HIP(hipMemcpyAsync(hbb, so.bb, C * sizeof(double), hipMemcpyDeviceToHost, s));
+  RB_HIP(hipMemcpyAsync(hflag, si.flag, C * sizeof(int), hipMemcpyDeviceToHost, s));
+  RB_HIP(hipMemcpyAsync(hit, si.iters, C * sizeof(int), hipMemcpyDeviceToHost, s));
   RB_HIP(hipStreamSynchronize(s));
   int fl = 0, mx = 0;
   double rel = 0;
@@ -2096,6 +2108,82 @@ extern "C" int rb_krylov_solve(int64_t n, int64_t nnz, const int32_t* row_ptr, c
                                                  check_every, work, work_bytes, iters_out, relres_out, flag_out, s);
   return solve_impl<double, float>(m, nsys, k, vals64, vals32, B, X, precision, tol, maxiter, use_x0, check_every,
                                    work, work_bytes, iters_out, relres_out, flag_out, s);
+}
+
+// ---------------------------------------------------------------------------
+// rb_krylov_solve_noalloc: the same solve on a fixed per-device scratch array
+// instead of a caller workspace, for callers that cannot allocate.  The
+// array below is one copy per device, held from module load to process end.
+// Host code never uses its name as an address: the device address comes from
+// hipGetSymbolAddress, once per device with that device current, and only
+// that pointer reaches kernels.  Calls that overlap on one device share the
+// array and race; every buffer the solve reads is written by the solve first
+// (fills, the initial residual, the state kernels), as with a fresh workspace.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[RB_NOALLOC_SCRATCH_BYTES];
+
+#define RB_SCRATCH_MAXDEV 64
+static std::mutex g_scratch_mu;
+static void* g_scratch_dev[RB_SCRATCH_MAXDEV];
+
+static int scratch_for_current_device(void** out) {
+  int dev = 0;
+  RB_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= RB_SCRATCH_MAXDEV) {
+    rb_set_error("rb_krylov_solve_noalloc: device %d beyond the %d devices with scratch slots", dev,
+                 RB_SCRATCH_MAXDEV);
+    return RB_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      rb_set_error("rb_krylov_solve_noalloc: hipGetSymbolAddress(scratch) on device %d -> %s", dev,
+                   hipGetErrorString(e));
+      return RB_ERR_HIP;
+    }
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return RB_OK;
+}
+
+struct NoAllocScope {
+  bool prev;
+  NoAllocScope() : prev(t_noalloc) { t_noalloc = true; }
+  ~NoAllocScope() { t_noalloc = prev; }
+};
+
+extern "C" int rb_krylov_solve_noalloc(int64_t n, int64_t nnz, const int32_t* row_ptr, const int32_t* col_idx,
+                                       const int32_t* diag_slot, int nsys, int k, int is_complex,
+                                       const void* vals64, const void* vals32, const void* B, void* X, int precision,
+                                       double tol, int maxiter, int use_x0, int check_every, int* iters_out,
+                                       double* relres_out, int* flag_out, void* stream) {
+  if (n <= 0 || nsys <= 0 || k <= 0 || !vals64 || !B || !X) {
+    rb_set_error("rb_krylov_solve_noalloc: bad arguments");
+    return RB_ERR_ARG;
+  }
+  if ((int64_t)nsys * k > 512) {
+    rb_set_error("rb_krylov_solve_noalloc: %lld columns exceed the 512-column limit of one launch",
+                 (long long)nsys * k);
+    return RB_ERR_SIZE;
+  }
+  size_t need = 0;
+  int rc = rb_krylov_workspace(n, nsys, k, is_complex, precision, &need);
+  if (rc) return rc;
+  if (need > (size_t)RB_NOALLOC_SCRATCH_BYTES) {
+    rb_set_error("rb_krylov_solve_noalloc: this solve needs %zu bytes of scratch, the fixed array holds %zu "
+                 "(use rb_krylov_solve with a workspace)", need, (size_t)RB_NOALLOC_SCRATCH_BYTES);
+    return RB_ERR_SIZE;
+  }
+  void* work = nullptr;
+  rc = scratch_for_current_device(&work);
+  if (rc) return rc;
+  NoAllocScope guard;
+  return rb_krylov_solve(n, nnz, row_ptr, col_idx, diag_slot, nsys, k, is_complex, vals64, vals32, B, X, precision,
+                         tol, maxiter, use_x0, check_every, work, (size_t)RB_NOALLOC_SCRATCH_BYTES, iters_out,
+                         relres_out, flag_out, stream);
 }
 
 // Y = A X for every system (dtype: 0 f64, 1 c128, 2 f32, 3 c64)
