@@ -304,7 +304,8 @@ int rb_bicgstab_solve(int64_t n, int64_t nnz, const int32_t* row_ptr, const int3
  *   rb_jt_cols: u[j] = sum_k JT[k][j] z[k][j/np] (mode 0, z: K x nw)
  *               u[j] = sum_k JT[k][j]^2          (mode 1, the Gram diagonal)
  * rb_jt_cols needs rb_jt_cols_workspace() bytes (per-block partials summed in
- * a fixed order).
+ * a fixed order).  Both need nw >= 1, np >= 1, nw*np <= M <= ldj; mode 0 of
+ * rb_jt_cols needs M == nw*np exactly (RB_ERR_ARG otherwise).
  */
 int rb_jt_rows(int64_t K, int64_t M, const void* JT, int64_t ldj, int in_dtype, int nw, int64_t np,
                const double* v, double* t, void* stream);

@@ -1320,10 +1320,10 @@ def runrecon(cfg, recon, detphi0, sd=None, extin=None, **kw):
         recon["prop"] = recon["prop"].reshape(1, -1)
     resid = np.zeros(maxiter)
     updates = []
-    Aregu = {}  # recon.py:127-133 (an explicit lmat or ltl; the prior() builder is product code)
+    Aregu = {}  # recon.py:127-133 (an explicit lmat, else ltl; the prior() builder is product code)
     if "lmat" in kw:
         Aregu["lmat"] = kw["lmat"]
-    if "ltl" in kw:
+    elif "ltl" in kw:
         Aregu["ltl"] = kw["ltl"]
     for it in range(maxiter):
         if "param" in recon or "prop" in recon:

@@ -427,15 +427,40 @@ def bicgstab_solve(row_ptr, col_idx, diag, n, nnz, vals64, B, is_complex, tol=1e
     return X, SolveInfo(it.value, rel.value, fl.value)
 
 
+class SolverWarning(RuntimeWarning):
+    """A forward solve stopped before every column reached its tolerance."""
+
+
+def warn_unconverged(info, where):
+    """Warn (SolverWarning) when a solve's flag is nonzero: 1 = iteration limit, 4 = breakdown."""
+    if info is not None and info.flag:
+        import warnings
+
+        why = {1: "reached the iteration limit", 4: "broke down"}.get(info.flag, "failed (flag %d)" % info.flag)
+        warnings.warn("%s: the Krylov solve %s after %d iterations; largest relative residual %.3e, so some "
+                      "fields and detector values are not at the requested tolerance"
+                      % (where, why, info.iters, info.relres), SolverWarning, stacklevel=3)
+
+
 def solve_system(sysm, B, k, precision="fp64", tol=1e-12, maxiter=2000, X0=None):
-    """CG/COCG over every system; a breakdown (flag 4) of a single-system solve is
-    finished by BiCGSTAB from the CG iterate."""
+    """CG/COCG over every system; after a breakdown (flag 4) every system is finished by
+    BiCGSTAB from its CG iterate (its columns that had converged stop at once)."""
     m = sysm.mesh
     X, info = krylov_solve(m.row_ptr, m.col_idx, m.diag, m.nn, m.nnz, sysm.vals64, sysm.vals32, B, k, sysm.nsys,
                            sysm.is_complex, precision, tol, maxiter, X0)
-    if info.flag == 4 and sysm.nsys == 1:
-        X, info = bicgstab_solve(m.row_ptr, m.col_idx, m.diag, m.nn, m.nnz, sysm.vals64, B, sysm.is_complex, tol,
-                                 maxiter, X)
+    if info.flag == 4:
+        if sysm.nsys == 1:
+            return bicgstab_solve(m.row_ptr, m.col_idx, m.diag, m.nn, m.nnz, sysm.vals64, B, sysm.is_complex, tol,
+                                  maxiter, X)
+        parts = []
+        for i in range(sysm.nsys):
+            sl = slice(i * k, (i + 1) * k)
+            Xi, inf = bicgstab_solve(m.row_ptr, m.col_idx, m.diag, m.nn, m.nnz, sysm.vals64[i], B[:, sl].contiguous(),
+                                     sysm.is_complex, tol, maxiter, X[:, sl].contiguous())
+            X[:, sl] = Xi
+            parts.append(inf)
+        info = SolveInfo(info.iters + max(p.iters for p in parts), max(p.relres for p in parts),
+                         max(p.flag for p in parts))
     return X, info
 
 

@@ -451,7 +451,11 @@ __global__ void k_sum_parts(int64_t M, int nparts, const double* __restrict__ pa
 extern "C" int rb_jt_rows(int64_t K, int64_t M, const void* JT, int64_t ldj, int in_dtype, int nw, int64_t np,
                           const double* v, double* t, void* stream) {
   hipStream_t s = (hipStream_t)stream;
-  if (K <= 0 || nw * np > M) return RB_ERR_ARG;
+  if (K <= 0 || nw <= 0 || np <= 0 || nw * np > M || ldj < M || !JT || !v || !t) {
+    rb_set_error("rb_jt_rows: bad sizes (K %lld, M %lld, ldj %lld, nw %d, np %lld)", (long long)K, (long long)M,
+                 (long long)ldj, nw, (long long)np);
+    return RB_ERR_ARG;
+  }
   unsigned g = (unsigned)ceil_div(K, 4);
   if (in_dtype == RB_F64)
     hipLaunchKernelGGL(k_jt_rows<double>, dim3(g), dim3(256), 0, s, K, M, (const double*)JT, ldj, nw, np, v, t);
@@ -473,9 +477,16 @@ extern "C" int rb_jt_cols(int64_t K, int64_t M, const void* JT, int64_t ldj, int
                           const double* z, int mode, double* u, void* work, size_t work_bytes, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   int64_t nparts = ceil_div(K, JCR);
-  if (K <= 0 || nw * np > M || (size_t)nparts * M * sizeof(double) > work_bytes || !work) {
-    rb_set_error("rb_jt_cols: bad sizes or workspace");
+  // mode 0 reads z[k][j / np] for every column j < M: M must be exactly nw blocks of np
+  if (K <= 0 || M <= 0 || nw <= 0 || np <= 0 || nw * np > M || (mode == 0 && (nw * np != M || !z)) ||
+      (mode != 0 && mode != 1) || ldj < M || !JT || !u) {
+    rb_set_error("rb_jt_cols: bad sizes (K %lld, M %lld, ldj %lld, nw %d, np %lld, mode %d)", (long long)K,
+                 (long long)M, (long long)ldj, nw, (long long)np, mode);
     return RB_ERR_ARG;
+  }
+  if ((size_t)nparts * M * sizeof(double) > work_bytes || !work) {
+    rb_set_error("rb_jt_cols: workspace of %zu bytes, %zu needed", work_bytes, (size_t)nparts * M * sizeof(double));
+    return RB_ERR_SIZE;
   }
   dim3 g((unsigned)ceil_div(M, 256), (unsigned)nparts);
   if (in_dtype == RB_F64)

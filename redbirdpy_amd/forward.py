@@ -241,8 +241,9 @@ def forward_batch(cfgs, precision="fp64", tol=1e-12, maxiter=None, jacobian=Fals
     assembled as one block-diagonal system, and every mesh puts its sources
     in the shared columns [0, TS) and its detectors in [TS, TS + TD) of its
     own rows.  The solve runs to tol / sqrt(number of meshes) relative to a
-    column's norm over the batch, which bounds each mesh's own relative
-    residual by tol.  Returns a BatchResult per cfg.
+    column's norm over the batch after every mesh's part of every column is
+    scaled to unit norm, which bounds each mesh's own relative residual by
+    tol whatever the meshes' RHS norms.  Returns a BatchResult per cfg.
     """
     if not cfgs:
         return []
@@ -321,7 +322,21 @@ def forward_batch(cfgs, precision="fp64", tol=1e-12, maxiter=None, jacobian=Fals
                                                                        device=mesh.dev, dtype=dt)
     if maxiter is None:
         maxiter = max(4000, 4 * int(np.sqrt(mesh.nn)) + 1000)
+    # every mesh's part of every column scaled to unit norm (the system is block-diagonal, so
+    # the solution block scales alike): a column's residual over the batch, <= tol/sqrt(M)
+    # times its norm sqrt(#meshes using it), then bounds each mesh's own relative residual by tol
+    row_blocks = [torch.as_tensor(mesh.iperm[noff[m] : noff[m] + np.asarray(c["node"]).shape[0]], device=mesh.dev)
+                  for m, c in enumerate(cfgs)]
+    scales = []
+    for rows in row_blocks:
+        nrm = torch.linalg.norm(B[rows], dim=0)
+        sc = torch.where(nrm > 0, nrm, torch.ones_like(nrm))
+        B[rows] = B[rows] / sc
+        scales.append(sc)
     X, info = _gpu.solve_system(sysm, B, C, precision, tol / np.sqrt(len(cfgs)), maxiter)
+    for rows, sc in zip(row_blocks, scales):
+        X[rows] = X[rows] * sc
+    _gpu.warn_unconverged(info, "forward_batch")
     Xu = mesh.user_rows(X)
     out = []
     for m, op in enumerate(ops):
@@ -342,17 +357,32 @@ def forward_batch(cfgs, precision="fp64", tol=1e-12, maxiter=None, jacobian=Fals
     return out
 
 
+def check_inputs(cfg, wvs):
+    """ValueError for NaN/Inf optical properties or optode positions (before any launch)."""
+    for wv in wvs:
+        props = _gpu.props_for(cfg, wv, 1)[0]
+        if not np.all(np.isfinite(props)):
+            raise ValueError("optical properties%s hold NaN or Inf" % ((" of wavelength %s" % wv) if wv else ""))
+    for key in ("srcpos", "detpos", "srcdir", "detdir", "widesrc", "widedet"):
+        v = cfg.get(key)
+        if v is not None and np.size(v) and not np.all(np.isfinite(np.asarray(v, dtype=float))):
+            raise ValueError("cfg['%s'] holds NaN or Inf" % key)
+
+
 def forward_device(cfg, rfcw=(1,), precision="fp64", tol=1e-12, maxiter=None, X0=None, optodes=None,
-                   keep_optodes=False):
+                   keep_optodes=False, where="runforward"):
     """Assemble, build RHS and solve all (wavelength, mode) systems on the GPU.
 
     optodes: located optodes of an earlier call, reused where their moved
     positions are unchanged (or as given, with keep_optodes=True: the
     optodes then stay where they were when the optical properties change).
+    A solve that stops short of ``tol`` (iteration limit, breakdown) raises
+    a SolverWarning naming ``where``; its flag is in the result's info.
     """
     if cfg.get("deldotdel") is None:
         cfg["deldotdel"], _ = deldotdel(cfg)
     wvs = list(cfg["prop"].keys()) if isinstance(cfg.get("prop"), dict) else [""]
+    check_inputs(cfg, wvs)
     systems = [(wv, md) for wv in wvs for md in rfcw]
     mesh = _gpu.mesh_for(cfg)
     sysm = _gpu.assemble(mesh, cfg, cfg["deldotdel"], systems, want32=(precision == "mixed"))
@@ -390,7 +420,8 @@ def forward_device(cfg, rfcw=(1,), precision="fp64", tol=1e-12, maxiter=None, X0
             x0 = None if X0 is None else X0[:, i * k : (i + 1) * k]
             Xi, inf = _gpu.solve_system(sub, B[:, i * k : (i + 1) * k].contiguous(), k, precision, tol, maxiter, x0)
             X[:, i * k : (i + 1) * k] = Xi
-            info = inf if info is None or inf.relres > info.relres else info
+            info = inf if info is None or (inf.flag, inf.relres) > (info.flag, info.relres) else info
+    _gpu.warn_unconverged(info, where)
     detvals = []
     for i, (wv, md) in enumerate(systems):
         op = optodes[wv]

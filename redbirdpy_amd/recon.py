@@ -20,6 +20,8 @@ device:
 __all__ = ["runrecon", "reginv", "reginvover", "reginvunder", "matreform", "matflat", "syncprop", "prior"]
 
 import time
+from collections.abc import Mapping
+
 import numpy as np
 import torch
 
@@ -181,31 +183,122 @@ def _chol_solve_normalized(H, rhs):
     return di * z
 
 
-def reginvunder(Amat, rhs, lambda_, invR=None, blocks=None, **kwargs):
-    """x = A^T (A A^T + lambda I)^-1 rhs (recon.py:429-509, no prior transform)."""
-    if invR is not None:
-        raise NotImplementedError("prior transforms are not supported")
+def _dense_inputs(Amat, rhs):
+    """Device copies of a dense A and rhs (float64, or complex128 for complex input)."""
     dev = _gpu.device()
-    A = torch.as_tensor(np.ascontiguousarray(Amat, dtype=np.float64), device=dev)
-    b = torch.as_tensor(np.asarray(rhs, dtype=np.float64).ravel(), device=dev)
-    AT = A.t().contiguous()  # (N x m) node-major, the Gram kernel's layout
-    H = _gpu.gram(AT)
-    H.diagonal().add_(lambda_)
-    y = _chol_solve_normalized(H, b)
-    return _gpu.jt_rows(AT, 1, AT.shape[1], y)[:, 0].cpu().numpy()
+    cplx = np.iscomplexobj(Amat) or np.iscomplexobj(rhs)
+    dt = np.complex128 if cplx else np.float64
+    A = torch.as_tensor(np.ascontiguousarray(Amat, dtype=dt), device=dev)
+    b = torch.as_tensor(np.ascontiguousarray(np.asarray(rhs, dtype=dt).ravel()), device=dev)
+    return A, b, cplx
+
+
+def _drop_zero(A, b):
+    """Columns and rows of A that are all zero (recon.py:375-390, :441-455): (A', b', kept columns)."""
+    n0 = A.shape[1]
+    keep = torch.nonzero(torch.sum(torch.abs(A), 0) != 0).reshape(-1)
+    if keep.numel() < n0:
+        A = A[:, keep]
+    rows = torch.sum(torch.abs(A), 1) != 0
+    if int(rows.sum()) < A.shape[0]:
+        A, b = A[rows], b[rows]
+    return A, b, keep, n0
+
+
+def _solve_sym(H, g, cplx):
+    """H^-1 g in the unit-diagonal form (recon.py:784-789): Cholesky for real SPD H, an LU solve for
+    the complex symmetric H of complex data."""
+    if not cplx:
+        return _chol_solve_normalized(H, g)
+    Hn, di = _normalize_diag(H)
+    return di * torch.linalg.solve(Hn, (di * g).reshape(-1, 1)).reshape(-1)
+
+
+def _spans(ncols, nx, blocks):
+    """Column ranges an R^-1 (or L^T L) of size nx applies to: all columns, or the unknown blocks of
+    that size (recon.py:458-470)."""
+    if nx == ncols:
+        return [(0, nx)]
+    if blocks is None:
+        return []
+    cum = np.cumsum([0] + [int(v[1]) for v in blocks.values()])
+    return [(int(cum[i]), int(cum[i + 1])) for i in range(len(cum) - 1) if cum[i + 1] - cum[i] == nx]
+
+
+def reginvunder(Amat, rhs, lambda_, invR=None, blocks=None, **kwargs):
+    """x = R^-1 A'^T (A' A'^T + lambda I)^-1 rhs with A' = A R^-1 (recon.py:429-509).
+
+    invR (the inverse of the prior's R factor) transforms all columns when
+    its size matches, else each unknown block of its size listed in
+    ``blocks`` ({name: (rows, cols)}); without it, x = A^T (A A^T + lambda I)^-1 rhs.
+    Zero columns and rows are dropped first, as in the reference.
+    """
+    A, b, cplx = _dense_inputs(Amat, rhs)
+    A, b, keep, n0 = _drop_zero(A, b)
+    spans = []
+    if invR is not None:
+        R = torch.as_tensor(np.asarray(invR, dtype=np.float64), device=A.device).to(A.dtype)
+        spans = _spans(A.shape[1], R.shape[0], blocks)
+        A = A.clone()
+        for a, e in spans:
+            A[:, a:e] = A[:, a:e] @ R
+    if cplx:
+        H = A @ A.t()  # unconjugated, as the reference's Amat @ Amat.T
+        H.diagonal().add_(lambda_)
+        res = A.t() @ _solve_sym(H, b, True)
+    else:
+        AT = A.t().contiguous()  # (N x m) node-major, the Gram kernel's layout
+        H = _gpu.gram(AT)
+        H.diagonal().add_(lambda_)
+        y = _chol_solve_normalized(H, b)
+        res = _gpu.jt_rows(AT, 1, AT.shape[1], y)[:, 0]
+    for a, e in spans:
+        res[a:e] = R @ res[a:e]
+    if keep.numel() < n0:
+        full = torch.zeros(n0, dtype=res.dtype, device=res.device)
+        full[keep] = res
+        res = full
+    return res.cpu().numpy()
 
 
 def reginvover(Amat, rhs, lambda_, LTL=None, blocks=None, **kwargs):
-    """x = (A^T A + lambda I)^-1 A^T rhs (recon.py:363-426, identity regulariser)."""
+    """x = (A^T A + lambda (I | L^T L))^-1 A^T rhs (recon.py:363-426).
+
+    An L^T L of the Hessian's size is added whole, a smaller one to each
+    diagonal block of its size; zero columns (and the matching L^T L rows)
+    and zero rows are dropped first, as in the reference.
+    """
+    A, b, cplx = _dense_inputs(Amat, rhs)
+    A, b, keep, n0 = _drop_zero(A, b)
     if LTL is not None:
-        raise NotImplementedError("prior matrices are not supported")
-    dev = _gpu.device()
-    A = torch.as_tensor(np.ascontiguousarray(Amat, dtype=np.float64), device=dev)
-    b = torch.as_tensor(np.asarray(rhs, dtype=np.float64).ravel(), device=dev)
-    H = _gpu.gram(A)
-    H.diagonal().add_(lambda_)
-    g = _gpu.jt_cols(A, 1, A.shape[1], b.reshape(-1, 1))  # A^T b
-    return _chol_solve_normalized(H, g).cpu().numpy()
+        LTL = np.asarray(LTL.toarray() if hasattr(LTL, "toarray") else LTL, dtype=np.float64)
+        if keep.numel() < n0 and LTL.shape[0] > keep.numel():
+            li = keep.cpu().numpy()
+            li = li[li < LTL.shape[0]]
+            LTL = LTL[np.ix_(li, li)]
+    if cplx:
+        H = A.t() @ A
+        g = A.t() @ b
+    else:
+        H = _gpu.gram(A)
+        g = _gpu.jt_cols(A, 1, A.shape[1], b.reshape(-1, 1))  # A^T b
+    if LTL is None:
+        H.diagonal().add_(lambda_)
+    else:
+        Lt = torch.as_tensor(LTL, device=A.device).to(H.dtype)
+        if Lt.shape[0] == H.shape[0]:
+            H = H + lambda_ * Lt
+        else:
+            nx = Lt.shape[0]
+            for i in range(0, H.shape[0], nx):
+                e = min(i + nx, H.shape[0])
+                H[i:e, i:e] += lambda_ * Lt[: e - i, : e - i]
+    x = _solve_sym(H, g, cplx)
+    if keep.numel() < n0:
+        full = torch.zeros(n0, dtype=x.dtype, device=x.device)
+        full[keep] = x
+        x = full
+    return x.cpu().numpy()
 
 
 def reginv(Amat, rhs, lambda_, Areg=None, blocks=None, **kwargs):
@@ -277,16 +370,29 @@ class GNStep:
         tq = mk("gn_gram", tq)
         H = s2 * (G * Wk)
         del G
-        y = self._factor_refine(H, misfit, lambda v: self._hv(v, s2, W), mk, tq)
+        y = self._factor_refine(H, misfit, lambda v: self._hv(v, s2, W), mk, tq,
+                                exact=lambda: s2 * (_gpu.gram(JT, None, "fp64") * Wk))
         T = _gpu.jt_rows(JT, nw, npairs, y)  # (nn, nw) = J_w^T y_w
         return s2 * (T @ E)
 
-    def _factor_refine(self, H, misfit, hv, mk, tq):
+    def _factor_refine(self, H, misfit, hv, mk, tq, exact=None):
         """y = (H + lambda I)^-1 misfit: Cholesky of the unit-diagonal form; the
-        mixed path refines with the float64 operator hv."""
+        mixed path refines with the float64 operator hv.  When the bf16x3 Gram
+        is not positive definite (its ~4e-6 entry error can exceed a tiny
+        lambda on a rank-deficient J J^T), ``exact()`` gives the float64 Gram
+        form of H to factor instead."""
         H.diagonal().add_(self.lam)
         Hn, di = _normalize_diag(H)
-        L = _gpu.cholesky(Hn)
+        try:
+            L = _gpu.cholesky(Hn)
+        except torch.linalg.LinAlgError:
+            if self.precision == "fp64" or exact is None:
+                raise
+            H = exact()
+            H.diagonal().add_(self.lam)
+            Hn, di = _normalize_diag(H)
+            L = _gpu.cholesky(Hn)
+            self.info["fp64_gram_fallback"] = True
         if self.precision == "fp64":
 
             def hsolve(r):
@@ -319,6 +425,12 @@ class GNStep:
                     break
                 y = y + hsolve(res)
             self.info["refine_relres"] = rel
+            if rel > self.reftol:
+                import warnings
+
+                warnings.warn("Gauss-Newton step: iterative refinement stopped at relative residual %.3e after "
+                              "%d passes (target %.1e)" % (rel, self.maxref, self.reftol), _gpu.SolverWarning,
+                              stacklevel=3)
         self.info["refinements"] = nref
         mk("gn_refine", tq)
         return y
@@ -367,7 +479,13 @@ class GNStepReIm(GNStep):
         H[m:, :m] = ir - ri
         H.mul_(s2)
         del G
-        y = self._factor_refine(H, misfit, lambda v: self._hv(v, s2), mk, tq)
+
+        def exact():
+            Ge = _gpu.gram(JT, None, "fp64")
+            rr_, ri_, ir_, ii_ = Ge[:m, :m], Ge[:m, m:], Ge[m:, :m], Ge[m:, m:]
+            return s2 * torch.cat([torch.cat([rr_ + ii_, ri_ - ir_], 1), torch.cat([ir_ - ri_, rr_ + ii_], 1)], 0)
+
+        y = self._factor_refine(H, misfit, lambda v: self._hv(v, s2), mk, tq, exact=exact)
         return s2 * _gpu.jt_rows(JT, 1, 2 * m, y)  # (nn, 1) = x
 
 
@@ -413,6 +531,48 @@ def _solve_over(JG, npairs, E, lam, misfit, LTL=None):
     return (s * x).reshape(nc, N).t()
 
 
+class JacobianDict(Mapping):
+    """runrecon's ``Jmua`` (recon.py:155-179, returned at :335): the node Jacobians of the
+    last iteration's forward fields, keyed like the reference -- chromophore -> the
+    wavelength-stacked E[w, c] J_w (jacchrome, forward.py:572-589) for multispectral
+    unknowns, else 'mua' (one wavelength) or the wavelength.  Each entry is a fresh
+    host array (pairs x nodes, user node order; complex for frequency-domain data),
+    built on access from the device-resident fields by the Jacobian kernel, so the
+    several-GB matrices exist on the host only when asked for.
+    """
+
+    def __init__(self, mesh, X, k, wvs, pairs, cplx, chs=None, E=None):
+        self._mesh, self._X, self._k = mesh, X, k
+        self._wvs, self._pairs, self._cplx = list(wvs), pairs, cplx
+        self._chs = list(chs) if chs else None
+        self._E = None if E is None else np.asarray(E, dtype=float)
+        self._keys = self._chs or (["mua"] if self._wvs == [""] else self._wvs)
+
+    def _jw(self, i):
+        s, d = self._pairs[i]
+        dt = torch.complex128 if self._cplx else torch.float64
+        JT, _ = _gpu.jac_node(self._mesh, self._X[:, i * self._k : (i + 1) * self._k], s, d, dt)
+        return self._mesh.user_rows(JT).t().cpu().numpy()
+
+    def __getitem__(self, key):
+        if key not in self._keys:
+            raise KeyError(key)
+        if self._chs:
+            c = self._chs.index(key)
+            return np.vstack([self._E[i, c] * self._jw(i) for i in range(len(self._wvs))])
+        return self._jw(0 if key == "mua" else self._wvs.index(key))
+
+    def __iter__(self):
+        return iter(self._keys)
+
+    def __len__(self):
+        return len(self._keys)
+
+
+# parameters runrecon updates (recon.py:278); others (aa3) keep their Jacobian block but no update
+_UPDATED_PARAMS = ("hbo", "hbr", "water", "lipids", "scatamp", "scatpow")
+
+
 def _chromophores(cfg):
     if isinstance(cfg.get("prop"), dict) and "param" in cfg:
         return [k for k in cfg["param"].keys() if k in CHROMOPHORES]
@@ -441,8 +601,9 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
     leaves values below ~solvertol of the peak at exactly 0, where
     'logphase' is undefined).
     Returns (recon, resid, cfg, updates, Jmua, detphi, phi) like the reference;
-    Jmua holds the node Jacobians per wavelength (numpy) only when
-    ``return_jac=True`` (they can be several GB).
+    Jmua is a JacobianDict with the reference's keys (chromophores, or 'mua',
+    or wavelengths) whose entries are materialised on access (they can be
+    several GB); ``return_jac=True`` materialises them all into a dict.
     """
     maxiter = kwargs.get("maxiter", 5)
     lam = kwargs.get("lambda_", recon.get("lambda", 0.05))
@@ -461,12 +622,12 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
     if rfcw != [1] or reform not in ("real", "reim", "complex", "logphase"):
         raise NotImplementedError("GPU runrecon supports rfcw=[1] with reform 'real', 'reim', 'logphase' "
                                   "(or 'complex' on CW)")
-    Aregu = {}  # recon.py:127-133
+    Aregu = {}  # recon.py:127-133: an lmat keyword, else ltl, else a prior built from recon['seg']
     if "lmat" in kwargs:
         Aregu["lmat"] = np.asarray(kwargs["lmat"], dtype=float)
-    if "ltl" in kwargs:
+    elif "ltl" in kwargs:
         Aregu["ltl"] = np.asarray(kwargs["ltl"], dtype=float)
-    elif "lmat" not in kwargs and kwargs.get("prior") and "seg" in recon:
+    elif kwargs.get("prior") and "seg" in recon:
         lm = prior(recon["seg"], kwargs["prior"], kwargs)
         if lm is not None:
             Aregu["lmat"] = lm.toarray() if hasattr(lm, "toarray") else np.asarray(lm, dtype=float)
@@ -480,7 +641,6 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
     detphi_out = phi_out = Jm_out = None
     timing = kwargs.get("timing", None)  # dict: per-phase seconds (synchronising) when given
     dual = "node" in recon and "elem" in recon and "mapid" in recon
-    jac_keep = {}  # unscaled (complex) Jacobians of the last iteration, for return_jac
     remap = None  # device CSR of the dual-mesh Jacobian map, built once
 
     def mark(name, t_prev):
@@ -499,7 +659,8 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
         if "param" in cfg and isinstance(cfg.get("prop"), dict):
             cfg["prop"] = updateprop(cfg, extin=extin)
         tp = mark("props", tp)
-        res = forward_device(cfg, rfcw, precision, stol, None, X0 if warm else None, optodes=optodes)
+        res = forward_device(cfg, rfcw, precision, stol, None, X0 if warm else None, optodes=optodes,
+                             where="runrecon iteration %d" % (it + 1))
         optodes = res.optodes  # reused next iteration where the moved optode positions are unchanged
         X0 = res.X
         tp = mark("forward", tp)
@@ -547,8 +708,6 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
                 _gpu.jac_node(mesh, res.phi(i), s, d, jdt, out=JT[:, blk])
                 continue
             Jw, _ = _gpu.jac_node(mesh, res.phi(i), s, d, torch.complex128 if cplx else jdt)
-            if return_jac:
-                jac_keep["mua" if wv == "" else wv] = mesh.user_rows(Jw).t().cpu().numpy()
             if rowscale is not None:
                 sc = _gpu.h2d(rowscale[blk] if cplx else np.real(rowscale[blk]), mesh.dev)
                 Jw = Jw * sc[None, :]
@@ -656,7 +815,7 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
                             pr[li, 0] += dx[li]
                     else:
                         pr[:, 0] += dx
-            else:
+            elif key in _UPDATED_PARAMS:
                 tgt = recon if ("param" in recon and key in recon["param"]) else cfg
                 if "param" not in tgt or key not in tgt["param"]:
                     continue
@@ -682,10 +841,10 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
         if it > 0 and abs(resid[it] - resid[it - 1]) / resid[0] < tol:
             resid = resid[: it + 1]
             break
+    pairs = [_active_pairs(sd.get(wv, sd) if isinstance(sd, dict) else sd) for wv in wvs]
+    Jm_out = JacobianDict(mesh, X0, res.k, wvs, pairs, res.is_complex, chs or None, E[: len(wvs)] if chs else None)
     if return_jac:
-        Jm_out = jac_keep or {
-            ("mua" if wv == "" else wv): mesh.user_rows(JT[:, i * npairs : (i + 1) * npairs]).t().cpu().numpy()
-            for i, wv in enumerate(wvs)}
+        Jm_out = dict(Jm_out.items())
     tp = time.perf_counter()
     Xh = _gpu.to_host(mesh.user_rows(X0))
     mark("outputs", tp)

@@ -61,3 +61,17 @@ def test_compute_entry_points_need_a_gpu():
         pytest.skip("GPU present")
     with pytest.raises(_abi.RbError):
         _abi.require_gpu()
+
+
+def test_jt_products_reject_bad_shapes_before_any_launch():
+    """rb_jt_cols mode 0 reads z[k][j / np] for every column j < M: M must be nw blocks of np, and
+    np >= 1 (checked on the host; the fake pointers below are never dereferenced)."""
+    import ctypes
+
+    L = _abi.load()
+    fake = ctypes.c_void_p(256)
+    for K, M, nw, npr, mode in ((10, 8, 2, 0, 0), (10, 8, 1, 4, 0), (10, 8, 3, 4, 0), (10, 8, 2, 4, 2)):
+        rc = L.rb_jt_cols(K, M, fake, M, _abi.RB_F64, nw, npr, fake, mode, fake, fake, 1 << 20, None)
+        assert rc == -1, (K, M, nw, npr, mode)
+    assert L.rb_jt_rows(10, 8, fake, 8, _abi.RB_F64, 2, 0, fake, fake, None) == -1
+    assert L.rb_jt_rows(10, 8, fake, 4, _abi.RB_F64, 2, 4, fake, fake, None) == -1  # ldj < M

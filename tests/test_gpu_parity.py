@@ -210,10 +210,13 @@ def test_femsolve_long_rows_and_widths(ncol):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("cap", ["8", None])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_femsolve_sweep_many_chunk_groups(precision, monkeypatch):
+def test_femsolve_sweep_many_chunk_groups(precision, cap, monkeypatch):
     """A capped SpMM grid gives each wavefront hundreds of row chunks (many
-    row-pointer groups per wavefront) on the FD box at 2 mm spacing."""
+    row-pointer groups per wavefront) on the FD box at 2 mm spacing; uncapped,
+    the grid is the default resident round (the mixed complex case is the
+    benchmark's k_cg_spmm_sweep_ru<complex double, complex float, 1> kernel)."""
     import redbirdpy_amd as rb
     from scipy.sparse.linalg import spsolve
     from redbirdpy_amd import workloads
@@ -225,7 +228,10 @@ def test_femsolve_sweep_many_chunk_groups(precision, monkeypatch):
     rng = np.random.default_rng(7)
     b = rng.standard_normal((n, 64)) + 1j * rng.standard_normal((n, 64))
     xr = spsolve(A.tocsc(), b)
-    monkeypatch.setenv("RB_SWEEP_MAX_GRID", "8")
+    if cap:
+        monkeypatch.setenv("RB_SWEEP_MAX_GRID", cap)
+    else:
+        monkeypatch.delenv("RB_SWEEP_MAX_GRID", raising=False)
     x, flag = rb.femsolve(A, b, tol=1e-12, precision=precision)
     assert flag == 0
     assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-9

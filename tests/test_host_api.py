@@ -1,5 +1,6 @@
 """Host-side helpers of the drop-in API (no GPU): prior, addnoise, musp2sasp, exports."""
 
+import copy
 import warnings
 
 import numpy as np
@@ -65,3 +66,39 @@ def test_reference_exports_present():
                  "getoptodes", "getreff", "getbulk", "updateprop", "extinction", "reginv", "matreform", "matflat",
                  "syncprop", "run"):
         assert hasattr(rb, name), name
+
+
+def test_extinction_updateprop_jacchrome_match_oracle_default_table():
+    """The built-in chromophore table path (property.py:30-213, forward.py:572-589): extinction with
+    interpolation and extrapolation, updateprop with its water/lipids defaults on label and node
+    params, and jacchrome's wavelength-stacked E[w, c] J_w, against the oracle."""
+    import redbirdpy_amd as rb
+    from oracle import rb_oracle as orc
+    from redbirdpy_amd.mesh import kuhn_box
+
+    wvs = ["650", "690", "785", "808", "830", "905", "1050"]
+    chs = ["hbo", "hbr", "water", "lipids", "aa3"]
+    np.testing.assert_allclose(rb.extinction(wvs, chs)[0], orc.extinction(wvs, chs)[0], rtol=1e-14)
+    node, elem = kuhn_box([0, 0, 0], [4, 4, 2], 1.0)
+    nn = node.shape[0]
+    rng = np.random.default_rng(4)
+    tab = np.array([[0, 0, 1, 1], [0.01, 1.0, 0.0, 1.37], [0.02, 1.2, 0.0, 1.37]])
+    for param in ({"hbo": rng.uniform(10, 30, nn), "hbr": rng.uniform(5, 15, nn)},
+                  {"hbo": np.array([20.0, 40.0]), "hbr": np.array([10.0, 12.0]), "aa3": np.array([1.0, 2.0])},
+                  {"hbo": rng.uniform(10, 30, nn), "hbr": rng.uniform(5, 15, nn),
+                   "scatamp": np.full(nn, 1.5), "scatpow": np.full(nn, 1.1)}):
+        c1 = {"node": node, "elem": elem, "prop": {w: tab.copy() for w in wvs[:3]}, "param": copy.deepcopy(param)}
+        c2 = copy.deepcopy(c1)
+        p1, p2 = rb.updateprop(c1), orc.updateprop(c2)
+        assert list(p1) == list(p2) == wvs[:3]
+        for w in wvs[:3]:
+            np.testing.assert_allclose(p1[w], p2[w], rtol=1e-14)
+        assert c1["param"]["water"] == 0.23 and c1["param"]["lipids"] == 0.58
+    J = {w: rng.standard_normal((6, nn)) for w in wvs[:3]}
+    Jc1 = rb.jacchrome(J, ["hbo", "hbr", "water"])
+    Jc2 = orc.jacchrome(J, ["hbo", "hbr", "water"])
+    for k in Jc2:
+        assert Jc1[k].shape == (18, nn)
+        np.testing.assert_allclose(Jc1[k], Jc2[k], rtol=1e-15)
+    with pytest.raises(ValueError):
+        rb.jacchrome(J["650"], ["hbo"])
