@@ -69,29 +69,134 @@ def _spmm_bytes(nn, nnz, C, sv):
     return nnz * (sv + 4) + (nn + 1) * 4 + 2 * nn * C * sv
 
 
+def cpu_threads():
+    """Host threads of the CPU baseline: every core this process may run on, capped by OMP_NUM_THREADS
+    when the host sets it (the GPU box gives one GPU's share of its cores)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return n
+
+
+def _blas_threads(n):
+    """Context that runs numpy/scipy BLAS on n threads (numpy is imported before the environment could say)."""
+    from threadpoolctl import threadpool_limits
+
+    return threadpool_limits(limits=n)
+
+
 def cpu_baseline(h):
     """Oracle (reference algorithm, scipy SuperLU) on the same FD box at spacing h."""
     from redbirdpy_amd import workloads
     from oracle import rb_oracle as orc
 
+    n = cpu_threads()
     cfg = workloads.fd_box(h)
     cfg, sd = orc.meshprep(cfg)
     rhs, loc, bary, _ = orc.femrhs(cfg, sd)
-    t0 = time.perf_counter()
-    A = orc.femlhs(cfg, cfg["deldotdel"])
-    x, _ = orc.femsolve(A, rhs)
-    det = orc.femgetdet(x, cfg, rhs, loc, bary)
-    dt = time.perf_counter() - t0
+    with _blas_threads(n):
+        t0 = time.perf_counter()
+        A = orc.femlhs(cfg, cfg["deldotdel"])
+        x, _ = orc.femsolve(A, rhs)
+        det = orc.femgetdet(x, cfg, rhs, loc, bary)
+        dt = time.perf_counter() - t0
     k = rhs.shape[1]
     return {
         "value": k / dt,
         "unit": "RHS-solves/s",
-        "cores": 1,
+        "cores": n,
         "kind": "port",
         "sample": "oracle femlhs + SuperLU factor/solve of %d complex RHS + femgetdet on the same FD box at %.1f mm "
-                  "(%d nodes), one pass, %.1f s; SuperLU is single-threaded" % (k, h, cfg["node"].shape[0], dt),
+                  "(%d nodes, not the 115k mesh), one pass, %.1f s; BLAS on %d threads, the SuperLU factorisation "
+                  "itself is single-threaded" % (k, h, cfg["node"].shape[0], dt, n),
         "seconds": dt,
         "nodes": int(cfg["node"].shape[0]),
+    }
+
+
+def cpu_gn_baseline(h=2.0, jac_pairs=128):
+    """One Gauss-Newton iteration of the oracle (the reference's default CPU path) on the 4-wavelength,
+    3-chromophore chromophore box at spacing h: runforward (femlhs + SuperLU + femgetdet per wavelength),
+    the numpy jac per wavelength, jacchrome/matflat and reginvunder.  The jac phase is timed on
+    jac_pairs of each wavelength's pairs and scaled to all of them (it is a loop over pairs); every
+    other phase runs in full."""
+    import copy
+    from redbirdpy_amd import workloads
+    from oracle import rb_oracle as orc
+
+    n = cpu_threads()
+    cfg0, bg, truth, extin, E = workloads.chrome_box(h)
+    c, sd = orc.meshprep(copy.deepcopy(cfg0))
+    c["param"] = {k: v.copy() for k, v in truth.items()}
+    c["prop"] = orc.updateprop(c, extin=extin)
+    wvs = list(c["prop"].keys())
+    with _blas_threads(n):
+        t0 = time.perf_counter()
+        det, phi = orc.runforward(c)
+        t_fwd = time.perf_counter() - t0
+        Jm = {}
+        t_jac = 0.0
+        npairs = 0
+        for wv in wvs:
+            sdw = sd[wv] if isinstance(sd, dict) else sd
+            act = sdw[sdw[:, 2] == 1] if sdw.shape[1] >= 3 else sdw
+            npairs = act.shape[0]
+            sub = act[:jac_pairs]
+            t1 = time.perf_counter()
+            Jn, _ = orc.jac(sub, phi[wv], c["deldotdel"], c["elem"], c["evol"])
+            t_jac += (time.perf_counter() - t1) * npairs / sub.shape[0]
+            Jm[wv] = np.tile(Jn, (npairs // sub.shape[0] + 1, 1))[:npairs]  # full-size stand-in for the step
+        chs = list(truth.keys())
+        Emat = np.array([[extin[wv][ch] for ch in chs] for wv in wvs])
+        t2 = time.perf_counter()
+        Jc = orc.jacchrome(Jm, chs, extin=Emat)
+        A = orc.matflat(Jc)
+        y = orc.flatten_detphi(det, sd, wvs, [1])
+        y0 = y * (1 + 0.005 * np.random.default_rng(40).standard_normal(y.shape))
+        A, mis, _ = orc.matreform(A, y0, y, "real")
+        sc = 1.0 / np.sqrt(np.sum(A ** 2))  # the blockscale of recon.py:235-236
+        m = A.shape[0]
+        dmu = orc.reginv(A * sc, mis, 1e-3 / m) * sc
+        t_step = time.perf_counter() - t2
+    t_iter = t_fwd + t_jac + t_step
+    return {
+        "value": 1.0 / t_iter,
+        "unit": "GN iters/s",
+        "cores": n,
+        "kind": "port",
+        "sample": "one oracle Gauss-Newton iteration, 4 wavelengths x %d pairs x 3 chromophores, on the chromophore "
+                  "box at %.1f mm (%d nodes, not the 115k mesh): runforward %.1f s (SuperLU, single-threaded), numpy "
+                  "jac %.1f s (timed on %d pairs per wavelength, scaled to %d), jacchrome + matflat + reginvunder "
+                  "(m=%d, N=%d) %.1f s; BLAS on %d threads" % (
+                      npairs, h, c["node"].shape[0], t_fwd, t_jac, min(jac_pairs, npairs), npairs, m, A.shape[1],
+                      t_step, n),
+        "seconds": t_iter,
+        "nodes": int(c["node"].shape[0]),
+    }
+
+
+def cpu_jac_baseline(phi, sd, cfg, pairs=32):
+    """The oracle's numpy jac (the reference's path without numba) on the 1024-pair CW box given the
+    fields, timed on `pairs` of the pairs and scaled to the full Jacobian."""
+    from oracle import rb_oracle as orc
+
+    n = cpu_threads()
+    act = sd[sd[:, 2] == 1] if sd.shape[1] >= 3 else sd
+    sub = act[:pairs]
+    with _blas_threads(n):
+        t0 = time.perf_counter()
+        orc.jac(sub, phi, cfg["deldotdel"], cfg["elem"], cfg["evol"])
+        dt = time.perf_counter() - t0
+    full = dt * act.shape[0] / sub.shape[0]
+    return {
+        "value": 1.0 / full,
+        "unit": "Jacobians/s",
+        "cores": n,
+        "kind": "port",
+        "sample": "oracle numpy jac (node and element Jacobians) for %d of the %d pairs on the %d-node CW box, "
+                  "%.1f s, scaled to all pairs" % (sub.shape[0], act.shape[0], phi.shape[0], dt),
+        "seconds_full_jacobian": full,
     }
 
 
@@ -147,6 +252,8 @@ def gn_bench(a):
                          / np.max(np.abs(results["fp64"]["param"][k]))) for k in bg}
         out["mixed"]["param_maxrel_err_vs_fp64"] = errs
         out["speedup_mixed_vs_fp64"] = out["mixed"]["gn_iters_per_s"] / out["fp64"]["gn_iters_per_s"]
+    if not a.skip_cpu:
+        out["cpu_baseline"] = cpu_gn_baseline()
     return out
 
 
@@ -217,8 +324,12 @@ def jac_bench(a, reps=5):
     dt = e0.elapsed_time(e1) / 1e3 / reps
     qptr, _ = _gpu.patch_matrices(mesh)
     nbytes = len(s) * mesh.nn * 8 + mesh.nn * res.k * 8 + int(qptr[-1].item()) * 8
-    return {"workload": "CW box 1 mm, %d pairs x %d nodes, float64" % (len(s), mesh.nn), "ms": dt * 1e3,
-            "bytes": nbytes, "achieved_gbs": nbytes / dt / 1e9, "frac_hbm": nbytes / dt / 1e9 / HBM_PEAK_GBS}
+    out = {"workload": "CW box %.1f mm, %d pairs x %d nodes, float64" % (a.h, len(s), mesh.nn), "ms": dt * 1e3,
+           "jacobians_per_s": 1.0 / dt, "bytes": nbytes, "achieved_gbs": nbytes / dt / 1e9,
+           "frac_hbm": nbytes / dt / 1e9 / HBM_PEAK_GBS}
+    if not a.skip_cpu:
+        out["cpu_baseline"] = cpu_jac_baseline(_gpu.to_host(mesh.user_rows(phi)), sd, cfg)
+    return out
 
 
 def main():
@@ -351,6 +462,7 @@ def main():
         out["gram"] = gram_bench(mesh.nn)
     if not a.skip_cpu:
         out["cpu_baseline"] = cpu_baseline(a.cpu_h)
+        out["config"]["cpu_sample_nodes"] = out["cpu_baseline"]["nodes"]
     print(json.dumps(out), flush=True)
 
 

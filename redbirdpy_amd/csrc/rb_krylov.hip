@@ -770,6 +770,175 @@ __device__ __forceinline__ void sweep_pair_body(const Csr& m, const TS* __restri
   block_partials_pair<NCH>(st, pv, C);
 }
 
+// Float32 solves of 64-column systems four at a time: one sweep covers a
+// quad of chunks (256 columns), lane l gathering columns 4l..4l+3 (16 bytes,
+// so one gather instruction brings a whole 1 KB row of the four systems,
+// half the instructions of two pair sweeps).  Lanes 16q..16q+15 belong to
+// system q of the quad and read that system's matrix value of each entry
+// (an LDS read of one float at a per-quarter address).  Schedule, LDS
+// staging and row-end handling as in sweep_pair_body.
+typedef __attribute__((ext_vector_type(4))) float fl4;
+
+template <int NCH>
+__device__ void block_partials_quad(const ColState<double>& st, double (&pv)[NCH / 4][4], int C) {
+  __shared__ double lds[KB / 64][256];
+  const int lane = threadIdx.x & 63, rg = threadIdx.x >> 6;
+  constexpr int RPB = KB / 64;
+#pragma unroll
+  for (int cq = 0; cq < NCH / 4; ++cq) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) lds[rg][4 * lane + e] = pv[cq][e];
+    __syncthreads();
+    for (int i = threadIdx.x; i < 256; i += KB) {
+      const int c = cq * 256 + i;
+      if (c < C) {
+        double acc = lds[0][i];
+        for (int r = 1; r < RPB; ++r) acc = acc + lds[r][i];
+        st.part[(int64_t)c * st.nb + blockIdx.x] = acc;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// FRG 4: eight 1 KB gathers in flight per wavefront, the pair sweep's 8 KB
+template <int NCH, int FRG = 4>
+__device__ __forceinline__ void sweep_quad_body(const Csr& m, const float* __restrict__ vals,
+                                                const float* __restrict__ P, float* __restrict__ Q, int C,
+                                                const ColState<double>& st) {
+  static_assert(NCH % 4 == 0, "quad sweeps cover whole groups of four 64-column systems");
+  __shared__ int32_t s_col[KB / 64][2][64];
+  __shared__ fl4 s_val[KB / 64][2][64];  // the four systems' values of an entry
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int q4 = lane >> 4;  // this lane's system within the quad
+  const int64_t seg = (m.n + NXCD - 1) / NXCD;
+  const int64_t s0 = (int64_t)(blockIdx.x % NXCD) * seg;
+  const int64_t s1 = min(m.n, s0 + seg);
+  const int64_t W = (int64_t)(gridDim.x / NXCD) * (KB / 64);
+  const int64_t wx = (int64_t)(blockIdx.x / NXCD) * (KB / 64) + wave;
+  const int64_t nq = (s1 > s0) ? (s1 - s0 + SRC - 1) / SRC : 0;
+  const int64_t nk = (nq > wx) ? (nq - wx + W - 1) / W : 0;
+  const uint32_t rowb = (uint32_t)C * (uint32_t)sizeof(float);
+  const __amdgpu_buffer_rsrc_t rsrc =
+      __builtin_amdgcn_make_buffer_rsrc((void*)P, (short)0, (int)(m.n * rowb), BUF_WORD3);
+  double pv[NCH / 4][4];
+#pragma unroll
+  for (int cq = 0; cq < NCH / 4; ++cq) pv[cq][0] = pv[cq][1] = pv[cq][2] = pv[cq][3] = 0.0;
+#pragma unroll
+  for (int cq = 0; cq < NCH / 4; ++cq) {
+    const int c = cq * 256 + 4 * lane;  // this lane's first column (C = NCH * 64)
+    if (!__any(st.active[c] || st.active[c + 1] || st.active[c + 2] || st.active[c + 3])) continue;
+    const uint32_t voff = (uint32_t)c * (uint32_t)sizeof(float);
+    const float* vA = vals + (int64_t)(4 * cq) * m.nnz;  // system = chunk
+    const float* vB = vA + m.nnz;
+    const float* vC = vB + m.nnz;
+    const float* vD = vC + m.nnz;
+    for (int64_t g0 = 0; g0 < nk; g0 += SGC) {
+      const int ng = (int)min((int64_t)SGC, nk - g0);
+      int32_t rp = 0;
+      {
+        const int kq = lane / (SRC + 1), o = lane % (SRC + 1);
+        if (kq < ng) rp = m.row_ptr[min(s0 + (wx + (g0 + kq) * W) * SRC + o, s1)];
+      }
+      int kc = 0;
+      int64_t cr0 = 0;
+      int cnr = 0, row = 0;
+      int32_t ce0 = 0, ce1 = 0, rend = 0;
+      fl4 acc = {0, 0, 0, 0}, pown = {0, 0, 0, 0};
+      auto chunk_at = [&](int kq, int64_t& r0, int& nr, int32_t& e0, int32_t& e1) {
+        r0 = s0 + (wx + (g0 + kq) * W) * SRC;
+        nr = (int)min((int64_t)SRC, s1 - r0);
+        e0 = __builtin_amdgcn_readlane(rp, kq * (SRC + 1));
+        e1 = __builtin_amdgcn_readlane(rp, kq * (SRC + 1) + nr);
+      };
+      auto drain = [&](int32_t e) {
+        while (row < cnr && rend == e) {
+          const int64_t o = (cr0 + row) * C + c;
+          *reinterpret_cast<fl4*>(Q + o) = acc;
+          pv[cq][0] += (double)pown.x * (double)acc.x;
+          pv[cq][1] += (double)pown.y * (double)acc.y;
+          pv[cq][2] += (double)pown.z * (double)acc.z;
+          pv[cq][3] += (double)pown.w * (double)acc.w;
+          acc = fl4{0, 0, 0, 0};
+          pown = fl4{0, 0, 0, 0};
+          ++row;
+          rend = (row < cnr) ? __builtin_amdgcn_readlane(rp, kc * (SRC + 1) + row + 1) : -1;
+        }
+      };
+      auto enter_chunk = [&](int kq) {
+        kc = kq;
+        chunk_at(kq, cr0, cnr, ce0, ce1);
+        row = 0;
+        rend = __builtin_amdgcn_readlane(rp, kq * (SRC + 1) + 1);
+        drain(ce0);
+      };
+      auto gather = [&](fl4(&x)[FRG], int sl, int jb) {
+#pragma unroll
+        for (int u = 0; u < FRG; ++u) {
+          const uint32_t cu = __builtin_amdgcn_readfirstlane(s_col[wave][sl][jb * FRG + u]);
+          x[u] = buf_ld<fl4>(rsrc, voff, cu * rowb);
+        }
+      };
+      auto consume = [&](const fl4(&x)[FRG], int sl, int jb, int32_t pb) {
+#pragma unroll
+        for (int u = 0; u < FRG; ++u) {
+          const float v = reinterpret_cast<const float*>(&s_val[wave][sl][jb * FRG + u])[q4];
+          acc = acc + v * x[u];
+          const int32_t cu = __builtin_amdgcn_readfirstlane(s_col[wave][sl][jb * FRG + u]);
+          if (cu == (int32_t)(cr0 + row)) pown = x[u];
+          drain(pb + jb * FRG + u + 1);
+        }
+      };
+      enter_chunk(0);
+      int32_t pb = ce0;
+      int sl = 0;
+      {
+        const int32_t e = pb + lane;
+        s_col[wave][0][lane] = (e < ce1) ? m.col_idx[e] : (int32_t)cr0;
+        s_val[wave][0][lane] = (e < ce1) ? fl4{vA[e], vB[e], vC[e], vD[e]} : fl4{0, 0, 0, 0};
+      }
+      fl4 xa[FRG], xb[FRG];
+      gather(xa, 0, 0);
+      for (;;) {
+        int kn = kc;
+        int64_t nr0 = cr0;
+        int nnr = cnr;
+        int32_t npb = pb + 64, ne0 = ce0, ne1 = ce1;
+        const bool more = npb < ce1;
+        const bool has_next = more || kc + 1 < ng;
+        if (!more && has_next) {
+          kn = kc + 1;
+          chunk_at(kn, nr0, nnr, ne0, ne1);
+          npb = ne0;
+        }
+        const int32_t en = min(npb + lane, max(ne1 - 1, 0));
+        const bool vn = has_next && npb + lane < ne1;
+        const int32_t tcol = m.col_idx[en];
+        const fl4 tval = fl4{vA[en], vB[en], vC[en], vD[en]};
+#pragma unroll
+        for (int pr = 0; pr < 64 / (2 * FRG); ++pr) {
+          gather(xb, sl, 2 * pr + 1);
+          consume(xa, sl, 2 * pr, pb);
+          if (pr + 1 < 64 / (2 * FRG)) {
+            gather(xa, sl, 2 * pr + 2);
+          } else {
+            s_col[wave][sl ^ 1][lane] = vn ? tcol : (int32_t)nr0;
+            s_val[wave][sl ^ 1][lane] = vn ? tval : fl4{0, 0, 0, 0};
+            gather(xa, sl ^ 1, 0);
+          }
+          consume(xb, sl, 2 * pr + 1, pb);
+        }
+        if (!has_next) break;
+        sl ^= 1;
+        pb = npb;
+        if (kn != kc) enter_chunk(kn);
+      }
+    }
+  }
+  block_partials_quad<NCH>(st, pv, C);
+}
+
 template <typename T, int NCH, bool PAIR = false>
 __global__ void __launch_bounds__(KB) k_cg_spmm_sweep(Csr m, const T* __restrict__ vals, const T* __restrict__ P,
                                                       T* __restrict__ Q, int C, int k,
@@ -803,7 +972,10 @@ __global__ void __launch_bounds__(KB) __attribute__((amdgpu_waves_per_eu(5))) k_
   if (!block_any_active(st.active, C)) return;
   if constexpr (PAIR) {
     static_assert(std::is_same<T32, float>::value && NCH % 2 == 0, "pair sweeps are for real 64-column systems");
-    sweep_pair_body<float, NCH>(m, vals32, P, Q, C, st);
+    if constexpr (NCH % 4 == 0)
+      sweep_quad_body<NCH>(m, vals32, P, Q, C, st);  // four float32 systems per 16-byte gather
+    else
+      sweep_pair_body<float, NCH>(m, vals32, P, Q, C, st);
   } else {
     SweepOps<T32, T32> op{vals32, P, nullptr, Q, nullptr, nullptr, nullptr, nullptr};
     sweep_body<T32, T32, NCH, 0>(m, op, C, k, st);
