@@ -15,15 +15,14 @@ extra   : the float64 path (complex128 COCG to 1e-12), the error of the
           iterations/s of the 4-wavelength chromophore reconstruction in
           both precisions.
 roofline: the SpMM kernel of the mixed path (q = A p with the p.q
-          reduction; the same launch also computes the float64 true
-          residual of the column chunks whose reliable update the previous
-          iteration requested, in about 1 launch in 15).  achieved = the
-          SpMM's algorithmic bytes per launch / mean launch duration,
-          measured with start/stop hipEvents carried by each dispatch on
-          its stream over a second run of the same steps (the events are
-          kept out of the timed steps, where they cost ~9 %); the mean
-          includes the residual passes and their bytes are not counted,
-          so the fraction is a lower bound.
+          reduction, LDS-staged: each block of rows copies its neighbour
+          rows of p into LDS once).  achieved = the SpMM's algorithmic
+          bytes per launch / mean launch duration, measured with start/stop
+          hipEvents carried by each dispatch on its stream over a second run
+          of the same steps (the events are kept out of the timed steps,
+          where they cost ~9 %).  The float64 true residual of the column
+          chunks whose reliable update the previous iteration requested runs
+          in its own gated launch right after (reported as resid64).
 """
 
 import argparse
@@ -38,7 +37,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-SPMM_KERNEL = "k_cg_spmm_sweep_ru<cplx<double>,cplx<float>,1>"
+SPMM_KERNEL = "k_cg_spmm_lds<cplx<float>,LdsCfg<128,768,2>>"
 
 
 def _args():
@@ -438,7 +437,7 @@ def main():
             "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated, mean over dispatches "
                               "(profiles/spmm_sweep_fd115k_c64_pmc.md)",
             "traffic_median_launch": traffic_plain,
-            "kernel": SPMM_KERNEL + " (q = A p with the p.q reduction; float64 residual of requested chunks)",
+            "kernel": SPMM_KERNEL + " (q = A p with the p.q reduction, neighbour rows staged in LDS)",
             "bytes_per_launch": nbytes,
             "avg_launch_us": spmm_avg_s * 1e6,
         },

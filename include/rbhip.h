@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define RB_ABI_VERSION 5
+#define RB_ABI_VERSION 6
 
 /* property layouts for rb_assemble */
 #define RB_PROP_ELEM 0   /* one row per element (label tables are expanded by the caller) */
@@ -176,6 +176,66 @@ int rb_krylov_solve_noalloc(int64_t n, int64_t nnz, const int32_t* row_ptr, cons
                             const void* vals32, const void* B, void* X, int precision, double tol, int maxiter,
                             int use_x0, int check_every, int* iters_out /* host */, double* relres_out /* host */,
                             int* flag_out /* host */, void* stream);
+
+/*
+ * LDS-staged SpMM plan (replaces the per-entry row gathers of the
+ * femsolve SpMM, solver.py:385-588, for 512-byte rows: 64 complex64 or
+ * float64 columns).  Consecutive CSR rows are grouped into blocks of at most
+ * RB_LDS_RMAX rows whose neighbour rows (the union of their columns, at most
+ * umax) are staged in LDS once per block, with at most recmax records (the
+ * kernel's limits: umax = RB_LDS_UMAX, recmax = RB_LDS_RECMAX; others are
+ * refused with RB_ERR_ARG); every row's entries become
+ * records padded to a multiple of 8 (a record: the LDS byte offset of its
+ * column's row and its matrix value; padding records have value 0).  The plan
+ * depends on the pattern only; `recv` is scratch the solve fills with the
+ * record values of the matrix being solved.
+ *
+ * rb_spmm_plan_count / rb_spmm_plan_fill run on the HOST over host copies of
+ * row_ptr / col_idx (device row order).  The caller allocates (device):
+ *   brow, bu: nblk + 1;  ulist: nunion + RB_LDS_UPAD;
+ *   rinfo: 2 * (n + 1) + RB_LDS_RPAD;  rmap, loff, recv: nrec + RB_LDS_RECPAD
+ * (host buffers of the same sizes for rb_spmm_plan_fill, which also writes
+ * the padding).  RB_ERR_SIZE: some row alone exceeds the block limits (the
+ * solve then keeps its gather SpMM).
+ */
+#define RB_LDS_UMAX 128 /* block limits of the kernel */
+#define RB_LDS_RECMAX 768
+#define RB_LDS_RMAX 64
+#define RB_LDS_UPAD 256
+#define RB_LDS_RPAD 512
+#define RB_LDS_RECPAD 1024
+typedef struct rb_spmm_plan {
+  int64_t nblk;         /* blocks of consecutive rows */
+  int64_t nrec;         /* entry records (padded rows) */
+  int32_t umax;         /* the block limits the plan was built with (selects the kernel) */
+  int32_t recmax;
+  const int32_t* brow;  /* nblk + 1: first row of each block */
+  const int32_t* bu;    /* nblk + 1: start of each block's union in ulist */
+  const int32_t* ulist; /* union rows of every block, ascending per block */
+  const int32_t* rinfo; /* per row (first record, LDS byte offset of the row's own p), row n = (nrec, 0) */
+  const int32_t* rmap;  /* record -> CSR slot, -1 for padding */
+  const int32_t* loff;  /* record -> LDS byte offset of its column's row within the block */
+  void* recv;           /* record values, written by each solve (8 bytes per record) */
+} rb_spmm_plan;
+int rb_spmm_plan_count(int64_t n, const int32_t* row_ptr /* host */, const int32_t* col_idx /* host */, int umax,
+                       int recmax, int64_t* nblk /* host */, int64_t* nunion /* host */, int64_t* nrec /* host */);
+int rb_spmm_plan_fill(int64_t n, const int32_t* row_ptr /* host */, const int32_t* col_idx /* host */, int umax,
+                      int recmax, int32_t* brow, int32_t* bu, int32_t* ulist, int32_t* rinfo, int32_t* rmap,
+                      int32_t* loff /* all host */);
+
+/*
+ * rb_krylov_solve with an LDS-staged SpMM plan (rb_spmm_plan, device
+ * pointers) for the 64-column solves of one system: mixed precision with
+ * complex64 inner solves, or plain float64 real systems.  Other shapes (and
+ * plan == NULL) run exactly as rb_krylov_solve.  Same arguments, workspace,
+ * results and flags.
+ */
+int rb_krylov_solve_planned(int64_t n, int64_t nnz, const int32_t* row_ptr, const int32_t* col_idx,
+                            const int32_t* diag_slot, int nsys, int k, int is_complex, const void* vals64,
+                            const void* vals32, const void* B, void* X, int precision, double tol, int maxiter,
+                            int use_x0, int check_every, void* work, size_t work_bytes, const rb_spmm_plan* plan,
+                            int* iters_out /* host */, double* relres_out /* host */, int* flag_out /* host */,
+                            void* stream);
 
 /*
  * Per-kernel timing of the Krylov loop/*

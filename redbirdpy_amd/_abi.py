@@ -23,6 +23,18 @@ RB_PROF_SPMM, RB_PROF_UPDATE, RB_PROF_PUPDATE, RB_PROF_RESID, RB_PROF_INIT = 0, 
 RB_WF_SINK, RB_WF_FACES = 0, 1
 RB_WF_UNIFORM, RB_WF_FOURIER, RB_WF_IMAGE = 0, 1, 2
 RB_WF_FRAME = 34
+# LDS-staged SpMM plan padding (include/rbhip.h)
+RB_LDS_UPAD, RB_LDS_RPAD, RB_LDS_RECPAD = 256, 512, 1024
+RB_LDS_LIMITS = (128, 768)  # (RB_LDS_UMAX, RB_LDS_RECMAX)
+
+
+class SpmmPlan(ctypes.Structure):
+    """rb_spmm_plan: device pointers of an LDS-staged SpMM plan (include/rbhip.h)."""
+
+    _fields_ = [("nblk", ctypes.c_int64), ("nrec", ctypes.c_int64), ("umax", ctypes.c_int32),
+                ("recmax", ctypes.c_int32), ("brow", ctypes.c_void_p),
+                ("bu", ctypes.c_void_p), ("ulist", ctypes.c_void_p), ("rinfo", ctypes.c_void_p),
+                ("rmap", ctypes.c_void_p), ("loff", ctypes.c_void_p), ("recv", ctypes.c_void_p)]
 
 _i32 = ctypes.c_int32
 _i64 = ctypes.c_int64
@@ -53,6 +65,12 @@ _SIGS = {
     "rb_krylov_solve_noalloc": [_i64, _i64, _vp, _vp, _vp, _int, _int, _int, _vp, _vp, _vp, _vp, _int, _dbl, _int,
                                 _int, _int, ctypes.POINTER(_int), ctypes.POINTER(_dbl), ctypes.POINTER(_int), _vp],
     "rb_spmm": [_i64, _i64, _vp, _vp, _int, _int, _int, _vp, _vp, _vp, _vp],
+    "rb_spmm_plan_count": [_i64, _vp, _vp, _int, _int, ctypes.POINTER(_i64), ctypes.POINTER(_i64),
+                           ctypes.POINTER(_i64)],
+    "rb_spmm_plan_fill": [_i64, _vp, _vp, _int, _int, _vp, _vp, _vp, _vp, _vp, _vp],
+    "rb_krylov_solve_planned": [_i64, _i64, _vp, _vp, _vp, _int, _int, _int, _vp, _vp, _vp, _vp, _int, _dbl, _int,
+                                _int, _int, _vp, _sz, _vp, ctypes.POINTER(_int), ctypes.POINTER(_dbl),
+                                ctypes.POINTER(_int), _vp],
     "rb_bicgstab_workspace": [_i64, _int, _int, ctypes.POINTER(_sz)],
     "rb_bicgstab_solve": [_i64, _i64, _vp, _vp, _vp, _int, _int, _vp, _vp, _vp, _dbl, _int, _int, _int, _vp, _sz,
                           ctypes.POINTER(_int), ctypes.POINTER(_dbl), ctypes.POINTER(_int), _vp],

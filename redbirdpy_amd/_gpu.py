@@ -7,6 +7,7 @@ inputs at their boundary and call into this module.
 """
 
 import ctypes
+import os
 import numpy as np
 import torch
 
@@ -154,6 +155,12 @@ class GpuMesh:
             self.area = _f64(area, self.dev)
             self._area_src = area
 
+    def spmm_plan(self):
+        """The LDS-staged SpMM plan of this pattern (built once); None when it does not apply."""
+        if getattr(self, "_plan", False) is False:
+            self._plan = spmm_plan(self.row_ptr.cpu().numpy(), self.col_idx[: self.nnz].cpu().numpy(), self.dev)
+        return None if os.environ.get("RB_LDS_SPMM", "1") == "0" else self._plan
+
     def pattern_host(self):
         """(indptr, indices, slot order) of the CSR pattern in the user's node numbering.
 
@@ -187,6 +194,46 @@ class GpuMesh:
     def mesh_nodes(self, a):
         """Host array indexed by user node -> device numbering."""
         return a if not self.reordered else np.asarray(a)[self.perm]
+
+
+class _Plan:
+    """An rb_spmm_plan (ctypes struct of device pointers) with the tensors that own its memory."""
+
+    def __init__(self, tensors, nblk, nrec, limits):
+        self.tensors = tensors
+        self.limits = limits
+        self.struct = _abi.SpmmPlan(nblk, nrec, limits[0], limits[1],
+                                    *[ctypes.c_void_p(t.data_ptr()) for t in tensors])
+
+
+def spmm_plan(row_ptr, col_idx, dev, limits=None):
+    """LDS-staged SpMM plan of a CSR pattern (host arrays, device row order): blocks of consecutive
+    rows with their neighbour-row unions and padded entry records (rb_spmm_plan_count/fill on the
+    host), uploaded to dev.  None when some row alone exceeds the block limits (RB_ERR_SIZE)."""
+    rp = np.ascontiguousarray(np.asarray(row_ptr).astype(np.int32))
+    ci = np.ascontiguousarray(np.asarray(col_idx).astype(np.int32))
+    n = rp.size - 1
+    nblk, nun, nrec = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
+    L = _abi.lib()
+    hp = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    limits = limits or _abi.RB_LDS_LIMITS
+    rc = L.rb_spmm_plan_count(n, hp(rp), hp(ci), limits[0], limits[1], ctypes.byref(nblk), ctypes.byref(nun),
+                              ctypes.byref(nrec))
+    if rc == -3:  # RB_ERR_SIZE: the gather SpMM stays
+        return None
+    _abi.check(rc, "rb_spmm_plan_count")
+    nb, nu, nr = nblk.value, nun.value, nrec.value
+    brow = np.zeros(nb + 1, np.int32)
+    bu = np.zeros(nb + 1, np.int32)
+    ul = np.zeros(nu + _abi.RB_LDS_UPAD, np.int32)
+    ri = np.zeros(2 * (n + 1) + _abi.RB_LDS_RPAD, np.int32)
+    rm = np.zeros(nr + _abi.RB_LDS_RECPAD, np.int32)
+    lo = np.zeros(nr + _abi.RB_LDS_RECPAD, np.int32)
+    _abi.check(L.rb_spmm_plan_fill(n, hp(rp), hp(ci), limits[0], limits[1], hp(brow), hp(bu), hp(ul), hp(ri),
+                                   hp(rm), hp(lo)), "rb_spmm_plan_fill")
+    t = [torch.as_tensor(a, device=dev) for a in (brow, bu, ul, ri, rm, lo)]
+    t.append(torch.empty(nr + _abi.RB_LDS_RECPAD, dtype=torch.float64, device=dev))  # 8-byte record values
+    return _Plan(t, nb, nr, limits)
 
 
 _MESH_CACHE = []  # [(key objects, GpuMesh)], small LRU
@@ -375,7 +422,7 @@ class SolveInfo:
 
 
 def krylov_solve(row_ptr, col_idx, diag, n, nnz, vals64, vals32, B, k, nsys, is_complex, precision="fp64",
-                 tol=1e-12, maxiter=2000, X0=None, check_every=16):
+                 tol=1e-12, maxiter=2000, X0=None, check_every=16, plan=None):
     """Solve A_s X = B for nsys systems x k columns; B is (n, nsys*k) on device."""
     dev = B.device
     prec = _PREC[precision]
@@ -398,9 +445,16 @@ def krylov_solve(row_ptr, col_idx, diag, n, nnz, vals64, vals32, B, k, nsys, is_
     fl = ctypes.c_int(0)
     if prec == _abi.RB_PREC_MIXED and vals32 is None:
         vals32 = vals64.to(torch.complex64 if is_complex else torch.float32)
-    call("rb_krylov_solve", n, nnz, ptr(row_ptr), ptr(col_idx), ptr(diag), nsys, k, int(is_complex), ptr(vals64),
-         ptr(vals32), ptr(B), ptr(X), prec, float(tol), int(maxiter), use_x0, int(check_every), ptr(work),
-         nbytes.value, ctypes.byref(it), ctypes.byref(rel), ctypes.byref(fl), _abi.stream_ptr())
+    if plan is not None:
+        plan = plan.struct
+        call("rb_krylov_solve_planned", n, nnz, ptr(row_ptr), ptr(col_idx), ptr(diag), nsys, k, int(is_complex),
+             ptr(vals64), ptr(vals32), ptr(B), ptr(X), prec, float(tol), int(maxiter), use_x0, int(check_every),
+             ptr(work), nbytes.value, ctypes.byref(plan), ctypes.byref(it), ctypes.byref(rel), ctypes.byref(fl),
+             _abi.stream_ptr())
+    else:
+        call("rb_krylov_solve", n, nnz, ptr(row_ptr), ptr(col_idx), ptr(diag), nsys, k, int(is_complex),
+             ptr(vals64), ptr(vals32), ptr(B), ptr(X), prec, float(tol), int(maxiter), use_x0, int(check_every),
+             ptr(work), nbytes.value, ctypes.byref(it), ctypes.byref(rel), ctypes.byref(fl), _abi.stream_ptr())
     return X, SolveInfo(it.value, rel.value, fl.value)
 
 
@@ -446,8 +500,11 @@ def solve_system(sysm, B, k, precision="fp64", tol=1e-12, maxiter=2000, X0=None)
     """CG/COCG over every system; after a breakdown (flag 4) every system is finished by
     BiCGSTAB from its CG iterate (its columns that had converged stop at once)."""
     m = sysm.mesh
+    # the LDS-staged SpMM serves 64-column solves of one system with 8-byte inner values
+    # (complex64 mixed, float64 plain)
+    lds = sysm.nsys == 1 and k == 64 and (sysm.is_complex == (precision == "mixed"))
     X, info = krylov_solve(m.row_ptr, m.col_idx, m.diag, m.nn, m.nnz, sysm.vals64, sysm.vals32, B, k, sysm.nsys,
-                           sysm.is_complex, precision, tol, maxiter, X0)
+                           sysm.is_complex, precision, tol, maxiter, X0, plan=m.spmm_plan() if lds else None)
     if info.flag == 4:
         if sysm.nsys == 1:
             return bicgstab_solve(m.row_ptr, m.col_idx, m.diag, m.nn, m.nnz, sysm.vals64, B, sysm.is_complex, tol,

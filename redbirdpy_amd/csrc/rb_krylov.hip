@@ -988,6 +988,225 @@ __global__ void __launch_bounds__(KB) __attribute__((amdgpu_waves_per_eu(5))) k_
   sweep_body<T64, T32, NCH, 1, 4>(m, oo, C, k, so);
 }
 
+// ---------------------------------------------------------------------------
+// LDS-staged SpMM for 512-byte rows (64 complex64 or float64 columns of one
+// system).  A workgroup owns a sequence of plan blocks (consecutive CSR rows
+// whose columns fall in at most RB_LDS_UMAX distinct rows).  Producer
+// wavefronts copy the next block's neighbour rows of p and its entry records
+// from global memory straight into one of two LDS buffers
+// (global_load_lds_dwordx4, 1 KB per instruction: two 512-byte rows, or a
+// 1 KB run of records) while the consumer wavefronts compute the current
+// block from the other buffer, one row at a time: each entry is a broadcast
+// LDS read of its record (offset, value) and one 512-byte LDS read of its
+// column's row.  Each p row therefore crosses the L2 once per block whose
+// union holds it (about 4 times for a tetrahedral mesh in Morton order)
+// instead of once per CSR entry (about 14.5 times).  Producers and consumers
+// run separate loops with one bare barrier per block each (a __syncthreads()
+// would make the producers drain their in-flight copies at every block),
+// and every producer issues the same number of copies per block (spare
+// copies land in a scratch KB), so one s_waitcnt vmcnt(0) before the barrier
+// hands a complete block over.  Consumers also form the per-column partials
+// of p.q from the row's own p (its union slot).
+// ---------------------------------------------------------------------------
+#define LDS_NW 12  // consumer wavefronts
+#define LDS_NP 4   // producer wavefronts
+#define LDS_RINB 1024  // (first record, own-row offset) of up to 128 rows of a block
+
+// block limits and buffer ring of one kernel configuration; a plan is built
+// for one (UMAX, RECMAX) pair and runs on the kernel configured for it
+template <int UMAX_, int RECMAX_, int NBUF_>
+struct LdsCfg {
+  static constexpr int UMAX = UMAX_;      // neighbour rows per block
+  static constexpr int RECMAX = RECMAX_;  // records per block
+  static constexpr int NBUF = NBUF_;      // block buffers in the ring (copies run NBUF-1 blocks ahead)
+  static constexpr int ROWB = (UMAX + 2) * 512;  // neighbour rows (+2 rows the odd copy may fill)
+  static constexpr int LOFB = RECMAX * 4;        // record offsets
+  static constexpr int VALB = RECMAX * 8;        // record values
+  static constexpr int BUFB = ROWB + LOFB + VALB + LDS_RINB;
+  static constexpr int TRASH = NBUF * BUFB;      // 1 KB written by spare copies, never read
+  static constexpr int HDR = TRASH + 1024;       // (first row, rows) of each buffer's block
+  static constexpr int BYTES = HDR + 8 * NBUF;
+  static constexpr int JB = (UMAX / 2 + LDS_NP - 1) / LDS_NP;  // row copies per producer and block
+  static constexpr int JR = (LOFB + VALB + LDS_RINB) / 1024;   // record copies per block
+  static constexpr int JRP = (JR + LDS_NP - 1) / LDS_NP;       // ... per producer
+  static_assert(BYTES <= 160 * 1024, "the block buffers must fit the 160 KB LDS of a gfx950 CU");
+  static_assert((LOFB + VALB) % 1024 == 0, "records are copied in 1 KB pieces");
+  static_assert(2 * (RB_LDS_RMAX + 1) * 4 <= LDS_RINB, "row table of a block");
+  static_assert(2 * LDS_NP * JB <= UMAX + RB_LDS_UPAD, "producers read union index runs within the padding");
+  static_assert((JB + JRP) * (NBUF - 2) < 64, "copies in flight behind the awaited block fit vmcnt");
+};
+// RB_LDS_UMAX / RB_LDS_RECMAX, two buffers: each p row crosses the L2 ~4 times per SpMM.  A ring of
+// three buffers with blocks of 80 rows (copies two blocks ahead) measured slower in the solve (68.8
+// against 63.5 us per launch at 115k nodes): the smaller unions load ~35% more rows.
+typedef LdsCfg<RB_LDS_UMAX, RB_LDS_RECMAX, 2> LdsWide;
+
+// s_waitcnt vmcnt(N), the other counters untouched (gfx9 encoding: vmcnt in bits 3:0 and 15:14)
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt is 6 bits");
+  __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
+}
+
+struct LdsArgs {
+  int64_t nblk;
+  int umax;
+  const int32_t* brow;
+  const int32_t* bu;
+  const int32_t* ulist;
+  const int32_t* rinfo;
+  const int32_t* loff;
+  const void* recv;
+};
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+// one direct 16-byte-per-lane copy from global memory into LDS: lane l's 16
+// bytes land at dst + 16 l
+__device__ __forceinline__ void dma16(const void* src, char* dst) {
+  __builtin_amdgcn_global_load_lds(src, (lds_void_t*)dst, 16, 0, 0);
+}
+
+// (the plan arrays come as separate read-only kernel arguments so their
+// uniform reads are scalar loads, outside the vector counter the copies use)
+template <typename T, typename CF>
+__global__ void __launch_bounds__((LDS_NW + LDS_NP) * 64)
+k_cg_spmm_lds(int64_t nblk, const int32_t* __restrict__ brow, const int32_t* __restrict__ bu,
+              const int32_t* __restrict__ ulist, const int32_t* __restrict__ rinfo, const int32_t* __restrict__ loff,
+              const T* __restrict__ recv, const T* __restrict__ P, T* __restrict__ Q,
+              ColState<typename sc<T>::acc> st) {
+  static_assert(sizeof(T) == 8, "512-byte rows: 64 columns of 8 bytes");
+  typedef typename sc<T>::acc A;
+  constexpr int JB = CF::JB, JR = CF::JR, JRP = CF::JRP, NBUF = CF::NBUF;
+  extern __shared__ __attribute__((aligned(16))) char lds[];
+  if (st.ru_req && blockIdx.x == 0 && threadIdx.x < MAXCHUNK) ru_cur(st)[threadIdx.x] = 0;
+  if (!block_any_active(st.active, 64)) return;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t seg = (nblk + NXCD - 1) / NXCD;
+  const int64_t b0 = (int64_t)(blockIdx.x % NXCD) * seg;
+  const int64_t b1 = min(nblk, b0 + seg);
+  const int64_t stride = gridDim.x / NXCD;
+  const int64_t bfirst = b0 + blockIdx.x / NXCD;
+  const int nmine = bfirst < b1 ? (int)((b1 - bfirst + stride - 1) / stride) : 0;
+  A pv = acc_zero(A());
+  if (wave >= LDS_NW) {
+    // producer: block k of this workgroup into buffer k & 1, JB + JRP copies every time
+    const int pw = wave - LDS_NW;
+    auto issue = [&](int kb) {
+      const bool real = kb < nmine;
+      const int64_t bb = real ? bfirst + (int64_t)kb * stride : 0;
+      const int u0 = real ? bu[bb] : 0;
+      const int ni = real ? (bu[bb + 1] - u0 + 1) / 2 : 0;  // row copies (two rows each)
+      const int r0 = real ? brow[bb] : 0;
+      const int e0 = real ? rinfo[2 * r0] : 0;
+      char* base = lds + (kb % NBUF) * CF::BUFB;
+      // the block's rows for the consumers (a plain LDS store, ordered by the next barrier; the
+      // consumers' own loads of brow would wait on their outstanding q stores)
+      if (pw == 0 && lane == 0) {
+        int* h = (int*)(lds + CF::HDR) + 2 * (kb % NBUF);
+        h[0] = r0;
+        h[1] = real ? brow[bb + 1] - r0 : 0;
+      }
+#pragma unroll
+      for (int j = 0; j < JRP; ++j) {
+        const int q = pw * JRP + j;  // 1 KB piece q: offsets, values, then the row table
+        const char* src;
+        if (q < CF::LOFB / 1024) src = (const char*)(loff + e0) + q * 1024;
+        else if (q < (CF::LOFB + CF::VALB) / 1024) src = (const char*)(recv + e0) + (q - CF::LOFB / 1024) * 1024;
+        else src = (const char*)(rinfo + 2 * r0);
+        dma16(src + lane * 16, q < JR ? base + CF::ROWB + q * 1024 : lds + CF::TRASH);
+      }
+      const int* ix = ulist + u0 + 2 * pw * JB;  // the padding keeps these runs readable
+#pragma unroll
+      for (int j = 0; j < JB; ++j) {
+        const int i = pw * JB + j;
+        // both row indices are uniform: scalar loads, kept apart from the copies' counter
+        const int ga = __builtin_amdgcn_readfirstlane(ix[2 * j]);
+        const int gb = __builtin_amdgcn_readfirstlane(ix[2 * j + 1]);
+        const int g = lane < 32 ? ga : gb;
+        const char* src = (const char*)(P + (int64_t)g * 64) + (lane & 31) * 16;
+        dma16(src, i < ni ? base + 2 * i * 512 : base + CF::UMAX * 512);
+      }
+    };
+    for (int kb = 0; kb < NBUF - 1; ++kb) issue(kb);
+    for (int kb = 0; kb < nmine; ++kb) {
+      wait_vmcnt<(JB + JRP) * (NBUF - 2)>();  // block kb has landed (the next NBUF-2 may still fly)
+      asm volatile("s_barrier" ::: "memory");
+      issue(kb + NBUF - 1);
+    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // spare copies land before the buffers are reused
+  } else {
+    for (int kb = 0; kb < nmine; ++kb) {
+      asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+      const char* buf = lds + (kb % NBUF) * CF::BUFB;
+      const int* h = (const int*)(lds + CF::HDR) + 2 * (kb % NBUF);
+      const int* lo = (const int*)(buf + CF::ROWB);
+      const T* lv = (const T*)(buf + CF::ROWB + CF::LOFB);
+      const int* ri = (const int*)(buf + CF::ROWB + CF::LOFB + CF::VALB);
+      const char* cb = buf + lane * (int)sizeof(T);
+      const int r0 = h[0], nr = h[1];
+      const int e0 = ri[0];
+      for (int i = wave; i < nr; i += LDS_NW) {
+        const int ea = ri[2 * i] - e0, ez = ri[2 * i + 2] - e0;
+        T acc = sc<T>::zero();
+        for (int e = ea; e < ez; e += 8) {
+          int la[8];
+          T va[8], pa[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            la[u] = lo[e + u];
+            va[u] = lv[e + u];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) pa[u] = *(const T*)(cb + la[u]);
+#pragma unroll
+          for (int u = 0; u < 8; ++u) acc = acc + va[u] * pa[u];
+        }
+        const T pown = *(const T*)(cb + ri[2 * i + 1]);
+        Q[(int64_t)(r0 + i) * 64 + lane] = acc;
+        pv = acc_add(pv, amul(pown, acc));
+      }
+    }
+  }
+  // per-column partials of this workgroup, consumer wavefronts summed in order
+  __syncthreads();
+  A* red = (A*)lds;
+  if (wave < LDS_NW) red[wave * 64 + lane] = pv;
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    A t = red[threadIdx.x];
+    for (int w = 1; w < LDS_NW; ++w) t = acc_add(t, red[w * 64 + threadIdx.x]);
+    st.part[(int64_t)threadIdx.x * st.nb + blockIdx.x] = t;
+  }
+}
+
+// record values of the matrix being solved: recv[i] = vals[rmap[i]] (0 for padding)
+template <typename T>
+__global__ void k_lds_pack(int64_t nrec, const int32_t* __restrict__ rmap, const T* __restrict__ vals,
+                           T* __restrict__ recv) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nrec; i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t e = rmap[i];
+    recv[i] = e >= 0 ? vals[e] : sc<T>::zero();
+  }
+}
+
+// mixed solves whose SpMM runs in its own launch: the float64 true residual
+// of the 64-column chunks whose replacement the previous iteration requested
+// (the second half of k_cg_spmm_sweep_ru).  Exits at once when none asked.
+template <typename T64, typename T32, int NCH>
+__global__ void __launch_bounds__(KB) __attribute__((amdgpu_waves_per_eu(5)))
+k_cg_resid_sweep(Csr m, const T64* __restrict__ vals64, const T64* __restrict__ B, const T64* __restrict__ X,
+                 const T32* __restrict__ D, const T32* __restrict__ dinv32, T32* __restrict__ R32, int C, int k,
+                 ColState<typename sc<T32>::acc> st, ColState<typename sc<T64>::acc> so) {
+  const int* gate = ru_prev(st);
+  int any = 0;
+  for (int ch = 0; ch < NCH; ++ch) any |= gate[ch];
+  if (!any) return;
+  if (!block_any_active(st.active, C)) return;
+  SweepOps<T64, T32> oo{vals64, X, D, nullptr, B, dinv32, R32, gate};
+  sweep_body<T64, T32, NCH, 1, 4>(m, oo, C, k, so);
+}
+
 // q = A_s p ; partial p.q -- one wavefront per contiguous run of rows.
 // The run's row pointers come in with one lane-parallel load; rows are taken
 // two at a time, each with its (col, val) entries loaded by the lanes and
@@ -2004,6 +2223,39 @@ static void spmm_plan(const Csr& m, int nsys, int C, int k, int64_t nb, bool* fl
     if (cap >= NXCD) *nbs = std::min(*nbs, cap);
   }
 }
+// one workgroup per CU (each holds both LDS block buffers), a multiple of NXCD
+static int64_t lds_grid(int64_t nb) {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < NXCD) cus = NXCD;
+  return std::max<int64_t>(NXCD, std::min<int64_t>(nb, cus) / NXCD * NXCD);
+}
+
+// the LDS-staged SpMM (its dynamic LDS opt-in is per kernel, set before each launch)
+template <typename T, typename CF, typename A>
+static void launch_spmm_lds_cf(const LdsArgs& pl, dim3 grid, hipStream_t s, const T* P, T* Q, ColState<A> st) {
+  (void)hipFuncSetAttribute((const void*)k_cg_spmm_lds<T, CF>, hipFuncAttributeMaxDynamicSharedMemorySize, CF::BYTES);
+  plaunch(k_cg_spmm_lds<T, CF>, grid, dim3((LDS_NW + LDS_NP) * 64), CF::BYTES, s, pl.nblk, pl.brow, pl.bu, pl.ulist,
+          pl.rinfo, pl.loff, (const T*)pl.recv, P, Q, st);
+}
+template <typename T, typename A>
+static void launch_spmm_lds(const LdsArgs& pl, dim3 grid, hipStream_t s, const T* P, T* Q, ColState<A> st) {
+  if constexpr (sizeof(T) == 8) {
+    launch_spmm_lds_cf<T, LdsWide>(pl, grid, s, P, Q, st);
+  }
+}
+
+// a plan applies to 64-column solves of one system with 8-byte values
+static bool lds_eligible(const rb_spmm_plan* plan, int nsys, int C, size_t value_bytes) {
+  const bool cfg_ok = plan && plan->umax == LdsWide::UMAX && plan->recmax == LdsWide::RECMAX;
+  return cfg_ok && plan->nblk > 0 && plan->brow && plan->bu && plan->ulist && plan->rinfo && plan->rmap &&
+         plan->loff && plan->recv && nsys == 1 && C == 64 && value_bytes == 8;
+}
+
+static LdsArgs lds_args(const rb_spmm_plan* plan) {
+  return LdsArgs{plan->nblk, plan->umax, plan->brow, plan->bu, plan->ulist, plan->rinfo, plan->loff, plan->recv};
+}
+
 #define L_UPD(G_, N_, ...) plaunch((k_cg_update<T, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
 #define L_PUP(G_, N_, ...) plaunch((k_cg_pupdate<T, TW, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
 
@@ -2011,13 +2263,14 @@ static void spmm_plan(const Csr& m, int nsys, int C, int k, int64_t nb, bool* fl
 template <typename T>
 static int run_cg(const Csr& m, int nsys, const T* vals, const T* dinv, const T* B, T* X, int use_x0,
                   SolveBufs<T> w, int C, int k, ColState<typename sc<T>::acc> st, int check_every, hipStream_t s,
-                  int* iters_done) {
+                  int* iters_done, const LdsArgs* lds = nullptr) {
   typedef typename sc<T>::acc A;
   int64_t nb = grid_for(m.n, C);
   st.nb = (int)nb;
   bool flat;
   int64_t nbs;
   spmm_plan<T>(m, nsys, C, k, nb, &flat, &nbs);
+  if (lds) nbs = lds_grid(nb);
   ColState<A> ss = st;  // the SpMM's partials come from its own grid
   ss.nb = (int)nbs;
   {
@@ -2035,7 +2288,10 @@ static int run_cg(const Csr& m, int nsys, const T* vals, const T* dinv, const T*
     for (int j = 0; j < chunk; ++j) {
       {
         ProfScope ps(RB_PROF_SPMM, s);
-        DISPATCH_GN(C, L_SPMM, m, vals, w.P, w.Q, C, k, ss);
+        if (lds)
+          launch_spmm_lds<T>(*lds, dim3(nbs), s, w.P, w.Q, ss);
+        else
+          DISPATCH_GN(C, L_SPMM, m, vals, w.P, w.Q, C, k, ss);
       }
       hipLaunchKernelGGL(k_fin_alpha<A>, dim3(C), dim3(FB), 0, s, C, ss);
       {
@@ -2064,7 +2320,8 @@ static int run_cg(const Csr& m, int nsys, const T* vals, const T* dinv, const T*
 template <typename T64, typename T32>
 static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const void* vals32, const void* Bv, void* Xv,
                       int precision, double tol, int maxiter, int use_x0, int check_every, void* work,
-                      size_t work_bytes, int* iters_out, double* relres_out, int* flag_out, hipStream_t s) {
+                      size_t work_bytes, int* iters_out, double* relres_out, int* flag_out, hipStream_t s,
+                      const rb_spmm_plan* plan = nullptr) {
   typedef typename sc<T64>::acc A;
   const int C = nsys * k;
   const int64_t n = m.n;
@@ -2093,7 +2350,15 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
     hipLaunchKernelGGL(k_dinv<T64>, dim3((unsigned)ceil_div(n * nsys, 256)), dim3(256), 0, s, m, A64, nsys, w.dinv);
     RB_CHECK_LAUNCH();
     int its = 0;
-    rc = run_cg<T64>(m, nsys, A64, w.dinv, B, X, use_x0, w, C, k, st, check_every, s, &its);
+    LdsArgs la{};
+    const bool use_lds = lds_eligible(plan, nsys, C, sizeof(T64));
+    if (use_lds) {  // this matrix's values into the plan's records
+      la = lds_args(plan);
+      const int64_t nr = plan->nrec + RB_LDS_RECPAD;
+      hipLaunchKernelGGL(k_lds_pack<T64>, dim3(ew_grid(nr)), dim3(256), 0, s, nr, plan->rmap, A64, (T64*)plan->recv);
+      RB_CHECK_LAUNCH();
+    }
+    rc = run_cg<T64>(m, nsys, A64, w.dinv, B, X, use_x0, w, C, k, st, check_every, s, &its, use_lds ? &la : nullptr);
     if (rc) return rc;
     RB_HIP(hipMemcpyAsync(hrr, st.rr, C * sizeof(double), hipMemcpyDeviceToHost, s));
     RB_HIP(hipMemcpyAsync(hbb, st.bb, C * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -2171,13 +2436,25 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
   spmm_plan<T32>(m, nsys, C, k, nb, &flat, &nbs);
   // real systems of 64 columns each, whole chunk pairs: one sweep per pair of systems
   const bool pair = std::is_same<T32, float>::value && pair_sweep<T32>(nsys, C, k);
-  if (flat) {  // the fused SpMM + residual launch has its own occupancy
+  // the LDS-staged SpMM (a plan for this 64-column system): its own launch, then the
+  // float64 residual of the requested chunks in a gated sweep launch
+  const bool use_lds = flat && lds_eligible(plan, nsys, C, sizeof(T32));
+  LdsArgs la{};
+  int64_t nbr = 0;
+  if (use_lds) {
+    la = lds_args(plan);
+    const int64_t nr = plan->nrec + RB_LDS_RECPAD;
+    hipLaunchKernelGGL(k_lds_pack<T32>, dim3(ew_grid(nr)), dim3(256), 0, s, nr, plan->rmap, A32, (T32*)plan->recv);
+    RB_CHECK_LAUNCH();
+    nbs = lds_grid(nb);
+    nbr = resident_grid((const void*)k_cg_resid_sweep<T64, T32, 1>, nb);
+  } else if (flat) {  // the fused SpMM + residual launch has its own occupancy
     nbs = std::min(nbs, resident_grid(spmm_ru_fn<T64, T32>(pick_NCH(C), pair), nb));
   }
   ColState<A> ss = si;  // the SpMM's partials come from its own grid
   ss.nb = (int)nbs;
-  ColState<A> sol = so;  // the in-loop residual's partials: the SpMM grid, or the regular one
-  sol.nb = (int)(flat ? nbs : nbo);
+  ColState<A> sol = so;  // the in-loop residual's partials: its sweep's grid, or the regular one
+  sol.nb = (int)(use_lds ? nbr : flat ? nbs : nbo);
   typedef T64 TW;
 #define L_SPMMRU(G_, N_, ...)                                                    \
   do {                                                                         \
@@ -2196,7 +2473,12 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
       si.ru_par = ss.ru_par = sol.ru_par = (it + j) & 1;
       {
         ProfScope ps(RB_PROF_SPMM, s);
-        if (flat) {
+        if (use_lds) {
+          launch_spmm_lds<T32>(la, dim3(nbs), s, w.P, w.Q, ss);
+          ProfScope pr(RB_PROF_RESID, s);
+          plaunch(k_cg_resid_sweep<T64, T32, 1>, dim3(nbr), dim3(KB), 0, s, m, A64, B, X, D, w.dinv, w.R, C, k, ss,
+                  sol);
+        } else if (flat) {
           DISPATCH_GN(C, L_SPMMRU, m, A32, w.P, w.Q, A64, B, X, D, w.dinv, w.R, C, k, ss, sol);
         } else {
           DISPATCH_GN(C, L_SPMM, m, A32, w.P, w.Q, C, k, ss);
@@ -2280,6 +2562,189 @@ extern "C" int rb_krylov_solve(int64_t n, int64_t nnz, const int32_t* row_ptr, c
                                                  check_every, work, work_bytes, iters_out, relres_out, flag_out, s);
   return solve_impl<double, float>(m, nsys, k, vals64, vals32, B, X, precision, tol, maxiter, use_x0, check_every,
                                    work, work_bytes, iters_out, relres_out, flag_out, s);
+}
+
+// ---------------------------------------------------------------------------
+// LDS SpMM plan (host): greedy blocks of consecutive rows, each closed when
+// one more row would take its union of columns past RB_LDS_UMAX, its records
+// past RB_LDS_RECMAX or its rows past RB_LDS_RMAX.  A block's union always
+// holds its own rows (the p.q partials read each row's own p there).
+// ---------------------------------------------------------------------------
+namespace {
+struct PlanScan {
+  int64_t n;
+  int umax = RB_LDS_UMAX, recmax = RB_LDS_RECMAX;
+  const int32_t* rp;
+  const int32_t* ci;
+  std::vector<int64_t> mark;  // block id + 1 that holds the column in its union
+  std::vector<int32_t> slot;  // the column's slot in that block's union
+  std::vector<int32_t> uni;
+  PlanScan(int64_t n_, const int32_t* rp_, const int32_t* ci_) : n(n_), rp(rp_), ci(ci_), mark(n_, 0), slot(n_, 0) {}
+  static int64_t pad8(int64_t x) { return (x + 7) / 8 * 8; }
+  // rows [r0, r1) of block blk; returns r1 (r0 when a single row does not fit)
+  int64_t grow(int64_t r0, int64_t blk, int64_t* recs) {
+    uni.clear();
+    int64_t r = r0, rec = 0;
+    while (r < n && r - r0 < RB_LDS_RMAX) {
+      int add = mark[r] == blk + 1 ? 0 : 1;  // the row itself
+      for (int32_t e = rp[r]; e < rp[r + 1]; ++e)
+        if (mark[ci[e]] != blk + 1 && ci[e] != r) ++add;
+      const int64_t pr = pad8(rp[r + 1] - rp[r]);
+      if ((int64_t)uni.size() + add > umax || rec + pr > recmax) break;
+      if (mark[r] != blk + 1) {
+        mark[r] = blk + 1;
+        uni.push_back((int32_t)r);
+      }
+      for (int32_t e = rp[r]; e < rp[r + 1]; ++e)
+        if (mark[ci[e]] != blk + 1) {
+          mark[ci[e]] = blk + 1;
+          uni.push_back(ci[e]);
+        }
+      rec += pr;
+      ++r;
+    }
+    *recs = rec;
+    std::sort(uni.begin(), uni.end());
+    for (size_t i = 0; i < uni.size(); ++i) slot[uni[i]] = (int32_t)i;
+    return r;
+  }
+};
+
+int plan_check(int64_t n, const int32_t* rp, const int32_t* ci) {
+  if (n <= 0 || !rp || !ci) return RB_ERR_ARG;
+  if (n >= (1LL << 31)) return RB_ERR_SIZE;
+  for (int64_t r = 0; r < n; ++r)
+    if (rp[r + 1] < rp[r]) return RB_ERR_ARG;
+  for (int64_t e = 0; e < rp[n]; ++e)
+    if (ci[e] < 0 || ci[e] >= n) return RB_ERR_ARG;
+  return RB_OK;
+}
+}  // namespace
+
+static bool plan_limits_ok(int umax, int recmax) {
+  return umax == LdsWide::UMAX && recmax == LdsWide::RECMAX;
+}
+
+extern "C" int rb_spmm_plan_count(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, int umax, int recmax,
+                                  int64_t* nblk, int64_t* nunion, int64_t* nrec) {
+  if (!nblk || !nunion || !nrec) return RB_ERR_ARG;
+  if (!plan_limits_ok(umax, recmax)) {
+    rb_set_error("rb_spmm_plan_count: no kernel for block limits (%d, %d)", umax, recmax);
+    return RB_ERR_ARG;
+  }
+  int rc = plan_check(n, row_ptr, col_idx);
+  if (rc) {
+    rb_set_error("rb_spmm_plan_count: bad pattern (n %lld)", (long long)n);
+    return rc;
+  }
+  PlanScan ps(n, row_ptr, col_idx);
+  ps.umax = umax;
+  ps.recmax = recmax;
+  int64_t r = 0, blk = 0, nu = 0, nr = 0;
+  while (r < n) {
+    int64_t recs = 0;
+    const int64_t r1 = ps.grow(r, blk, &recs);
+    if (r1 == r) {
+      rb_set_error("rb_spmm_plan_count: row %lld alone exceeds the block limits (%d neighbour rows, %d records)",
+                   (long long)r, umax, recmax);
+      return RB_ERR_SIZE;
+    }
+    nu += (int64_t)ps.uni.size();
+    nr += recs;
+    r = r1;
+    ++blk;
+  }
+  *nblk = blk;
+  *nunion = nu;
+  *nrec = nr;
+  return RB_OK;
+}
+
+extern "C" int rb_spmm_plan_fill(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, int umax, int recmax,
+                                 int32_t* brow, int32_t* bu, int32_t* ulist, int32_t* rinfo, int32_t* rmap,
+                                 int32_t* loff) {
+  if (!brow || !bu || !ulist || !rinfo || !rmap || !loff) return RB_ERR_ARG;
+  if (!plan_limits_ok(umax, recmax)) {
+    rb_set_error("rb_spmm_plan_fill: no kernel for block limits (%d, %d)", umax, recmax);
+    return RB_ERR_ARG;
+  }
+  int rc = plan_check(n, row_ptr, col_idx);
+  if (rc) {
+    rb_set_error("rb_spmm_plan_fill: bad pattern (n %lld)", (long long)n);
+    return rc;
+  }
+  PlanScan ps(n, row_ptr, col_idx);
+  ps.umax = umax;
+  ps.recmax = recmax;
+  int64_t r = 0, blk = 0, nu = 0, nr = 0;
+  brow[0] = 0;
+  bu[0] = 0;
+  while (r < n) {
+    int64_t recs = 0;
+    const int64_t r1 = ps.grow(r, blk, &recs);
+    if (r1 == r) {
+      rb_set_error("rb_spmm_plan_fill: row %lld alone exceeds the block limits", (long long)r);
+      return RB_ERR_SIZE;
+    }
+    for (size_t i = 0; i < ps.uni.size(); ++i) ulist[nu + (int64_t)i] = ps.uni[i];
+    nu += (int64_t)ps.uni.size();
+    for (int64_t q = r; q < r1; ++q) {
+      rinfo[2 * q] = (int32_t)nr;
+      rinfo[2 * q + 1] = ps.slot[q] * 512;
+      const int32_t e0 = row_ptr[q], e1 = row_ptr[q + 1];
+      const int64_t pr = PlanScan::pad8(e1 - e0);
+      for (int64_t j = 0; j < pr; ++j) {
+        const bool real = e0 + j < e1;
+        rmap[nr + j] = real ? (int32_t)(e0 + j) : -1;
+        loff[nr + j] = real ? ps.slot[col_idx[e0 + j]] * 512 : 0;
+      }
+      nr += pr;
+    }
+    r = r1;
+    ++blk;
+    brow[blk] = (int32_t)r;
+    bu[blk] = (int32_t)nu;
+  }
+  rinfo[2 * n] = (int32_t)nr;
+  rinfo[2 * n + 1] = 0;
+  for (int i = 0; i < RB_LDS_RPAD; ++i) rinfo[2 * (n + 1) + i] = 0;
+  for (int i = 0; i < RB_LDS_UPAD; ++i) ulist[nu + i] = 0;
+  for (int i = 0; i < RB_LDS_RECPAD; ++i) {
+    rmap[nr + i] = -1;
+    loff[nr + i] = 0;
+  }
+  return RB_OK;
+}
+
+extern "C" int rb_krylov_solve_planned(int64_t n, int64_t nnz, const int32_t* row_ptr, const int32_t* col_idx,
+                                       const int32_t* diag_slot, int nsys, int k, int is_complex,
+                                       const void* vals64, const void* vals32, const void* B, void* X, int precision,
+                                       double tol, int maxiter, int use_x0, int check_every, void* work,
+                                       size_t work_bytes, const rb_spmm_plan* plan, int* iters_out,
+                                       double* relres_out, int* flag_out, void* stream) {
+  if (n <= 0 || nsys <= 0 || k <= 0 || !vals64 || !B || !X) {
+    rb_set_error("rb_krylov_solve_planned: bad arguments");
+    return RB_ERR_ARG;
+  }
+  if ((int64_t)nsys * k > 512) {
+    rb_set_error("rb_krylov_solve_planned: %lld columns exceed the 512-column limit of one launch",
+                 (long long)nsys * k);
+    return RB_ERR_SIZE;
+  }
+  if (plan && (plan->nblk <= 0 || plan->nrec <= 0)) {
+    rb_set_error("rb_krylov_solve_planned: empty plan");
+    return RB_ERR_ARG;
+  }
+  if (check_every <= 0) check_every = 16;
+  if (maxiter <= 0) maxiter = 1000;
+  Csr m{row_ptr, col_idx, diag_slot, n, nnz};
+  hipStream_t s = (hipStream_t)stream;
+  if (is_complex)
+    return solve_impl<cplx<double>, cplx<float>>(m, nsys, k, vals64, vals32, B, X, precision, tol, maxiter, use_x0,
+                                                 check_every, work, work_bytes, iters_out, relres_out, flag_out, s,
+                                                 plan);
+  return solve_impl<double, float>(m, nsys, k, vals64, vals32, B, X, precision, tol, maxiter, use_x0, check_every,
+                                   work, work_bytes, iters_out, relres_out, flag_out, s, plan);
 }
 
 // ---------------------------------------------------------------------------

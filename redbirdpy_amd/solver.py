@@ -78,6 +78,13 @@ def femsolve(Amat, rhs, method="auto", **kwargs):
     vals = torch.as_tensor(A.data.astype(complex if is_complex else float), device=dev).reshape(1, -1)
     x = np.zeros((n, ncol), dtype=complex if is_complex else float)
     x0 = kwargs.get("x0", None)
+    # the LDS-staged SpMM serves 64-column chunks with 8-byte inner values (complex64 mixed, float64 plain)
+    plan = None
+    if not use_bicgstab and is_complex == (precision == "mixed") and ncol % 512 == 64:
+        import os
+
+        if os.environ.get("RB_LDS_SPMM", "1") != "0":
+            plan = _gpu.spmm_plan(A.indptr, A.indices, dev)
     flag = 0
     info = None
     for c0 in range(0, ncol, 512):
@@ -92,7 +99,7 @@ def femsolve(Amat, rhs, method="auto", **kwargs):
             X, info = _gpu.bicgstab_solve(row_ptr, col_idx, diag, n, A.nnz, vals, B, is_complex, tol, maxiter, X0)
         else:
             X, info = _gpu.krylov_solve(row_ptr, col_idx, diag, n, A.nnz, vals, None, B, c1 - c0, 1, is_complex,
-                                        precision, tol, maxiter, X0)
+                                        precision, tol, maxiter, X0, plan=plan if c1 - c0 == 64 else None)
             if info.flag == 4:  # CG/COCG breakdown: BiCGSTAB from the last iterate
                 X, info = _gpu.bicgstab_solve(row_ptr, col_idx, diag, n, A.nnz, vals, B, is_complex, tol, maxiter,
                                               X)
