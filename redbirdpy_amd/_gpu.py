@@ -285,6 +285,10 @@ def props_for(cfg, wavelength="", mode=1):
         om = cfg.get("omega", 0)
     if mode == 2:
         om = 0
+    props = np.asarray(props)
+    if np.iscomplexobj(props):  # e.g. concentrations after a reform='complex' update (recon.py:305-313)
+        raise NotImplementedError("complex-valued optical properties%s: the GPU assembly takes real mua, mus, g "
+                                  "and n" % ((" of wavelength %s" % wavelength) if wavelength else ""))
     return np.asarray(props, dtype=np.float64), float(reff), float(om)
 
 

@@ -360,7 +360,7 @@ def forward_batch(cfgs, precision="fp64", tol=1e-12, maxiter=None, jacobian=Fals
 def check_inputs(cfg, wvs):
     """ValueError for NaN/Inf optical properties or optode positions (before any launch)."""
     for wv in wvs:
-        props = _gpu.props_for(cfg, wv, 1)[0]
+        props = _gpu.props_for(cfg, wv, 1)[0]  # refuses complex-valued properties
         if not np.all(np.isfinite(props)):
             raise ValueError("optical properties%s hold NaN or Inf" % ((" of wavelength %s" % wv) if wv else ""))
     for key in ("srcpos", "detpos", "srcdir", "detdir", "widesrc", "widedet"):

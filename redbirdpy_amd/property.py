@@ -120,7 +120,8 @@ def updateprop(cfg, wv=None, extin=None):
         elif musp is not None:
             nref = tmpl[1, 3] if tmpl.shape[0] > 1 else 1.37
             if hasattr(mua, "__len__"):
-                newp = np.empty((len(mua), 4))
+                # column_stack in the reference: complex concentrations give complex rows
+                newp = np.empty((len(mua), 4), dtype=np.result_type(np.asarray(mua), np.asarray(musp), float))
                 newp[:, 0] = mua
                 newp[:, 1] = musp
                 newp[:, 2] = 0.0
@@ -128,7 +129,7 @@ def updateprop(cfg, wv=None, extin=None):
             else:
                 newp = np.array([[mua, musp, 0, nref]])
         elif hasattr(mua, "__len__"):
-            newp = np.empty((len(mua), tmpl.shape[1]))
+            newp = np.empty((len(mua), tmpl.shape[1]), dtype=np.result_type(np.asarray(mua), tmpl))
             newp[:, 0] = mua
             newp[:, 1:] = tmpl[1, 1:]
         else:

@@ -437,56 +437,105 @@ class GNStep:
 
 
 class GNStepReIm(GNStep):
-    """'reim' rows for one complex unknown block (recon.py:537-541 with :243-251).
+    """'reim' rows for complex data (recon.py:537-541 with :243-251).
 
     The real system is A = [[Jr, -Ji], [Ji, Jr]] on the unknowns [x; y] (the
-    real and imaginary parts of a complex update); the reference's update
-    parsing keeps the first N entries, x.  With the node-major [Re | Im]
-    Jacobian JT (K x 2m) and its Gram G = [[Grr, Gri], [Gir, Gii]]:
-    A A^T = [[Grr + Gii, Gri - Gir], [Gir - Gri, Grr + Gii]], ||A||_F^2 = 2 ||J||_F^2,
-    and x = s^2 (Jr^T y1 + Ji^T y2).
+    real and imaginary parts of a complex update).  The flattened Jacobian is
+    J[(w,p),(c,n)] = E[w,c] J_w[p,n] (wavelength row blocks w, unknown blocks
+    c); the reference's update parsing walks the unknown blocks over the
+    first N = nc*nn entries of [x; y], so every block receives its own x (the
+    real part of its complex Tikhonov step) and y is never read.  With the
+    node-major [Re | Im] Jacobian JT (K x 2m, m = nw*npairs), its Gram
+    G = [[Grr, Gri], [Gir, Gii]] and Wk = (E E^T)[w,w'] over the pair blocks:
+    A A^T = [[Wk o (Grr + Gii), Wk o (Gri - Gir)], [Wk o (Gir - Gri), Wk o (Grr + Gii)]],
+    ||A||_F^2 = 2 ||J||_F^2, and x_c = s^2 sum_w E[w,c] (Jr_w^T y1_w + Ji_w^T y2_w).
     """
 
-    def __init__(self, JT, npairs, lam, precision="fp64", reftol=1e-10, maxref=20):
-        super().__init__(JT, npairs, np.ones((2, 1)), lam, precision, reftol, maxref)
+    def __init__(self, JT, npairs, E, lam, precision="fp64", reftol=1e-10, maxref=20):
+        super().__init__(JT, npairs, E, lam, precision, reftol, maxref)
 
-    def _hv(self, v, s2, W=None):
+    def _halves(self, v):
+        """(nn, nw) = Jr_w^T v1_w + Ji_w^T v2_w for v = [v1; v2] (each nw*npairs)."""
+        nw = self.E.shape[0]
+        t = _gpu.jt_rows(self.JT, 2 * nw, self.np, v)
+        return t[:, :nw] + t[:, nw:]
+
+    def _hv(self, v, s2, W):
         """(s^2 A A^T + lambda) v with float64 accumulation (four reads of J)."""
-        JT, m = self.JT, self.np
-        a = _gpu.jt_rows(JT, 1, 2 * m, v)[:, 0]  # Jr^T v1 + Ji^T v2
-        b = _gpu.jt_rows(JT, 1, 2 * m, torch.cat([v[m:], -v[:m]]))[:, 0]  # Jr^T v2 - Ji^T v1
-        u1 = _gpu.jt_cols(JT, 2, m, torch.stack([a, -b], 1))  # [Jr a; -Ji b]
-        u2 = _gpu.jt_cols(JT, 2, m, torch.stack([b, a], 1))  # [Jr b; Ji a]
+        m = v.shape[0] // 2
+        nw = W.shape[0]
+        za = self._halves(v) @ W  # Jr^T v1 + Ji^T v2, mixed over wavelengths by E E^T
+        zb = self._halves(torch.cat([v[m:], -v[:m]])) @ W  # Jr^T v2 - Ji^T v1
+        u1 = _gpu.jt_cols(self.JT, 2 * nw, self.np, torch.cat([za, -zb], 1))  # [Jr za; -Ji zb]
+        u2 = _gpu.jt_cols(self.JT, 2 * nw, self.np, torch.cat([zb, za], 1))  # [Jr zb; Ji za]
         return s2 * torch.cat([u1[:m] + u1[m:], u2[:m] + u2[m:]]) + self.lam * v
 
-    def solve(self, misfit):
-        JT, m = self.JT, self.np
-        mk = getattr(self, "mark", None) or (lambda name, t: t)
-        tq = time.perf_counter()
+    def _gram_blocks(self):
+        """Gram blocks rr, ri, ir, ii of the [Re | Im] Jacobian and the Kronecker weights Wk."""
+        JT, E = self.JT, self.E
+        m = JT.shape[1] // 2
         if self.precision == "fp64":
-            colsq = _gpu.jt_cols(JT, 2, m)
+            colsq = _gpu.jt_cols(JT, 2 * E.shape[0], self.np)
             G = _gpu.gram(JT, None, "fp64")
         else:
             G, colsq = _gpu.gram(JT, None, "bf16x3", want_colsq=True)
             G.diagonal().copy_(colsq)
-        s2 = 1.0 / (2.0 * torch.sum(colsq).item())
+        W = E @ E.t()
+        Wk = torch.repeat_interleave(torch.repeat_interleave(W, self.np, 0), self.np, 1)
+        return G, colsq, W, Wk, m
+
+    def solve(self, misfit):
+        JT, E = self.JT, self.E
+        mk = getattr(self, "mark", None) or (lambda name, t: t)
+        tq = time.perf_counter()
+        G, colsq, W, Wk, m = self._gram_blocks()
+        wdiag = torch.repeat_interleave(torch.diagonal(W), self.np)
+        s2 = 1.0 / (2.0 * torch.sum(wdiag * (colsq[:m] + colsq[m:])).item())
         tq = mk("gn_gram", tq)
-        rr, ri, ir, ii = G[:m, :m], G[:m, m:], G[m:, :m], G[m:, m:]
-        H = torch.empty_like(G)
-        H[:m, :m] = rr + ii
-        H[m:, m:] = rr + ii
-        H[:m, m:] = ri - ir
-        H[m:, :m] = ir - ri
-        H.mul_(s2)
+
+        def assemble(Gm):
+            rr, ri, ir, ii = Gm[:m, :m], Gm[:m, m:], Gm[m:, :m], Gm[m:, m:]
+            d, o = Wk * (rr + ii), Wk * (ri - ir)
+            return s2 * torch.cat([torch.cat([d, o], 1), torch.cat([-o, d], 1)], 0)
+
+        H = assemble(G)
         del G
+        y = self._factor_refine(H, misfit, lambda v: self._hv(v, s2, W), mk, tq,
+                                exact=lambda: assemble(_gpu.gram(JT, None, "fp64")))
+        return s2 * (self._halves(y) @ E)  # (nn, nc) = the x of every block
 
-        def exact():
-            Ge = _gpu.gram(JT, None, "fp64")
-            rr_, ri_, ir_, ii_ = Ge[:m, :m], Ge[:m, m:], Ge[m:, :m], Ge[m:, m:]
-            return s2 * torch.cat([torch.cat([rr_ + ii_, ri_ - ir_], 1), torch.cat([ir_ - ri_, rr_ + ii_], 1)], 0)
 
-        y = self._factor_refine(H, misfit, lambda v: self._hv(v, s2), mk, tq, exact=exact)
-        return s2 * _gpu.jt_rows(JT, 1, 2 * m, y)  # (nn, 1) = x
+class GNStepComplex(GNStepReIm):
+    """reform 'complex' (recon.py:192-198, :235-240, :474-501): the complex
+    normal equations of the unconjugated Gauss-Newton step, float64.
+
+    blockscale s = 1/sqrt(sum J^2) with the complex square (the reference's
+    np.sqrt(np.sum(Jflat**2)) on complex Jflat, principal branch), H = s^2 J J^T
+    + lambda I (complex symmetric: J^T, not J^H), y = D solve(D H D, D r) with
+    D = 1/sqrt(|diag H| + 1e-16), and dmu = s^2 J^T y, complex.  From the
+    [Re | Im] Gram blocks: J J^T = Wk o ((Grr - Gii) + i (Gri + Gir)).
+    """
+
+    def solve(self, misfit):
+        JT, E = self.JT, self.E
+        self.precision = "fp64"
+        G, colsq, W, Wk, m = self._gram_blocks()
+        wdiag = torch.repeat_interleave(torch.diagonal(W), self.np)
+        sumsq = complex(torch.sum(wdiag * (colsq[:m] - colsq[m:])).item(),
+                        2.0 * torch.sum(wdiag * torch.diagonal(G[:m, m:])).item())
+        s2 = 1.0 / sumsq  # (1/sqrt(sum J^2))^2
+        rr, ri, ir, ii = G[:m, :m], G[:m, m:], G[m:, :m], G[m:, m:]
+        H = s2 * torch.complex(Wk * (rr - ii), Wk * (ri + ir))
+        del G
+        H.diagonal().add_(self.lam)
+        Hn, di = _normalize_diag(H)
+        r = misfit.to(torch.complex128)
+        y = di * torch.linalg.solve(Hn, di * r)
+        yr, yi = y.real.contiguous(), y.imag.contiguous()
+        re = self._halves(torch.cat([yr, -yi]))  # Jr^T yr - Ji^T yi
+        im = self._halves(torch.cat([yi, yr]))  # Jr^T yi + Ji^T yr
+        self.info["refinements"] = 0
+        return s2 * (torch.complex(re, im) @ E.to(torch.complex128))
 
 
 def _solve_over(JG, npairs, E, lam, misfit, LTL=None):
@@ -586,8 +635,9 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
     (recon['prop'] with nn rows), and multispectral chromophore unknowns
     (recon['param'] node arrays); CW or frequency-domain data with reform
     'real' ([Re; Im] rows for complex data, recon.py:532-546), 'reim'
-    ([[Re, -Im], [Im, Re]] rows, :537-541, one unknown block on one mesh) or
-    'logphase' (log-amplitude and phase rows, :548-565); priors (lmat, ltl, or
+    ([[Re, -Im], [Im, Re]] rows, :537-541, node unknowns on one mesh),
+    'complex' (complex normal equations, :192-198, float64) or 'logphase'
+    (log-amplitude and phase rows, :548-565); priors (lmat, ltl, or
     prior= with recon['seg'], :127-133, :225-233) and label compression by a 1-D
     recon['seg'] (:210-218), with the overdetermined step when the unknowns are
     few (:363-426); a dual mesh (recon['node'],
@@ -619,9 +669,12 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
     optodes = kwargs.get("optodes", None)
     if isinstance(rfcw, int):
         rfcw = [rfcw]
-    if rfcw != [1] or reform not in ("real", "reim", "complex", "logphase"):
-        raise NotImplementedError("GPU runrecon supports rfcw=[1] with reform 'real', 'reim', 'logphase' "
-                                  "(or 'complex' on CW)")
+    if reform not in ("real", "reim", "complex", "logphase"):
+        raise ValueError("Unknown form: %s" % reform)  # matreform's error (recon.py:567)
+    if rfcw != [1]:
+        # the reference passes runforward's per-mode dict of fields to jac (recon.py:151-158,
+        # forward.py:513), which fails there; one mode per reconstruction here
+        raise NotImplementedError("runrecon reconstructs one mode at a time (rfcw=[1])")
     Aregu = {}  # recon.py:127-133: an lmat keyword, else ltl, else a prior built from recon['seg']
     if "lmat" in kwargs:
         Aregu["lmat"] = np.asarray(kwargs["lmat"], dtype=float)
@@ -675,8 +728,6 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
         y0 = _flatten_detphi(detphi0, sd, wvs, rfcw)
         y = _flatten_detphi(detphi, sd, wvs, rfcw)
         cplx = res.is_complex
-        if cplx and reform == "complex":
-            raise NotImplementedError("GPU runrecon solves real normal equations: use reform 'real' or 'logphase'")
         # rows of the real system: (part, wavelength, pair) with part Re [, Im]
         # (recon.py:532-565); 'logphase' scales each complex row by
         # conj(y)/|y|^2 before splitting
@@ -687,6 +738,8 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
             misfit = np.log(np.abs(y0)) - np.log(np.abs(y))
             if cplx:
                 misfit = np.concatenate([misfit, np.angle(y0) - np.angle(y)])
+        elif cplx and reform == "complex":
+            misfit = y0 - y  # the reference skips matreform for 'complex' (recon.py:192-198)
         elif cplx:
             misfit = np.concatenate([np.real(y0 - y), np.imag(y0 - y)])
         else:
@@ -728,7 +781,8 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
                 raise NotImplementedError("multi-wavelength mua reconstruction needs chromophore params")
             E = np.ones((1, 1))
             keys = ["mua"]
-        if nparts == 2:
+        cform = cplx and reform in ("reim", "complex")  # Re/Im parts of complex rows and unknowns
+        if nparts == 2 and not cform:
             E = np.vstack([E, E])  # the Im row blocks carry the same extinction weights
         JG = JT
         if dual:
@@ -759,13 +813,13 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
                 from scipy.linalg import qr
 
                 Aregu["lir"] = np.linalg.inv(np.triu(qr(Aregu["lmat"], mode="r")[0]))
-        if reform == "reim" and cplx:
-            # [[Jr, -Ji], [Ji, Jr]] rows; with several unknown blocks the reference's
-            # update parsing (recon.py:243-251) mixes real and imaginary parts
-            if len(keys) != 1 or nw != 1 or dual or labelled or Aregu:
-                raise NotImplementedError("reform 'reim' is supported for one unknown block on one mesh, "
-                                          "without priors or label compression")
-        misfit_t = _gpu.h2d(misfit, mesh.dev, np.float64)
+        if cform:
+            # 'reim' ([[Jr, -Ji], [Ji, Jr]] rows) and 'complex' (complex normal equations) keep
+            # the node unknowns of every block on one mesh
+            if dual or labelled or Aregu or over:
+                raise NotImplementedError("reform '%s' is supported for node unknowns on one mesh, without "
+                                          "priors or label compression" % reform)
+        misfit_t = _gpu.h2d(misfit, mesh.dev, np.complex128 if np.iscomplexobj(misfit) else np.float64)
         step = None
         if over:
             tp = mark("misfit", tp)
@@ -789,8 +843,10 @@ def runrecon(cfg, recon, detphi0, sd=None, **kwargs):
                 JG = invR.t() @ JG  # A' = A R^-1 per block: node-major J' = R^-T J
             else:
                 s2pre = None
-            if reform == "reim" and cplx:
-                step = GNStepReIm(JG, npairs, lam, precision, reftol=kwargs.get("reftol", 1e-10))
+            if reform == "complex" and cplx:
+                step = GNStepComplex(JG, npairs, E, lam, "fp64")
+            elif reform == "reim" and cplx:
+                step = GNStepReIm(JG, npairs, E, lam, precision, reftol=kwargs.get("reftol", 1e-10))
             else:
                 step = GNStep(JG, npairs, E, lam, precision, reftol=kwargs.get("reftol", 1e-10), scale2=s2pre)
             step.mark = (lambda name, t: mark(name, t)) if timing is not None else None

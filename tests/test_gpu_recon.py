@@ -87,9 +87,12 @@ def _fd_chrome(h=5.0, small=False):
 
 
 @pytest.mark.parametrize("reform,precision,tol", [("real", "fp64", 1e-7), ("real", "mixed", 1e-6),
-                                                  ("logphase", "fp64", 1e-6), ("logphase", "mixed", 1e-6)])
+                                                  ("logphase", "fp64", 1e-6), ("logphase", "mixed", 1e-6),
+                                                  ("reim", "fp64", 1e-7), ("reim", "mixed", 1e-6)])
 def test_runrecon_fd_chromophores_match_oracle(reform, precision, tol):
-    """Frequency-domain data: [Re; Im] rows (reform 'real') or log-amplitude/phase rows.
+    """Frequency-domain data: [Re; Im] rows (reform 'real'), log-amplitude/phase rows, or
+    [[Re, -Im], [Im, Re]] rows on [Re; Im] unknowns ('reim': 4 wavelengths x 3 chromophore blocks,
+    each block taking the real part of its complex Tikhonov step, recon.py:243-251).
 
     Log-amplitude rows are scaled by 1/|y|: the solver's 1e-12 relative
     residual becomes ~1e-12 * max|y|/|y| relative error of a weak detector
@@ -117,6 +120,33 @@ def test_runrecon_fd_chromophores_match_oracle(reform, precision, tol):
         assert err < tol, (k, err)
     np.testing.assert_allclose(out1[1], out2[1], rtol=1e-7 if (precision, reform) == ("fp64", "real") else 1e-5)
     assert out1[1][-1] < out1[1][0]
+
+
+def test_runrecon_fd_chromophores_complex_matches_oracle():
+    """reform 'complex' (recon.py:192-198): the unconjugated complex normal equations with the
+    complex blockscale 1/sqrt(sum J^2); the concentrations become complex, as in the reference.
+    A second iteration would assemble complex optical properties, which the GPU refuses."""
+    import redbirdpy_amd as rb
+    from oracle import rb_oracle as orc
+
+    cfg0, bg, extin, y0 = _fd_chrome()
+    c1, sd1 = rb.meshprep(copy.deepcopy(cfg0))
+    c2, sd2 = orc.meshprep(copy.deepcopy(cfg0))
+    lam = 1e-3 / sum(v.size for v in y0.values())
+    r1 = {"param": {k: v.copy() for k, v in bg.items()}}
+    r2 = {"param": {k: v.copy() for k, v in bg.items()}}
+    out1 = rb.runrecon(c1, r1, y0, sd1, maxiter=1, lambda_=lam, extin=extin, reform="complex", report=False)
+    out2 = orc.runrecon(c2, r2, y0, sd2, maxiter=1, lambda_=lam, extin=extin, reform="complex")
+    for k in bg:
+        a, b = r1["param"][k], r2["param"][k]
+        assert np.iscomplexobj(a) and np.iscomplexobj(b)
+        d1, d2 = a - bg[k], b - bg[k]  # the complex update itself
+        err = np.max(np.abs(d1 - d2)) / np.max(np.abs(d2))
+        assert err < 1e-7, (k, err)
+        assert np.max(np.abs(d2.imag)) > 1e-3 * np.max(np.abs(d2.real))  # a genuinely complex step
+    np.testing.assert_allclose(out1[1], out2[1], rtol=1e-9)
+    with pytest.raises(NotImplementedError):
+        rb.runrecon(c1, r1, y0, sd1, maxiter=1, lambda_=lam, extin=extin, reform="complex", report=False)
 
 
 def _mua_problem(omega, dual):
@@ -204,8 +234,12 @@ def test_runrecon_unsupported_forms_raise():
         rb.runrecon(c1, copy.deepcopy(recon), y0, sd1, maxiter=1, reform="reim", report=False)
     cfg0, recon, y0 = _mua_problem(2 * np.pi * 1e8, False)
     c1, sd1 = rb.meshprep(copy.deepcopy(cfg0))
-    with pytest.raises(NotImplementedError):
+    # a complex mua step added in place to the real prop array: numpy's casting error, as the
+    # reference's recon['prop'][:, 0] += dx (recon.py:276) raises
+    with pytest.raises(TypeError):
         rb.runrecon(c1, copy.deepcopy(recon), y0, sd1, maxiter=1, reform="complex", report=False)
+    with pytest.raises(NotImplementedError):  # the reference's runrecon cannot run two modes (jac gets a dict)
+        rb.runrecon(c1, copy.deepcopy(recon), y0, sd1, maxiter=1, rfcw=[1, 2], report=False)
 
 
 def _label_problem():

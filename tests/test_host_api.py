@@ -102,3 +102,43 @@ def test_extinction_updateprop_jacchrome_match_oracle_default_table():
         np.testing.assert_allclose(Jc1[k], Jc2[k], rtol=1e-15)
     with pytest.raises(ValueError):
         rb.jacchrome(J["650"], ["hbo"])
+
+
+@pytest.mark.parametrize("multi", [False, True])
+def test_sdmap_maxdist_and_exclusions_brute_force(multi):
+    """sdmap's excludesrc / excludedet (keyword or cfg field) and maxdist (utility.py:719-796)
+    against a brute-force table: excluded optodes leave the grid, pair row = d*Ns' + s over the
+    kept ones, a point pair is active iff its distance is below maxdist, wide-field pairs always
+    are; multi-wavelength cfgs give one 4-column table per wavelength.  Product and oracle."""
+    import redbirdpy_amd as rb
+    from oracle import rb_oracle as orc
+
+    rng = np.random.default_rng(5)
+    src = rng.uniform(0, 40, (6, 3))
+    det = rng.uniform(0, 40, (5, 3))
+    cfg = {"srcpos": src, "detpos": det, "prop": np.zeros((2, 4)),
+           "widesrc": np.zeros((10, 2)), "excludedet": [3]}  # nodes x 2 patterns
+    if multi:
+        cfg["prop"] = {"690": np.zeros((2, 4)), "830": np.zeros((2, 4))}
+    maxdist = 25.0
+    ks, kd = [0, 2, 3, 5], [0, 1, 2, 4]  # excludesrc [1, 4] by keyword, excludedet [3] from cfg
+    srcs = ks + [6, 7]  # the two wide-field sources follow the point sources
+    want = []
+    for d in kd:
+        for s in srcs:
+            act = 1.0 if s >= 6 else float(np.linalg.norm(src[s] - det[d]) < maxdist)
+            want.append([s, 8 + d, act])
+    want = np.array(want)
+    assert 0 < want[:, 2].sum() < len(want)  # the distance cut bites
+    for mod in (rb, orc):
+        sd = mod.sdmap(cfg, maxdist=maxdist, excludesrc=[1, 4])
+        tabs = sd.values() if multi else [sd]
+        if multi:
+            assert list(sd.keys()) == ["690", "830"]
+        for t in tabs:
+            assert t.shape == (len(want), 4 if multi else 3)
+            np.testing.assert_array_equal(t[:, :3], want)
+            if multi:
+                assert np.all(t[:, 3] == 1)
+    # no cut: every kept pair active
+    assert np.all(rb.sdmap({**cfg, "prop": np.zeros((2, 4))})[:, 2] == 1)
