@@ -344,6 +344,8 @@ __global__ void __launch_bounds__(KB) k_cg_spmm(Csr m, const T* __restrict__ val
     } else if (G == 64 && k % 64 == 0) {  // every chunk of 64 columns belongs to one system
 #pragma unroll
       for (int ch = 0; ch < NCH; ++ch) {
+        q[ch] = sc<T>::zero();
+        if (ch * G >= C) continue;  // a chunk past the last column has no system (and no values)
         T q1[1];
         row_gather_w64<T, 1>(m.col_idx, vals + (int64_t)((ch * G) / k) * m.nnz, rs, re, row, P + ch * G, C, lane,
                              C - ch * G, q1);
@@ -1606,6 +1608,8 @@ __global__ void __launch_bounds__(KB) k_outer_resid(Csr m, const T64* __restrict
       for (int ch = 0; ch < NCH; ++ch) {
         const int c = lane + ch * G;
         const T64* v = vals + (int64_t)((c < C ? c : 0) / k) * m.nnz;
+        ax[ch] = sc<T64>::zero();
+        if (ch * G >= C) continue;  // a chunk past the last column has no system (and no values)
         if (G == 64 && (k % 64 == 0 || k == C))  // each 64-column chunk within one system
           ax[ch] = row_gather_w64_xd<T64, T32>(m.col_idx, vals + (int64_t)((ch * G) / k) * m.nnz, rs, re, row, X,
                                                D, C, lane, c, c < C);
@@ -2206,8 +2210,12 @@ static int64_t resident_grid(const void* fn, int64_t nb) {
 
 template <typename T>
 static void spmm_plan(const Csr& m, int nsys, int C, int k, int64_t nb, bool* flat, int64_t* nbs) {
+  // the sweep addresses P through a buffer descriptor: under 2 GiB (RB_SPMM_FLAT_MAX_BYTES lowers the
+  // limit, so tests reach the row-per-group kernels of larger solves on small meshes)
+  int64_t lim = (int64_t)1 << 31;
+  if (const char* e = getenv("RB_SPMM_FLAT_MAX_BYTES")) lim = std::min(lim, (int64_t)atoll(e));
   *flat = pick_G(C) == 64 && (nsys == 1 || k % 64 == 0) && m.nnz > 0 &&
-          m.n * (int64_t)C * (int64_t)sizeof(T) < ((int64_t)1 << 31);
+          m.n * (int64_t)C * (int64_t)sizeof(T) < lim;
   *nbs = nb;
   if (!*flat) return;
   const int nch = pick_NCH(C);

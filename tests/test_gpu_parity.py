@@ -322,3 +322,34 @@ def test_irregular_mesh_forward_and_jacobian(omega):
     Jn2, Je2 = orc.jac(sd2, p2, c2["deldotdel"], c2["elem"], c2["evol"])
     assert np.linalg.norm(Jn1 - Jn2) / np.linalg.norm(Jn2) < 1e-12
     assert np.linalg.norm(Je1 - Je2) / np.linalg.norm(Je2) < 1e-12
+
+
+@pytest.mark.parametrize("precision,nw,ncol", [("mixed", 1, 272), ("fp64", 3, 64), ("mixed", 3, 64)])
+def test_row_group_spmm_chunks_past_the_last_column(precision, nw, ncol, monkeypatch):
+    """Solves too large for the sweep's buffer descriptor run the row-per-group SpMM and float64
+    residual kernels over 64-column chunks rounded up to a power of two: one system of 272
+    columns (chunks 5-7 lie past the last column) and three 64-column systems (chunk 3 past it)
+    have no values there.  RB_SPMM_FLAT_MAX_BYTES forces that path on a small box; results against
+    the oracle's SuperLU solves."""
+    import copy
+    from oracle import rb_oracle as orc
+    import redbirdpy_amd as rb
+
+    monkeypatch.setenv("RB_SPMM_FLAT_MAX_BYTES", "4096")
+    ns = ncol // 2
+    xs, ys = np.meshgrid(np.linspace(3, 17, 8), np.linspace(3, 17, (ns + 7) // 8))
+    src = np.column_stack([xs.ravel(), ys.ravel(), np.zeros(xs.size)])[:ns]
+    cfg = box_cfg((20, 20, 10), 2.0, src=src, det=src + [1.0, 1.0, 0.0])
+    if nw > 1:
+        cfg["prop"] = {str(650 + 25 * i): np.array([[0, 0, 1, 1], [0.004 + 0.003 * i, 1.0, 0, 1.37]])
+                       for i in range(nw)}
+    c1, _ = rb.meshprep(copy.deepcopy(cfg))
+    c2, _ = orc.meshprep(copy.deepcopy(cfg))
+    d1, f1 = rb.runforward(c1, precision=precision, tol=1e-12)
+    d2, f2 = orc.runforward(c2)
+    if nw == 1:
+        d1, d2, f1, f2 = {"": d1}, {"": d2}, {"": f1}, {"": f2}
+    for wv in d2:
+        assert f1[wv].shape[1] == ncol
+        assert _relmax(d1[wv], d2[wv]) < 1e-8
+        assert np.linalg.norm(f1[wv] - f2[wv]) / np.linalg.norm(f2[wv]) < 1e-9
