@@ -70,6 +70,30 @@ int rb_csr_pattern(const int32_t* el, int64_t ne, int64_t nn, const int32_t* inc
                    int32_t* diag_slot, void* stream);
 
 /*
+ * Mesh preparation on the GPU (replaces redbirdpy utility.py meshprep's
+ * host steps for one mesh or a batch of meshes concatenated with node
+ * offsets).  All indices 0-based.
+ * rb_tet_prep: per tet, local nodes 0 and 1 swapped when the signed volume
+ *   is negative (_meshreorient_fallback, utility.py:1047-1063), evol = |det|/6
+ *   (_elemvolume_fallback, :1090-1109) and deldotdel (ne x 10, :53-98).
+ * rb_boundary_faces: faces that lie in exactly one tet, oriented by the local
+ *   faces (0,2,1),(0,1,3),(0,3,2),(1,2,3), in lexicographic order of their
+ *   sorted vertex triples (_volface_fallback, :1066-1087); tinc from
+ *   rb_incidence (nv 4).  faces == NULL: counts into face_ptr (nn + 1,
+ *   exclusive scan) and *nf_out (host); then call again with faces (nf x 3).
+ *   RB_ERR_SIZE when a node owns more than 32 boundary faces.
+ * rb_face_area: half the cross-product norm per triangle (:1090-1109).
+ * rb_node_volume: a quarter of the incident tets' volumes (_nodevolume, :1005-1017).
+ */
+int rb_tet_prep(int64_t nn, int64_t ne, const double* node, const int32_t* el_in, int32_t* el_out, double* evol,
+                double* deldotdel, void* stream);
+int rb_boundary_faces(int64_t nn, const int32_t* el, const int32_t* tinc_ptr, const int32_t* tinc, int32_t* face_ptr,
+                      int32_t* faces, int64_t* nf_out /* host */, void* stream);
+int rb_face_area(int64_t nf, const double* node, const int32_t* faces, double* area, void* stream);
+int rb_node_volume(int64_t nn, const double* evol, const int32_t* tinc_ptr, const int32_t* tinc, double* nvol,
+                   void* stream);
+
+/*
  * Assembly slot map: for each CSR slot, the ascending list of contributing
  * (tet, i, j) / (face, i, j) codes; fixes the summation order of
  * rb_assemble.  tinc/finc are the tet (nv=4) and face (nv=3) incidences.

@@ -51,6 +51,10 @@ _SIGS = {
     "rb_zero": [_vp, _sz, _vp],
     "rb_incidence": [_vp, _i64, _int, _i64, _vp, _vp, _vp],
     "rb_csr_pattern": [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _i64, ctypes.POINTER(_i64), _vp, _vp],
+    "rb_tet_prep": [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
+    "rb_boundary_faces": [_i64, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_i64), _vp],
+    "rb_face_area": [_i64, _vp, _vp, _vp, _vp],
+    "rb_node_volume": [_i64, _vp, _vp, _vp, _vp, _vp],
     "rb_slot_map": [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp],
     "rb_assemble": [_vp, _vp, _i64, _vp, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _int, _int, _vp, _vp, _vp,
                     _i64, _i64, _vp, _vp, _int, _vp, _vp, _vp],

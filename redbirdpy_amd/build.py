@@ -15,7 +15,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
-SOURCES = ["rb_mesh.hip", "rb_assemble.hip", "rb_krylov.hip", "rb_jac.hip", "rb_gram.hip", "rb_widefield.hip",
+SOURCES = ["rb_mesh.hip", "rb_meshprep.hip", "rb_assemble.hip", "rb_krylov.hip", "rb_jac.hip", "rb_gram.hip", "rb_widefield.hip",
            "rb_bicgstab.hip", "rb_chol.hip"]
 OUT = os.path.join(HERE, "librbhip.so")
 FLAGS = [

@@ -7,6 +7,7 @@ kernel; the remaining geometry is vectorised host code, as in the reference.
 
 __all__ = [
     "meshprep",
+    "meshprep_batch",
     "deldotdel",
     "sdmap",
     "getoptodes",
@@ -23,7 +24,10 @@ __all__ = [
 
 import warnings
 
+import ctypes
+
 import numpy as np
+import torch
 
 # iso2mesh is not used: plane cuts, point location and boundary faces are
 # computed here and on the GPU (the flag keeps the reference's name)
@@ -468,6 +472,99 @@ def meshinterp(fromval, elemid, elembary, fromelem, toval=None):
         acc += w[:, j : j + 1] * fv[corners[:, j]]
     out[inside] = acc
     return out if out.shape[1] > 1 else out[:, 0]
+
+
+_MESH_FIELDS = ("isreoriented", "face", "area", "evol", "nvol", "cols", "idxsum", "deldotdel")
+
+
+def meshprep_batch(cfgs):
+    """meshprep for many meshes (utility.py:101-246), their geometry in one GPU launch sequence.
+
+    The meshes are concatenated with node offsets (faces never cross meshes)
+    and reoriented, measured and given their deldotdel (rb_tet_prep), node
+    incidence, boundary faces in the reference's order (rb_boundary_faces),
+    face areas, node volumes and CSR pattern on the device; the results are
+    split back per mesh.  The rest of meshprep (properties, Reff, wide-field
+    sources, the sd map) runs per mesh as in meshprep.  A cfg that already
+    carries mesh fields goes through meshprep itself.  Returns [(cfg, sd)].
+    """
+    from . import _gpu
+    from ._abi import call, ptr, stream_ptr
+
+    cfgs = [forcearray(c, _ARRAY_KEYS) for c in cfgs]
+    fresh = [i for i, c in enumerate(cfgs) if not any(c.get(k) is not None and c.get(k) is not False
+                                                       for k in _MESH_FIELDS)]
+    out = [None] * len(cfgs)
+    for i, c in enumerate(cfgs):
+        if "node" not in c or "elem" not in c:
+            raise ValueError("cfg must contain 'node' and 'elem'")
+        if i not in fresh:
+            out[i] = meshprep(c)
+    if not fresh:
+        return out
+    nodes, elems, noff, eoff = [], [], [0], [0]
+    for i in fresh:
+        c = cfgs[i]
+        if c["elem"].shape[1] > 4 and c.get("seg") is None:
+            c["seg"] = c["elem"][:, 4].astype(int)
+        nodes.append(np.asarray(c["node"], dtype=np.float64))
+        elems.append(np.asarray(c["elem"])[:, :4].astype(np.int64) - 1 + noff[-1])
+        noff.append(noff[-1] + nodes[-1].shape[0])
+        eoff.append(eoff[-1] + elems[-1].shape[0])
+    nn, ne = noff[-1], eoff[-1]
+    dev = _gpu.device()
+    s = stream_ptr()
+    nd = _gpu.h2d(np.vstack(nodes), dev)
+    el_in = _gpu.h2d(np.vstack(elems).astype(np.int32), dev)
+    el = torch.empty_like(el_in)
+    evol = torch.empty(ne, dtype=torch.float64, device=dev)
+    dd = torch.empty((ne, 10), dtype=torch.float64, device=dev)
+    call("rb_tet_prep", nn, ne, ptr(nd), ptr(el_in), ptr(el), ptr(evol), ptr(dd), s)
+    tinc_ptr = torch.empty(nn + 1, dtype=torch.int32, device=dev)
+    tinc = torch.empty(4 * ne, dtype=torch.int32, device=dev)
+    call("rb_incidence", ptr(el), ne, 4, nn, ptr(tinc_ptr), ptr(tinc), s)
+    face_ptr = torch.empty(nn + 1, dtype=torch.int32, device=dev)
+    nf = ctypes.c_int64(0)
+    call("rb_boundary_faces", nn, ptr(el), ptr(tinc_ptr), ptr(tinc), ptr(face_ptr), None, ctypes.byref(nf), s)
+    faces = torch.empty((max(nf.value, 1), 3), dtype=torch.int32, device=dev)
+    call("rb_boundary_faces", nn, ptr(el), ptr(tinc_ptr), ptr(tinc), ptr(face_ptr), ptr(faces), ctypes.byref(nf), s)
+    area = torch.empty(max(nf.value, 1), dtype=torch.float64, device=dev)
+    call("rb_face_area", nf.value, ptr(nd), ptr(faces), ptr(area), s)
+    nvol = torch.empty(nn, dtype=torch.float64, device=dev)
+    call("rb_node_volume", nn, ptr(evol), ptr(tinc_ptr), ptr(tinc), ptr(nvol), s)
+    row_ptr = torch.empty(nn + 1, dtype=torch.int32, device=dev)
+    nnz = ctypes.c_int64(0)
+    call("rb_csr_pattern", ptr(el), ne, nn, ptr(tinc_ptr), ptr(tinc), ptr(row_ptr), None, 0, ctypes.byref(nnz), None,
+         s)
+    col_idx = torch.empty(max(nnz.value, 1), dtype=torch.int32, device=dev)
+    call("rb_csr_pattern", ptr(el), ne, nn, ptr(tinc_ptr), ptr(tinc), ptr(row_ptr), ptr(col_idx), nnz.value,
+         ctypes.byref(nnz), None, s)
+    el_h, evol_h, dd_h = el.cpu().numpy(), evol.cpu().numpy(), dd.cpu().numpy()
+    fp_h, fc_h, ar_h = face_ptr.cpu().numpy(), faces[: nf.value].cpu().numpy(), area[: nf.value].cpu().numpy()
+    nv_h, rp_h, ci_h = nvol.cpu().numpy(), row_ptr.cpu().numpy(), col_idx[: nnz.value].cpu().numpy()
+    for j, i in enumerate(fresh):
+        c = cfgs[i]
+        n0, n1, e0, e1 = noff[j], noff[j + 1], eoff[j], eoff[j + 1]
+        ro = (el_h[e0:e1] - n0 + 1).astype(c["elem"].dtype if np.issubdtype(c["elem"].dtype, np.integer) else np.int64)
+        c["elem"] = np.column_stack([ro, c["elem"][:, 4:]]) if c["elem"].shape[1] > 4 else ro
+        c["isreoriented"] = True
+        ev = evol_h[e0:e1].copy()
+        if np.any(ev == 0):
+            raise ValueError("Degenerate elements detected at indices: %s" % np.flatnonzero(ev == 0))
+        c["evol"] = ev
+        f0, f1 = fp_h[n0], fp_h[n1]
+        c["face"] = fc_h[f0:f1].astype(np.int64) - n0 + 1
+        c["area"] = ar_h[f0:f1].copy()
+        c["nvol"] = nv_h[n0:n1].copy()
+        rp = rp_h[n0 : n1 + 1].astype(np.int64) - rp_h[n0]
+        ci = ci_h[rp_h[n0] : rp_h[n1]].astype(np.int64) - n0
+        rows = np.repeat(np.arange(n1 - n0), np.diff(rp))
+        off = ci != rows
+        c["rows"], c["cols"] = rows[off], ci[off]
+        c["idxcount"] = np.bincount(rows[off], minlength=n1 - n0)
+        c["deldotdel"] = dd_h[e0:e1].copy()
+        out[i] = meshprep(c)  # properties, Reff, wide-field sources, idxsum, omega and the sd map
+    return out
 
 
 def meshprep(cfg):
