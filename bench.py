@@ -56,6 +56,8 @@ def _args():
                    help="forward solver tolerance of both Gauss-Newton legs (redbirdpy femsolve's default)")
     p.add_argument("--gn-precisions", default="mixed,fp64", help="comma list of GN legs to run")
     p.add_argument("--no-kernels", action="store_true", help="skip the Jacobian and Gram kernel legs")
+    p.add_argument("--no-batch", dest="batch", action="store_false", help="skip the 64-mesh uneven batch leg")
+    p.add_argument("--batch-meshes", type=int, default=64)
     p.add_argument("--gn-phases", action="store_true", help="add a synchronising per-phase timing run")
     p.add_argument("--cpu-h", type=float, default=1.3, help="mesh spacing of the bounded CPU sample")
     p.add_argument("--copy-calibration", action="store_true",
@@ -253,6 +255,65 @@ def gn_bench(a):
         out["speedup_mixed_vs_fp64"] = out["mixed"]["gn_iters_per_s"] / out["fp64"]["gn_iters_per_s"]
     if not a.skip_cpu:
         out["cpu_baseline"] = cpu_gn_baseline()
+    return out
+
+
+def batch_bench(a):
+    """The uneven batch: 64 seeded Kuhn meshes (2,000-200,000 nodes each), point sources plus 16
+    Fourier patterns and as many detectors per mesh; GPU mesh preparation for all meshes
+    (meshprep_batch), then every mesh's forward solve in one block-diagonal multi-RHS solve
+    (BatchForward), float64 and mixed, tolerance 1e-10.  RHS-solves/s counts each mesh's own
+    source and detector columns; the timed solve includes assembly, the Krylov solve and the
+    detector values (the batch's mesh, optodes and right-hand sides stay resident)."""
+    import ctypes
+    import torch
+    import redbirdpy_amd as rb
+    from redbirdpy_amd import workloads, _abi
+    from redbirdpy_amd.forward import BatchForward
+
+    cfgs = workloads.batch_boxes(count=a.batch_meshes)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    prepped = [c for c, _ in rb.meshprep_batch(cfgs)]
+    torch.cuda.synchronize()
+    t_prep = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    bf = BatchForward(prepped)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    mesh = bf.mesh
+    out = {"workload": "%d seeded Kuhn boxes, %d nodes / %d tets in all (%d-%d nodes each), %d columns per mesh "
+                       "at most, %d per-mesh columns in all" % (len(cfgs), mesh.nn, mesh.ne,
+                                                                min(c["node"].shape[0] for c in cfgs),
+                                                                max(c["node"].shape[0] for c in cfgs), bf.C,
+                                                                bf.columns),
+           "meshprep_batch_s": t_prep, "setup_s": t_setup, "tol": 1e-10}
+    L = _abi.lib()
+    dets = {}
+    for prec in ("fp64", "mixed"):
+        bf.solve(prec, 1e-10, fields=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = bf.solve(prec, 1e-10, fields=False)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        dets[prec] = res
+        L.rb_prof_reset()
+        L.rb_prof_enable(1)
+        bf.solve(prec, 1e-10, fields=False)
+        torch.cuda.synchronize()
+        L.rb_prof_enable(0)
+        ms, cnt = ctypes.c_double(0), ctypes.c_int64(0)
+        _abi.call("rb_prof_read", _abi.RB_PROF_SPMM, ctypes.byref(ms), ctypes.byref(cnt))
+        sv = 8 if prec == "fp64" else 4
+        avg = ms.value / 1e3 / max(cnt.value, 1)
+        nbytes = _spmm_bytes(mesh.nn, mesh.nnz, bf.C, sv)
+        out[prec] = {"rhs_solves_per_s": bf.columns / dt, "s": dt, "iters": bf.info.iters, "relres": bf.info.relres,
+                     "spmm_avg_us": avg * 1e6, "spmm_bytes": nbytes, "spmm_frac_hbm": nbytes / avg / 1e9 / HBM_PEAK_GBS}
+    out["speedup_mixed_vs_fp64"] = out["fp64"]["s"] / out["mixed"]["s"]
+    out["mixed"]["err_detval_maxrel_vs_fp64"] = max(
+        float(np.max(np.abs(x.detval - y.detval) / np.abs(y.detval))) for x, y in zip(dets["mixed"], dets["fp64"])
+        if y.detval is not None)
     return out
 
 
@@ -456,6 +517,8 @@ def main():
         out["speedup_mixed_vs_fp64"] = dt_d / dt_m
     if a.gn:
         out["gn"] = gn_bench(a)
+    if a.batch:
+        out["batch"] = batch_bench(a)
     if not a.no_kernels:
         out["jacobian"] = jac_bench(a)
         out["gram"] = gram_bench(mesh.nn)

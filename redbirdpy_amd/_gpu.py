@@ -500,10 +500,42 @@ def warn_unconverged(info, where):
                       % (where, why, info.iters, info.relres), SolverWarning, stacklevel=3)
 
 
+SWEEP_MAX_BYTES = 1 << 31
+
+
+def sweep_columns(n, k, nsys, is_complex):
+    """Columns per solve call so that a one-system solve of k columns keeps its float64 vectors
+    (n rows) under the column sweep's 2 GiB buffer descriptor: k itself when it fits (or when
+    no multiple of 64 columns fits, or several systems share the call), else the largest
+    multiple of 64 that fits.  RB_SPMM_FLAT_MAX_BYTES lowers the limit as in the library."""
+    lim = SWEEP_MAX_BYTES
+    env = os.environ.get("RB_SPMM_FLAT_MAX_BYTES")
+    if env:
+        lim = min(lim, int(env))
+    row = n * (16 if is_complex else 8)
+    if nsys != 1 or k <= 64 or row * k < lim:
+        return k
+    g = (lim - 1) // row // 64 * 64
+    return g if g >= 64 else k
+
+
 def solve_system(sysm, B, k, precision="fp64", tol=1e-12, maxiter=2000, X0=None):
     """CG/COCG over every system; after a breakdown (flag 4) every system is finished by
     BiCGSTAB from its CG iterate (its columns that had converged stop at once)."""
     m = sysm.mesh
+    grp = sweep_columns(m.nn, k, sysm.nsys, sysm.is_complex)
+    if grp < k:
+        # the column sweep addresses a solve's vectors through 32-bit buffer offsets (float64 rows
+        # under 2 GiB); a wider solve of one system runs as column groups that fit it
+        parts, infos = [], []
+        for c0 in range(0, k, grp):
+            c1 = min(k, c0 + grp)
+            Xi, inf = solve_system(sysm, B[:, c0:c1].contiguous(), c1 - c0, precision, tol, maxiter,
+                                   None if X0 is None else X0[:, c0:c1].contiguous())
+            parts.append(Xi)
+            infos.append(inf)
+        return torch.cat(parts, dim=1), SolveInfo(max(i.iters for i in infos), max(i.relres for i in infos),
+                                                  max(i.flag for i in infos))
     # the LDS-staged SpMM serves 64-column solves of one system with 8-byte inner values
     # (complex64 mixed, float64 plain)
     lds = sysm.nsys == 1 and k == 64 and (sysm.is_complex == (precision == "mixed"))

@@ -2442,6 +2442,14 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
   bool flat;
   int64_t nbs;
   spmm_plan<T32>(m, nsys, C, k, nb, &flat, &nbs);
+  {
+    // the fused residual pass addresses x64 through a buffer descriptor as well: its float64
+    // rows must fit the same limit (else both kernels take the row-per-group path)
+    bool flat64;
+    int64_t unused;
+    spmm_plan<T64>(m, nsys, C, k, nb, &flat64, &unused);
+    flat = flat && flat64;
+  }
   // real systems of 64 columns each, whole chunk pairs: one sweep per pair of systems
   const bool pair = std::is_same<T32, float>::value && pair_sweep<T32>(nsys, C, k);
   // the LDS-staged SpMM (a plan for this 64-column system): its own launch, then the

@@ -9,7 +9,7 @@ numpy/scipy objects like the reference; ``forward_device`` keeps results on
 the GPU for the reconstruction loop and for autograd.
 """
 
-__all__ = ["runforward", "femlhs", "femrhs", "femgetdet", "jac", "jacchrome", "C0", "forward_batch"]
+__all__ = ["runforward", "femlhs", "femrhs", "femgetdet", "jac", "jacchrome", "C0", "forward_batch", "BatchForward"]
 
 import numpy as np
 import torch
@@ -231,8 +231,149 @@ class BatchResult:
         self.jac = jac
 
 
-def forward_batch(cfgs, precision="fp64", tol=1e-12, maxiter=None, jacobian=False):
+class BatchForward:
     """Forward problems of several meshes in one multi-RHS solve (one launch sequence for all).
+
+    cfgs: meshprep'ed single-wavelength cfgs with label or element
+    properties, the same omega and the same Reff (the Robin coefficient is
+    one per assembly).  The meshes are translated apart (FEM matrices do not
+    depend on translation; the node order then keeps each mesh contiguous),
+    assembled as one block-diagonal system, and every mesh puts its sources
+    in the shared columns [0, TS) and its detectors in [TS, TS + TD) of its
+    own rows.  The solve runs to tol / sqrt(number of meshes) relative to a
+    column's norm over the batch after every mesh's part of every column is
+    scaled to unit norm, which bounds each mesh's own relative residual by
+    tol whatever the meshes' RHS norms.  Construction keeps the batch's mesh,
+    located optodes and right-hand sides on the device; solve() assembles and
+    solves (each call the same work) and returns a BatchResult per cfg.
+    """
+
+    def __init__(self, cfgs):
+        if not cfgs:
+            raise ValueError("BatchForward needs at least one cfg")
+        for c in cfgs:
+            if isinstance(c.get("prop"), dict):
+                raise NotImplementedError("forward_batch takes single-wavelength cfgs")
+        om = float(cfgs[0].get("omega", 0))
+        reff = float(cfgs[0]["reff"])
+        if any(float(c.get("omega", 0)) != om or float(c["reff"]) != reff for c in cfgs):
+            raise NotImplementedError("forward_batch needs one omega and one Reff for the whole batch")
+        nodes, elems, faces, evol, area, dd, props, shifts, noff = [], [], [], [], [], [], [], [], []
+        x0, nn = 0.0, 0
+        for c in cfgs:
+            nd = np.asarray(c["node"], dtype=float)
+            shift = np.array([x0 - nd[:, 0].min(), 0.0, 0.0])
+            x0 += (nd[:, 0].max() - nd[:, 0].min()) + 10.0
+            nodes.append(nd + shift)
+            elems.append(np.asarray(c["elem"])[:, :4].astype(np.int64) + nn)
+            faces.append(np.asarray(c["face"])[:, :3].astype(np.int64) + nn)
+            evol.append(np.asarray(c["evol"], dtype=float))
+            area.append(np.asarray(c["area"], dtype=float))
+            if c.get("deldotdel") is None:
+                c["deldotdel"], _ = deldotdel(c)
+            dd.append(np.asarray(c["deldotdel"], dtype=float))
+            props.append(_element_props(c))
+            shifts.append(shift)
+            noff.append(nn)
+            nn += nd.shape[0]
+        big = {"node": np.vstack(nodes), "elem": np.vstack(elems), "face": np.vstack(faces),
+               "evol": np.concatenate(evol), "area": np.concatenate(area), "prop": np.vstack(props),
+               "omega": om, "reff": reff, "isreoriented": True}
+        big["deldotdel"] = np.vstack(dd)
+        mesh = _gpu.mesh_for(big)
+        ops = []
+        for m, c in enumerate(cfgs):
+            src, det, wsrc, wdet = getoptodes(c)
+            op = _Optodes.__new__(_Optodes)
+            op.ns, op.nd, op.ws, op.wd = optode_counts(c)
+            op.src = np.atleast_2d(src) + shifts[m] if op.ns else None
+            op.det = np.atleast_2d(det) + shifts[m] if op.nd else None
+            op.sloc, op.sbary = _locate(mesh, op.src)
+            op.dloc, op.dbary = _locate(mesh, op.det)
+            nm = np.asarray(c["node"]).shape[0]
+
+            def embed(w):  # this mesh's wide columns in the batch's node rows
+                if w is None:
+                    return None
+                e = np.zeros((nn, w.shape[1]))
+                e[noff[m] : noff[m] + nm] = w
+                return e
+
+            op.wsrc = embed(np.asarray(wsrc)) if op.ws else None
+            op.wdet = embed(np.asarray(wdet)) if op.wd else None
+            op.k = op.ns + op.ws + op.nd + op.wd
+            ops.append(op)
+        TS = max(o.ns + o.ws for o in ops)
+        TD = max(o.nd + o.wd for o in ops)
+        C = TS + TD
+        if C > 512:
+            raise ValueError("forward_batch: at most 512 source + detector columns per mesh")
+        self.is_complex = om > 0
+        dt = torch.complex128 if self.is_complex else torch.float64
+        B = _gpu.zeros((mesh.nn, C), dt, mesh.dev)
+        for m, op in enumerate(ops):
+            rows = torch.as_tensor(mesh.iperm[noff[m] : noff[m] + np.asarray(cfgs[m]["node"]).shape[0]],
+                                   device=mesh.dev)
+            if op.ns:
+                _gpu.rhs_block(mesh, B, 0, _loc0(op.sloc), op.sbary, self.is_complex)
+            if op.ws:
+                B[rows, op.ns : op.ns + op.ws] = torch.as_tensor(op.wsrc[noff[m] : noff[m] + len(rows)],
+                                                                 device=mesh.dev, dtype=dt)
+            if op.nd:
+                _gpu.rhs_block(mesh, B, TS, _loc0(op.dloc), op.dbary, self.is_complex)
+            if op.wd:
+                B[rows, TS + op.nd : TS + op.nd + op.wd] = torch.as_tensor(op.wdet[noff[m] : noff[m] + len(rows)],
+                                                                           device=mesh.dev, dtype=dt)
+        # every mesh's part of every column scaled to unit norm (the system is block-diagonal, so
+        # the solution block scales alike): a column's residual over the batch, <= tol/sqrt(M)
+        # times its norm sqrt(#meshes using it), then bounds each mesh's own relative residual by tol
+        self.row_blocks = [torch.as_tensor(mesh.iperm[noff[m] : noff[m] + np.asarray(c["node"]).shape[0]],
+                                           device=mesh.dev) for m, c in enumerate(cfgs)]
+        self.scales = []
+        for rows in self.row_blocks:
+            nrm = torch.linalg.norm(B[rows], dim=0)
+            sc = torch.where(nrm > 0, nrm, torch.ones_like(nrm))
+            B[rows] = B[rows] / sc
+            self.scales.append(sc)
+        self.detcols = [op.det_columns(big, mesh) if (op.ns + op.ws) and (op.nd + op.wd) else None for op in ops]
+        self.cfgs, self.big, self.mesh, self.ops, self.noff = cfgs, big, mesh, ops, noff
+        self.TS, self.TD, self.C, self.B = TS, TD, C, B
+        self.columns = sum(op.k for op in ops)  # each mesh's own source and detector columns
+        self.info = None
+
+    def solve(self, precision="fp64", tol=1e-12, maxiter=None, fields=True, jacobian=False):
+        mesh, big, TS, C = self.mesh, self.big, self.TS, self.C
+        sysm = _gpu.assemble(mesh, big, big["deldotdel"], [("", 1)], want32=(precision == "mixed"))
+        if maxiter is None:
+            maxiter = max(4000, 4 * int(np.sqrt(mesh.nn)) + 1000)
+        X, info = _gpu.solve_system(sysm, self.B, C, precision, tol / np.sqrt(len(self.cfgs)), maxiter)
+        for rows, sc in zip(self.row_blocks, self.scales):
+            X[rows] = X[rows] * sc
+        _gpu.warn_unconverged(info, "forward_batch")
+        self.info = info
+        self.X = X
+        Xu = mesh.user_rows(X) if fields or jacobian else None
+        out = []
+        for m, op in enumerate(self.ops):
+            ts, td = op.ns + op.ws, op.nd + op.wd
+            det_val = None
+            if self.detcols[m] is not None:
+                p, r, v = self.detcols[m]
+                det_val = _gpu.getdet(p, r, v, X, 0, ts, sysm.is_complex).cpu().numpy()
+            phi = J = None
+            if Xu is not None:
+                nm = np.asarray(self.cfgs[m]["node"]).shape[0]
+                blk = Xu[self.noff[m] : self.noff[m] + nm]
+                phi = torch.cat([blk[:, :ts], blk[:, TS : TS + td]], dim=1).cpu().numpy()
+                if jacobian and ts and td:
+                    c = self.cfgs[m]
+                    J, _ = jac(sdmap(c), phi, c["deldotdel"], c["elem"], c["evol"], return_elem=False)
+            out.append(BatchResult(det_val, phi, J))
+        return out
+
+
+def forward_batch(cfgs, precision="fp64", tol=1e-12, maxiter=None, jacobian=False):
+    """Forward problems of several meshes in one multi-RHS solve (BatchForward(cfgs).solve(...)).
 
     cfgs: meshprep'ed single-wavelength cfgs with label or element
     properties, the same omega and the same Reff (the Robin coefficient is
@@ -247,113 +388,9 @@ def forward_batch(cfgs, precision="fp64", tol=1e-12, maxiter=None, jacobian=Fals
     """
     if not cfgs:
         return []
-    for c in cfgs:
-        if isinstance(c.get("prop"), dict):
-            raise NotImplementedError("forward_batch takes single-wavelength cfgs")
-    om = float(cfgs[0].get("omega", 0))
-    reff = float(cfgs[0]["reff"])
-    if any(float(c.get("omega", 0)) != om or float(c["reff"]) != reff for c in cfgs):
-        raise NotImplementedError("forward_batch needs one omega and one Reff for the whole batch")
-    nodes, elems, faces, evol, area, dd, props, shifts, noff = [], [], [], [], [], [], [], [], []
-    x0, nn = 0.0, 0
-    for c in cfgs:
-        nd = np.asarray(c["node"], dtype=float)
-        shift = np.array([x0 - nd[:, 0].min(), 0.0, 0.0])
-        x0 += (nd[:, 0].max() - nd[:, 0].min()) + 10.0
-        nodes.append(nd + shift)
-        elems.append(np.asarray(c["elem"])[:, :4].astype(np.int64) + nn)
-        faces.append(np.asarray(c["face"])[:, :3].astype(np.int64) + nn)
-        evol.append(np.asarray(c["evol"], dtype=float))
-        area.append(np.asarray(c["area"], dtype=float))
-        if c.get("deldotdel") is None:
-            c["deldotdel"], _ = deldotdel(c)
-        dd.append(np.asarray(c["deldotdel"], dtype=float))
-        props.append(_element_props(c))
-        shifts.append(shift)
-        noff.append(nn)
-        nn += nd.shape[0]
-    big = {"node": np.vstack(nodes), "elem": np.vstack(elems), "face": np.vstack(faces),
-           "evol": np.concatenate(evol), "area": np.concatenate(area), "prop": np.vstack(props),
-           "omega": om, "reff": reff, "isreoriented": True}
-    big["deldotdel"] = np.vstack(dd)
-    mesh = _gpu.mesh_for(big)
-    sysm = _gpu.assemble(mesh, big, big["deldotdel"], [("", 1)], want32=(precision == "mixed"))
-    ops = []
-    for m, c in enumerate(cfgs):
-        src, det, wsrc, wdet = getoptodes(c)
-        op = _Optodes.__new__(_Optodes)
-        op.ns, op.nd, op.ws, op.wd = optode_counts(c)
-        op.src = np.atleast_2d(src) + shifts[m] if op.ns else None
-        op.det = np.atleast_2d(det) + shifts[m] if op.nd else None
-        op.sloc, op.sbary = _locate(mesh, op.src)
-        op.dloc, op.dbary = _locate(mesh, op.det)
-        nm = np.asarray(c["node"]).shape[0]
-
-        def embed(w):  # this mesh's wide columns in the batch's node rows
-            if w is None:
-                return None
-            e = np.zeros((nn, w.shape[1]))
-            e[noff[m] : noff[m] + nm] = w
-            return e
-
-        op.wsrc = embed(np.asarray(wsrc)) if op.ws else None
-        op.wdet = embed(np.asarray(wdet)) if op.wd else None
-        op.k = op.ns + op.ws + op.nd + op.wd
-        ops.append(op)
-    TS = max(o.ns + o.ws for o in ops)
-    TD = max(o.nd + o.wd for o in ops)
-    C = TS + TD
-    if C > 512:
-        raise ValueError("forward_batch: at most 512 source + detector columns per mesh")
-    dt = torch.complex128 if sysm.is_complex else torch.float64
-    B = _gpu.zeros((mesh.nn, C), dt, mesh.dev)
-    for m, op in enumerate(ops):
-        rows = torch.as_tensor(mesh.iperm[noff[m] : noff[m] + np.asarray(cfgs[m]["node"]).shape[0]],
-                               device=mesh.dev)
-        if op.ns:
-            _gpu.rhs_block(mesh, B, 0, _loc0(op.sloc), op.sbary, sysm.is_complex)
-        if op.ws:
-            B[rows, op.ns : op.ns + op.ws] = torch.as_tensor(op.wsrc[noff[m] : noff[m] + len(rows)], device=mesh.dev,
-                                                             dtype=dt)
-        if op.nd:
-            _gpu.rhs_block(mesh, B, TS, _loc0(op.dloc), op.dbary, sysm.is_complex)
-        if op.wd:
-            B[rows, TS + op.nd : TS + op.nd + op.wd] = torch.as_tensor(op.wdet[noff[m] : noff[m] + len(rows)],
-                                                                       device=mesh.dev, dtype=dt)
-    if maxiter is None:
-        maxiter = max(4000, 4 * int(np.sqrt(mesh.nn)) + 1000)
-    # every mesh's part of every column scaled to unit norm (the system is block-diagonal, so
-    # the solution block scales alike): a column's residual over the batch, <= tol/sqrt(M)
-    # times its norm sqrt(#meshes using it), then bounds each mesh's own relative residual by tol
-    row_blocks = [torch.as_tensor(mesh.iperm[noff[m] : noff[m] + np.asarray(c["node"]).shape[0]], device=mesh.dev)
-                  for m, c in enumerate(cfgs)]
-    scales = []
-    for rows in row_blocks:
-        nrm = torch.linalg.norm(B[rows], dim=0)
-        sc = torch.where(nrm > 0, nrm, torch.ones_like(nrm))
-        B[rows] = B[rows] / sc
-        scales.append(sc)
-    X, info = _gpu.solve_system(sysm, B, C, precision, tol / np.sqrt(len(cfgs)), maxiter)
-    for rows, sc in zip(row_blocks, scales):
-        X[rows] = X[rows] * sc
-    _gpu.warn_unconverged(info, "forward_batch")
-    Xu = mesh.user_rows(X)
-    out = []
-    for m, op in enumerate(ops):
-        ts, td = op.ns + op.ws, op.nd + op.wd
-        det_val = None
-        if ts and td:
-            p, r, v = op.det_columns(big, mesh)
-            det_val = _gpu.getdet(p, r, v, X, 0, ts, sysm.is_complex).cpu().numpy()
-        nm = np.asarray(cfgs[m]["node"]).shape[0]
-        blk = Xu[noff[m] : noff[m] + nm]
-        phi = torch.cat([blk[:, :ts], blk[:, TS : TS + td]], dim=1).cpu().numpy()
-        J = None
-        if jacobian and ts and td:
-            sd = sdmap(cfgs[m])
-            J, _ = jac(sd, phi, cfgs[m]["deldotdel"], cfgs[m]["elem"], cfgs[m]["evol"], return_elem=False)
-        out.append(BatchResult(det_val, phi, J))
-    forward_batch.last_info = info
+    bf = BatchForward(cfgs)
+    out = bf.solve(precision, tol, maxiter, fields=True, jacobian=jacobian)
+    forward_batch.last_info = bf.info
     return out
 
 

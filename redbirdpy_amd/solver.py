@@ -78,17 +78,20 @@ def femsolve(Amat, rhs, method="auto", **kwargs):
     vals = torch.as_tensor(A.data.astype(complex if is_complex else float), device=dev).reshape(1, -1)
     x = np.zeros((n, ncol), dtype=complex if is_complex else float)
     x0 = kwargs.get("x0", None)
+    # chunks of at most 512 columns; the column sweep's buffer descriptor (float64 vectors under
+    # 2 GiB) may ask for narrower ones on large meshes
+    step = max(1, 512 if use_bicgstab else _gpu.sweep_columns(n, min(ncol, 512), 1, is_complex))
     # the LDS-staged SpMM serves 64-column chunks with 8-byte inner values (complex64 mixed, float64 plain)
     plan = None
-    if not use_bicgstab and is_complex == (precision == "mixed") and ncol % 512 == 64:
+    if not use_bicgstab and is_complex == (precision == "mixed") and (step == 64 or ncol % step == 64):
         import os
 
         if os.environ.get("RB_LDS_SPMM", "1") != "0":
             plan = _gpu.spmm_plan(A.indptr, A.indices, dev)
     flag = 0
     info = None
-    for c0 in range(0, ncol, 512):
-        c1 = min(ncol, c0 + 512)
+    for c0 in range(0, ncol, step):
+        c1 = min(ncol, c0 + step)
         B = torch.as_tensor(np.ascontiguousarray(rhs[:, c0:c1]), device=dev).to(rdt)
         X0 = None
         if x0 is not None:

@@ -353,3 +353,32 @@ def test_row_group_spmm_chunks_past_the_last_column(precision, nw, ncol, monkeyp
         assert f1[wv].shape[1] == ncol
         assert _relmax(d1[wv], d2[wv]) < 1e-8
         assert np.linalg.norm(f1[wv] - f2[wv]) / np.linalg.norm(f2[wv]) < 1e-9
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_wide_solve_runs_as_column_groups_that_fit_the_sweep(precision, monkeypatch):
+    """A one-system solve whose float64 vectors exceed the sweep's descriptor limit runs as
+    64-column groups (the limit lowered so 272 columns of a 726-node box do not fit and 64 do):
+    femsolve and the batch path's solve_system both, against the oracle's SuperLU solves."""
+    import copy
+    from oracle import rb_oracle as orc
+    import redbirdpy_amd as rb
+    from redbirdpy_amd import _gpu
+    from redbirdpy_amd.forward import BatchForward
+
+    xs, ys = np.meshgrid(np.linspace(3, 17, 8), np.linspace(3, 17, 17))
+    src = np.column_stack([xs.ravel(), ys.ravel(), np.zeros(xs.size)])
+    cfg = box_cfg((20, 20, 10), 2.0, src=src, det=src + [1.0, 1.0, 0.0])
+    c1, _ = rb.meshprep(copy.deepcopy(cfg))
+    c2, _ = orc.meshprep(copy.deepcopy(cfg))
+    n = c1["node"].shape[0]
+    monkeypatch.setenv("RB_SPMM_FLAT_MAX_BYTES", str(n * 8 * 100))
+    assert _gpu.sweep_columns(n, 272, 1, False) == 64
+    d1, f1 = rb.runforward(c1, precision=precision, tol=1e-12)
+    d2, f2 = orc.runforward(c2)
+    assert f1.shape[1] == 272
+    assert _relmax(d1, d2) < 1e-8
+    assert np.linalg.norm(f1 - f2) / np.linalg.norm(f2) < 1e-9
+    res = BatchForward([c1]).solve(precision, 1e-12)
+    assert _relmax(res[0].detval, d2) < 1e-8
+    assert np.linalg.norm(res[0].phi - f2) / np.linalg.norm(f2) < 1e-9

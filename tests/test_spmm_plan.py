@@ -113,3 +113,21 @@ def test_plan_refuses_a_row_beyond_the_limits():
                               ctypes.byref(a), ctypes.byref(a), ctypes.byref(a))
     assert rc == -1  # no kernel for these limits
     assert _gpu.spmm_plan(A.indptr, A.indices, "cpu") is None
+
+
+def test_sweep_columns_groups_of_64_under_the_descriptor_limit(monkeypatch):
+    """Columns per solve call: the whole width while its float64 vectors fit 2 GiB, else the
+    largest multiple of 64 that fits; several systems or no fitting 64-column group keep the width."""
+    from redbirdpy_amd import _gpu
+
+    monkeypatch.delenv("RB_SPMM_FLAT_MAX_BYTES", raising=False)
+    n = 2_610_000
+    assert _gpu.sweep_columns(n, 272, 1, False) == 64  # 2^31 / (8 n) = 102 columns
+    assert _gpu.sweep_columns(n, 272, 1, True) == 272  # complex: not even 64 columns fit
+    assert _gpu.sweep_columns(n, 272, 4, False) == 272
+    assert _gpu.sweep_columns(115_000, 512, 1, True) == 512
+    assert _gpu.sweep_columns(1_000_000, 512, 1, False) == 256  # 268 columns fit
+    assert _gpu.sweep_columns(n, 64, 1, False) == 64
+    monkeypatch.setenv("RB_SPMM_FLAT_MAX_BYTES", str(726 * 8 * 100))
+    assert _gpu.sweep_columns(726, 272, 1, False) == 64
+    assert _gpu.sweep_columns(726, 99, 1, False) == 99
