@@ -416,10 +416,11 @@ _PREC = {"fp64": _abi.RB_PREC_FP64, "mixed": _abi.RB_PREC_MIXED}
 
 
 class SolveInfo:
-    def __init__(self, iters, relres, flag):
+    def __init__(self, iters, relres, flag, groups=None):
         self.iters = iters
         self.relres = relres
         self.flag = flag
+        self.groups = groups  # [(columns, iterations)] of the column groups a wide solve ran as
 
     def __repr__(self):
         return "SolveInfo(iters=%d, relres=%.3e, flag=%d)" % (self.iters, self.relres, self.flag)
@@ -535,7 +536,8 @@ def solve_system(sysm, B, k, precision="fp64", tol=1e-12, maxiter=2000, X0=None)
             parts.append(Xi)
             infos.append(inf)
         return torch.cat(parts, dim=1), SolveInfo(max(i.iters for i in infos), max(i.relres for i in infos),
-                                                  max(i.flag for i in infos))
+                                                  max(i.flag for i in infos),
+                                                  [(p.shape[1], i.iters) for p, i in zip(parts, infos)])
     # the LDS-staged SpMM serves 64-column solves of one system with 8-byte inner values
     # (complex64 mixed, float64 plain)
     lds = sysm.nsys == 1 and k == 64 and (sysm.is_complex == (precision == "mixed"))
