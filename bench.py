@@ -520,10 +520,14 @@ def main():
         err_phi = (torch.linalg.norm(phi_m - phi_d) / torch.linalg.norm(phi_d)).item()
         dm, dd = res_m.detvals[0], res_d.detvals[0]
         err_det = (torch.abs(dm - dd) / torch.abs(dd)).max().item()
+        # relative to each source's largest detector value (the element-wise maximum is set by the
+        # farthest detectors, whose values lie ~1e-6 below the largest)
+        err_colmax = (torch.abs(dm - dd) / torch.abs(dd).amax(dim=0, keepdim=True)).max().item()
         out["fp64"] = {"rhs_solves_per_s": k / dt_d, "ms_per_step": dt_d * 1e3, "iters": res_d.info.iters,
                        "relres": res_d.info.relres, "tol": a.tol_fp64}
         out["mixed"]["err_phi_relL2_vs_fp64"] = err_phi
         out["mixed"]["err_detval_maxrel_vs_fp64"] = err_det
+        out["mixed"]["err_detval_colmax_vs_fp64"] = err_colmax
         out["speedup_mixed_vs_fp64"] = dt_d / dt_m
     if a.gn:
         out["gn"] = gn_bench(a)

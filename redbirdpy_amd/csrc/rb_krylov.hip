@@ -62,7 +62,11 @@ struct ColState {
   int* ru_req;
   int ru_par;
   double delta2;  // request a replacement when rr <= delta2 * rmax2 (0: plain CG)
-  double tol2ru;  // ... or when rr <= tol2ru * bb
+  double tol2ru;  // ... or when every active column of the chunk has rr <= tol2ru * bb
+  // per chunk: [0, MAXCHUNK) columns arrived in this k_fin_beta launch,
+  // [MAXCHUNK, 2 MAXCHUNK) active columns above the tolerance, [2 MAXCHUNK,
+  // 3 MAXCHUNK) active columns below it; the chunk's last column resets them
+  int* ru_cnt;
 };
 
 template <typename A>
@@ -1680,11 +1684,35 @@ __global__ void __launch_bounds__(FB) k_fin_alpha(int C, ColState<A> st) {
   }
 }
 
+// one column's arrival at the chunk-wide tolerance trigger (thread 0 of its k_fin_beta
+// block): the chunk's last column to arrive requests a replacement when no active
+// column is above the tolerance and at least one is below it, then clears the counters
+template <typename A>
+__device__ void ru_arrive(const ColState<A>& st, int C, int c, bool live, bool below) {
+  const int ch = c >> 6;
+  int* cnt = st.ru_cnt;
+  if (live) atomicAdd(cnt + (below ? 2 : 1) * MAXCHUNK + ch, 1);
+  __threadfence();
+  const int ncol = min(64, C - (ch << 6));
+  if (atomicAdd(cnt + ch, 1) == ncol - 1) {
+    __threadfence();
+    const int above = atomicAdd(cnt + MAXCHUNK + ch, 0);
+    const int nbelow = atomicAdd(cnt + 2 * MAXCHUNK + ch, 0);
+    if (above == 0 && nbelow > 0) ru_cur(st)[ch] = 1;
+    atomicExch(cnt + ch, 0);
+    atomicExch(cnt + MAXCHUNK + ch, 0);
+    atomicExch(cnt + 2 * MAXCHUNK + ch, 0);
+  }
+}
+
 template <typename A>
 __global__ void __launch_bounds__(FB) k_fin_beta(int C, ColState<A> st) {
   int c = blockIdx.x;
   if (!st.active[c]) {
-    if (threadIdx.x == 0) st.beta[c] = acc_zero(A());
+    if (threadIdx.x == 0) {
+      st.beta[c] = acc_zero(A());
+      if (st.delta2 > 0.0) ru_arrive(st, C, c, false, false);
+    }
     return;
   }
   A v[2];
@@ -1696,7 +1724,12 @@ __global__ void __launch_bounds__(FB) k_fin_beta(int C, ColState<A> st) {
     st.iters[c] = it;
     A rz = v[0];
     if (st.delta2 > 0.0) {  // mixed: the float64 true residual decides (next iteration's k_fin_alpha_ru)
-      if (rr <= st.delta2 * st.rmax2[c] || rr <= st.tol2ru * st.bb[c]) ru_cur(st)[c >> 6] = 1;
+      // a column below the tolerance keeps iterating until every active column of its chunk is
+      // (one replacement confirms them together); above it, a drop by delta since the last
+      // replacement requests one at once
+      const bool below = rr <= st.tol2ru * st.bb[c];
+      if (!below && rr <= st.delta2 * st.rmax2[c]) ru_cur(st)[c >> 6] = 1;
+      bool live = false;
       if (it >= st.maxiter) {
         st.active[c] = 0;
         st.flag[c] = 1;
@@ -1709,7 +1742,9 @@ __global__ void __launch_bounds__(FB) k_fin_beta(int C, ColState<A> st) {
         st.rho_prev[c] = st.rho[c];
         st.beta[c] = acc_div(rz, st.rho[c]);
         st.rho[c] = rz;
+        live = true;
       }
+      ru_arrive(st, C, c, live, below);
     } else if (rr <= st.tol2 * st.bb[c]) {
       st.active[c] = 0;
       st.beta[c] = acc_zero(A());
@@ -1936,6 +1971,7 @@ static ColState<A> state_at(char* base, const size_t* o, int64_t nblocks) {
   s.rho_prev = (A*)(base + o[10]);
   s.rmax2 = (double*)(base + o[11]);
   s.ru_req = nullptr;
+  s.ru_cnt = nullptr;
   s.ru_par = 0;
   s.nb = (int)nblocks;
   s.tol2 = 0;
@@ -1974,7 +2010,7 @@ static size_t ws_layout(int64_t n, int64_t C, int nsys, int mixed, size_t* o) {
     o[4] = L.take(n * C * sizeof(T32));  // D (inner solution)
     state_layout<A>(L, C, nb, o + 8);
     state_layout<A>(L, C, nb, o + 20);
-    o[6] = L.take(64);  // replacement request flag
+    o[6] = L.take(5 * MAXCHUNK * sizeof(int));  // replacement requests (2 parities) + chunk counters
   }
   return L.off;
 }
@@ -2396,7 +2432,12 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
     rb_set_error("rb_krylov_solve: mixed precision needs the 32-bit matrix values");
     return RB_ERR_ARG;
   }
-  const double RU_DELTA = 0.02;
+  // RB_RU_DELTA overrides the drop factor (tuning runs)
+  static const double RU_DELTA = [] {
+    const char* e = getenv("RB_RU_DELTA");
+    const double d = e ? atof(e) : 0.0;
+    return d > 0.0 && d < 1.0 ? d : 0.02;
+  }();
   SolveBufs<T32> w{(T32*)(base + o[0]), (T32*)(base + o[1]), (T32*)(base + o[2]), (T32*)(base + o[3])};
   T32* D = (T32*)(base + o[4]);
   int* ru_req = (int*)(base + o[6]);
@@ -2407,6 +2448,7 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
   si.delta2 = RU_DELTA * RU_DELTA;
   si.tol2ru = tol2;
   si.ru_req = ru_req;
+  si.ru_cnt = ru_req + 2 * MAXCHUNK;
   so.tol2 = tol2;
   so.maxiter = maxiter;
   const T64* A64 = (const T64*)vals64;
@@ -2417,7 +2459,7 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
   if (rc) return rc;
   rc = rb_fill_bytes0(D, (size_t)n * C * sizeof(T32), s);
   if (rc) return rc;
-  rc = rb_fill_i32(ru_req, 0, 2 * MAXCHUNK, s);
+  rc = rb_fill_i32(ru_req, 0, 5 * MAXCHUNK, s);
   if (rc) return rc;
   if (!use_x0) {
     rc = rb_fill_bytes0(X, (size_t)n * C * sizeof(T64), s);
