@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define RB_ABI_VERSION 6
+#define RB_ABI_VERSION 7
 
 /* property layouts for rb_assemble */
 #define RB_PROP_ELEM 0   /* one row per element (label tables are expanded by the caller) */
@@ -115,6 +115,28 @@ int rb_assemble(const int32_t* slot_ptr, const int32_t* codes, int64_t nnz, cons
                 const double* area, int nsys, int kind, const double* mua, const double* musp, const double* nref,
                 int64_t plen, int64_t nlen, const double* omega, const double* bc, int is_complex, void* out64,
                 void* out32, void* stream);
+
+/*
+ * rb_assemble in two passes (the same arguments and results; redbirdpy
+ * forward.py:90-256 `femlhs`): one writes the properties of each (element,
+ * system) into work (rb_assemble_workspace bytes); then each workgroup takes
+ * the slots that start in its bins of 1024 codes, its threads stage the
+ * codes' contributions in LDS and one thread per slot sums its slot's
+ * codes in the slot map's order (every slot holds at least one code, as in a
+ * pattern built from the tets).  RB_PROP_VOLUME runs rb_assemble.  Values
+ * agree with rb_assemble to rounding (the P1 mass weights are split into a
+ * per-code term and the slot's nodal coefficient times its summed volume).
+ */
+int rb_assemble_workspace(int64_t nn, int64_t ne, int nsys, size_t* bytes);
+/* the workgroup bins of a slot map (once per mesh): rb_assemble_bin_count
+ * entries, bins[b] = first slot whose first code is at or after 256 b */
+int rb_assemble_bin_count(int64_t ne, int64_t nf, int64_t* nbins);
+int rb_assemble_bins(const int32_t* slot_ptr, int64_t nnz, int64_t ne, int64_t nf, int32_t* bins, void* stream);
+int rb_assemble_ws(const int32_t* slot_ptr, const int32_t* codes, int64_t nnz, const int32_t* el, int64_t ne,
+                   const int32_t* fc, int64_t nf, int64_t nn, const double* deldotdel, const double* evol,
+                   const double* area, int nsys, int kind, const double* mua, const double* musp, const double* nref,
+                   int64_t plen, int64_t nlen, const double* omega, const double* bc, int is_complex, void* out64,
+                   void* out32, const int32_t* bins, void* work, size_t work_bytes, void* stream);
 
 /* wide-field sources (rb_widefield) */
 #define RB_WF_SINK 0    /* collimated: the cut of the mesh by the sink plane */

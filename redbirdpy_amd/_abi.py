@@ -58,6 +58,11 @@ _SIGS = {
     "rb_slot_map": [_vp, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp],
     "rb_assemble": [_vp, _vp, _i64, _vp, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _int, _int, _vp, _vp, _vp,
                     _i64, _i64, _vp, _vp, _int, _vp, _vp, _vp],
+    "rb_assemble_workspace": [_i64, _i64, _int, ctypes.POINTER(_sz)],
+    "rb_assemble_ws": [_vp, _vp, _i64, _vp, _i64, _vp, _i64, _i64, _vp, _vp, _vp, _int, _int, _vp, _vp, _vp,
+                       _i64, _i64, _vp, _vp, _int, _vp, _vp, _vp, _vp, _sz, _vp],
+    "rb_assemble_bin_count": [_i64, _i64, ctypes.POINTER(_i64)],
+    "rb_assemble_bins": [_vp, _i64, _i64, _i64, _vp, _vp],
     "rb_tsearch": [_vp, _vp, _i64, _vp, _int, _dbl, _vp, _vp, _vp],
     "rb_rhs_points": [_vp, _vp, _vp, _int, _vp, _i64, _int, _vp, _vp],
     "rb_widefield": [_i64, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _int, _vp, _int, _int, _int, _int,

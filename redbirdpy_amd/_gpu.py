@@ -155,6 +155,28 @@ class GpuMesh:
             self.area = _f64(area, self.dev)
             self._area_src = area
 
+    def asm_workspace(self, nsys):
+        """Property tables of rb_assemble_ws for nsys systems (kept for reuse; stream-ordered, so one
+        assembly at a time per stream)."""
+        nbytes = ctypes.c_size_t(0)
+        call("rb_assemble_workspace", self.nn, self.ne, nsys, ctypes.byref(nbytes))
+        w = getattr(self, "_asm_work", None)
+        if w is None or w.numel() < nbytes.value:
+            w = torch.empty(int(nbytes.value), dtype=torch.uint8, device=self.dev)
+            self._asm_work = w
+        return w
+
+    def asm_bins(self):
+        """Workgroup bins of the slot map for rb_assemble_ws (built once)."""
+        b = getattr(self, "_asm_bins", None)
+        if b is None:
+            cnt = ctypes.c_int64(0)
+            call("rb_assemble_bin_count", self.ne, self.nf, ctypes.byref(cnt))
+            b = torch.empty(int(cnt.value), dtype=torch.int32, device=self.dev)
+            call("rb_assemble_bins", ptr(self.slot_ptr), self.nnz, self.ne, self.nf, ptr(b), _abi.stream_ptr())
+            self._asm_bins = b
+        return b
+
     def spmm_plan(self):
         """The LDS-staged SpMM plan of this pattern (built once); None when it does not apply."""
         if getattr(self, "_plan", False) is False:
@@ -346,11 +368,17 @@ class System:
         self.is_complex = is_complex
 
 
-def assemble(mesh, cfg, deldotdel, systems, want32=False):
-    """Assemble femlhs matrices for a list of (wavelength, mode) systems in one launch.
+class AsmInputs:
+    """Device inputs of one assembly (rb_assemble / rb_assemble_ws arguments after the mesh's)."""
 
-    Returns a System with values (nsys x nnz) float64/complex128 (+ 32-bit copy).
-    """
+    def __init__(self, kind, plen, nlen, mua, musp, nref, om, bc, is_complex, nsys):
+        self.kind, self.plen, self.nlen = kind, plen, nlen
+        self.mua, self.musp, self.nref, self.om, self.bc = mua, musp, nref, om, bc
+        self.is_complex, self.nsys = is_complex, nsys
+
+
+def assemble_inputs(mesh, cfg, deldotdel, systems):
+    """Property arrays of (wavelength, mode) systems on the device, in the layout rb_assemble takes."""
     dev = mesh.dev
     mesh.set_geometry(deldotdel, cfg["evol"], cfg.get("area"))
     cols = {"mua": [], "musp": [], "nref": []}
@@ -383,21 +411,37 @@ def assemble(mesh, cfg, deldotdel, systems, want32=False):
             nlen = max(nlen, nref.numel())
         oms.append(om)
         bcs.append((1 - reff) / (12.0 * (1 + reff)))
-    nsys = len(systems)
     nref_rows = [x if x.numel() == nlen else x.expand(nlen) for x in cols["nref"]]
-    mua_d = torch.stack(cols["mua"]).contiguous()
-    musp_d = torch.stack(cols["musp"]).contiguous()
-    nref_d = torch.stack(nref_rows).contiguous()
-    is_complex = any(o > 0 for o in oms)
-    om_d = _f64(np.array(oms), dev)
-    bc_d = _f64(np.array(bcs), dev)
-    shape = (nsys, mesh.nnz)
-    v64 = torch.empty(shape, dtype=torch.complex128 if is_complex else torch.float64, device=dev)
-    v32 = torch.empty(shape, dtype=torch.complex64 if is_complex else torch.float32, device=dev) if want32 else None
-    call("rb_assemble", ptr(mesh.slot_ptr), ptr(mesh.codes), mesh.nnz, ptr(mesh.el), mesh.ne, ptr(mesh.fc),
-         mesh.nf, mesh.nn, ptr(mesh.dd), ptr(mesh.evol), ptr(mesh.area), nsys, kind, ptr(mua_d), ptr(musp_d),
-         ptr(nref_d), plen, nlen, ptr(om_d), ptr(bc_d), int(is_complex), ptr(v64), ptr(v32), _abi.stream_ptr())
-    return System(mesh, v64, v32, nsys, is_complex)
+    return AsmInputs(kind, plen, nlen, torch.stack(cols["mua"]).contiguous(), torch.stack(cols["musp"]).contiguous(),
+                     torch.stack(nref_rows).contiguous(), _f64(np.array(oms), dev), _f64(np.array(bcs), dev),
+                     any(o > 0 for o in oms), len(systems))
+
+
+def assemble_call(mesh, a, want32=False, entry="rb_assemble_ws"):
+    """Run one assembly of AsmInputs a through entry (rb_assemble_ws, or rb_assemble): (v64, v32)."""
+    shape = (a.nsys, mesh.nnz)
+    dev = mesh.dev
+    v64 = torch.empty(shape, dtype=torch.complex128 if a.is_complex else torch.float64, device=dev)
+    v32 = torch.empty(shape, dtype=torch.complex64 if a.is_complex else torch.float32, device=dev) if want32 else None
+    args = (ptr(mesh.slot_ptr), ptr(mesh.codes), mesh.nnz, ptr(mesh.el), mesh.ne, ptr(mesh.fc), mesh.nf, mesh.nn,
+            ptr(mesh.dd), ptr(mesh.evol), ptr(mesh.area), a.nsys, a.kind, ptr(a.mua), ptr(a.musp), ptr(a.nref),
+            a.plen, a.nlen, ptr(a.om), ptr(a.bc), int(a.is_complex), ptr(v64), ptr(v32))
+    if entry == "rb_assemble_ws":
+        work = mesh.asm_workspace(a.nsys)
+        call("rb_assemble_ws", *args, ptr(mesh.asm_bins()), ptr(work), work.numel(), _abi.stream_ptr())
+    else:
+        call(entry, *args, _abi.stream_ptr())
+    return v64, v32
+
+
+def assemble(mesh, cfg, deldotdel, systems, want32=False):
+    """Assemble femlhs matrices for a list of (wavelength, mode) systems in one launch sequence.
+
+    Returns a System with values (nsys x nnz) float64/complex128 (+ 32-bit copy).
+    """
+    a = assemble_inputs(mesh, cfg, deldotdel, systems)
+    v64, v32 = assemble_call(mesh, a, want32)
+    return System(mesh, v64, v32, a.nsys, a.is_complex)
 
 
 def volume_weights(mesh):
