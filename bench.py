@@ -55,7 +55,8 @@ def _args():
     p.add_argument("--gn-tol", type=float, default=1e-10,
                    help="forward solver tolerance of both Gauss-Newton legs (redbirdpy femsolve's default)")
     p.add_argument("--gn-precisions", default="mixed,fp64", help="comma list of GN legs to run")
-    p.add_argument("--no-kernels", action="store_true", help="skip the Jacobian and Gram kernel legs")
+    p.add_argument("--no-kernels", action="store_true", help="skip the Jacobian, Gram and assembly kernel legs")
+    p.add_argument("--only-assembly", action="store_true", help="only the assembly leg (for kernel profiles)")
     p.add_argument("--no-batch", dest="batch", action="store_false", help="skip the 64-mesh uneven batch leg")
     p.add_argument("--batch-meshes", type=int, default=64)
     p.add_argument("--gn-phases", action="store_true", help="add a synchronising per-phase timing run")
@@ -365,6 +366,53 @@ def gram_bench(nn, m=4096, reps=3):
     return out
 
 
+def assembly_bench(a, cfg, reps=20):
+    """femlhs assembly at 115k nodes: the FD element-property system (1 complex system + its
+    complex64 copy) and the 4-wavelength node-property CW systems of the Gauss-Newton leg (+ float32
+    copies), through rb_assemble_ws (property pass + binned slot pass) and the per-slot rb_assemble,
+    event-timed per call.  slot_pass_bytes: algorithmic bytes of the slot pass (codes, slot_ptr,
+    deldotdel rows, property records, [evol, el, nodal mua], values written)."""
+    import copy
+    import torch
+    import redbirdpy_amd as rb
+    from redbirdpy_amd import _gpu, workloads
+
+    def timeit(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e3
+
+    out = {}
+    cases = [("fd_elem_1sys", cfg, [("", 1)])]
+    cfg0, bg, truth, extin, E = workloads.chrome_box(a.h)
+    ct, _ = rb.meshprep(copy.deepcopy(cfg0))
+    ct["param"] = {k: v.copy() for k, v in truth.items()}
+    ct["prop"] = rb.updateprop(ct, extin=extin)
+    cases.append(("cw_node_4sys", ct, [(wv, 1) for wv in ct["prop"]]))
+    for name, c, systems in cases:
+        mesh = _gpu.mesh_for(c)
+        ai = _gpu.assemble_inputs(mesh, c, c["deldotdel"], systems)
+        ncodes, ne, nnz, ns = mesh.codes.numel(), mesh.ne, mesh.nnz, ai.nsys
+        vb = (16 + 8) if ai.is_complex else (8 + 4)
+        if ai.kind == 0:  # element records of 32 bytes
+            nbytes = 4 * ncodes + 4 * (nnz + 1) + 80 * ne + 32 * ne * ns + vb * nnz * ns
+        else:  # node records of 16 bytes, evol, el and the nodal mua
+            nbytes = (4 * ncodes + 4 * (nnz + 1) + 80 * ne + 16 * ne * ns + 8 * ne + 16 * ne + 8 * mesh.nn * ns
+                      + vb * nnz * ns)
+        t_ws = timeit(lambda: _gpu.assemble_call(mesh, ai, want32=True))
+        t_old = timeit(lambda: _gpu.assemble_call(mesh, ai, want32=True, entry="rb_assemble"))
+        out[name] = {"nodes": mesh.nn, "tets": ne, "nnz": nnz, "systems": ns, "us_per_call": t_ws,
+                     "us_per_call_per_slot_kernel": t_old, "slot_pass_bytes": nbytes}
+    return out
+
+
 def jac_bench(a, reps=5):
     """Node Jacobian of 1024 pairs on the 1 mm CW box (115,351 nodes), float64 out.
 
@@ -424,6 +472,9 @@ def main():
     cfg = workloads.fd_box(a.h)
     cfg, sd = rb.meshprep(cfg)
     mesh = _gpu.mesh_for(cfg)
+    if a.only_assembly:
+        print(json.dumps({"assembly": assembly_bench(a, cfg)}), flush=True)
+        return
     opt = locate_optodes(cfg)
     k = opt[""].k
     L = _abi.lib()
@@ -536,6 +587,7 @@ def main():
     if not a.no_kernels:
         out["jacobian"] = jac_bench(a)
         out["gram"] = gram_bench(mesh.nn)
+        out["assembly"] = assembly_bench(a, cfg)
     if not a.skip_cpu:
         out["cpu_baseline"] = cpu_baseline(a.cpu_h)
         out["config"]["cpu_sample_nodes"] = out["cpu_baseline"]["nodes"]

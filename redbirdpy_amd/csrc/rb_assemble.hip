@@ -199,42 +199,40 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// node properties, per node: rows[n * nsys + s] = {1/(3(mua+musp)), mua, n} (coalesced reads of
-// the system-major inputs, the systems of a node written side by side)
+// node properties, per (node, system): rows[n * nsys + s] = {1/(3(mua+musp)), mua, n}, the
+// systems of a node side by side (consecutive threads write consecutive rows)
 __global__ void __launch_bounds__(256)
     k_asm_node_rows(int64_t nn, int nsys, const double* __restrict__ mua, const double* __restrict__ musp,
                     const double* __restrict__ nref, int64_t nlen, double* __restrict__ rows) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nn * nsys) return;
-  const int64_t s = i / nn, n = i - s * nn;
-  const double m = mua[i];
-  double* r = rows + 3 * (n * nsys + s);
-  r[0] = 1.0 / (3.0 * (m + musp[i]));
+  const int64_t n = i / nsys, s = i - n * nsys;
+  const double m = mua[s * nn + n];
+  double* r = rows + 3 * i;
+  r[0] = 1.0 / (3.0 * (m + musp[s * nn + n]));
   r[1] = m;
   r[2] = nlen > 1 ? nref[s * nlen + n] : 0.0;
 }
 
-// node properties, per element: the mean of its nodes' D and the sums of their mua and n
+// node properties, per (element, system) (consecutive threads write consecutive records):
+// the mean of the element's nodal D and the sums of its nodal mua and n
 __global__ void __launch_bounds__(256)
     k_asm_props_node(int64_t ne, int nsys, const int32_t* __restrict__ el, const double* __restrict__ rows,
                      double2* __restrict__ ntab, double* __restrict__ nsn) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne) return;
-  int32_t q[4];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ne * nsys) return;
+  const int64_t e = i / nsys;
+  const int s = (int)(i - e * nsys);
+  double Ds = 0.0, S = 0.0, Sn = 0.0;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) q[k] = el[4 * e + k];
-  for (int s = 0; s < nsys; ++s) {
-    double Ds = 0.0, S = 0.0, Sn = 0.0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const double* r = rows + 3 * ((int64_t)q[k] * nsys + s);
-      Ds += r[0];
-      S += r[1];
-      Sn += r[2];
-    }
-    ntab[e * nsys + s] = make_double2(Ds / 4.0, S);
-    nsn[e * nsys + s] = Sn;
+  for (int k = 0; k < 4; ++k) {
+    const double* r = rows + 3 * ((int64_t)el[4 * e + k] * nsys + s);
+    Ds += r[0];
+    S += r[1];
+    Sn += r[2];
   }
+  ntab[i] = make_double2(Ds / 4.0, S);
+  nsn[i] = Sn;
 }
 
 // first slot whose first code is at or after position v (slot_ptr[0..nnz] ascending)
@@ -361,7 +359,12 @@ __global__ void __launch_bounds__(256) k_assemble_bin(AsmTabArgs a) {
   __shared__ double l_im[CPLX ? NS : 1][CPLX ? LCAP : 1];
   __shared__ double l_v[KIND == RB_PROP_NODE ? LCAP : 1];
   const int t = threadIdx.x;
-  const int64_t b0 = (int64_t)blockIdx.x * BM, b1 = b0 + BM < a.nbin ? b0 + BM : a.nbin;
+  // workgroups go to the 8 accelerator dies (each with its own L2) round-robin: die x takes
+  // one contiguous run of bins, so the element rows that neighbouring bins share are fetched
+  // into one L2
+  const int64_t G = gridDim.x, q8 = G / 8, r8 = G % 8, x8 = blockIdx.x % 8, k8 = blockIdx.x / 8;
+  const int64_t lb = (x8 < r8 ? x8 * (q8 + 1) : r8 * (q8 + 1) + (x8 - r8) * q8) + k8;
+  const int64_t b0 = lb * BM, b1 = b0 + BM < a.nbin ? b0 + BM : a.nbin;
   const int64_t sb = a.bins[b0], se = a.bins[b1];
   if (sb >= se) return;  // no slot starts in these bins (uniform over the workgroup)
   const int64_t cb = a.slot_ptr[sb], ce = a.slot_ptr[se];
@@ -532,8 +535,8 @@ extern "C" int rb_assemble_ws(const int32_t* slot_ptr, const int32_t* codes, int
     double* rows = (double*)(wb + off[2]);
     hipLaunchKernelGGL(k_asm_node_rows, dim3((unsigned)ceil_div(nn * nsys, 256)), dim3(256), 0, s, nn, nsys, mua, musp,
                        nref, nlen, rows);
-    hipLaunchKernelGGL(k_asm_props_node, dim3(ge), dim3(256), 0, s, ne, nsys, el, rows, (double2*)(wb + off[0]),
-                       (double*)(wb + off[1]));
+    hipLaunchKernelGGL(k_asm_props_node, dim3((unsigned)ceil_div(ne * nsys, 256)), dim3(256), 0, s, ne, nsys, el, rows,
+                       (double2*)(wb + off[0]), (double*)(wb + off[1]));
   }
   RB_CHECK_LAUNCH();
   AsmTabArgs a;
@@ -559,15 +562,15 @@ extern "C" int rb_assemble_ws(const int32_t* slot_ptr, const int32_t* codes, int
   a.nsys = nsys;
   a.out64 = out64;
   a.out32 = out32;
-  // every tet contributes 16 codes and every face 9 (the slot map holds each once)
-  constexpr int BM = 4;
-  const unsigned gb = (unsigned)ceil_div(a.nbin, BM);
+  // every tet contributes 16 codes and every face 9 (the slot map holds each once); four
+  // bins per workgroup, two where the staged contributions of 3-4 systems would cap occupancy
+  const unsigned g4 = (unsigned)ceil_div(a.nbin, 4), g2 = (unsigned)ceil_div(a.nbin, 2);
 #define RB_ASM_NS(K_, C_)                                                                                 \
   switch (a.ns) {                                                                                       \
-    case 1: hipLaunchKernelGGL((k_assemble_bin<K_, C_, 1, BM>), dim3(gb), dim3(256), 0, s, a); break;   \
-    case 2: hipLaunchKernelGGL((k_assemble_bin<K_, C_, 2, BM>), dim3(gb), dim3(256), 0, s, a); break;   \
-    case 3: hipLaunchKernelGGL((k_assemble_bin<K_, C_, 3, BM>), dim3(gb), dim3(256), 0, s, a); break;   \
-    default: hipLaunchKernelGGL((k_assemble_bin<K_, C_, 4, BM>), dim3(gb), dim3(256), 0, s, a); break;  \
+    case 1: hipLaunchKernelGGL((k_assemble_bin<K_, C_, 1, 4>), dim3(g4), dim3(256), 0, s, a); break;    \
+    case 2: hipLaunchKernelGGL((k_assemble_bin<K_, C_, 2, 4>), dim3(g4), dim3(256), 0, s, a); break;    \
+    case 3: hipLaunchKernelGGL((k_assemble_bin<K_, C_, 3, 2>), dim3(g2), dim3(256), 0, s, a); break;    \
+    default: hipLaunchKernelGGL((k_assemble_bin<K_, C_, 4, 2>), dim3(g2), dim3(256), 0, s, a); break;   \
   }
   for (int s0 = 0; s0 < nsys; s0 += RB_ASM_MAXSYS) {
     a.s0 = s0;
