@@ -284,7 +284,18 @@ int rb_krylov_solve_planned(int64_t n, int64_t nnz, const int32_t* row_ptr, cons
                             void* stream);
 
 /*
- * Per-kernel timing of the Krylov loop/*
+ * The column sweep's grid (the q = A p launches of rb_krylov_solve and
+ * rb_krylov_solve_planned when every 64-column chunk lies in one system):
+ * at most cap workgroups (rounded down to a multiple of 8, the accelerator
+ * dies), 0 = one resident round of the device (the default).  Process-wide;
+ * the autotuner sets it around the solves it has measured.  No counterpart
+ * in the reference, whose CPU solvers split right-hand sides with
+ * rhsblock / nthread (solver.py:600-639).
+ */
+int rb_krylov_set_sweep_grid(int64_t cap);
+int64_t rb_krylov_get_sweep_grid(void);
+
+/*
  * Per-kernel timing of the Krylov loop: when enabled, hipEvents are
  * recorded on the launch stream around every launch of the kernels below;
  * rb_prof_read returns the summed milliseconds and the launch count.

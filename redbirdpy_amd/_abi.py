@@ -68,6 +68,8 @@ _SIGS = {
     "rb_widefield": [_i64, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _int, _vp, _int, _int, _int, _int,
                      _vp, _int, _int, _vp, _vp, _vp],
     "rb_krylov_workspace": [_i64, _int, _int, _int, _int, ctypes.POINTER(_sz)],
+    "rb_krylov_set_sweep_grid": [_i64],
+    "rb_krylov_get_sweep_grid": [],
     "rb_krylov_solve": [_i64, _i64, _vp, _vp, _vp, _int, _int, _int, _vp, _vp, _vp, _vp, _int, _dbl, _int,
                         _int, _int, _vp, _sz, ctypes.POINTER(_int), ctypes.POINTER(_dbl), ctypes.POINTER(_int),
                         _vp],
@@ -104,7 +106,7 @@ _SIGS = {
     "rb_prof_reset": [],
     "rb_prof_read": [_int, ctypes.POINTER(_dbl), ctypes.POINTER(_i64)],
 }
-_RESTYPE = {"rb_last_error": ctypes.c_char_p}
+_RESTYPE = {"rb_last_error": ctypes.c_char_p, "rb_krylov_get_sweep_grid": _i64}
 
 _lib = None
 

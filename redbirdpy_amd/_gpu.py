@@ -564,9 +564,12 @@ def sweep_columns(n, k, nsys, is_complex):
     return g if g >= 64 else k
 
 
-def solve_system(sysm, B, k, precision="fp64", tol=1e-12, maxiter=2000, X0=None):
+def solve_system(sysm, B, k, precision="fp64", tol=1e-12, maxiter=2000, X0=None, tuned=None):
     """CG/COCG over every system; after a breakdown (flag 4) every system is finished by
-    BiCGSTAB from its CG iterate (its columns that had converged stop at once)."""
+    BiCGSTAB from its CG iterate (its columns that had converged stop at once).
+
+    tuned: launch choices {"lds", "grid", "check_every"} (tune.py); None takes the autotuner's
+    cached entry for this shape, if any (RB_AUTOTUNE=1 measures one on a shape's first solve)."""
     m = sysm.mesh
     grp = sweep_columns(m.nn, k, sysm.nsys, sysm.is_complex)
     if grp < k:
@@ -582,11 +585,21 @@ def solve_system(sysm, B, k, precision="fp64", tol=1e-12, maxiter=2000, X0=None)
         return torch.cat(parts, dim=1), SolveInfo(max(i.iters for i in infos), max(i.relres for i in infos),
                                                   max(i.flag for i in infos),
                                                   [(p.shape[1], i.iters) for p, i in zip(parts, infos)])
+    from . import tune
+
+    if tuned is None:
+        tuned = tune.lookup(sysm, k, precision)
+        if tuned is None and os.environ.get("RB_AUTOTUNE") == "1":
+            tuned = tune.tune_system(sysm, B, k, precision, tol if tol > 0 else 1e-10, maxiter)
     # the LDS-staged SpMM serves 64-column solves of one system with 8-byte inner values
     # (complex64 mixed, float64 plain)
     lds = sysm.nsys == 1 and k == 64 and (sysm.is_complex == (precision == "mixed"))
-    X, info = krylov_solve(m.row_ptr, m.col_idx, m.diag, m.nn, m.nnz, sysm.vals64, sysm.vals32, B, k, sysm.nsys,
-                           sysm.is_complex, precision, tol, maxiter, X0, plan=m.spmm_plan() if lds else None)
+    lds = lds and (tuned is None or bool(tuned.get("lds", True)))
+    check_every = int(tuned.get("check_every", 16)) if tuned else 16
+    with tune.applied(tuned):
+        X, info = krylov_solve(m.row_ptr, m.col_idx, m.diag, m.nn, m.nnz, sysm.vals64, sysm.vals32, B, k,
+                               sysm.nsys, sysm.is_complex, precision, tol, maxiter, X0,
+                               check_every=check_every, plan=m.spmm_plan() if lds else None)
     if info.flag == 4:
         if sysm.nsys == 1:
             return bicgstab_solve(m.row_ptr, m.col_idx, m.diag, m.nn, m.nnz, sysm.vals64, B, sysm.is_complex, tol,

@@ -24,6 +24,7 @@
 #include "rb_common.h"
 #include "../../include/rbhip.h"
 #include <vector>
+#include <atomic>
 #include <mutex>
 #include <stdlib.h>
 #include <algorithm>
@@ -2244,6 +2245,20 @@ static int64_t resident_grid(const void* fn, int64_t nb) {
   return std::min(g, nb);
 }
 
+// the column sweep's grid cap set by rb_krylov_set_sweep_grid (0: one resident round)
+static std::atomic<int64_t> g_sweep_grid{0};
+
+extern "C" int rb_krylov_set_sweep_grid(int64_t cap) {
+  if (cap < 0) {
+    rb_set_error("rb_krylov_set_sweep_grid: negative cap %lld", (long long)cap);
+    return RB_ERR_ARG;
+  }
+  g_sweep_grid.store(cap);
+  return RB_OK;
+}
+
+extern "C" int64_t rb_krylov_get_sweep_grid(void) { return g_sweep_grid.load(); }
+
 template <typename T>
 static void spmm_plan(const Csr& m, int nsys, int C, int k, int64_t nb, bool* flat, int64_t* nbs) {
   // the sweep addresses P through a buffer descriptor: under 2 GiB (RB_SPMM_FLAT_MAX_BYTES lowers the
@@ -2266,6 +2281,8 @@ static void spmm_plan(const Csr& m, int nsys, int C, int k, int64_t nb, bool* fl
     const int64_t cap = atoll(e) / NXCD * NXCD;
     if (cap >= NXCD) *nbs = std::min(*nbs, cap);
   }
+  const int64_t tuned = g_sweep_grid.load() / NXCD * NXCD;
+  if (tuned >= NXCD) *nbs = std::min(*nbs, tuned);
 }
 // one workgroup per CU (each holds both LDS block buffers), a multiple of NXCD
 static int64_t lds_grid(int64_t nb) {

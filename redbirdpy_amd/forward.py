@@ -406,16 +406,9 @@ def check_inputs(cfg, wvs):
             raise ValueError("cfg['%s'] holds NaN or Inf" % key)
 
 
-def forward_device(cfg, rfcw=(1,), precision="fp64", tol=1e-12, maxiter=None, X0=None, optodes=None,
-                   keep_optodes=False, where="runforward"):
-    """Assemble, build RHS and solve all (wavelength, mode) systems on the GPU.
-
-    optodes: located optodes of an earlier call, reused where their moved
-    positions are unchanged (or as given, with keep_optodes=True: the
-    optodes then stay where they were when the optical properties change).
-    A solve that stops short of ``tol`` (iteration limit, breakdown) raises
-    a SolverWarning naming ``where``; its flag is in the result's info.
-    """
+def forward_system(cfg, rfcw=(1,), precision="fp64", optodes=None, keep_optodes=False):
+    """The device systems of a forward run: (mesh, System, B, k, systems, optodes, wavelengths), B the
+    right-hand sides of every (wavelength, mode) system, k columns each (optodes as in forward_device)."""
     if cfg.get("deldotdel") is None:
         cfg["deldotdel"], _ = deldotdel(cfg)
     wvs = list(cfg["prop"].keys()) if isinstance(cfg.get("prop"), dict) else [""]
@@ -444,6 +437,21 @@ def forward_device(cfg, rfcw=(1,), precision="fp64", tol=1e-12, maxiter=None, X0
     B = _gpu.zeros((mesh.nn, nsys * k), dt, mesh.dev)
     for i, (wv, md) in enumerate(systems):
         optodes[wv].fill_block(mesh, B, i * k, sysm.is_complex)
+    return mesh, sysm, B, k, systems, optodes, wvs
+
+
+def forward_device(cfg, rfcw=(1,), precision="fp64", tol=1e-12, maxiter=None, X0=None, optodes=None,
+                   keep_optodes=False, where="runforward"):
+    """Assemble, build RHS and solve all (wavelength, mode) systems on the GPU.
+
+    optodes: located optodes of an earlier call, reused where their moved
+    positions are unchanged (or as given, with keep_optodes=True: the
+    optodes then stay where they were when the optical properties change).
+    A solve that stops short of ``tol`` (iteration limit, breakdown) raises
+    a SolverWarning naming ``where``; its flag is in the result's info.
+    """
+    mesh, sysm, B, k, systems, optodes, wvs = forward_system(cfg, rfcw, precision, optodes, keep_optodes)
+    nsys = len(systems)
     if maxiter is None:
         maxiter = max(4000, 4 * int(np.sqrt(mesh.nn)) + 1000)
     if nsys * k <= 512:

@@ -1,0 +1,199 @@
+"""Launch-parameter autotuner for the multi-RHS Krylov solves.
+
+The reference splits its right-hand sides for the CPU solvers with ``rhsblock``
+and ``nthread`` (redbirdpy solver.py:600-639, ``_blqmr_parallel`` :233-307).
+Here one multi-RHS solve covers every column, and what is left to choose is
+how the GPU runs it:
+
+* ``lds``: the LDS-staged SpMM (its plan exists for 64-column solves of one
+  system with 8-byte inner values) or the die-wide column sweep;
+* ``grid``: the column sweep's workgroup cap (0 = one resident round of the
+  device, else a multiple of 8; rb_krylov_set_sweep_grid);
+* ``check_every``: iterations between the host's convergence polls (a poll
+  is a short host round trip; a column that converges between polls runs on
+  with its kernels gated off until the next one).
+
+``tune_system`` times each candidate on the system itself: the SpMM choices
+with short solves whose tolerance is 0 (a fixed number of iterations), the
+poll interval with full solves to the requested tolerance, and keeps the
+fastest.  Results are cached per (device name, nodes, nonzeros, columns,
+systems, real/complex, precision) in a JSON file (``RB_TUNE_CACHE``, default
+``~/.cache/redbirdpy_amd/tune.json``); ``_gpu.solve_system`` applies a cached
+entry to every later solve of that shape, and with ``RB_AUTOTUNE=1`` tunes a
+shape on its first solve.  Every candidate runs the same arithmetic: the
+choice changes the launch shape, not the result beyond rounding.
+"""
+
+import json
+import os
+import threading
+import time
+
+import torch
+
+from . import _abi
+
+_LOCK = threading.Lock()
+_CACHE = None
+_CACHE_FILE = None
+
+GRID_CAPS = (0, 512, 256)
+CHECK_EVERY = (8, 16, 32)
+DEFAULT = {"lds": True, "grid": 0, "check_every": 16}
+
+
+def cache_path():
+    return os.environ.get("RB_TUNE_CACHE") or os.path.join(os.path.expanduser("~"), ".cache", "redbirdpy_amd",
+                                                            "tune.json")
+
+
+def _load():
+    global _CACHE, _CACHE_FILE
+    path = cache_path()
+    if _CACHE is None or _CACHE_FILE != path:
+        try:
+            with open(path) as f:
+                data = json.load(f)
+            _CACHE = data if isinstance(data, dict) else {}
+        except (OSError, ValueError):
+            _CACHE = {}
+        _CACHE_FILE = path
+    return _CACHE
+
+
+def _save():
+    path = cache_path()
+    os.makedirs(os.path.dirname(path) or ".", exist_ok=True)
+    tmp = "%s.%d.tmp" % (path, os.getpid())
+    with open(tmp, "w") as f:
+        json.dump(_CACHE, f, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+
+
+def key(device_name, n, nnz, k, nsys, is_complex, precision):
+    return "%s|n=%d|nnz=%d|k=%d|nsys=%d|%s|%s" % (device_name, n, nnz, k, nsys, "complex" if is_complex else "real",
+                                                  precision)
+
+
+_NAMES = {}
+
+
+def _device_name(dev):
+    i = dev.index if dev.index is not None else torch.cuda.current_device()
+    if i not in _NAMES:
+        _NAMES[i] = torch.cuda.get_device_name(i)
+    return _NAMES[i]
+
+
+def system_key(sysm, k, precision):
+    m = sysm.mesh
+    return key(_device_name(m.dev), m.nn, m.nnz, k, sysm.nsys, sysm.is_complex, precision)
+
+
+def lookup(sysm, k, precision):
+    """The cached launch choices for this system shape, or None."""
+    with _LOCK:
+        return _load().get(system_key(sysm, k, precision))
+
+
+def store(sysm, k, precision, choice):
+    with _LOCK:
+        _load()[system_key(sysm, k, precision)] = dict(choice)
+        _save()
+
+
+def clear():
+    """Forget every cached choice (the file too)."""
+    global _CACHE
+    with _LOCK:
+        _CACHE = {}
+        try:
+            os.remove(cache_path())
+        except OSError:
+            pass
+
+
+class applied:
+    """Context that sets the column sweep's grid cap for the solves inside it (nothing to do for
+    the default cap)."""
+
+    def __init__(self, choice):
+        self.grid = int((choice or {}).get("grid", 0))
+
+    def __enter__(self):
+        if self.grid:
+            self.prev = int(_abi.lib().rb_krylov_get_sweep_grid())
+            _abi.call("rb_krylov_set_sweep_grid", self.grid)
+        return self
+
+    def __exit__(self, *exc):
+        if self.grid:
+            _abi.call("rb_krylov_set_sweep_grid", self.prev)
+        return False
+
+
+def _timed(fn, reps):
+    fn()  # warm (workspace, plan, first-launch costs)
+    torch.cuda.synchronize()
+    best = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    return best
+
+
+def tune_system(sysm, B, k, precision="mixed", tol=1e-10, maxiter=2000, iters=48, reps=2, save=True):
+    """Measure the launch choices for solving sysm (nsys systems x k columns of B) and cache the
+    fastest.  Returns {"lds", "grid", "check_every", "seconds_per_iter", "seconds_per_solve",
+    "candidates": [...]}."""
+    from . import _gpu
+
+    m = sysm.mesh
+    plan_ok = (sysm.nsys == 1 and k == 64 and sysm.is_complex == (precision == "mixed")
+               and m.spmm_plan() is not None)
+    flat = _gpu.sweep_columns(m.nn, k, sysm.nsys, sysm.is_complex) == k
+    cands = []
+    if plan_ok:
+        cands.append({"lds": True, "grid": 0})
+    if flat or not plan_ok:
+        cands += [{"lds": False, "grid": g} for g in (GRID_CAPS if flat else (0,))]
+    res = []
+    for c in cands:
+        with applied(c):
+            t = _timed(lambda: _gpu.solve_system(sysm, B, k, precision, 0.0, iters, tuned=dict(c, check_every=16)),
+                       reps)
+        res.append(dict(c, seconds_per_iter=t / iters))
+    best = min(res, key=lambda r: r["seconds_per_iter"])
+    polls = []
+    for ce in CHECK_EVERY:
+        ch = {"lds": best["lds"], "grid": best["grid"], "check_every": ce}
+        with applied(ch):
+            t = _timed(lambda: _gpu.solve_system(sysm, B, k, precision, tol, maxiter, tuned=ch), reps)
+        polls.append(dict(ch, seconds_per_solve=t))
+    pbest = min(polls, key=lambda r: r["seconds_per_solve"])
+    choice = {"lds": best["lds"], "grid": best["grid"], "check_every": pbest["check_every"]}
+    if save:
+        store(sysm, k, precision, choice)
+    return dict(choice, seconds_per_iter=best["seconds_per_iter"], seconds_per_solve=pbest["seconds_per_solve"],
+                candidates=res + polls)
+
+
+def tune_forward(cfg, rfcw=(1,), precision="mixed", tol=1e-12, maxiter=None, optodes=None, **kw):
+    """Tune the solves of cfg's forward run (forward_device's systems and right-hand sides): one
+    entry for the whole multi-system solve, or, past 512 columns, for its one-system solves.
+    Returns tune_system's result."""
+    import numpy as np
+    from . import _gpu
+    from .forward import forward_system
+
+    mesh, sysm, B, k, systems, _, _ = forward_system(cfg, rfcw, precision, optodes)
+    if maxiter is None:
+        maxiter = max(4000, 4 * int(np.sqrt(mesh.nn)) + 1000)
+    if len(systems) * k > 512:
+        sysm = _gpu.System(mesh, sysm.vals64[:1], None if sysm.vals32 is None else sysm.vals32[:1], 1,
+                           sysm.is_complex)
+        B = B[:, :k].contiguous()
+    return tune_system(sysm, B, k, precision, tol, maxiter, **kw)

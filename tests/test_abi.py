@@ -14,7 +14,7 @@ HEADER = os.path.join(ROOT, "include", "rbhip.h")
 
 def _declared():
     txt = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(rb_\w+)\s*\(", txt, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(rb_\w+)\s*\(", txt, flags=re.M)))
 
 
 def _exported():
