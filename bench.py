@@ -14,6 +14,9 @@ extra   : the float64 path (complex128 COCG to 1e-12), the error of the
           mixed solution against it, and (unless --no-gn) Gauss-Newton
           iterations/s of the 4-wavelength chromophore reconstruction in
           both precisions.
+autotune: before its timed steps each workload's launch choices (SpMM path,
+          sweep grid, poll interval) are tuned on it (tune.py); --no-autotune
+          keeps the defaults.
 roofline: the SpMM kernel of the mixed path (q = A p with the p.q
           reduction, LDS-staged: each block of rows copies its neighbour
           rows of p into LDS once).  achieved = the SpMM's algorithmic
@@ -55,6 +58,10 @@ def _args():
     p.add_argument("--gn-tol", type=float, default=1e-10,
                    help="forward solver tolerance of both Gauss-Newton legs (redbirdpy femsolve's default)")
     p.add_argument("--gn-precisions", default="mixed,fp64", help="comma list of GN legs to run")
+    p.add_argument("--gn-reftol", type=float, default=1e-8,
+                   help="relative residual of the mixed leg's refined Gauss-Newton step (its float64 normal equations)")
+    p.add_argument("--no-autotune", dest="autotune", action="store_false",
+                   help="solve with the default launch choices instead of tuning each workload first")
     p.add_argument("--no-kernels", action="store_true", help="skip the Jacobian, Gram and assembly kernel legs")
     p.add_argument("--only-assembly", action="store_true", help="only the assembly leg (for kernel profiles)")
     p.add_argument("--no-batch", dest="batch", action="store_false", help="skip the 64-mesh uneven batch leg")
@@ -202,6 +209,18 @@ def cpu_jac_baseline(phi, sd, cfg, pairs=32):
     }
 
 
+def _tuned(cfg, precision, tol):
+    """Tune cfg's forward solves (tune.tune_forward, cached for the run) and report the choice."""
+    from redbirdpy_amd import tune
+
+    t0 = time.perf_counter()
+    out = tune.tune_forward(cfg, precision=precision, tol=tol)
+    return {"lds": out["lds"], "grid": out["grid"], "check_every": out["check_every"],
+            "tuning_s": time.perf_counter() - t0,
+            "us_per_iter": {("lds" if c["lds"] else "sweep grid %d" % c["grid"]): c["seconds_per_iter"] * 1e6
+                            for c in out["candidates"] if "seconds_per_iter" in c}}
+
+
 def gn_bench(a):
     """Gauss-Newton iterations/s: 4 wavelengths x 3 chromophores, 1024 pairs per wavelength.
 
@@ -223,20 +242,23 @@ def gn_bench(a):
     rng = np.random.default_rng(40)
     y0 = {wv: det[wv] * (1 + 0.005 * rng.standard_normal(det[wv].shape)) for wv in workloads.WAVELENGTHS}
     m = sum(v.size for v in y0.values())
-    out = {"m": m, "nodes": int(ct["node"].shape[0]), "iters": a.gn_iters, "solver_tol": a.gn_tol}
+    out = {"m": m, "nodes": int(ct["node"].shape[0]), "iters": a.gn_iters, "solver_tol": a.gn_tol,
+           "mixed_step_reftol": a.gn_reftol}
+    if a.autotune:
+        out["autotune"] = {prec: _tuned(ct, prec, a.gn_tol) for prec in a.gn_precisions.split(",")}
     results = {}
     for prec in a.gn_precisions.split(","):
         # one untimed iteration first (workspace and mesh-structure allocation)
         c, sd = rb.meshprep(copy.deepcopy(cfg0))
         r = {"param": {k: v.copy() for k, v in bg.items()}}
         rb.runrecon(c, r, y0, sd, maxiter=1, lambda_=1e-3 / m, extin=extin, precision=prec, report=False,
-                    solvertol=a.gn_tol)
+                    solvertol=a.gn_tol, reftol=a.gn_reftol)
         c, sd = rb.meshprep(copy.deepcopy(cfg0))
         r = {"param": {k: v.copy() for k, v in bg.items()}}
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         _, resid, *_ = rb.runrecon(c, r, y0, sd, maxiter=a.gn_iters, lambda_=1e-3 / m, extin=extin,
-                                   precision=prec, report=False, solvertol=a.gn_tol)
+                                   precision=prec, report=False, solvertol=a.gn_tol, reftol=a.gn_reftol)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         # phase breakdown from a separate, synchronising run (not the timed one)
@@ -245,7 +267,7 @@ def gn_bench(a):
             c2, sd2 = rb.meshprep(copy.deepcopy(cfg0))
             r2 = {"param": {k: v.copy() for k, v in bg.items()}}
             rb.runrecon(c2, r2, y0, sd2, maxiter=a.gn_iters, lambda_=1e-3 / m, extin=extin, precision=prec,
-                        report=False, timing=tm, solvertol=a.gn_tol)
+                        report=False, timing=tm, solvertol=a.gn_tol, reftol=a.gn_reftol)
         results[prec] = r
         out[prec] = {"gn_iters_per_s": a.gn_iters / dt, "s_per_iter": dt / a.gn_iters,
                      "resid": [float(x) for x in resid], "phases_s": tm}
@@ -478,6 +500,15 @@ def main():
     opt = locate_optodes(cfg)
     k = opt[""].k
     L = _abi.lib()
+    tuned = {}
+    if a.autotune:
+        import tempfile
+
+        # the run's own cache file (a user's cache is left alone)
+        os.environ.setdefault("RB_TUNE_CACHE", os.path.join(tempfile.gettempdir(), "rb_bench_tune_%d.json" % os.getpid()))
+        tuned["mixed"] = _tuned(cfg, "mixed", a.tol_mixed)
+        if not a.skip_fp64:
+            tuned["fp64"] = _tuned(cfg, "fp64", a.tol_fp64)
 
     def step(prec, tol):
         return forward_device(cfg, [1], prec, tol, optodes=opt)
@@ -549,6 +580,7 @@ def main():
         },
         "mixed": {"iters": res_m.info.iters, "relres": res_m.info.relres, "flag": res_m.info.flag,
                   "kernels": per_kernel, "ms_per_step_with_kernel_events": dt_prof * 1e3},
+        "autotune": tuned,
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,
