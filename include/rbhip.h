@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define RB_ABI_VERSION 7
+#define RB_ABI_VERSION 8
 
 /* property layouts for rb_assemble */
 #define RB_PROP_ELEM 0   /* one row per element (label tables are expanded by the caller) */
@@ -262,6 +262,8 @@ typedef struct rb_spmm_plan {
   const int32_t* rmap;  /* record -> CSR slot, -1 for padding */
   const int32_t* loff;  /* record -> LDS byte offset of its column's row within the block */
   void* recv;           /* record values, written by each solve (8 bytes per record) */
+  int64_t recv_sets;    /* record value sets recv holds, nrec + RB_LDS_RECPAD records each: one per
+                           solve of one system, one per system pair for real float32 systems */
 } rb_spmm_plan;
 int rb_spmm_plan_count(int64_t n, const int32_t* row_ptr /* host */, const int32_t* col_idx /* host */, int umax,
                        int recmax, int64_t* nblk /* host */, int64_t* nunion /* host */, int64_t* nrec /* host */);
@@ -272,9 +274,10 @@ int rb_spmm_plan_fill(int64_t n, const int32_t* row_ptr /* host */, const int32_
 /*
  * rb_krylov_solve with an LDS-staged SpMM plan (rb_spmm_plan, device
  * pointers) for the 64-column solves of one system: mixed precision with
- * complex64 inner solves, or plain float64 real systems.  Other shapes (and
- * plan == NULL) run exactly as rb_krylov_solve.  Same arguments, workspace,
- * results and flags.
+ * complex64 inner solves, or plain float64 real systems; and for mixed
+ * solves of an even number of real 64-column systems (one launch per system
+ * pair, plan->recv_sets >= nsys / 2).  Other shapes (and plan == NULL) run
+ * exactly as rb_krylov_solve.  Same arguments, workspace, results and flags.
  */
 int rb_krylov_solve_planned(int64_t n, int64_t nnz, const int32_t* row_ptr, const int32_t* col_idx,
                             const int32_t* diag_slot, int nsys, int k, int is_complex, const void* vals64,

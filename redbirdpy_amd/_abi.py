@@ -34,7 +34,8 @@ class SpmmPlan(ctypes.Structure):
     _fields_ = [("nblk", ctypes.c_int64), ("nrec", ctypes.c_int64), ("umax", ctypes.c_int32),
                 ("recmax", ctypes.c_int32), ("brow", ctypes.c_void_p),
                 ("bu", ctypes.c_void_p), ("ulist", ctypes.c_void_p), ("rinfo", ctypes.c_void_p),
-                ("rmap", ctypes.c_void_p), ("loff", ctypes.c_void_p), ("recv", ctypes.c_void_p)]
+                ("rmap", ctypes.c_void_p), ("loff", ctypes.c_void_p), ("recv", ctypes.c_void_p),
+                ("recv_sets", ctypes.c_int64)]
 
 _i32 = ctypes.c_int32
 _i64 = ctypes.c_int64

@@ -177,11 +177,14 @@ class GpuMesh:
             self._asm_bins = b
         return b
 
-    def spmm_plan(self):
-        """The LDS-staged SpMM plan of this pattern (built once); None when it does not apply."""
+    def spmm_plan(self, sets=1):
+        """The LDS-staged SpMM plan of this pattern (built once) with room for `sets` record value
+        sets; None when it does not apply."""
         if getattr(self, "_plan", False) is False:
             self._plan = spmm_plan(self.row_ptr.cpu().numpy(), self.col_idx[: self.nnz].cpu().numpy(), self.dev)
-        return None if os.environ.get("RB_LDS_SPMM", "1") == "0" else self._plan
+        if os.environ.get("RB_LDS_SPMM", "1") == "0" or self._plan is None:
+            return None
+        return self._plan.ensure_sets(sets)
 
     def pattern_host(self):
         """(indptr, indices, slot order) of the CSR pattern in the user's node numbering.
@@ -224,8 +227,19 @@ class _Plan:
     def __init__(self, tensors, nblk, nrec, limits):
         self.tensors = tensors
         self.limits = limits
+        self.nrec = nrec
         self.struct = _abi.SpmmPlan(nblk, nrec, limits[0], limits[1],
-                                    *[ctypes.c_void_p(t.data_ptr()) for t in tensors])
+                                    *[ctypes.c_void_p(t.data_ptr()) for t in tensors], 1)
+
+    def ensure_sets(self, sets):
+        """Room for `sets` record value sets (one per real float32 system pair)."""
+        if self.struct.recv_sets < sets:
+            recv = torch.empty(sets * (self.nrec + _abi.RB_LDS_RECPAD), dtype=torch.float64,
+                               device=self.tensors[-1].device)
+            self.tensors[-1] = recv
+            self.struct.recv = ctypes.c_void_p(recv.data_ptr())
+            self.struct.recv_sets = sets
+        return self
 
 
 def spmm_plan(row_ptr, col_idx, dev, limits=None):
@@ -548,6 +562,19 @@ def warn_unconverged(info, where):
 SWEEP_MAX_BYTES = 1 << 31
 
 
+def lds_plan_sets(nsys, k, is_complex, precision):
+    """Record value sets an LDS-staged SpMM plan needs for this solve shape (0: it does not apply):
+    one for 64-column solves of one system with 8-byte values (complex64 mixed, float64 real), one per
+    pair for mixed solves of an even number of real 64-column systems."""
+    if k != 64:
+        return 0
+    if nsys == 1 and is_complex == (precision == "mixed"):
+        return 1
+    if precision == "mixed" and not is_complex and nsys >= 2 and nsys % 2 == 0:
+        return nsys // 2
+    return 0
+
+
 def sweep_columns(n, k, nsys, is_complex):
     """Columns per solve call so that a one-system solve of k columns keeps its float64 vectors
     (n rows) under the column sweep's 2 GiB buffer descriptor: k itself when it fits (or when
@@ -592,14 +619,14 @@ def solve_system(sysm, B, k, precision="fp64", tol=1e-12, maxiter=2000, X0=None,
         if tuned is None and os.environ.get("RB_AUTOTUNE") == "1":
             tuned = tune.tune_system(sysm, B, k, precision, tol if tol > 0 else 1e-10, maxiter)
     # the LDS-staged SpMM serves 64-column solves of one system with 8-byte inner values
-    # (complex64 mixed, float64 plain)
-    lds = sysm.nsys == 1 and k == 64 and (sysm.is_complex == (precision == "mixed"))
-    lds = lds and (tuned is None or bool(tuned.get("lds", True)))
+    # (complex64 mixed, float64 plain) and mixed solves of real 64-column systems in pairs
+    lds_sets = lds_plan_sets(sysm.nsys, k, sysm.is_complex, precision)
+    lds = lds_sets > 0 and (tuned is None or bool(tuned.get("lds", True)))
     check_every = int(tuned.get("check_every", 16)) if tuned else 16
     with tune.applied(tuned):
         X, info = krylov_solve(m.row_ptr, m.col_idx, m.diag, m.nn, m.nnz, sysm.vals64, sysm.vals32, B, k,
                                sysm.nsys, sysm.is_complex, precision, tol, maxiter, X0,
-                               check_every=check_every, plan=m.spmm_plan() if lds else None)
+                               check_every=check_every, plan=m.spmm_plan(lds_sets) if lds else None)
     if info.flag == 4:
         if sysm.nsys == 1:
             return bicgstab_solve(m.row_ptr, m.col_idx, m.diag, m.nn, m.nnz, sysm.vals64, B, sysm.is_complex, tol,

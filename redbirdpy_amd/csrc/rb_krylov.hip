@@ -1073,21 +1073,41 @@ __device__ __forceinline__ void dma16(const void* src, char* dst) {
   __builtin_amdgcn_global_load_lds(src, (lds_void_t*)dst, 16, 0, 0);
 }
 
+// partial-sum type of an LDS SpMM launch: the column type's accumulator, or for a pair of
+// real float32 systems (RPAIR) double, two columns per lane
+template <typename T, bool RPAIR>
+struct LdsAcc {
+  typedef typename sc<T>::acc type;
+};
+template <>
+struct LdsAcc<cplx<float>, true> {
+  typedef double type;
+};
+
 // (the plan arrays come as separate read-only kernel arguments so their
 // uniform reads are scalar loads, outside the vector counter the copies use)
-template <typename T, typename CF>
+//
+// Each staged row is 512 bytes at P + row * rs + coff (T units): one system's 64 columns of 8
+// bytes, or with RPAIR the 128 float32 columns of a pair of real 64-column systems, lane l
+// holding columns 2l and 2l+1 (lanes 0-31 the first system's, scaled by the record's .x,
+// lanes 32-63 the second's, by .y).  The launch's columns are [cbase, cbase + ncol) of st.
+template <typename T, typename CF, bool RPAIR>
 __global__ void __launch_bounds__((LDS_NW + LDS_NP) * 64)
 k_cg_spmm_lds(int64_t nblk, const int32_t* __restrict__ brow, const int32_t* __restrict__ bu,
               const int32_t* __restrict__ ulist, const int32_t* __restrict__ rinfo, const int32_t* __restrict__ loff,
-              const T* __restrict__ recv, const T* __restrict__ P, T* __restrict__ Q,
-              ColState<typename sc<T>::acc> st) {
-  static_assert(sizeof(T) == 8, "512-byte rows: 64 columns of 8 bytes");
+              const T* __restrict__ recv, const T* __restrict__ P, T* __restrict__ Q, int64_t rs, int64_t coff,
+              int cbase, ColState<typename LdsAcc<T, RPAIR>::type> st) {
+  static_assert(sizeof(T) == 8, "512-byte rows: 64 lanes of 8 bytes");
+  static_assert(!RPAIR || std::is_same<T, cplx<float>>::value, "a real pair is staged as 8-byte (float, float)");
   typedef typename sc<T>::acc A;
+  typedef typename LdsAcc<T, RPAIR>::type AS;
+  constexpr int NC = RPAIR ? 128 : 64;  // columns of this launch
   constexpr int JB = CF::JB, JR = CF::JR, JRP = CF::JRP, NBUF = CF::NBUF;
   extern __shared__ __attribute__((aligned(16))) char lds[];
   if (st.ru_req && blockIdx.x == 0 && threadIdx.x < MAXCHUNK) ru_cur(st)[threadIdx.x] = 0;
-  if (!block_any_active(st.active, 64)) return;
+  if (!block_any_active(st.active + cbase, NC)) return;
   const int lane = threadIdx.x & 63;
+  const bool second = lane >= 32;  // RPAIR: this lane's columns belong to the pair's second system
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t seg = (nblk + NXCD - 1) / NXCD;
   const int64_t b0 = (int64_t)(blockIdx.x % NXCD) * seg;
@@ -1095,7 +1115,7 @@ k_cg_spmm_lds(int64_t nblk, const int32_t* __restrict__ brow, const int32_t* __r
   const int64_t stride = gridDim.x / NXCD;
   const int64_t bfirst = b0 + blockIdx.x / NXCD;
   const int nmine = bfirst < b1 ? (int)((b1 - bfirst + stride - 1) / stride) : 0;
-  A pv = acc_zero(A());
+  A pv = acc_zero(A());  // RPAIR: .x column 2l, .y column 2l+1
   if (wave >= LDS_NW) {
     // producer: block k of this workgroup into buffer k & 1, JB + JRP copies every time
     const int pw = wave - LDS_NW;
@@ -1131,7 +1151,7 @@ k_cg_spmm_lds(int64_t nblk, const int32_t* __restrict__ brow, const int32_t* __r
         const int ga = __builtin_amdgcn_readfirstlane(ix[2 * j]);
         const int gb = __builtin_amdgcn_readfirstlane(ix[2 * j + 1]);
         const int g = lane < 32 ? ga : gb;
-        const char* src = (const char*)(P + (int64_t)g * 64) + (lane & 31) * 16;
+        const char* src = (const char*)(P + (int64_t)g * rs + coff) + (lane & 31) * 16;
         dma16(src, i < ni ? base + 2 * i * 512 : base + CF::UMAX * 512);
       }
     };
@@ -1166,24 +1186,57 @@ k_cg_spmm_lds(int64_t nblk, const int32_t* __restrict__ brow, const int32_t* __r
           }
 #pragma unroll
           for (int u = 0; u < 8; ++u) pa[u] = *(const T*)(cb + la[u]);
+          if constexpr (RPAIR) {
+            // select between values read into registers (a select between the fields' addresses
+            // would index va dynamically and put it in scratch memory)
 #pragma unroll
-          for (int u = 0; u < 8; ++u) acc = acc + va[u] * pa[u];
+            for (int u = 0; u < 8; ++u) {
+              const float vx = va[u].x, vy = va[u].y;
+              const float v = second ? vy : vx;
+              acc.x += v * pa[u].x;
+              acc.y += v * pa[u].y;
+            }
+          } else {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc = acc + va[u] * pa[u];
+          }
         }
         const T pown = *(const T*)(cb + ri[2 * i + 1]);
-        Q[(int64_t)(r0 + i) * 64 + lane] = acc;
-        pv = acc_add(pv, amul(pown, acc));
+        Q[(int64_t)(r0 + i) * rs + coff + lane] = acc;
+        if constexpr (RPAIR) {
+          pv.x += (double)pown.x * (double)acc.x;
+          pv.y += (double)pown.y * (double)acc.y;
+        } else {
+          pv = acc_add(pv, amul(pown, acc));
+        }
       }
     }
   }
   // per-column partials of this workgroup, consumer wavefronts summed in order
   __syncthreads();
-  A* red = (A*)lds;
-  if (wave < LDS_NW) red[wave * 64 + lane] = pv;
+  AS* red = (AS*)lds;
+  if (wave < LDS_NW) {
+    if constexpr (RPAIR) {
+      red[wave * NC + 2 * lane] = pv.x;
+      red[wave * NC + 2 * lane + 1] = pv.y;
+    } else {
+      red[wave * NC + lane] = pv;
+    }
+  }
   __syncthreads();
-  if (threadIdx.x < 64) {
-    A t = red[threadIdx.x];
-    for (int w = 1; w < LDS_NW; ++w) t = acc_add(t, red[w * 64 + threadIdx.x]);
-    st.part[(int64_t)threadIdx.x * st.nb + blockIdx.x] = t;
+  if (threadIdx.x < NC) {
+    AS t = red[threadIdx.x];
+    for (int w = 1; w < LDS_NW; ++w) t = acc_add(t, red[w * NC + threadIdx.x]);
+    st.part[(int64_t)(cbase + threadIdx.x) * st.nb + blockIdx.x] = t;
+  }
+}
+
+// records of a pair of real float32 systems: recv[i] = (va[rmap[i]], vb[rmap[i]]) (0 for padding)
+__global__ void k_lds_pack_pair(int64_t nrec, const int32_t* __restrict__ rmap, const float* __restrict__ va,
+                                const float* __restrict__ vb, cplx<float>* __restrict__ recv) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nrec; i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t e = rmap[i];
+    recv[i] = e >= 0 ? mk(va[e], vb[e]) : mk(0.0f, 0.0f);
   }
 }
 
@@ -2292,25 +2345,68 @@ static int64_t lds_grid(int64_t nb) {
   return std::max<int64_t>(NXCD, std::min<int64_t>(nb, cus) / NXCD * NXCD);
 }
 
-// the LDS-staged SpMM (its dynamic LDS opt-in is per kernel, set before each launch)
-template <typename T, typename CF, typename A>
-static void launch_spmm_lds_cf(const LdsArgs& pl, dim3 grid, hipStream_t s, const T* P, T* Q, ColState<A> st) {
-  (void)hipFuncSetAttribute((const void*)k_cg_spmm_lds<T, CF>, hipFuncAttributeMaxDynamicSharedMemorySize, CF::BYTES);
-  plaunch(k_cg_spmm_lds<T, CF>, grid, dim3((LDS_NW + LDS_NP) * 64), CF::BYTES, s, pl.nblk, pl.brow, pl.bu, pl.ulist,
-          pl.rinfo, pl.loff, (const T*)pl.recv, P, Q, st);
+// the gated float64 residual sweep of an LDS-staged solve, by 64-column chunks per launch
+template <typename T64, typename T32>
+static const void* resid_sweep_fn(int nch) {
+  return nch == 1   ? (const void*)k_cg_resid_sweep<T64, T32, 1>
+         : nch == 2 ? (const void*)k_cg_resid_sweep<T64, T32, 2>
+         : nch == 4 ? (const void*)k_cg_resid_sweep<T64, T32, 4>
+                    : (const void*)k_cg_resid_sweep<T64, T32, 8>;
 }
+template <typename T64, typename T32, typename A32, typename A64>
+static void launch_resid_sweep(int nch, dim3 grid, hipStream_t s, const Csr& m, const T64* A64v, const T64* B,
+                               const T64* X, const T32* D, const T32* dinv, T32* R, int C, int k, ColState<A32> ss,
+                               ColState<A64> sol) {
+  if (nch == 1)
+    plaunch(k_cg_resid_sweep<T64, T32, 1>, grid, dim3(KB), 0, s, m, A64v, B, X, D, dinv, R, C, k, ss, sol);
+  else if (nch == 2)
+    plaunch(k_cg_resid_sweep<T64, T32, 2>, grid, dim3(KB), 0, s, m, A64v, B, X, D, dinv, R, C, k, ss, sol);
+  else if (nch == 4)
+    plaunch(k_cg_resid_sweep<T64, T32, 4>, grid, dim3(KB), 0, s, m, A64v, B, X, D, dinv, R, C, k, ss, sol);
+  else
+    plaunch(k_cg_resid_sweep<T64, T32, 8>, grid, dim3(KB), 0, s, m, A64v, B, X, D, dinv, R, C, k, ss, sol);
+}
+
+// the LDS-staged SpMM (its dynamic LDS opt-in is per kernel, set before each launch)
+template <typename T, typename CF, bool RPAIR, typename A>
+static void launch_spmm_lds_cf(const LdsArgs& pl, dim3 grid, hipStream_t s, const T* recv, const T* P, T* Q,
+                               int64_t rs, int64_t coff, int cbase, ColState<A> st) {
+  (void)hipFuncSetAttribute((const void*)k_cg_spmm_lds<T, CF, RPAIR>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            CF::BYTES);
+  plaunch(k_cg_spmm_lds<T, CF, RPAIR>, grid, dim3((LDS_NW + LDS_NP) * 64), CF::BYTES, s, pl.nblk, pl.brow, pl.bu,
+          pl.ulist, pl.rinfo, pl.loff, recv, P, Q, rs, coff, cbase, st);
+}
+// one 64-column system with 8-byte values (rows of 64 columns)
 template <typename T, typename A>
 static void launch_spmm_lds(const LdsArgs& pl, dim3 grid, hipStream_t s, const T* P, T* Q, ColState<A> st) {
   if constexpr (sizeof(T) == 8) {
-    launch_spmm_lds_cf<T, LdsWide>(pl, grid, s, P, Q, st);
+    launch_spmm_lds_cf<T, LdsWide, false>(pl, grid, s, (const T*)pl.recv, P, Q, 64, 0, 0, st);
   }
+}
+// real float32 systems of 64 columns each, in pairs: one launch per pair, record set p for pair p
+template <typename A>
+static void launch_spmm_lds_pairs(const LdsArgs& pl, int64_t nrec_set, dim3 grid, hipStream_t s, int nsys,
+                                  const float* P, float* Q, ColState<A> st) {
+  const int C = 64 * nsys;
+  for (int p = 0; p < nsys / 2; ++p)
+    launch_spmm_lds_cf<cplx<float>, LdsWide, true>(pl, grid, s, (const cplx<float>*)pl.recv + p * nrec_set,
+                                                   (const cplx<float>*)P, (cplx<float>*)Q, C / 2, 64 * p, 128 * p,
+                                                   st);
 }
 
 // a plan applies to 64-column solves of one system with 8-byte values
-static bool lds_eligible(const rb_spmm_plan* plan, int nsys, int C, size_t value_bytes) {
+static bool lds_plan_ok(const rb_spmm_plan* plan) {
   const bool cfg_ok = plan && plan->umax == LdsWide::UMAX && plan->recmax == LdsWide::RECMAX;
   return cfg_ok && plan->nblk > 0 && plan->brow && plan->bu && plan->ulist && plan->rinfo && plan->rmap &&
-         plan->loff && plan->recv && nsys == 1 && C == 64 && value_bytes == 8;
+         plan->loff && plan->recv && plan->recv_sets >= 1;
+}
+static bool lds_eligible(const rb_spmm_plan* plan, int nsys, int C, size_t value_bytes) {
+  return lds_plan_ok(plan) && nsys == 1 && C == 64 && value_bytes == 8;
+}
+// ... and to real float32 systems of 64 columns each in pairs, one record set per pair
+static bool lds_pairs_eligible(const rb_spmm_plan* plan, int nsys, int k, size_t value_bytes) {
+  return lds_plan_ok(plan) && k == 64 && nsys >= 2 && nsys % 2 == 0 && value_bytes == 4 &&
+         plan->recv_sets >= nsys / 2;
 }
 
 static LdsArgs lds_args(const rb_spmm_plan* plan) {
@@ -2513,16 +2609,28 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
   const bool pair = std::is_same<T32, float>::value && pair_sweep<T32>(nsys, C, k);
   // the LDS-staged SpMM (a plan for this 64-column system): its own launch, then the
   // float64 residual of the requested chunks in a gated sweep launch
-  const bool use_lds = flat && lds_eligible(plan, nsys, C, sizeof(T32));
+  // (or, for real float32 systems of 64 columns in pairs, one LDS launch per pair)
+  const bool lds_one = flat && lds_eligible(plan, nsys, C, sizeof(T32));
+  const bool lds_pairs = flat && pair && lds_pairs_eligible(plan, nsys, k, sizeof(T32));
+  const bool use_lds = lds_one || lds_pairs;
+  const int rnch = pick_NCH(C);
   LdsArgs la{};
-  int64_t nbr = 0;
+  int64_t nbr = 0, nrec_set = 0;
   if (use_lds) {
     la = lds_args(plan);
-    const int64_t nr = plan->nrec + RB_LDS_RECPAD;
-    hipLaunchKernelGGL(k_lds_pack<T32>, dim3(ew_grid(nr)), dim3(256), 0, s, nr, plan->rmap, A32, (T32*)plan->recv);
+    nrec_set = plan->nrec + RB_LDS_RECPAD;
+    if (lds_one) {
+      hipLaunchKernelGGL(k_lds_pack<T32>, dim3(ew_grid(nrec_set)), dim3(256), 0, s, nrec_set, plan->rmap, A32,
+                         (T32*)plan->recv);
+    } else if constexpr (std::is_same<T32, float>::value) {
+      for (int pp = 0; pp < nsys / 2; ++pp)
+        hipLaunchKernelGGL(k_lds_pack_pair, dim3(ew_grid(nrec_set)), dim3(256), 0, s, nrec_set, plan->rmap,
+                           A32 + (int64_t)(2 * pp) * m.nnz, A32 + (int64_t)(2 * pp + 1) * m.nnz,
+                           (cplx<float>*)plan->recv + pp * nrec_set);
+    }
     RB_CHECK_LAUNCH();
     nbs = lds_grid(nb);
-    nbr = resident_grid((const void*)k_cg_resid_sweep<T64, T32, 1>, nb);
+    nbr = resident_grid(resid_sweep_fn<T64, T32>(rnch), nb);
   } else if (flat) {  // the fused SpMM + residual launch has its own occupancy
     nbs = std::min(nbs, resident_grid(spmm_ru_fn<T64, T32>(pick_NCH(C), pair), nb));
   }
@@ -2549,10 +2657,14 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
       {
         ProfScope ps(RB_PROF_SPMM, s);
         if (use_lds) {
-          launch_spmm_lds<T32>(la, dim3(nbs), s, w.P, w.Q, ss);
+          if (lds_pairs) {
+            if constexpr (std::is_same<T32, float>::value)
+              launch_spmm_lds_pairs(la, nrec_set, dim3(nbs), s, nsys, w.P, w.Q, ss);
+          } else {
+            launch_spmm_lds<T32>(la, dim3(nbs), s, w.P, w.Q, ss);
+          }
           ProfScope pr(RB_PROF_RESID, s);
-          plaunch(k_cg_resid_sweep<T64, T32, 1>, dim3(nbr), dim3(KB), 0, s, m, A64, B, X, D, w.dinv, w.R, C, k, ss,
-                  sol);
+          launch_resid_sweep<T64, T32>(rnch, dim3(nbr), s, m, A64, B, X, D, w.dinv, w.R, C, k, ss, sol);
         } else if (flat) {
           DISPATCH_GN(C, L_SPMMRU, m, A32, w.P, w.Q, A64, B, X, D, w.dinv, w.R, C, k, ss, sol);
         } else {

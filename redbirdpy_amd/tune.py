@@ -152,8 +152,8 @@ def tune_system(sysm, B, k, precision="mixed", tol=1e-10, maxiter=2000, iters=48
     from . import _gpu
 
     m = sysm.mesh
-    plan_ok = (sysm.nsys == 1 and k == 64 and sysm.is_complex == (precision == "mixed")
-               and m.spmm_plan() is not None)
+    sets = _gpu.lds_plan_sets(sysm.nsys, k, sysm.is_complex, precision)
+    plan_ok = sets > 0 and m.spmm_plan(sets) is not None
     flat = _gpu.sweep_columns(m.nn, k, sysm.nsys, sysm.is_complex) == k
     cands = []
     if plan_ok:

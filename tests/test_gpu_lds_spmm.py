@@ -93,3 +93,30 @@ def test_irregular_mesh_64col(precision, monkeypatch):
     d3, f3 = orc.runforward(c3)
     assert _rel(f1, f3) < 1e-10 and _relmax(d1, d3) < 1e-8
     assert _rel(f1, f2) < 1e-10 and abs(it1 - it2) <= 3
+
+
+@pytest.mark.parametrize("nw", [2, 4, 6])
+def test_cw_mixed_wavelength_pairs_against_oracle_and_gather(nw, monkeypatch):
+    """Mixed solves of nw real 64-column systems (one per wavelength, the Gauss-Newton forward's
+    shape): one LDS launch per pair of systems (float32 pair records, lanes 0-31 the first system's
+    columns, 32-63 the second's), then the gated float64 residual over all 64-column chunks.  Every
+    wavelength's fields and detector values against the oracle and against the gather SpMM."""
+    from oracle import rb_oracle as orc
+    from redbirdpy_amd import _gpu, workloads
+    import redbirdpy_amd as rb
+
+    cfg = workloads.jac_box(2.0)
+    cfg["prop"] = {str(650 + 30 * i): np.array([[0, 0, 1, 1], [0.006 + 0.004 * i, 0.8 + 0.1 * i, 0, 1.37]])
+                   for i in range(nw)}
+    c0, _ = rb.meshprep(copy.deepcopy(cfg))
+    assert _gpu.lds_plan_sets(nw, 64, False, "mixed") == nw // 2
+    assert _gpu.mesh_for(c0).spmm_plan(nw // 2).struct.recv_sets >= nw // 2
+    (d1, f1, it1), (d2, f2, it2) = _forward_both(cfg, "mixed", monkeypatch)
+    c3, _ = orc.meshprep(copy.deepcopy(cfg))
+    d3, f3 = orc.runforward(c3)
+    assert sorted(d1) == sorted(d3)
+    for wv in d3:
+        assert f1[wv].shape[1] == 64
+        assert _rel(f1[wv], f3[wv]) < 1e-10 and _relmax(d1[wv], d3[wv]) < 1e-8
+        assert _rel(f1[wv], f2[wv]) < 1e-10
+    assert abs(it1 - it2) <= 3
