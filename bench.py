@@ -209,16 +209,16 @@ def cpu_jac_baseline(phi, sd, cfg, pairs=32):
     }
 
 
-def _tuned(cfg, precision, tol):
+def _tuned(cfg, precision, tol, warm_from=None):
     """Tune cfg's forward solves (tune.tune_forward, cached for the run) and report the choice."""
     from redbirdpy_amd import tune
 
     t0 = time.perf_counter()
-    out = tune.tune_forward(cfg, precision=precision, tol=tol)
+    out = tune.tune_forward(cfg, precision=precision, tol=tol, warm_from=warm_from)
     return {"lds": out["lds"], "grid": out["grid"], "check_every": out["check_every"],
             "tuning_s": time.perf_counter() - t0,
-            "us_per_iter": {("lds" if c["lds"] else "sweep grid %d" % c["grid"]): c["seconds_per_iter"] * 1e6
-                            for c in out["candidates"] if "seconds_per_iter" in c}}
+            "ms_per_solve": {"%s, poll %d" % ("lds" if c["lds"] else "sweep grid %d" % c["grid"], c["check_every"]):
+                             c["seconds_per_solve"] * 1e3 for c in out["candidates"]}}
 
 
 def gn_bench(a):
@@ -245,7 +245,12 @@ def gn_bench(a):
     out = {"m": m, "nodes": int(ct["node"].shape[0]), "iters": a.gn_iters, "solver_tol": a.gn_tol,
            "mixed_step_reftol": a.gn_reftol}
     if a.autotune:
-        out["autotune"] = {prec: _tuned(ct, prec, a.gn_tol) for prec in a.gn_precisions.split(",")}
+        # the Gauss-Newton forward solves start from the previous iteration's fields: tuned warm-started
+        # from the fields at the starting (background) concentrations
+        cb = {k: v for k, v in ct.items()}
+        cb["param"] = {k: v.copy() for k, v in bg.items()}
+        cb["prop"] = rb.updateprop(cb, extin=extin)
+        out["autotune"] = {prec: _tuned(ct, prec, a.gn_tol, warm_from=cb) for prec in a.gn_precisions.split(",")}
     results = {}
     for prec in a.gn_precisions.split(","):
         # one untimed iteration first (workspace and mesh-structure allocation)

@@ -242,12 +242,42 @@ class _Plan:
         return self
 
 
+_PLAN_CACHE = []  # [(key, plan)]: plans by pattern content, most recent first
+
+
+def _pattern_key(rp, ci, dev, limits):
+    try:
+        import xxhash
+
+        h = xxhash.xxh3_64()
+    except ImportError:  # pragma: no cover - xxhash ships with this environment
+        import hashlib
+
+        h = hashlib.blake2b(digest_size=16)
+    h.update(rp.tobytes())
+    h.update(ci.tobytes())
+    return (str(dev), tuple(limits), rp.size, ci.size, h.hexdigest())
+
+
 def spmm_plan(row_ptr, col_idx, dev, limits=None):
     """LDS-staged SpMM plan of a CSR pattern (host arrays, device row order): blocks of consecutive
     rows with their neighbour-row unions and padded entry records (rb_spmm_plan_count/fill on the
-    host), uploaded to dev.  None when some row alone exceeds the block limits (RB_ERR_SIZE)."""
+    host), uploaded to dev.  None when some row alone exceeds the block limits (RB_ERR_SIZE).  Plans
+    are kept by pattern content (a small LRU): a mesh rebuilt from the same arrays reuses its plan."""
     rp = np.ascontiguousarray(np.asarray(row_ptr).astype(np.int32))
     ci = np.ascontiguousarray(np.asarray(col_idx).astype(np.int32))
+    key = _pattern_key(rp, ci, dev, limits or _abi.RB_LDS_LIMITS)
+    for i, (k, p) in enumerate(_PLAN_CACHE):
+        if k == key:
+            _PLAN_CACHE.insert(0, _PLAN_CACHE.pop(i))
+            return p
+    p = _build_spmm_plan(rp, ci, dev, limits)
+    _PLAN_CACHE.insert(0, (key, p))
+    del _PLAN_CACHE[4:]
+    return p
+
+
+def _build_spmm_plan(rp, ci, dev, limits=None):
     n = rp.size - 1
     nblk, nun, nrec = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
     L = _abi.lib()

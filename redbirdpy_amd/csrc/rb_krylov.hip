@@ -24,6 +24,7 @@
 #include "rb_common.h"
 #include "../../include/rbhip.h"
 #include <vector>
+#include <cstring>
 #include <atomic>
 #include <mutex>
 #include <stdlib.h>
@@ -1887,17 +1888,6 @@ __global__ void __launch_bounds__(FB) k_fin_alpha_ru(int C, double tol2, ColStat
   }
 }
 
-// count active columns (one block), for the host's convergence polls
-__global__ void k_count_active(const int* active, int C, int* nactive) {
-  __shared__ int s;
-  if (threadIdx.x == 0) s = 0;
-  __syncthreads();
-  int loc = 0;
-  for (int c = threadIdx.x; c < C; c += blockDim.x) loc += active[c] ? 1 : 0;
-  atomicAdd(&s, loc);
-  __syncthreads();
-  if (threadIdx.x == 0) *nactive = s;
-}
 
 template <typename T32>
 __global__ void k_mask_cols(int64_t total, int C, const int* __restrict__ active, T32* __restrict__ R32) {
@@ -2082,21 +2072,68 @@ static int* g_pinned = nullptr;  // host flag for convergence polls
 // allocation (no pinned poll buffer, no timing events)
 static thread_local bool t_noalloc = false;
 
-static int poll_nactive(const int* active, int C, int* d_nactive, hipStream_t s, int* out) {
-  hipLaunchKernelGGL(k_count_active, dim3(1), dim3(256), 0, s, active, C, d_nactive);
+// active columns and the largest rr / (tol2 bb) among them (one block): the poll the
+// iteration loops size their next chunk from
+__global__ void k_poll(const int* __restrict__ active, const double* __restrict__ rr, const double* __restrict__ bb,
+                       double tol2, int C, int* out) {
+  __shared__ int s_n[256];
+  __shared__ double s_r[256];
+  int n = 0;
+  double r = 0.0;
+  for (int c = threadIdx.x; c < C; c += blockDim.x) {
+    if (!active[c]) continue;
+    ++n;
+    const double den = tol2 * bb[c];
+    const double q = den > 0.0 ? rr[c] / den : 0.0;
+    r = q > r ? q : r;
+  }
+  s_n[threadIdx.x] = n;
+  s_r[threadIdx.x] = r;
+  __syncthreads();
+  for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      s_n[threadIdx.x] += s_n[threadIdx.x + h];
+      s_r[threadIdx.x] = fmax(s_r[threadIdx.x], s_r[threadIdx.x + h]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = s_n[0];
+    *(double*)(out + 2) = s_r[0];
+  }
+}
+
+// poll: active columns and the largest ratio (out_ratio; 0 when none is known)
+template <typename A>
+static int poll_state(const ColState<A>& st, int C, double tol2, hipStream_t s, int* out, double* out_ratio) {
+  hipLaunchKernelGGL(k_poll, dim3(1), dim3(256), 0, s, st.active, st.rr, st.bb, tol2, C, st.nactive);
   RB_CHECK_LAUNCH();
   if (!g_pinned && !t_noalloc) RB_HIP(hipHostMalloc((void**)&g_pinned, 64, hipHostMallocDefault));
-  if (g_pinned) {
-    RB_HIP(hipMemcpyAsync(g_pinned, d_nactive, sizeof(int), hipMemcpyDeviceToHost, s));
-    RB_HIP(hipStreamSynchronize(s));
-    *out = *g_pinned;
-  } else {  // pageable destination: the copy completes before it returns
-    int v = 0;
-    RB_HIP(hipMemcpyAsync(&v, d_nactive, sizeof(int), hipMemcpyDeviceToHost, s));
-    RB_HIP(hipStreamSynchronize(s));
-    *out = v;
-  }
+  int buf[4] = {0, 0, 0, 0};
+  int* dst = g_pinned ? g_pinned : buf;  // a pageable destination completes before the copy returns
+  RB_HIP(hipMemcpyAsync(dst, st.nactive, 16, hipMemcpyDeviceToHost, s));
+  RB_HIP(hipStreamSynchronize(s));
+  *out = dst[0];
+  double r;
+  memcpy(&r, dst + 2, sizeof(double));
+  *out_ratio = r;
   return RB_OK;
+}
+
+// iterations to run before the next poll: check_every, or fewer when the decay of the largest
+// residual ratio since the last poll says the active columns reach the tolerance sooner (a
+// converged solve then stops within a few iterations instead of running up to check_every - 1
+// rounds of gated launches); at least 2
+static int next_chunk(int check_every, double ratio, double prev_ratio, int prev_chunk) {
+  const int lo = check_every < 2 ? check_every : 2;
+  if (!(ratio > 0.0) || !(prev_ratio > 0.0) || prev_chunk <= 0) return check_every;
+  if (ratio <= 1.0) return lo;  // below the tolerance: the confirming replacement is next
+  if (!(prev_ratio > ratio)) return check_every;
+  const double per_it = log(prev_ratio / ratio) / prev_chunk;
+  const double need = log(ratio) / per_it;
+  if (!(need < (double)check_every)) return check_every;
+  const int c = (int)ceil(need) + 1;
+  return c < lo ? lo : (c > check_every ? check_every : c);
 }
 
 // ---------------------------------------------------------------------------
@@ -2437,11 +2474,12 @@ static int run_cg(const Csr& m, int nsys, const T* vals, const T* dinv, const T*
   }
   RB_CHECK_LAUNCH();
   int nact = 0;
-  int rc = poll_nactive(st.active, C, st.nactive, s, &nact);
+  double ratio = 0.0, prev_ratio = 0.0;
+  int rc = poll_state(st, C, st.tol2, s, &nact, &ratio);
   if (rc) return rc;
-  int it = 0;
+  int it = 0, last = 0;
   while (nact > 0 && it < st.maxiter) {
-    int chunk = std::min(check_every, st.maxiter - it);
+    int chunk = std::min(next_chunk(check_every, ratio, prev_ratio, last), st.maxiter - it);
     for (int j = 0; j < chunk; ++j) {
       {
         ProfScope ps(RB_PROF_SPMM, s);
@@ -2467,7 +2505,9 @@ static int run_cg(const Csr& m, int nsys, const T* vals, const T* dinv, const T*
     }
     RB_CHECK_LAUNCH();
     it += chunk;
-    rc = poll_nactive(st.active, C, st.nactive, s, &nact);
+    last = chunk;
+    prev_ratio = ratio;
+    rc = poll_state(st, C, st.tol2, s, &nact, &ratio);
     if (rc) return rc;
   }
   *iters_done = it;
@@ -2646,11 +2686,14 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
   DISPATCH_GN(C, L_PUP, m.n, w.dinv, D, w.R, w.P, C, k, X, si);  // p = z (P is zero, alpha 0, no flags)
   RB_CHECK_LAUNCH();
   int nact = 0;
-  rc = poll_nactive(si.active, C, si.nactive, s, &nact);
+  // (the inner recursive residual against the float64 tolerance sizes the chunks; the float64
+  // replacements alone decide convergence)
+  double ratio = 0.0, prev_ratio = 0.0;
+  rc = poll_state(si, C, tol2, s, &nact, &ratio);
   if (rc) return rc;
-  int it = 0;
+  int it = 0, last = 0;
   while (nact > 0 && it < maxiter) {
-    int chunk = std::min(check_every, maxiter - it);
+    int chunk = std::min(next_chunk(check_every, ratio, prev_ratio, last), maxiter - it);
     for (int j = 0; j < chunk; ++j) {
       // request flags: this iteration raises half (it+j)&1 and serves the other
       si.ru_par = ss.ru_par = sol.ru_par = (it + j) & 1;
@@ -2688,7 +2731,9 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
     }
     RB_CHECK_LAUNCH();
     it += chunk;
-    rc = poll_nactive(si.active, C, si.nactive, s, &nact);
+    last = chunk;
+    prev_ratio = ratio;
+    rc = poll_state(si, C, tol2, s, &nact, &ratio);
     if (rc) return rc;
   }
 #undef T

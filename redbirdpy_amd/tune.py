@@ -13,10 +13,13 @@ how the GPU runs it:
   is a short host round trip; a column that converges between polls runs on
   with its kernels gated off until the next one).
 
-``tune_system`` times each candidate on the system itself: the SpMM choices
-with short solves whose tolerance is 0 (a fixed number of iterations), the
-poll interval with full solves to the requested tolerance, and keeps the
-fastest.  Results are cached per (device name, nodes, nonzeros, columns,
+``tune_system`` times each candidate on the system itself with full solves to
+the requested tolerance (columns converge at different iterations and the
+kernels skip converged 64-column chunks, so a fixed-iteration timing with
+every column active ranks the SpMM paths differently: at 115k nodes it put
+the 4-system LDS pairs 1% ahead of the sweep, whose Gauss-Newton forward
+solves are 6% faster), first the SpMM choices, then the poll interval for
+the fastest, and keeps the fastest.  Results are cached per (device name, nodes, nonzeros, columns,
 systems, real/complex, precision) in a JSON file (``RB_TUNE_CACHE``, default
 ``~/.cache/redbirdpy_amd/tune.json``); ``_gpu.solve_system`` applies a cached
 entry to every later solve of that shape, and with ``RB_AUTOTUNE=1`` tunes a
@@ -145,10 +148,11 @@ def _timed(fn, reps):
     return best
 
 
-def tune_system(sysm, B, k, precision="mixed", tol=1e-10, maxiter=2000, iters=48, reps=2, save=True):
-    """Measure the launch choices for solving sysm (nsys systems x k columns of B) and cache the
-    fastest.  Returns {"lds", "grid", "check_every", "seconds_per_iter", "seconds_per_solve",
-    "candidates": [...]}."""
+def tune_system(sysm, B, k, precision="mixed", tol=1e-10, maxiter=2000, reps=2, save=True, X0=None):
+    """Measure the launch choices for solving sysm (nsys systems x k columns of B) to tol and cache
+    the fastest.  X0: the warm start the workload's solves begin from (a Gauss-Newton forward starts
+    from the previous iteration's fields; warm and cold solves can rank the SpMM paths differently).
+    Returns {"lds", "grid", "check_every", "seconds_per_solve", "candidates": [...]}."""
     from . import _gpu
 
     m = sysm.mesh
@@ -160,35 +164,38 @@ def tune_system(sysm, B, k, precision="mixed", tol=1e-10, maxiter=2000, iters=48
         cands.append({"lds": True, "grid": 0})
     if flat or not plan_ok:
         cands += [{"lds": False, "grid": g} for g in (GRID_CAPS if flat else (0,))]
-    res = []
-    for c in cands:
-        with applied(c):
-            t = _timed(lambda: _gpu.solve_system(sysm, B, k, precision, 0.0, iters, tuned=dict(c, check_every=16)),
-                       reps)
-        res.append(dict(c, seconds_per_iter=t / iters))
-    best = min(res, key=lambda r: r["seconds_per_iter"])
-    polls = []
-    for ce in CHECK_EVERY:
-        ch = {"lds": best["lds"], "grid": best["grid"], "check_every": ce}
+    def solve_time(ch):
         with applied(ch):
-            t = _timed(lambda: _gpu.solve_system(sysm, B, k, precision, tol, maxiter, tuned=ch), reps)
-        polls.append(dict(ch, seconds_per_solve=t))
+            return _timed(lambda: _gpu.solve_system(sysm, B, k, precision, tol, maxiter, X0=X0, tuned=ch), reps)
+
+    res = [dict(c, check_every=16, seconds_per_solve=solve_time(dict(c, check_every=16))) for c in cands]
+    best = min(res, key=lambda r: r["seconds_per_solve"])
+    polls = [best]
+    for ce in CHECK_EVERY:
+        if ce != 16:
+            ch = {"lds": best["lds"], "grid": best["grid"], "check_every": ce}
+            polls.append(dict(ch, seconds_per_solve=solve_time(ch)))
     pbest = min(polls, key=lambda r: r["seconds_per_solve"])
-    choice = {"lds": best["lds"], "grid": best["grid"], "check_every": pbest["check_every"]}
+    choice = {"lds": pbest["lds"], "grid": pbest["grid"], "check_every": pbest["check_every"]}
     if save:
         store(sysm, k, precision, choice)
-    return dict(choice, seconds_per_iter=best["seconds_per_iter"], seconds_per_solve=pbest["seconds_per_solve"],
-                candidates=res + polls)
+    return dict(choice, seconds_per_solve=pbest["seconds_per_solve"], candidates=res + polls[1:])
 
 
-def tune_forward(cfg, rfcw=(1,), precision="mixed", tol=1e-12, maxiter=None, optodes=None, **kw):
+def tune_forward(cfg, rfcw=(1,), precision="mixed", tol=1e-12, maxiter=None, optodes=None, warm_from=None, **kw):
     """Tune the solves of cfg's forward run (forward_device's systems and right-hand sides): one
     entry for the whole multi-system solve, or, past 512 columns, for its one-system solves.
-    Returns tune_system's result."""
+    warm_from: a cfg on the same mesh whose forward fields start each timed solve (as the next
+    Gauss-Newton iteration starts from the last one's).  Returns tune_system's result."""
     import numpy as np
     from . import _gpu
     from .forward import forward_system
 
+    from .forward import forward_device
+
+    X0 = None
+    if warm_from is not None:
+        X0 = forward_device(warm_from, rfcw, "fp64", tol, maxiter, optodes=optodes, where="tune_forward").X
     mesh, sysm, B, k, systems, _, _ = forward_system(cfg, rfcw, precision, optodes)
     if maxiter is None:
         maxiter = max(4000, 4 * int(np.sqrt(mesh.nn)) + 1000)
@@ -196,4 +203,5 @@ def tune_forward(cfg, rfcw=(1,), precision="mixed", tol=1e-12, maxiter=None, opt
         sysm = _gpu.System(mesh, sysm.vals64[:1], None if sysm.vals32 is None else sysm.vals32[:1], 1,
                            sysm.is_complex)
         B = B[:, :k].contiguous()
-    return tune_system(sysm, B, k, precision, tol, maxiter, **kw)
+        X0 = None if X0 is None else X0[:, :k].contiguous()
+    return tune_system(sysm, B, k, precision, tol, maxiter, X0=X0, **kw)

@@ -24,9 +24,9 @@ def test_tune_forward_caches_a_choice_and_results_stay_exact(tmp_path, monkeypat
     c2, _ = orc.meshprep(copy.deepcopy(cfg))
     d2, _ = orc.runforward(c2)
     out = tune.tune_forward(c1, precision="mixed", tol=1e-12)
-    assert set(out) >= {"lds", "grid", "check_every", "seconds_per_iter", "candidates"}
+    assert set(out) >= {"lds", "grid", "check_every", "seconds_per_solve", "candidates"}
     assert out["grid"] in tune.GRID_CAPS and out["check_every"] in tune.CHECK_EVERY
-    assert len(out["candidates"]) >= 1 + len(tune.CHECK_EVERY)
+    assert len(out["candidates"]) >= len(tune.CHECK_EVERY)
     assert _abi.lib().rb_krylov_get_sweep_grid() == 0  # restored after every candidate
     mesh, sysm, B, k, *_ = forward_system(c1, (1,), "mixed")
     assert tune.lookup(sysm, k, "mixed") == {key: out[key] for key in ("lds", "grid", "check_every")}

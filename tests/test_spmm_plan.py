@@ -131,3 +131,21 @@ def test_sweep_columns_groups_of_64_under_the_descriptor_limit(monkeypatch):
     monkeypatch.setenv("RB_SPMM_FLAT_MAX_BYTES", str(726 * 8 * 100))
     assert _gpu.sweep_columns(726, 272, 1, False) == 64
     assert _gpu.sweep_columns(726, 99, 1, False) == 99
+
+
+def test_plans_are_kept_by_pattern_content():
+    """The same pattern from other arrays (a mesh rebuilt from copies) returns the kept plan; another
+    pattern builds its own."""
+    import numpy as np
+    import scipy.sparse as sp
+    from redbirdpy_amd import _gpu
+
+    A = sp.random(300, 300, density=0.02, format="csr", random_state=3) + sp.eye(300, format="csr")
+    A.sort_indices()
+    p1 = _gpu.spmm_plan(A.indptr, A.indices, "cpu")
+    p2 = _gpu.spmm_plan(A.indptr.copy(), A.indices.copy(), "cpu")
+    assert p1 is not None and p2 is p1
+    B = A + sp.eye(300, k=5, format="csr")
+    B.sort_indices()
+    p3 = _gpu.spmm_plan(B.indptr, B.indices, "cpu")
+    assert p3 is not p1
