@@ -216,8 +216,9 @@ def _tuned(cfg, precision, tol, warm_from=None):
     t0 = time.perf_counter()
     out = tune.tune_forward(cfg, precision=precision, tol=tol, warm_from=warm_from)
     return {"lds": out["lds"], "grid": out["grid"], "check_every": out["check_every"],
-            "tuning_s": time.perf_counter() - t0,
-            "ms_per_solve": {"%s, poll %d" % ("lds" if c["lds"] else "sweep grid %d" % c["grid"], c["check_every"]):
+            "precond": ("jacobi", "poly")[out["precond"]], "tuning_s": time.perf_counter() - t0,
+            "ms_per_solve": {"%s, %s, poll %d" % ("lds" if c["lds"] else "sweep grid %d" % c["grid"],
+                                                  ("jacobi", "poly")[c["precond"]], c["check_every"]):
                              c["seconds_per_solve"] * 1e3 for c in out["candidates"]}}
 
 

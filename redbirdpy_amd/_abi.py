@@ -18,6 +18,8 @@ RB_PROP_NODE = 1
 RB_PROP_VOLUME = 2
 RB_PREC_FP64 = 0
 RB_PREC_MIXED = 1
+RB_PRECOND_JACOBI = 0
+RB_PRECOND_POLY = 1
 RB_F64, RB_C128, RB_F32, RB_C64 = 0, 1, 2, 3
 RB_PROF_SPMM, RB_PROF_UPDATE, RB_PROF_PUPDATE, RB_PROF_RESID, RB_PROF_INIT = 0, 1, 2, 3, 4
 RB_WF_SINK, RB_WF_FACES = 0, 1
@@ -71,6 +73,8 @@ _SIGS = {
     "rb_krylov_workspace": [_i64, _int, _int, _int, _int, ctypes.POINTER(_sz)],
     "rb_krylov_set_sweep_grid": [_i64],
     "rb_krylov_get_sweep_grid": [],
+    "rb_krylov_set_precond": [_int],
+    "rb_krylov_get_precond": [],
     "rb_krylov_solve": [_i64, _i64, _vp, _vp, _vp, _int, _int, _int, _vp, _vp, _vp, _vp, _int, _dbl, _int,
                         _int, _int, _vp, _sz, ctypes.POINTER(_int), ctypes.POINTER(_dbl), ctypes.POINTER(_int),
                         _vp],

@@ -1,0 +1,105 @@
+"""The degree-1 polynomial preconditioner (rb_krylov_set_precond(RB_PRECOND_POLY)) in every solve
+path it serves: mixed FD through the LDS-staged SpMM, float64 FD through the column sweep, float64
+CW through the LDS SpMM, a mixed real single system, two complex systems in one solve, and the
+row-per-group SpMM of a solve past the sweep's descriptor limit.  Each against the oracle's SuperLU
+solves at the same tolerances as the Jacobi solves, and with fewer iterations than Jacobi."""
+
+import copy
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _relmax(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _both(cfg, precision, monkeypatch, tol=1e-12):
+    import redbirdpy_amd as rb
+    from redbirdpy_amd import tune
+
+    monkeypatch.setenv("RB_TUNE_CACHE", "/nonexistent/dir/tune.json")  # no cached choices apply
+    out = {}
+    for pc in (0, 1):
+        c1, _ = rb.meshprep(copy.deepcopy(cfg))
+        with tune.applied({"precond": pc}):
+            d, f = rb.runforward(c1, precision=precision, tol=tol)
+        out[pc] = (d, f, rb.forward.last_info.iters)
+    return out
+
+
+def _oracle(cfg):
+    from oracle import rb_oracle as orc
+
+    c2, _ = orc.meshprep(copy.deepcopy(cfg))
+    return orc.runforward(c2)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_poly_fd_64col(precision, monkeypatch):
+    from redbirdpy_amd import workloads
+
+    cfg = workloads.fd_box(2.0)
+    out = _both(cfg, precision, monkeypatch)
+    d3, f3 = _oracle(cfg)
+    for pc in (0, 1):
+        d, f, _ = out[pc]
+        assert _rel(f, f3) < 1e-10 and _relmax(d, d3) < 1e-8
+    assert out[1][2] < 0.8 * out[0][2]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_poly_cw_64col(precision, monkeypatch):
+    from redbirdpy_amd import workloads
+
+    cfg = workloads.jac_box(2.0)
+    out = _both(cfg, precision, monkeypatch)
+    d3, f3 = _oracle(cfg)
+    d, f, _ = out[1]
+    assert _rel(f, f3) < 1e-10 and _relmax(d, d3) < 1e-8
+    assert out[1][2] < 0.8 * out[0][2]
+
+
+def test_poly_two_complex_systems_one_solve(monkeypatch):
+    from redbirdpy_amd import workloads
+
+    cfg = workloads.fd_box(2.0)
+    cfg["prop"] = {"690": np.array([[0, 0, 1, 1], [0.008, 1.0, 0, 1.37], [0.02, 1.2, 0, 1.37]]),
+                   "830": np.array([[0, 0, 1, 1], [0.012, 0.9, 0, 1.37], [0.03, 1.1, 0, 1.37]])}
+    out = _both(cfg, "mixed", monkeypatch)
+    d3, f3 = _oracle(cfg)
+    d, f, _ = out[1]
+    for wv in d3:
+        assert _rel(f[wv], f3[wv]) < 1e-10 and _relmax(d[wv], d3[wv]) < 1e-8
+
+
+def test_poly_row_group_spmm(monkeypatch):
+    """Past the sweep's descriptor limit (lowered for the test) the row-per-group SpMM serves both
+    products of an iteration."""
+    from redbirdpy_amd import workloads
+
+    monkeypatch.setenv("RB_SPMM_FLAT_MAX_BYTES", "4096")
+    monkeypatch.setenv("RB_LDS_SPMM", "0")
+    cfg = workloads.fd_box(2.0)
+    out = _both(cfg, "mixed", monkeypatch)
+    d3, f3 = _oracle(cfg)
+    d, f, _ = out[1]
+    assert _rel(f, f3) < 1e-10 and _relmax(d, d3) < 1e-8
+
+
+def test_precond_setting_round_trip():
+    from redbirdpy_amd import _abi
+
+    L = _abi.lib()
+    assert L.rb_krylov_get_precond() == _abi.RB_PRECOND_JACOBI
+    _abi.call("rb_krylov_set_precond", _abi.RB_PRECOND_POLY)
+    assert L.rb_krylov_get_precond() == _abi.RB_PRECOND_POLY
+    _abi.call("rb_krylov_set_precond", _abi.RB_PRECOND_JACOBI)
+    with pytest.raises(_abi.RbError):
+        _abi.call("rb_krylov_set_precond", 7)
