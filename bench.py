@@ -547,7 +547,7 @@ def main():
     _abi.call("rb_prof_read", _abi.RB_PROF_SPMM, ctypes.byref(ms), ctypes.byref(cnt))
     spmm_avg_s = ms.value / 1e3 / max(cnt.value, 1)
     per_kernel = {}
-    for name, kid in (("spmm", 0), ("update", 1), ("pupdate", 2), ("resid64", 3), ("init", 4)):
+    for name, kid in (("spmm", 0), ("update", 1), ("pupdate", 2), ("resid64", 3), ("init", 4), ("poly", 5)):
         m2 = ctypes.c_double(0)
         c2 = ctypes.c_int64(0)
         _abi.call("rb_prof_read", kid, ctypes.byref(m2), ctypes.byref(c2))
@@ -555,6 +555,13 @@ def main():
     C = k
     nbytes = _spmm_bytes(mesh.nn, mesh.nnz, C, 8)  # complex64 inner solve
     achieved = nbytes / spmm_avg_s / 1e9
+    pk = per_kernel["poly"]
+    if pk["launches_per_step"] > 0:
+        # the polynomial preconditioner's product, LDS-staged with its z epilogue: the records of
+        # A D^-1, r read once, z written once, plus D^-1
+        pbytes = nbytes + mesh.nn * 8
+        pavg = pk["ms_per_step"] / 1e3 / pk["launches_per_step"]
+        pk.update({"bytes_per_launch": pbytes, "avg_launch_us": pavg * 1e6, "achieved_GBs": pbytes / pavg / 1e9})
 
     traffic = traffic_plain = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")

@@ -266,8 +266,9 @@ typedef struct rb_spmm_plan {
   const int32_t* rmap;  /* record -> CSR slot, -1 for padding */
   const int32_t* loff;  /* record -> LDS byte offset of its column's row within the block */
   void* recv;           /* record values, written by each solve (8 bytes per record) */
-  int64_t recv_sets;    /* record value sets recv holds, nrec + RB_LDS_RECPAD records each: one per
-                           solve of one system, one per system pair for real float32 systems */
+  int64_t recv_sets;    /* record value sets recv holds, nrec + RB_LDS_RECPAD records each: for one
+                           system, A and (second, used by the polynomial preconditioner when
+                           present) A D^-1; one per system pair for real float32 systems */
 } rb_spmm_plan;
 int rb_spmm_plan_count(int64_t n, const int32_t* row_ptr /* host */, const int32_t* col_idx /* host */, int umax,
                        int recmax, int64_t* nblk /* host */, int64_t* nunion /* host */, int64_t* nrec /* host */);
@@ -326,7 +327,8 @@ int rb_krylov_get_precond(void);
 #define RB_PROF_PUPDATE 2 /* p = z + beta p */
 #define RB_PROF_RESID 3   /* float64 residual of the mixed-precision refinement */
 #define RB_PROF_INIT 4    /* initial residual */
-#define RB_PROF_NKERNELS 5
+#define RB_PROF_POLY 5    /* z = p(D^-1 A) D^-1 r of the polynomial preconditioner, r.z and r.r */
+#define RB_PROF_NKERNELS 6
 int rb_prof_enable(int on);
 int rb_prof_reset(void);
 int rb_prof_read(int id, double* total_ms /* host */, int64_t* count /* host */);

@@ -1065,6 +1065,7 @@ struct LdsArgs {
   const int32_t* rinfo;
   const int32_t* loff;
   const void* recv;
+  const void* recv_ad;  // the records of A D^-1 (the polynomial preconditioner's product), or null
 };
 
 // real type of a column type, and a column value scaled by it
@@ -1122,16 +1123,17 @@ struct LdsAcc<cplx<float>, true> {
 // lanes 32-63 the second's, by .y).  The launch's columns are [cbase, cbase + ncol) of st.
 //
 // POLY (one system): the product of the polynomial preconditioner with its epilogue fused: P
-// holds y = D^-1 r, the row's w = (A y)_row gives z = c0 y + c1 D^-1 w (c0, c1 from the
-// Gershgorin bound lbound[0]), z goes to Q, and the partials are r.z and r.r (R the residual)
-// in st's slots 0 and 1 of C columns, as k_cg_poly_z writes them.
+// holds the residual r and recv the records of A D^-1, so the row's w = (A D^-1 r)_row and its
+// own staged r give z = D^-1 (c0 r + c1 w) (c0, c1 from the Gershgorin bound lbound[0]); z goes
+// to Q, and the partials are r.z and r.r in st's slots 0 and 1 of C columns, as k_cg_poly_z
+// writes them.
 template <typename T, typename CF, bool RPAIR, bool POLY>
 __global__ void __launch_bounds__((LDS_NW + LDS_NP) * 64)
 k_cg_spmm_lds(int64_t nblk, const int32_t* __restrict__ brow, const int32_t* __restrict__ bu,
               const int32_t* __restrict__ ulist, const int32_t* __restrict__ rinfo, const int32_t* __restrict__ loff,
               const T* __restrict__ recv, const T* __restrict__ P, T* __restrict__ Q, int64_t rs, int64_t coff,
-              int cbase, ColState<typename LdsAcc<T, RPAIR>::type> st, const T* __restrict__ R,
-              const T* __restrict__ dinvp, const float* __restrict__ lbound, int C) {
+              int cbase, ColState<typename LdsAcc<T, RPAIR>::type> st, const T* __restrict__ dinvp,
+              const float* __restrict__ lbound, int C) {
   static_assert(!(RPAIR && POLY), "the polynomial epilogue serves one system of 64 columns");
   static_assert(sizeof(T) == 8, "512-byte rows: 64 lanes of 8 bytes");
   static_assert(!RPAIR || std::is_same<T, cplx<float>>::value, "a real pair is staged as 8-byte (float, float)");
@@ -1246,13 +1248,11 @@ k_cg_spmm_lds(int64_t nblk, const int32_t* __restrict__ brow, const int32_t* __r
           }
         }
         const T pown = *(const T*)(cb + ri[2 * i + 1]);
-        if constexpr (POLY) {
-          const int64_t ro = (int64_t)(r0 + i) * rs + coff + lane;
-          const T z = rscale(pown, pc0) + rscale(dinvp[r0 + i] * acc, pc1);
-          Q[ro] = z;
-          const T rv = R[ro];
-          pv = acc_add(pv, amul(rv, z));
-          pv2 = acc_add(pv2, acc_from_real(sc<T>::norm2(rv), A()));
+        if constexpr (POLY) {  // pown is this row's r, acc its (A D^-1 r)
+          const T z = dinvp[r0 + i] * (rscale(pown, pc0) + rscale(acc, pc1));
+          Q[(int64_t)(r0 + i) * rs + coff + lane] = z;
+          pv = acc_add(pv, amul(pown, z));
+          pv2 = acc_add(pv2, acc_from_real(sc<T>::norm2(pown), A()));
           continue;
         }
         Q[(int64_t)(r0 + i) * rs + coff + lane] = acc;
@@ -1310,6 +1310,17 @@ __global__ void k_lds_pack(int64_t nrec, const int32_t* __restrict__ rmap, const
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nrec; i += (int64_t)gridDim.x * blockDim.x) {
     const int32_t e = rmap[i];
     recv[i] = e >= 0 ? vals[e] : sc<T>::zero();
+  }
+}
+
+// records of A D^-1 (system 0): recv[i] = a_e / a_cc for slot e = rmap[i] in column c (0 for padding)
+template <typename T>
+__global__ void k_lds_pack_ad(int64_t nrec, const int32_t* __restrict__ rmap, const T* __restrict__ vals,
+                              const int32_t* __restrict__ col_idx, const T* __restrict__ dinv,
+                              T* __restrict__ recv) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nrec; i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t e = rmap[i];
+    recv[i] = e >= 0 ? vals[e] * dinv[col_idx[e]] : sc<T>::zero();
   }
 }
 
@@ -1425,7 +1436,8 @@ __global__ void __launch_bounds__(KB) k_cg_spmm_w64(Csr m, const T* __restrict__
   block_partials<A, 64, NCH>(st, pv, 1, C);
 }
 
-// r -= alpha q ; z = dinv r ; partials rz, rr
+// r -= alpha q ; z = dinv r ; partials rz, rr.  With the polynomial preconditioner (st.poly)
+// z and the partials come later: Y given, y = D^-1 r goes there; else only r is updated.
 template <typename T, int G, int NCH>
 __global__ void __launch_bounds__(KB) k_cg_update(int64_t n, const T* __restrict__ dinv, T* __restrict__ R,
                                                   const T* __restrict__ Q, int C, int k,
@@ -1455,17 +1467,17 @@ __global__ void __launch_bounds__(KB) k_cg_update(int64_t n, const T* __restrict
           r = r - scale_to(Q[o], al[ch]);
           R[o] = r;
         }
-        T z = dinv[(int64_t)s * n + row] * r;
-        if (Y) {  // polynomial preconditioner: y = D^-1 r for the second product; k_cg_poly_z forms z
-          Y[o] = z;
-        } else {
+        if (Y) {  // y = D^-1 r for the preconditioner's product; k_cg_poly_z forms z
+          Y[o] = dinv[(int64_t)s * n + row] * r;
+        } else if (!st.poly) {
+          const T z = dinv[(int64_t)s * n + row] * r;
           pv[ch][0] = acc_add(pv[ch][0], amul(r, z));
           pv[ch][1] = acc_add(pv[ch][1], acc_from_real(sc<T>::norm2(r), A()));
         }
       }
     }
   }
-  if (!Y) block_partials<A, G, NCH>(st, pv, 2, C);
+  if (!Y && !st.poly) block_partials<A, G, NCH>(st, pv, 2, C);
 }
 
 
@@ -1513,24 +1525,27 @@ __global__ void __launch_bounds__(KB) k_cg_poly_z(int64_t n, const T* __restrict
   block_partials<A, G, NCH>(st, pv, 2, C);
 }
 
-// Gershgorin bound of D^-1 A per system: max over rows of sum_j |a_rj| / |a_rr| (a float
-// atomic max on the bit pattern, monotonic for positive floats); out[s] starts at 0
+// Gershgorin bound of D^-1 A per system: max over rows of sum_j |a_rj| / |a_rr|; workgroup
+// (x, s) takes 256 rows of system s, and its maximum goes out by one atomic max on the bit
+// pattern (monotonic for non-negative floats); out[s] starts at 0
 template <typename T>
-__global__ void k_gersh(Csr m, const T* __restrict__ vals, int nsys, unsigned* __restrict__ out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_gersh(Csr m, const T* __restrict__ vals, unsigned* __restrict__ out) {
+  const int sy = blockIdx.y;
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   float v = 0.0f;
-  int s = 0;
-  if (i < m.n * nsys) {
-    s = (int)(i / m.n);
-    const int64_t row = i - (int64_t)s * m.n;
-    const T* a = vals + (int64_t)s * m.nnz;
+  if (row < m.n) {
+    const T* a = vals + (int64_t)sy * m.nnz;
     double sum = 0.0;
     for (int32_t j = m.row_ptr[row]; j < m.row_ptr[row + 1]; ++j) sum += sqrt((double)sc<T>::norm2(a[j]));
     const int32_t d = m.diag[row];
     const double ad = d >= 0 ? sqrt((double)sc<T>::norm2(a[d])) : 0.0;
     v = ad > 0.0 ? (float)(sum / ad) * 1.0001f : 0.0f;  // rounded up past the float conversion
   }
-  if (i < m.n * nsys) atomicMax(out + s, __float_as_uint(v));
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+  __shared__ float wmax[4];
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicMax(out + sy, __float_as_uint(fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]))));
 }
 
 // y = D^-1 r over every column (the start of a polynomial-preconditioned solve)
@@ -2589,20 +2604,20 @@ static void launch_resid_sweep(int nch, dim3 grid, hipStream_t s, const Csr& m, 
 // the LDS-staged SpMM (its dynamic LDS opt-in is per kernel, set before each launch)
 template <typename T, typename CF, bool RPAIR, typename A, bool POLY = false>
 static void launch_spmm_lds_cf(const LdsArgs& pl, dim3 grid, hipStream_t s, const T* recv, const T* P, T* Q,
-                               int64_t rs, int64_t coff, int cbase, ColState<A> st, const T* R = nullptr,
-                               const T* dinvp = nullptr, const float* lbound = nullptr, int C = 64) {
+                               int64_t rs, int64_t coff, int cbase, ColState<A> st, const T* dinvp = nullptr,
+                               const float* lbound = nullptr, int C = 64) {
   (void)hipFuncSetAttribute((const void*)k_cg_spmm_lds<T, CF, RPAIR, POLY>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, CF::BYTES);
   plaunch(k_cg_spmm_lds<T, CF, RPAIR, POLY>, grid, dim3((LDS_NW + LDS_NP) * 64), CF::BYTES, s, pl.nblk, pl.brow,
-          pl.bu, pl.ulist, pl.rinfo, pl.loff, recv, P, Q, rs, coff, cbase, st, R, dinvp, lbound, C);
+          pl.bu, pl.ulist, pl.rinfo, pl.loff, recv, P, Q, rs, coff, cbase, st, dinvp, lbound, C);
 }
-// the polynomial preconditioner's product with z formed in the epilogue (one 64-column system):
-// Y = D^-1 r in, z out to Z, r.z and r.r partials into st
+// the polynomial preconditioner's product with z formed in the epilogue (one 64-column system,
+// pl.recv_ad packed): the residual R in, z out to Z, r.z and r.r partials into st
 template <typename T, typename A>
-static void launch_spmm_lds_poly(const LdsArgs& pl, dim3 grid, hipStream_t s, const T* Y, T* Z, const T* R,
-                                 const T* dinv, const unsigned* lbound, ColState<A> st) {
+static void launch_spmm_lds_poly(const LdsArgs& pl, dim3 grid, hipStream_t s, const T* R, T* Z, const T* dinv,
+                                 const unsigned* lbound, ColState<A> st) {
   if constexpr (sizeof(T) == 8) {
-    launch_spmm_lds_cf<T, LdsWide, false, A, true>(pl, grid, s, (const T*)pl.recv, Y, Z, 64, 0, 0, st, R, dinv,
+    launch_spmm_lds_cf<T, LdsWide, false, A, true>(pl, grid, s, (const T*)pl.recv_ad, R, Z, 64, 0, 0, st, dinv,
                                                    (const float*)lbound, 64);
   }
 }
@@ -2640,7 +2655,8 @@ static bool lds_pairs_eligible(const rb_spmm_plan* plan, int nsys, int k, size_t
 }
 
 static LdsArgs lds_args(const rb_spmm_plan* plan) {
-  return LdsArgs{plan->nblk, plan->umax, plan->brow, plan->bu, plan->ulist, plan->rinfo, plan->loff, plan->recv};
+  return LdsArgs{plan->nblk, plan->umax, plan->brow, plan->bu, plan->ulist, plan->rinfo, plan->loff, plan->recv,
+                 nullptr};
 }
 
 #define L_UPD(G_, N_, ...) plaunch((k_cg_update<T, G_, N_>), dim3(nb), dim3(KB), 0, s, __VA_ARGS__)
@@ -2662,8 +2678,11 @@ static int run_cg(const Csr& m, int nsys, const T* vals, const T* dinv, const T*
   if (lds) nbs = lds_grid(nb);
   ColState<A> ss = st;  // the SpMM's partials come from its own grid
   ss.nb = (int)nbs;
-  // polynomial preconditioner (not in the pair layout of real 64-column systems)
+  // polynomial preconditioner (not in the pair layout of real 64-column systems); with the LDS
+  // plan's A D^-1 records its product stages r itself and forms z in its epilogue
   const bool poly = YZ && lb && !pair_sweep<T>(nsys, C, k);
+  const bool lds_poly = poly && lds && lds->recv_ad;
+  st.poly = ss.poly = poly ? 1 : 0;
   auto spmm = [&](const T* Pin, T* Qout) {
     if (lds)
       launch_spmm_lds<T>(*lds, dim3(nbs), s, Pin, Qout, ss);
@@ -2677,13 +2696,14 @@ static int run_cg(const Csr& m, int nsys, const T* vals, const T* dinv, const T*
     if (poly) {  // z0 = M r0, rho = r0.z0, p0 = z0
       int rcf = rb_fill_i32((int32_t*)lb, 0, nsys, s);
       if (rcf) return rcf;
-      hipLaunchKernelGGL(k_gersh<T>, dim3((unsigned)ceil_div(m.n * nsys, 256)), dim3(256), 0, s, m, vals, nsys, lb);
-      hipLaunchKernelGGL(k_poly_y<T>, dim3(ew_grid(m.n * C)), dim3(256), 0, s, m.n, dinv, (const T*)w.R, YZ, C, k);
-      if (lds) {  // z formed in the LDS product's epilogue, into the spent q
-        launch_spmm_lds_poly<T>(*lds, dim3(nbs), s, (const T*)YZ, w.Q, (const T*)w.R, dinv, lb, ss);
+      hipLaunchKernelGGL(k_gersh<T>, dim3((unsigned)ceil_div(m.n, 256), nsys), dim3(256), 0, s, m, vals, lb);
+      if (lds_poly) {  // z formed in the LDS product's epilogue, into the spent q
+        launch_spmm_lds_poly<T>(*lds, dim3(nbs), s, (const T*)w.R, w.Q, dinv, lb, ss);
         hipLaunchKernelGGL(k_fin_rho<A>, dim3(C), dim3(FB), 0, s, C, ss);
         DISPATCH_GN(C, L_PUP, m.n, dinv, X, w.R, w.P, C, k, (TW*)nullptr, st, (const T*)w.Q);
       } else {
+        hipLaunchKernelGGL(k_poly_y<T>, dim3(ew_grid(m.n * C)), dim3(256), 0, s, m.n, dinv, (const T*)w.R, YZ, C,
+                           k);
         spmm(YZ, w.Q);
         DISPATCH_GN(C, L_PZ, m.n, dinv, (const T*)w.R, YZ, (const T*)w.Q, (const float*)lb, C, k, st);
         hipLaunchKernelGGL(k_fin_rho<A>, dim3(C), dim3(FB), 0, s, C, st);
@@ -2708,21 +2728,18 @@ static int run_cg(const Csr& m, int nsys, const T* vals, const T* dinv, const T*
       {
         ProfScope ps(RB_PROF_UPDATE, s);
         if (poly || !pair_sweep<T>(nsys, C, k) || !launch_update_pair<T>(C, nb, s, m.n, dinv, w.R, w.Q, st))
-          DISPATCH_GN(C, L_UPD, m.n, dinv, w.R, w.Q, C, k, st, poly ? YZ : (T*)nullptr);
+          DISPATCH_GN(C, L_UPD, m.n, dinv, w.R, w.Q, C, k, st, poly && !lds_poly ? YZ : (T*)nullptr);
       }
       const T* zin = nullptr;  // the polynomial preconditioner's z for the p-update
       bool zlds = false;       // its r.z, r.r partials on the LDS grid
-      if (poly && lds) {       // w = A y with z formed in the epilogue, into the spent q
-        ProfScope ps(RB_PROF_SPMM, s);
-        launch_spmm_lds_poly<T>(*lds, dim3(nbs), s, (const T*)YZ, w.Q, (const T*)w.R, dinv, lb, ss);
+      if (lds_poly) {          // w = A D^-1 r with z formed in the epilogue, into the spent q
+        ProfScope ps(RB_PROF_POLY, s);
+        launch_spmm_lds_poly<T>(*lds, dim3(nbs), s, (const T*)w.R, w.Q, dinv, lb, ss);
         zin = w.Q;
         zlds = true;
       } else if (poly) {  // w = A y into the spent q, then z and the r.z, r.r partials
-        {
-          ProfScope ps(RB_PROF_SPMM, s);
-          spmm(YZ, w.Q);
-        }
-        ProfScope ps(RB_PROF_UPDATE, s);
+        ProfScope ps(RB_PROF_POLY, s);
+        spmm(YZ, w.Q);
         DISPATCH_GN(C, L_PZ, m.n, dinv, (const T*)w.R, YZ, (const T*)w.Q, (const float*)lb, C, k, st);
         zin = YZ;
       }
@@ -2787,6 +2804,13 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
       RB_CHECK_LAUNCH();
     }
     const bool poly = g_precond.load() == RB_PRECOND_POLY;
+    if (use_lds && poly && plan->recv_sets >= 2) {  // the second record set: A D^-1
+      const int64_t nr = plan->nrec + RB_LDS_RECPAD;
+      la.recv_ad = (const T64*)plan->recv + nr;
+      hipLaunchKernelGGL(k_lds_pack_ad<T64>, dim3(ew_grid(nr)), dim3(256), 0, s, nr, plan->rmap, A64, m.col_idx,
+                         (const T64*)w.dinv, (T64*)la.recv_ad);
+      RB_CHECK_LAUNCH();
+    }
     rc = run_cg<T64>(m, nsys, A64, w.dinv, B, X, use_x0, w, C, k, st, check_every, s, &its, use_lds ? &la : nullptr,
                      poly ? (T64*)(base + o[5]) : nullptr, poly ? (unsigned*)(base + o[7]) : nullptr);
     if (rc) return rc;
@@ -2929,19 +2953,26 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
     else
       DISPATCH_GN(C, L_SPMM, m, A32, Pin, Qout, C, k, ss);
   };
+  // with the plan's second record set (A D^-1) the preconditioner's product stages r itself
+  const bool lds_poly = poly && lds_one && plan->recv_sets >= 2;
+  if (lds_poly) {
+    la.recv_ad = (const T32*)plan->recv + nrec_set;
+    hipLaunchKernelGGL(k_lds_pack_ad<T32>, dim3(ew_grid(nrec_set)), dim3(256), 0, s, nrec_set, plan->rmap, A32,
+                       m.col_idx, (const T32*)w.dinv, (T32*)la.recv_ad);
+  }
   if (poly) {  // z0 = M r0, rho = r0.z0, p0 = z0 (P is zero, alpha and beta 0)
     rc = rb_fill_i32((int32_t*)lb, 0, nsys, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_gersh<T32>, dim3((unsigned)ceil_div(n * nsys, 256)), dim3(256), 0, s, m, A32, nsys, lb);
-    hipLaunchKernelGGL(k_poly_y<T32>, dim3(ew_grid(n * C)), dim3(256), 0, s, n, (const T32*)w.dinv, (const T32*)w.R,
-                       YZ, C, k);
-    if (lds_one) {  // z formed in the LDS product's epilogue, into the spent q
+    hipLaunchKernelGGL(k_gersh<T32>, dim3((unsigned)ceil_div(n, 256), nsys), dim3(256), 0, s, m, A32, lb);
+    if (lds_poly) {  // z formed in the LDS product's epilogue, into the spent q
       ColState<A> sz = si;
       sz.nb = (int)nbs;
-      launch_spmm_lds_poly<T32>(la, dim3(nbs), s, (const T32*)YZ, w.Q, (const T32*)w.R, (const T32*)w.dinv, lb, sz);
+      launch_spmm_lds_poly<T32>(la, dim3(nbs), s, (const T32*)w.R, w.Q, (const T32*)w.dinv, lb, sz);
       hipLaunchKernelGGL(k_fin_rho<A>, dim3(C), dim3(FB), 0, s, C, sz);
       DISPATCH_GN(C, L_PUP, m.n, w.dinv, D, w.R, w.P, C, k, X, si, (const T32*)w.Q);
     } else {
+      hipLaunchKernelGGL(k_poly_y<T32>, dim3(ew_grid(n * C)), dim3(256), 0, s, n, (const T32*)w.dinv,
+                         (const T32*)w.R, YZ, C, k);
       spmm2(YZ, w.Q);
       DISPATCH_GN(C, L_PZ, m.n, w.dinv, (const T32*)w.R, YZ, (const T32*)w.Q, (const float*)lb, C, k, si);
       hipLaunchKernelGGL(k_fin_rho<A>, dim3(C), dim3(FB), 0, s, C, si);
@@ -2987,21 +3018,18 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
       {
         ProfScope ps(RB_PROF_UPDATE, s);
         if (poly || !pair || !launch_update_pair<T32>(C, nb, s, m.n, w.dinv, w.R, w.Q, si))
-          DISPATCH_GN(C, L_UPD, m.n, w.dinv, w.R, w.Q, C, k, si, poly ? YZ : (T32*)nullptr);
+          DISPATCH_GN(C, L_UPD, m.n, w.dinv, w.R, w.Q, C, k, si, poly && !lds_poly ? YZ : (T32*)nullptr);
       }
       const T32* zin = nullptr;  // the polynomial preconditioner's z for the p-update
       ColState<A> sz = si;       // ... and where its r.z, r.r partials are
-      if (poly && lds_one) {     // w = A y with z formed in the epilogue, into the spent q
+      if (lds_poly) {            // w = A D^-1 r with z formed in the epilogue, into the spent q
         sz.nb = (int)nbs;
-        ProfScope ps(RB_PROF_SPMM, s);
-        launch_spmm_lds_poly<T32>(la, dim3(nbs), s, (const T32*)YZ, w.Q, (const T32*)w.R, (const T32*)w.dinv, lb, sz);
+        ProfScope ps(RB_PROF_POLY, s);
+        launch_spmm_lds_poly<T32>(la, dim3(nbs), s, (const T32*)w.R, w.Q, (const T32*)w.dinv, lb, sz);
         zin = w.Q;
       } else if (poly) {  // w = A y into the spent q, then z and the r.z, r.r partials
-        {
-          ProfScope ps(RB_PROF_SPMM, s);
-          spmm2(YZ, w.Q);
-        }
-        ProfScope ps(RB_PROF_UPDATE, s);
+        ProfScope ps(RB_PROF_POLY, s);
+        spmm2(YZ, w.Q);
         DISPATCH_GN(C, L_PZ, m.n, w.dinv, (const T32*)w.R, YZ, (const T32*)w.Q, (const float*)lb, C, k, si);
         zin = YZ;
       }

@@ -88,6 +88,8 @@ def femsolve(Amat, rhs, method="auto", **kwargs):
 
         if os.environ.get("RB_LDS_SPMM", "1") != "0":
             plan = _gpu.spmm_plan(A.indptr, A.indices, dev)
+            if plan is not None:  # A and A D^-1 (the polynomial preconditioner's product)
+                plan.ensure_sets(_gpu.lds_plan_sets(1, 64, is_complex, precision))
     flag = 0
     info = None
     for c0 in range(0, ncol, step):

@@ -79,6 +79,39 @@ def test_poly_two_complex_systems_one_solve(monkeypatch):
         assert _rel(f[wv], f3[wv]) < 1e-10 and _relmax(d[wv], d3[wv]) < 1e-8
 
 
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_poly_lds_plan_without_the_scaled_records(precision, monkeypatch):
+    """A plan with one record set (no A D^-1 records) runs the preconditioner's product through
+    the LDS SpMM on y = D^-1 r and forms z in its own kernel; both forms converge to the same
+    solution in the same number of iterations (give or take one)."""
+    import torch
+    import redbirdpy_amd as rb
+    from redbirdpy_amd import _gpu, tune, workloads
+    from redbirdpy_amd.forward import forward_system
+
+    monkeypatch.setenv("RB_TUNE_CACHE", "/nonexistent/dir/tune.json")
+    cfg = workloads.fd_box(2.0) if precision == "mixed" else workloads.jac_box(2.0)
+    c1, _ = rb.meshprep(copy.deepcopy(cfg))
+    mesh, sysm, B, k, *_ = forward_system(c1, (1,), precision)
+    assert k == 64
+    m = sysm.mesh
+    two = m.spmm_plan(2)
+    assert two is not None and two.struct.recv_sets >= 2
+    one = _gpu._build_spmm_plan(m.row_ptr.cpu().numpy(), m.col_idx[: m.nnz].cpu().numpy(), m.dev)
+    assert one.struct.recv_sets == 1
+    res = {}
+    with tune.applied({"precond": 1}):
+        for name, plan in (("fused", two), ("split", one)):
+            res[name] = _gpu.krylov_solve(m.row_ptr, m.col_idx, m.diag, m.nn, m.nnz, sysm.vals64, sysm.vals32, B, k,
+                                          1, sysm.is_complex, precision, 1e-12, 4000, plan=plan)
+    torch.cuda.synchronize()
+    (xf, inf_f), (xs, inf_s) = res["fused"], res["split"]
+    assert inf_f.flag == 0 and inf_s.flag == 0
+    assert inf_f.relres < 1e-12 and inf_s.relres < 1e-12
+    assert abs(inf_f.iters - inf_s.iters) <= 1
+    assert (torch.linalg.norm(xf - xs) / torch.linalg.norm(xs)).item() < 1e-10
+
+
 def test_poly_row_group_spmm(monkeypatch):
     """Past the sweep's descriptor limit (lowered for the test) the row-per-group SpMM serves both
     products of an iteration."""
