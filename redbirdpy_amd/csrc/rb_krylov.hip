@@ -2493,11 +2493,12 @@ extern "C" int rb_prof_read(int id, double* total_ms, int64_t* count) {
 // q = A p launch over the solve's columns: the sweep kernel when every
 // 64-column chunk lies in one system and P fits its buffer descriptor
 // (< 2 GiB), on a grid of exactly one resident round of workgroups; the
-// row-per-group kernel otherwise.  `flat` and `nbs` come from spmm_plan.
+// row-per-group kernel otherwise.  `flat` and `nbs` come from spmm_plan; `spmm_pair`
+// (real 64-column systems in chunk pairs) is the caller's layout choice.
 #define L_SPMM(G_, N_, ...)                                                                                 \
   do {                                                                                                    \
     if (G_ == 64 && flat)                                                                                 \
-      launch_spmm_sweep<T, N_>(pair_sweep<T>(nsys, C, k), dim3(nbs), s, __VA_ARGS__);                       \
+      launch_spmm_sweep<T, N_>(spmm_pair, dim3(nbs), s, __VA_ARGS__);                                       \
     else if (nsys == 1)                                                                                   \
       plaunch((k_cg_spmm<T, G_, N_, false>), dim3(nbs), dim3(KB), 0, s, __VA_ARGS__);                      \
     else                                                                                                  \
@@ -2678,9 +2679,11 @@ static int run_cg(const Csr& m, int nsys, const T* vals, const T* dinv, const T*
   if (lds) nbs = lds_grid(nb);
   ColState<A> ss = st;  // the SpMM's partials come from its own grid
   ss.nb = (int)nbs;
-  // polynomial preconditioner (not in the pair layout of real 64-column systems); with the LDS
-  // plan's A D^-1 records its product stages r itself and forms z in its epilogue
-  const bool poly = YZ && lb && !pair_sweep<T>(nsys, C, k);
+  // polynomial preconditioner (real 64-column systems then run in the one-column layout rather
+  // than in chunk pairs); with the LDS plan's A D^-1 records its product stages r itself and
+  // forms z in its epilogue
+  const bool poly = YZ && lb;
+  const bool spmm_pair = pair_sweep<T>(nsys, C, k) && !poly;
   const bool lds_poly = poly && lds && lds->recv_ad;
   st.poly = ss.poly = poly ? 1 : 0;
   auto spmm = [&](const T* Pin, T* Qout) {
@@ -2727,7 +2730,7 @@ static int run_cg(const Csr& m, int nsys, const T* vals, const T* dinv, const T*
       hipLaunchKernelGGL(k_fin_alpha<A>, dim3(C), dim3(FB), 0, s, C, ss);
       {
         ProfScope ps(RB_PROF_UPDATE, s);
-        if (poly || !pair_sweep<T>(nsys, C, k) || !launch_update_pair<T>(C, nb, s, m.n, dinv, w.R, w.Q, st))
+        if (!spmm_pair || !launch_update_pair<T>(C, nb, s, m.n, dinv, w.R, w.Q, st))
           DISPATCH_GN(C, L_UPD, m.n, dinv, w.R, w.Q, C, k, st, poly && !lds_poly ? YZ : (T*)nullptr);
       }
       const T* zin = nullptr;  // the polynomial preconditioner's z for the p-update
@@ -2746,8 +2749,7 @@ static int run_cg(const Csr& m, int nsys, const T* vals, const T* dinv, const T*
       hipLaunchKernelGGL(k_fin_beta<A>, dim3(C), dim3(FB), 0, s, C, zlds ? ss : st);
       {
         ProfScope ps(RB_PROF_PUPDATE, s);
-        if (poly || !pair_sweep<T>(nsys, C, k) ||
-            !launch_pupdate_pair<T>(C, nb, s, m.n, dinv, X, w.R, w.P, (TW*)nullptr, st))
+        if (!spmm_pair || !launch_pupdate_pair<T>(C, nb, s, m.n, dinv, X, w.R, w.P, (TW*)nullptr, st))
           DISPATCH_GN(C, L_PUP, m.n, dinv, X, w.R, w.P, C, k, (TW*)nullptr, st, zin);
       }
     }
@@ -2859,9 +2861,9 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
   si.tol2ru = tol2;
   si.ru_req = ru_req;
   si.ru_cnt = ru_req + 2 * MAXCHUNK;
-  // the polynomial preconditioner (not in the pair layout of real 64-column systems)
-  const bool poly = g_precond.load() == RB_PRECOND_POLY &&
-                    !(std::is_same<T32, float>::value && pair_sweep<T32>(nsys, C, k));
+  // the polynomial preconditioner (real 64-column systems then run in the one-column layout
+  // rather than in chunk pairs)
+  const bool poly = g_precond.load() == RB_PRECOND_POLY;
   si.poly = poly ? 1 : 0;
   T32* YZ = poly ? (T32*)(base + o[5]) : nullptr;
   unsigned* lb = poly ? (unsigned*)(base + o[7]) : nullptr;
@@ -2909,7 +2911,8 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
     flat = flat && flat64;
   }
   // real systems of 64 columns each, whole chunk pairs: one sweep per pair of systems
-  const bool pair = std::is_same<T32, float>::value && pair_sweep<T32>(nsys, C, k);
+  const bool pair = std::is_same<T32, float>::value && pair_sweep<T32>(nsys, C, k) && !poly;
+  const bool spmm_pair = pair;
   // the LDS-staged SpMM (a plan for this 64-column system): its own launch, then the
   // float64 residual of the requested chunks in a gated sweep launch
   // (or, for real float32 systems of 64 columns in pairs, one LDS launch per pair)
@@ -3017,7 +3020,7 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
       hipLaunchKernelGGL(k_fin_alpha_ru<A>, dim3(C), dim3(FB), 0, s, C, tol2, ss, sol);
       {
         ProfScope ps(RB_PROF_UPDATE, s);
-        if (poly || !pair || !launch_update_pair<T32>(C, nb, s, m.n, w.dinv, w.R, w.Q, si))
+        if (!pair || !launch_update_pair<T32>(C, nb, s, m.n, w.dinv, w.R, w.Q, si))
           DISPATCH_GN(C, L_UPD, m.n, w.dinv, w.R, w.Q, C, k, si, poly && !lds_poly ? YZ : (T32*)nullptr);
       }
       const T32* zin = nullptr;  // the polynomial preconditioner's z for the p-update
@@ -3036,7 +3039,7 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
       hipLaunchKernelGGL(k_fin_beta<A>, dim3(C), dim3(FB), 0, s, C, sz);
       {
         ProfScope ps(RB_PROF_PUPDATE, s);
-        if (poly || !pair || !launch_pupdate_pair<T32>(C, nb, s, m.n, w.dinv, D, w.R, w.P, X, si))
+        if (!pair || !launch_pupdate_pair<T32>(C, nb, s, m.n, w.dinv, D, w.R, w.P, X, si))
           DISPATCH_GN(C, L_PUP, m.n, w.dinv, D, w.R, w.P, C, k, X, si, zin);
       }
     }

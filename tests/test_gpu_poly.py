@@ -1,6 +1,7 @@
 """The degree-1 polynomial preconditioner (rb_krylov_set_precond(RB_PRECOND_POLY)) in every solve
 path it serves: mixed FD through the LDS-staged SpMM, float64 FD through the column sweep, float64
-CW through the LDS SpMM, a mixed real single system, two complex systems in one solve, and the
+CW through the LDS SpMM, a mixed real single system, two complex systems in one solve, four real
+systems taken out of the chunk-pair layout, an LDS plan without the A D^-1 records, and the
 row-per-group SpMM of a solve past the sweep's descriptor limit.  Each against the oracle's SuperLU
 solves at the same tolerances as the Jacobi solves, and with fewer iterations than Jacobi."""
 
@@ -77,6 +78,24 @@ def test_poly_two_complex_systems_one_solve(monkeypatch):
     d, f, _ = out[1]
     for wv in d3:
         assert _rel(f[wv], f3[wv]) < 1e-10 and _relmax(d[wv], d3[wv]) < 1e-8
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_poly_four_real_systems_leave_the_pair_layout(precision, monkeypatch):
+    """Four real 64-column systems (the Gauss-Newton forward's shape) run in chunk pairs under
+    Jacobi; the polynomial preconditioner runs them in the one-column sweep instead."""
+    from redbirdpy_amd import workloads
+
+    cfg = workloads.jac_box(2.0)
+    cfg["prop"] = {str(690 + 40 * i): np.array([[0, 0, 1, 1], [0.008 + 0.003 * i, 1.0 - 0.05 * i, 0, 1.37],
+                                                [0.02, 1.2, 0, 1.37]]) for i in range(4)}
+    out = _both(cfg, precision, monkeypatch)
+    d3, f3 = _oracle(cfg)
+    for pc in (0, 1):
+        d, f, _ = out[pc]
+        for wv in d3:
+            assert _rel(f[wv], f3[wv]) < 1e-10 and _relmax(d[wv], d3[wv]) < 1e-8
+    assert out[1][2] < 0.8 * out[0][2]
 
 
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
