@@ -40,7 +40,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-SPMM_KERNEL = "k_cg_spmm_lds<cplx<float>,LdsCfg<128,768,2>>"
+SPMM_KERNEL = "k_cg_spmm_lds<cplx<float>,LdsCfg<128,768,2>,false,false>"
 
 
 def _args():
@@ -566,10 +566,14 @@ def main():
     traffic = traffic_plain = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
-        rec = json.load(open(tpath)).get(SPMM_KERNEL)
+        tj = json.load(open(tpath))
+        rec = tj.get(SPMM_KERNEL)
         if rec and a.h == 1.0:
             traffic = rec["hbm_bytes_per_launch"]
             traffic_plain = rec.get("hbm_bytes_median_launch")
+        prec_ = tj.get(SPMM_KERNEL.replace(",false,false>", ",false,true>"))
+        if prec_ and a.h == 1.0 and "bytes_per_launch" in per_kernel["poly"]:
+            per_kernel["poly"]["traffic"] = prec_["hbm_bytes_per_launch"]
     out = {
         "metric": "RHS-solves/s + GN iters/s @115k nodes: FP32+refine vs FP64, 1 GPU, <=1e-6",
         "value": k / dt_m,
@@ -602,7 +606,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated, mean over dispatches "
-                              "(profiles/spmm_sweep_fd115k_c64_pmc.md)",
+                              "(profiles/spmm_lds_fd115k_c64_pmc.md)",
             "traffic_median_launch": traffic_plain,
             "kernel": SPMM_KERNEL + " (q = A p with the p.q reduction, neighbour rows staged in LDS)",
             "bytes_per_launch": nbytes,

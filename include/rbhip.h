@@ -310,9 +310,12 @@ int64_t rb_krylov_get_sweep_grid(void);
  * solve's width).  RB_PRECOND_POLY applies z = p(D^-1 A) D^-1 r with
  * p(x) = (3 / L)(1 - 0.8 x / L), L the Gershgorin bound of D^-1 A per system:
  * one more q = A y product per iteration for ~1.8x fewer iterations on 3-D
- * FEM systems.  Real systems of 64 columns in the pair layout keep Jacobi.
- * No counterpart in the reference (its iterative solvers take ILU / none,
- * solver.py:791-888); the autotuner chooses per shape.
+ * FEM systems.  Real systems of 64 columns each then run in the one-column
+ * layout instead of chunk pairs.  With an LDS plan holding two record sets
+ * (rb_spmm_plan.recv_sets >= 2) the product is fused into the LDS-staged SpMM.
+ * No counterpart in the reference (its iterative solvers take a caller's M,
+ * pyamg's smoothed aggregation for 'cg+amg' / 'minres+amg', solver.py:409-414
+ * and :697-700, or none); the autotuner chooses per shape.
  */
 int rb_krylov_set_precond(int kind);
 int rb_krylov_get_precond(void);
