@@ -319,7 +319,15 @@ def batch_bench(a):
            "meshprep_batch_s": t_prep, "setup_s": t_setup, "tol": 1e-10}
     L = _abi.lib()
     dets = {}
+    if a.autotune:
+        out["autotune"] = {}
     for prec in ("fp64", "mixed"):
+        if a.autotune:  # the first 64-column group's launch choices (every full group shares them)
+            t0 = time.perf_counter()
+            tr = bf.tune(prec, 1e-10)
+            out["autotune"][prec] = {"lds": tr["lds"], "grid": tr["grid"], "check_every": tr["check_every"],
+                                     "precond": ("jacobi", "poly")[tr["precond"]],
+                                     "tuning_s": time.perf_counter() - t0}
         bf.solve(prec, 1e-10, fields=False)
         torch.cuda.synchronize()
         t0 = time.perf_counter()

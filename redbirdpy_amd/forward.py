@@ -341,6 +341,19 @@ class BatchForward:
         self.columns = sum(op.k for op in ops)  # each mesh's own source and detector columns
         self.info = None
 
+    def tune(self, precision="fp64", tol=1e-12, maxiter=None, reps=1):
+        """Tune the launch choices of this batch's solve and cache them (tune.tune_system): timed on
+        its first column group, whose shape every full-width group of the solve shares."""
+        from . import tune as _tune
+
+        mesh = self.mesh
+        sysm = _gpu.assemble(mesh, self.big, self.big["deldotdel"], [("", 1)], want32=(precision == "mixed"))
+        if maxiter is None:
+            maxiter = max(4000, 4 * int(np.sqrt(mesh.nn)) + 1000)
+        grp = _gpu.sweep_columns(mesh.nn, self.C, 1, sysm.is_complex)
+        return _tune.tune_system(sysm, self.B[:, :grp].contiguous(), grp, precision,
+                                 tol / np.sqrt(len(self.cfgs)), maxiter, reps=reps)
+
     def solve(self, precision="fp64", tol=1e-12, maxiter=None, fields=True, jacobian=False):
         mesh, big, TS, C = self.mesh, self.big, self.TS, self.C
         sysm = _gpu.assemble(mesh, big, big["deldotdel"], [("", 1)], want32=(precision == "mixed"))

@@ -45,3 +45,23 @@ def test_forward_batch_rejects_mixed_omega():
     cfgs = [rb.meshprep(copy.deepcopy(c))[0] for c in (a, b)]
     with pytest.raises(NotImplementedError):
         rb.forward_batch(cfgs)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_forward_batch_tune_caches_a_choice_and_detector_values_stay(precision, tmp_path, monkeypatch):
+    """BatchForward.tune times the batch's solve (its first column group) and caches the choice;
+    the later solves under it give the same detector values to the solve tolerance."""
+    import redbirdpy_amd as rb
+    from redbirdpy_amd import tune
+    from redbirdpy_amd.forward import BatchForward
+
+    monkeypatch.setenv("RB_TUNE_CACHE", str(tmp_path / "tune.json"))
+    cfgs = [rb.meshprep(copy.deepcopy(c))[0] for c in _batch(2 * np.pi * 1e8)]
+    bf = BatchForward(cfgs)
+    ref = bf.solve(precision, 1e-11, fields=False)
+    out = bf.tune(precision, 1e-11)
+    assert out["precond"] in tune.PRECONDS and out["check_every"] in tune.CHECK_EVERY
+    assert (tmp_path / "tune.json").exists()
+    res = bf.solve(precision, 1e-11, fields=False)
+    for a, b in zip(res, ref):
+        assert np.max(np.abs(a.detval - b.detval)) <= 1e-9 * np.max(np.abs(b.detval))
