@@ -1130,7 +1130,8 @@ struct LdsAcc<cplx<float>, true> {
 template <typename T, typename CF, bool RPAIR, bool POLY>
 __global__ void __launch_bounds__((LDS_NW + LDS_NP) * 64)
 k_cg_spmm_lds(int64_t nblk, const int32_t* __restrict__ brow, const int32_t* __restrict__ bu,
-              const int32_t* __restrict__ ulist, const int32_t* __restrict__ rinfo, const int32_t* __restrict__ loff,
+              const int32_t* __restrict__ ulist, const int32_t* __restrict__ rinfo,
+              const int32_t* __restrict__ rinfo_m, const int32_t* __restrict__ loff,
               const T* __restrict__ recv, const T* __restrict__ P, T* __restrict__ Q, int64_t rs, int64_t coff,
               int cbase, ColState<typename LdsAcc<T, RPAIR>::type> st, const T* __restrict__ dinvp,
               const float* __restrict__ lbound, int C) {
@@ -1163,49 +1164,66 @@ k_cg_spmm_lds(int64_t nblk, const int32_t* __restrict__ brow, const int32_t* __r
     pc1 = (R1)(-3.0 * POLY_K) / (L * L);
   }
   if (wave >= LDS_NW) {
-    // producer: block k of this workgroup into buffer k & 1, JB + JRP copies every time
+    // producer: block k of this workgroup into buffer k % NBUF, JB + JRP copies every time.  A
+    // block's metadata costs two dependent scalar-load round trips (its rows and union start,
+    // then the union's row indices and first record); they are made one block ahead, while the
+    // previous copies are in flight, so a block's copies start right after its barrier
     const int pw = wave - LDS_NW;
-    auto issue = [&](int kb) {
+    struct Meta {
+      int ni, r0, nr, e0;
+      int g[2 * JB];
+    };
+    auto meta = [&](int kb) {
+      Meta mt;
       const bool real = kb < nmine;
       const int64_t bb = real ? bfirst + (int64_t)kb * stride : 0;
       const int u0 = real ? bu[bb] : 0;
-      const int ni = real ? (bu[bb + 1] - u0 + 1) / 2 : 0;  // row copies (two rows each)
-      const int r0 = real ? brow[bb] : 0;
-      const int e0 = real ? rinfo[2 * r0] : 0;
+      mt.ni = real ? (bu[bb + 1] - u0 + 1) / 2 : 0;  // row copies (two rows each)
+      mt.r0 = real ? brow[bb] : 0;
+      mt.nr = real ? brow[bb + 1] - mt.r0 : 0;
+      // rinfo_m is rinfo again: rinfo is also a source of the copies, which the compiler takes as
+      // possibly written, so its own loads could not use the scalar cache and would make the wave
+      // wait on its copies in flight
+      mt.e0 = real ? rinfo_m[2 * mt.r0] : 0;
+      const int* ix = ulist + u0 + 2 * pw * JB;  // the padding keeps these runs readable
+      // the row indices are uniform: scalar loads, kept apart from the copies' counter
+#pragma unroll
+      for (int j = 0; j < 2 * JB; ++j) mt.g[j] = __builtin_amdgcn_readfirstlane(ix[j]);
+      return mt;
+    };
+    auto issue = [&](const Meta& mt, int kb) {
       char* base = lds + (kb % NBUF) * CF::BUFB;
       // the block's rows for the consumers (a plain LDS store, ordered by the next barrier; the
       // consumers' own loads of brow would wait on their outstanding q stores)
       if (pw == 0 && lane == 0) {
         int* h = (int*)(lds + CF::HDR) + 2 * (kb % NBUF);
-        h[0] = r0;
-        h[1] = real ? brow[bb + 1] - r0 : 0;
+        h[0] = mt.r0;
+        h[1] = mt.nr;
       }
 #pragma unroll
       for (int j = 0; j < JRP; ++j) {
         const int q = pw * JRP + j;  // 1 KB piece q: offsets, values, then the row table
         const char* src;
-        if (q < CF::LOFB / 1024) src = (const char*)(loff + e0) + q * 1024;
-        else if (q < (CF::LOFB + CF::VALB) / 1024) src = (const char*)(recv + e0) + (q - CF::LOFB / 1024) * 1024;
-        else src = (const char*)(rinfo + 2 * r0);
+        if (q < CF::LOFB / 1024) src = (const char*)(loff + mt.e0) + q * 1024;
+        else if (q < (CF::LOFB + CF::VALB) / 1024) src = (const char*)(recv + mt.e0) + (q - CF::LOFB / 1024) * 1024;
+        else src = (const char*)(rinfo + 2 * mt.r0);
         dma16(src + lane * 16, q < JR ? base + CF::ROWB + q * 1024 : lds + CF::TRASH);
       }
-      const int* ix = ulist + u0 + 2 * pw * JB;  // the padding keeps these runs readable
 #pragma unroll
       for (int j = 0; j < JB; ++j) {
         const int i = pw * JB + j;
-        // both row indices are uniform: scalar loads, kept apart from the copies' counter
-        const int ga = __builtin_amdgcn_readfirstlane(ix[2 * j]);
-        const int gb = __builtin_amdgcn_readfirstlane(ix[2 * j + 1]);
-        const int g = lane < 32 ? ga : gb;
+        const int g = lane < 32 ? mt.g[2 * j] : mt.g[2 * j + 1];
         const char* src = (const char*)(P + (int64_t)g * rs + coff) + (lane & 31) * 16;
-        dma16(src, i < ni ? base + 2 * i * 512 : base + CF::UMAX * 512);
+        dma16(src, i < mt.ni ? base + 2 * i * 512 : base + CF::UMAX * 512);
       }
     };
-    for (int kb = 0; kb < NBUF - 1; ++kb) issue(kb);
+    for (int kb = 0; kb < NBUF - 1; ++kb) issue(meta(kb), kb);
+    Meta nxt = meta(NBUF - 1);
     for (int kb = 0; kb < nmine; ++kb) {
       wait_vmcnt<(JB + JRP) * (NBUF - 2)>();  // block kb has landed (the next NBUF-2 may still fly)
       asm volatile("s_barrier" ::: "memory");
-      issue(kb + NBUF - 1);
+      issue(nxt, kb + NBUF - 1);
+      nxt = meta(kb + NBUF);
     }
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // spare copies land before the buffers are reused
   } else {
@@ -2610,7 +2628,7 @@ static void launch_spmm_lds_cf(const LdsArgs& pl, dim3 grid, hipStream_t s, cons
   (void)hipFuncSetAttribute((const void*)k_cg_spmm_lds<T, CF, RPAIR, POLY>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, CF::BYTES);
   plaunch(k_cg_spmm_lds<T, CF, RPAIR, POLY>, grid, dim3((LDS_NW + LDS_NP) * 64), CF::BYTES, s, pl.nblk, pl.brow,
-          pl.bu, pl.ulist, pl.rinfo, pl.loff, recv, P, Q, rs, coff, cbase, st, dinvp, lbound, C);
+          pl.bu, pl.ulist, pl.rinfo, pl.rinfo, pl.loff, recv, P, Q, rs, coff, cbase, st, dinvp, lbound, C);
 }
 // the polynomial preconditioner's product with z formed in the epilogue (one 64-column system,
 // pl.recv_ad packed): the residual R in, z out to Z, r.z and r.r partials into st
