@@ -595,14 +595,14 @@ SWEEP_MAX_BYTES = 1 << 31
 def lds_plan_sets(nsys, k, is_complex, precision):
     """Record value sets an LDS-staged SpMM plan needs for this solve shape (0: it does not apply):
     for 64-column solves of one system with 8-byte values (complex64 mixed, float64 real) two, A and
-    A D^-1 (the polynomial preconditioner's product); one per pair for mixed solves of an even number
-    of real 64-column systems."""
+    A D^-1 (the polynomial preconditioner's product); for mixed solves of an even number of real
+    64-column systems one per pair, then one per pair for the pairs' A D^-1."""
     if k != 64:
         return 0
     if nsys == 1 and is_complex == (precision == "mixed"):
         return 2
     if precision == "mixed" and not is_complex and nsys >= 2 and nsys % 2 == 0:
-        return nsys // 2
+        return nsys
     return 0
 
 

@@ -1122,20 +1122,20 @@ struct LdsAcc<cplx<float>, true> {
 // holding columns 2l and 2l+1 (lanes 0-31 the first system's, scaled by the record's .x,
 // lanes 32-63 the second's, by .y).  The launch's columns are [cbase, cbase + ncol) of st.
 //
-// POLY (one system): the product of the polynomial preconditioner with its epilogue fused: P
-// holds the residual r and recv the records of A D^-1, so the row's w = (A D^-1 r)_row and its
-// own staged r give z = D^-1 (c0 r + c1 w) (c0, c1 from the Gershgorin bound lbound[0]); z goes
-// to Q, and the partials are r.z and r.r in st's slots 0 and 1 of C columns, as k_cg_poly_z
-// writes them.
+// POLY: the product of the polynomial preconditioner with its epilogue fused: P holds the
+// residual r and recv the records of A D^-1, so the row's w = (A D^-1 r)_row and its own staged
+// r give z = D^-1 (c0 r + c1 w) (c0, c1 from the system's Gershgorin bound in lbound); z goes to
+// Q, and the partials are r.z and r.r in st's slots 0 and 1 of C columns, as k_cg_poly_z writes
+// them.  dinvp is the system's D^-1 (T), or with RPAIR the float D^-1 of every system, system s
+// at dinvp + s * dstride (the lane's system is cbase / 64 + its half of the pair).
 template <typename T, typename CF, bool RPAIR, bool POLY>
 __global__ void __launch_bounds__((LDS_NW + LDS_NP) * 64)
 k_cg_spmm_lds(int64_t nblk, const int32_t* __restrict__ brow, const int32_t* __restrict__ bu,
               const int32_t* __restrict__ ulist, const int32_t* __restrict__ rinfo,
               const int32_t* __restrict__ rinfo_m, const int32_t* __restrict__ loff,
               const T* __restrict__ recv, const T* __restrict__ P, T* __restrict__ Q, int64_t rs, int64_t coff,
-              int cbase, ColState<typename LdsAcc<T, RPAIR>::type> st, const T* __restrict__ dinvp,
-              const float* __restrict__ lbound, int C) {
-  static_assert(!(RPAIR && POLY), "the polynomial epilogue serves one system of 64 columns");
+              int cbase, ColState<typename LdsAcc<T, RPAIR>::type> st, const void* __restrict__ dinvp,
+              int64_t dstride, const float* __restrict__ lbound, int C) {
   static_assert(sizeof(T) == 8, "512-byte rows: 64 lanes of 8 bytes");
   static_assert(!RPAIR || std::is_same<T, cplx<float>>::value, "a real pair is staged as 8-byte (float, float)");
   typedef typename sc<T>::acc A;
@@ -1158,8 +1158,9 @@ k_cg_spmm_lds(int64_t nblk, const int32_t* __restrict__ brow, const int32_t* __r
   A pv2 = acc_zero(A());  // POLY: r.r
   typedef typename RealOf<T>::type R1;
   R1 pc0 = 0, pc1 = 0;
+  const int psys = RPAIR ? cbase / 64 + (second ? 1 : 0) : 0;  // POLY: this lane's system
   if constexpr (POLY) {
-    const R1 L = (R1)lbound[0];
+    const R1 L = (R1)lbound[psys];
     pc0 = (R1)3.0 / L;
     pc1 = (R1)(-3.0 * POLY_K) / (L * L);
   }
@@ -1266,8 +1267,19 @@ k_cg_spmm_lds(int64_t nblk, const int32_t* __restrict__ brow, const int32_t* __r
           }
         }
         const T pown = *(const T*)(cb + ri[2 * i + 1]);
-        if constexpr (POLY) {  // pown is this row's r, acc its (A D^-1 r)
-          const T z = dinvp[r0 + i] * (rscale(pown, pc0) + rscale(acc, pc1));
+        if constexpr (POLY && RPAIR) {  // pown: this row's r in columns 2l, 2l+1; acc: (A D^-1 r)
+          const float d = ((const float*)dinvp)[(int64_t)psys * dstride + r0 + i];
+          T z;
+          z.x = d * (pown.x * pc0 + acc.x * pc1);
+          z.y = d * (pown.y * pc0 + acc.y * pc1);
+          Q[(int64_t)(r0 + i) * rs + coff + lane] = z;
+          pv.x += (double)pown.x * (double)z.x;
+          pv.y += (double)pown.y * (double)z.y;
+          pv2.x += (double)pown.x * (double)pown.x;
+          pv2.y += (double)pown.y * (double)pown.y;
+          continue;
+        } else if constexpr (POLY) {  // pown is this row's r, acc its (A D^-1 r)
+          const T z = ((const T*)dinvp)[r0 + i] * (rscale(pown, pc0) + rscale(acc, pc1));
           Q[(int64_t)(r0 + i) * rs + coff + lane] = z;
           pv = acc_add(pv, amul(pown, z));
           pv2 = acc_add(pv2, acc_from_real(sc<T>::norm2(pown), A()));
@@ -1302,7 +1314,14 @@ k_cg_spmm_lds(int64_t nblk, const int32_t* __restrict__ brow, const int32_t* __r
   }
   if constexpr (POLY) {  // the r.r partials in slot 1
     __syncthreads();
-    if (wave < LDS_NW) red[wave * NC + lane] = pv2;
+    if (wave < LDS_NW) {
+      if constexpr (RPAIR) {
+        red[wave * NC + 2 * lane] = pv2.x;
+        red[wave * NC + 2 * lane + 1] = pv2.y;
+      } else {
+        red[wave * NC + lane] = pv2;
+      }
+    }
     __syncthreads();
     if (threadIdx.x < NC) {
       AS t = red[threadIdx.x];
@@ -1328,6 +1347,24 @@ __global__ void k_lds_pack(int64_t nrec, const int32_t* __restrict__ rmap, const
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nrec; i += (int64_t)gridDim.x * blockDim.x) {
     const int32_t e = rmap[i];
     recv[i] = e >= 0 ? vals[e] : sc<T>::zero();
+  }
+}
+
+// records of A D^-1 for a pair of real float32 systems: (va[e] / a_cc, vb[e] / b_cc) for slot
+// e = rmap[i] in column c, da and db the two systems' D^-1 (0 for padding)
+__global__ void k_lds_pack_pair_ad(int64_t nrec, const int32_t* __restrict__ rmap, const float* __restrict__ va,
+                                   const float* __restrict__ vb, const int32_t* __restrict__ col_idx,
+                                   const float* __restrict__ da, const float* __restrict__ db,
+                                   cplx<float>* __restrict__ recv) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nrec; i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t e = rmap[i];
+    cplx<float> v{0.0f, 0.0f};
+    if (e >= 0) {
+      const int32_t c = col_idx[e];
+      v.x = va[e] * da[c];
+      v.y = vb[e] * db[c];
+    }
+    recv[i] = v;
   }
 }
 
@@ -1710,6 +1747,7 @@ __global__ void __launch_bounds__(KB) k_cg_update_pair(int64_t n, const TS* __re
         if (al[cp][1] != 0.0) r.y = r.y - scale_to(q.y, al[cp][1]);
         *reinterpret_cast<fl2*>(R + o) = r;
       }
+      if (st.poly) continue;  // the preconditioner's product forms z and the partials
       const TS d = dinv[(int64_t)sys * n + row];
       if (act[cp][0]) {
         pv[cp][0][0] += amul(r.x, (TS)(d * r.x));
@@ -1721,14 +1759,14 @@ __global__ void __launch_bounds__(KB) k_cg_update_pair(int64_t n, const TS* __re
       }
     }
   }
-  block_partials_pair2<NCH>(st, pv, C);
+  if (!st.poly) block_partials_pair2<NCH>(st, pv, C);
 }
 
 template <typename TS, int NCH>
 __global__ void __launch_bounds__(KB) k_cg_pupdate_pair(int64_t n, const TS* __restrict__ dinv,
                                                         TS* __restrict__ X, const TS* __restrict__ R,
                                                         TS* __restrict__ P, int C, double* __restrict__ X64,
-                                                        ColState<double> st) {
+                                                        ColState<double> st, const TS* __restrict__ Z) {
   typedef typename pair_of<TS>::type fl2;
   constexpr int G = 64;
   constexpr int NP = NCH / 2;
@@ -1771,11 +1809,18 @@ __global__ void __launch_bounds__(KB) k_cg_pupdate_pair(int64_t n, const TS* __r
         *reinterpret_cast<fl2*>(X + o) = x;
       }
       if (act[cp][0] || act[cp][1]) {
-        const fl2 r = *reinterpret_cast<const fl2*>(R + o);
-        const TS d = dinv[(int64_t)sys * n + row];
+        fl2 z;  // the preconditioned residual: given (polynomial), or D^-1 r
+        if (Z) {
+          z = *reinterpret_cast<const fl2*>(Z + o);
+        } else {
+          const fl2 r = *reinterpret_cast<const fl2*>(R + o);
+          const TS d = dinv[(int64_t)sys * n + row];
+          z.x = d * r.x;
+          z.y = d * r.y;
+        }
         fl2 pn = p;
-        if (act[cp][0]) pn.x = d * r.x + scale_to(p.x, be[cp][0]);
-        if (act[cp][1]) pn.y = d * r.y + scale_to(p.y, be[cp][1]);
+        if (act[cp][0]) pn.x = z.x + scale_to(p.x, be[cp][0]);
+        if (act[cp][1]) pn.y = z.y + scale_to(p.y, be[cp][1]);
         *reinterpret_cast<fl2*>(P + o) = pn;
       }
     }
@@ -2444,13 +2489,13 @@ static bool launch_update_pair(int C, int64_t nb, hipStream_t s, int64_t n, cons
 
 template <typename T32, typename A, typename TW>
 static bool launch_pupdate_pair(int C, int64_t nb, hipStream_t s, int64_t n, const T32* dinv, T32* X, const T32* R,
-                                T32* P, TW* X64, ColState<A> st) {
+                                T32* P, TW* X64, ColState<A> st, const T32* Z = nullptr) {
   if constexpr (std::is_same<T32, float>::value || std::is_same<T32, double>::value) {
     const int nch = C / 64;
     double* x64 = (double*)X64;  // the float64 solution (mixed) or null
-    if (nch == 2) plaunch(k_cg_pupdate_pair<T32, 2>, dim3(nb), dim3(KB), 0, s, n, dinv, X, R, P, C, x64, st);
-    else if (nch == 4) plaunch(k_cg_pupdate_pair<T32, 4>, dim3(nb), dim3(KB), 0, s, n, dinv, X, R, P, C, x64, st);
-    else plaunch(k_cg_pupdate_pair<T32, 8>, dim3(nb), dim3(KB), 0, s, n, dinv, X, R, P, C, x64, st);
+    if (nch == 2) plaunch(k_cg_pupdate_pair<T32, 2>, dim3(nb), dim3(KB), 0, s, n, dinv, X, R, P, C, x64, st, Z);
+    else if (nch == 4) plaunch(k_cg_pupdate_pair<T32, 4>, dim3(nb), dim3(KB), 0, s, n, dinv, X, R, P, C, x64, st, Z);
+    else plaunch(k_cg_pupdate_pair<T32, 8>, dim3(nb), dim3(KB), 0, s, n, dinv, X, R, P, C, x64, st, Z);
     return true;
   }
   return false;
@@ -2623,12 +2668,12 @@ static void launch_resid_sweep(int nch, dim3 grid, hipStream_t s, const Csr& m, 
 // the LDS-staged SpMM (its dynamic LDS opt-in is per kernel, set before each launch)
 template <typename T, typename CF, bool RPAIR, typename A, bool POLY = false>
 static void launch_spmm_lds_cf(const LdsArgs& pl, dim3 grid, hipStream_t s, const T* recv, const T* P, T* Q,
-                               int64_t rs, int64_t coff, int cbase, ColState<A> st, const T* dinvp = nullptr,
-                               const float* lbound = nullptr, int C = 64) {
+                               int64_t rs, int64_t coff, int cbase, ColState<A> st, const void* dinvp = nullptr,
+                               int64_t dstride = 0, const float* lbound = nullptr, int C = 64) {
   (void)hipFuncSetAttribute((const void*)k_cg_spmm_lds<T, CF, RPAIR, POLY>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, CF::BYTES);
   plaunch(k_cg_spmm_lds<T, CF, RPAIR, POLY>, grid, dim3((LDS_NW + LDS_NP) * 64), CF::BYTES, s, pl.nblk, pl.brow,
-          pl.bu, pl.ulist, pl.rinfo, pl.rinfo, pl.loff, recv, P, Q, rs, coff, cbase, st, dinvp, lbound, C);
+          pl.bu, pl.ulist, pl.rinfo, pl.rinfo, pl.loff, recv, P, Q, rs, coff, cbase, st, dinvp, dstride, lbound, C);
 }
 // the polynomial preconditioner's product with z formed in the epilogue (one 64-column system,
 // pl.recv_ad packed): the residual R in, z out to Z, r.z and r.r partials into st
@@ -2636,8 +2681,8 @@ template <typename T, typename A>
 static void launch_spmm_lds_poly(const LdsArgs& pl, dim3 grid, hipStream_t s, const T* R, T* Z, const T* dinv,
                                  const unsigned* lbound, ColState<A> st) {
   if constexpr (sizeof(T) == 8) {
-    launch_spmm_lds_cf<T, LdsWide, false, A, true>(pl, grid, s, (const T*)pl.recv_ad, R, Z, 64, 0, 0, st, dinv,
-                                                   (const float*)lbound, 64);
+    launch_spmm_lds_cf<T, LdsWide, false, A, true>(pl, grid, s, (const T*)pl.recv_ad, R, Z, 64, 0, 0, st,
+                                                   (const void*)dinv, 0, (const float*)lbound, 64);
   }
 }
 // one 64-column system with 8-byte values (rows of 64 columns)
@@ -2656,6 +2701,20 @@ static void launch_spmm_lds_pairs(const LdsArgs& pl, int64_t nrec_set, dim3 grid
     launch_spmm_lds_cf<cplx<float>, LdsWide, true>(pl, grid, s, (const cplx<float>*)pl.recv + p * nrec_set,
                                                    (const cplx<float>*)P, (cplx<float>*)Q, C / 2, 64 * p, 128 * p,
                                                    st);
+}
+
+// ... and the polynomial preconditioner's product of such pairs (pl.recv_ad: the pairs' A D^-1
+// records, set p for pair p): the residual R in, z out to Z, r.z and r.r partials into st; dinv
+// holds every system's float D^-1, system s at dinv + s * n
+template <typename A>
+static void launch_spmm_lds_pairs_poly(const LdsArgs& pl, int64_t nrec_set, dim3 grid, hipStream_t s, int nsys,
+                                       int64_t n, const float* R, float* Z, const float* dinv,
+                                       const unsigned* lbound, ColState<A> st) {
+  const int C = 64 * nsys;
+  for (int p = 0; p < nsys / 2; ++p)
+    launch_spmm_lds_cf<cplx<float>, LdsWide, true, A, true>(
+        pl, grid, s, (const cplx<float>*)pl.recv_ad + p * nrec_set, (const cplx<float>*)R, (cplx<float>*)Z, C / 2,
+        64 * p, 128 * p, st, (const void*)dinv, n, (const float*)lbound, C);
 }
 
 // a plan applies to 64-column solves of one system with 8-byte values
@@ -2928,8 +2987,13 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
     spmm_plan<T64>(m, nsys, C, k, nb, &flat64, &unused);
     flat = flat && flat64;
   }
-  // real systems of 64 columns each, whole chunk pairs: one sweep per pair of systems
-  const bool pair = std::is_same<T32, float>::value && pair_sweep<T32>(nsys, C, k) && !poly;
+  // real systems of 64 columns each, whole chunk pairs: one sweep per pair of systems (under the
+  // polynomial preconditioner only where its pair-form LDS product applies: a plan holding the
+  // pairs' A D^-1 records as well, sets nsys/2 .. nsys-1)
+  const bool pair_shape = std::is_same<T32, float>::value && pair_sweep<T32>(nsys, C, k);
+  const bool pair_poly = poly && pair_shape && flat && lds_pairs_eligible(plan, nsys, k, sizeof(T32)) &&
+                         plan->recv_sets >= nsys;
+  const bool pair = pair_shape && (!poly || pair_poly);
   const bool spmm_pair = pair;
   // the LDS-staged SpMM (a plan for this 64-column system): its own launch, then the
   // float64 residual of the requested chunks in a gated sweep launch
@@ -2981,6 +3045,23 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
     hipLaunchKernelGGL(k_lds_pack_ad<T32>, dim3(ew_grid(nrec_set)), dim3(256), 0, s, nrec_set, plan->rmap, A32,
                        m.col_idx, (const T32*)w.dinv, (T32*)la.recv_ad);
   }
+  if constexpr (std::is_same<T32, float>::value) {
+    if (pair_poly) {  // the pairs' A D^-1 records after their A records
+      la.recv_ad = (const cplx<float>*)plan->recv + (int64_t)(nsys / 2) * nrec_set;
+      for (int pp = 0; pp < nsys / 2; ++pp)
+        hipLaunchKernelGGL(k_lds_pack_pair_ad, dim3(ew_grid(nrec_set)), dim3(256), 0, s, nrec_set, plan->rmap,
+                           A32 + (int64_t)(2 * pp) * m.nnz, A32 + (int64_t)(2 * pp + 1) * m.nnz, m.col_idx,
+                           (const float*)w.dinv + (int64_t)(2 * pp) * n,
+                           (const float*)w.dinv + (int64_t)(2 * pp + 1) * n,
+                           (cplx<float>*)la.recv_ad + pp * nrec_set);
+    }
+  }
+  // the pair-form product of the polynomial preconditioner (float32 systems only)
+  auto pair_poly_product = [&](ColState<A> sz) {
+    if constexpr (std::is_same<T32, float>::value)
+      launch_spmm_lds_pairs_poly(la, nrec_set, dim3(nbs), s, nsys, n, (const float*)w.R, (float*)w.Q,
+                                 (const float*)w.dinv, lb, sz);
+  };
   if (poly) {  // z0 = M r0, rho = r0.z0, p0 = z0 (P is zero, alpha and beta 0)
     rc = rb_fill_i32((int32_t*)lb, 0, nsys, s);
     if (rc) return rc;
@@ -2989,6 +3070,12 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
       ColState<A> sz = si;
       sz.nb = (int)nbs;
       launch_spmm_lds_poly<T32>(la, dim3(nbs), s, (const T32*)w.R, w.Q, (const T32*)w.dinv, lb, sz);
+      hipLaunchKernelGGL(k_fin_rho<A>, dim3(C), dim3(FB), 0, s, C, sz);
+      DISPATCH_GN(C, L_PUP, m.n, w.dinv, D, w.R, w.P, C, k, X, si, (const T32*)w.Q);
+    } else if (pair_poly) {  // the same with the pairs' product
+      ColState<A> sz = si;
+      sz.nb = (int)nbs;
+      pair_poly_product(sz);
       hipLaunchKernelGGL(k_fin_rho<A>, dim3(C), dim3(FB), 0, s, C, sz);
       DISPATCH_GN(C, L_PUP, m.n, w.dinv, D, w.R, w.P, C, k, X, si, (const T32*)w.Q);
     } else {
@@ -3039,7 +3126,8 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
       {
         ProfScope ps(RB_PROF_UPDATE, s);
         if (!pair || !launch_update_pair<T32>(C, nb, s, m.n, w.dinv, w.R, w.Q, si))
-          DISPATCH_GN(C, L_UPD, m.n, w.dinv, w.R, w.Q, C, k, si, poly && !lds_poly ? YZ : (T32*)nullptr);
+          DISPATCH_GN(C, L_UPD, m.n, w.dinv, w.R, w.Q, C, k, si,
+                      poly && !lds_poly && !pair_poly ? YZ : (T32*)nullptr);
       }
       const T32* zin = nullptr;  // the polynomial preconditioner's z for the p-update
       ColState<A> sz = si;       // ... and where its r.z, r.r partials are
@@ -3047,6 +3135,11 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
         sz.nb = (int)nbs;
         ProfScope ps(RB_PROF_POLY, s);
         launch_spmm_lds_poly<T32>(la, dim3(nbs), s, (const T32*)w.R, w.Q, (const T32*)w.dinv, lb, sz);
+        zin = w.Q;
+      } else if (pair_poly) {  // the same, one launch per system pair
+        sz.nb = (int)nbs;
+        ProfScope ps(RB_PROF_POLY, s);
+        pair_poly_product(sz);
         zin = w.Q;
       } else if (poly) {  // w = A y into the spent q, then z and the r.z, r.r partials
         ProfScope ps(RB_PROF_POLY, s);
@@ -3057,7 +3150,7 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
       hipLaunchKernelGGL(k_fin_beta<A>, dim3(C), dim3(FB), 0, s, C, sz);
       {
         ProfScope ps(RB_PROF_PUPDATE, s);
-        if (!pair || !launch_pupdate_pair<T32>(C, nb, s, m.n, w.dinv, D, w.R, w.P, X, si))
+        if (!pair || !launch_pupdate_pair<T32>(C, nb, s, m.n, w.dinv, D, w.R, w.P, X, si, zin))
           DISPATCH_GN(C, L_PUP, m.n, w.dinv, D, w.R, w.P, C, k, X, si, zin);
       }
     }

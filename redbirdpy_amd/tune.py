@@ -14,9 +14,8 @@ how the GPU runs it:
   about to converge);
 * ``precond``: Jacobi, or the degree-1 polynomial preconditioner
   (rb_krylov_set_precond): one more SpMM per iteration for ~1.8x fewer
-  iterations on 3-D FEM systems (real 64-column systems then leave the
-  chunk-pair layout, and their LDS-staged pair SpMM, for the one-column
-  sweep).
+  iterations on 3-D FEM systems (real 64-column systems keep the chunk-pair
+  layout with the LDS-staged pair SpMM, else run in the one-column sweep).
 
 ``tune_system`` times each candidate on the system itself with full solves to
 the requested tolerance (columns converge at different iterations and the
@@ -175,12 +174,15 @@ def tune_system(sysm, B, k, precision="mixed", tol=1e-10, maxiter=2000, reps=2, 
         spmms.append({"lds": True, "grid": 0})
     if flat or not plan_ok:
         spmms += [{"lds": False, "grid": g} for g in (GRID_CAPS if flat else (0,))]
-    # real 64-column systems in pairs: the polynomial preconditioner runs them in the one-column
-    # sweep (the LDS-staged pair SpMM has no polynomial form)
+    # real 64-column systems in pairs: the polynomial preconditioner keeps the pair layout with the
+    # LDS-staged pair SpMM (mixed, a plan with the pairs' A D^-1 records), else it runs them in the
+    # one-column sweep
     pairs = not sysm.is_complex and k == 64 and sysm.nsys % 2 == 0
     cands = [dict(c, precond=_abi.RB_PRECOND_JACOBI) for c in spmms]
     if pairs:
         cands += [{"lds": False, "grid": g, "precond": _abi.RB_PRECOND_POLY} for g in (GRID_CAPS if flat else (0,))]
+        if plan_ok:
+            cands.append({"lds": True, "grid": 0, "precond": _abi.RB_PRECOND_POLY})
     else:
         cands += [dict(c, precond=_abi.RB_PRECOND_POLY) for c in spmms]
     def solve_time(ch):

@@ -109,8 +109,9 @@ def test_cw_mixed_wavelength_pairs_against_oracle_and_gather(nw, monkeypatch):
     cfg["prop"] = {str(650 + 30 * i): np.array([[0, 0, 1, 1], [0.006 + 0.004 * i, 0.8 + 0.1 * i, 0, 1.37]])
                    for i in range(nw)}
     c0, _ = rb.meshprep(copy.deepcopy(cfg))
-    assert _gpu.lds_plan_sets(nw, 64, False, "mixed") == nw // 2
-    assert _gpu.mesh_for(c0).spmm_plan(nw // 2).struct.recv_sets >= nw // 2
+    # one record set per pair, then one per pair for the pairs' A D^-1
+    assert _gpu.lds_plan_sets(nw, 64, False, "mixed") == nw
+    assert _gpu.mesh_for(c0).spmm_plan(nw).struct.recv_sets >= nw
     (d1, f1, it1), (d2, f2, it2) = _forward_both(cfg, "mixed", monkeypatch)
     c3, _ = orc.meshprep(copy.deepcopy(cfg))
     d3, f3 = orc.runforward(c3)

@@ -1,7 +1,7 @@
 """The degree-1 polynomial preconditioner (rb_krylov_set_precond(RB_PRECOND_POLY)) in every solve
 path it serves: mixed FD through the LDS-staged SpMM, float64 FD through the column sweep, float64
-CW through the LDS SpMM, a mixed real single system, two complex systems in one solve, four real
-systems taken out of the chunk-pair layout, an LDS plan without the A D^-1 records, and the
+CW through the LDS SpMM, a mixed real single system, two complex systems in one solve, two and four
+real systems in chunk pairs (LDS pair product, mixed), an LDS plan without the A D^-1 records, and the
 row-per-group SpMM of a solve past the sweep's descriptor limit.  Each against the oracle's SuperLU
 solves at the same tolerances as the Jacobi solves, and with fewer iterations than Jacobi."""
 
@@ -80,15 +80,18 @@ def test_poly_two_complex_systems_one_solve(monkeypatch):
         assert _rel(f[wv], f3[wv]) < 1e-10 and _relmax(d[wv], d3[wv]) < 1e-8
 
 
+@pytest.mark.parametrize("nw", [2, 4])
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
-def test_poly_four_real_systems_leave_the_pair_layout(precision, monkeypatch):
-    """Four real 64-column systems (the Gauss-Newton forward's shape) run in chunk pairs under
-    Jacobi; the polynomial preconditioner runs them in the one-column sweep instead."""
+def test_poly_real_64_column_systems_in_pairs(precision, nw, monkeypatch):
+    """Real 64-column systems (the Gauss-Newton forward's shape) run in chunk pairs under Jacobi.
+    Under the polynomial preconditioner the mixed solve keeps the pairs with the LDS-staged pair
+    product (the pairs' A D^-1 records, z for both systems of a pair in its epilogue), checked
+    also against the one-column sweep (LDS SpMM off); float64 runs them in the one-column sweep."""
     from redbirdpy_amd import workloads
 
     cfg = workloads.jac_box(2.0)
     cfg["prop"] = {str(690 + 40 * i): np.array([[0, 0, 1, 1], [0.008 + 0.003 * i, 1.0 - 0.05 * i, 0, 1.37],
-                                                [0.02, 1.2, 0, 1.37]]) for i in range(4)}
+                                                [0.02, 1.2, 0, 1.37]]) for i in range(nw)}
     out = _both(cfg, precision, monkeypatch)
     d3, f3 = _oracle(cfg)
     for pc in (0, 1):
@@ -96,6 +99,13 @@ def test_poly_four_real_systems_leave_the_pair_layout(precision, monkeypatch):
         for wv in d3:
             assert _rel(f[wv], f3[wv]) < 1e-10 and _relmax(d[wv], d3[wv]) < 1e-8
     assert out[1][2] < 0.8 * out[0][2]
+    if precision == "mixed":
+        monkeypatch.setenv("RB_LDS_SPMM", "0")
+        sweep = _both(cfg, precision, monkeypatch)
+        d, f, it = sweep[1]
+        for wv in d3:
+            assert _rel(f[wv], out[1][1][wv]) < 1e-10
+        assert abs(it - out[1][2]) <= 3
 
 
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
