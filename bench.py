@@ -40,7 +40,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-SPMM_KERNEL = "k_cg_spmm_lds<cplx<float>,LdsCfg<128,768,2>,false,false>"
+SPMM_KERNEL = "k_cg_spmm_lds<cplx<float>,LdsCfg<128,768,2,8>,false,false>"
 
 
 def _args():

@@ -279,9 +279,10 @@ int rb_spmm_plan_fill(int64_t n, const int32_t* row_ptr /* host */, const int32_
 /*
  * rb_krylov_solve with an LDS-staged SpMM plan (rb_spmm_plan, device
  * pointers) for the 64-column solves of one system: mixed precision with
- * complex64 inner solves, or plain float64 real systems; and for mixed
- * solves of an even number of real 64-column systems (one launch per system
- * pair, plan->recv_sets >= nsys / 2).  Other shapes (and plan == NULL) run
+ * complex64 or float32 inner solves, or plain float64 real systems; and for
+ * mixed solves of an even number of real 64-column systems (one launch per
+ * system pair, plan->recv_sets >= nsys / 2; >= nsys for the polynomial
+ * preconditioner's pair product).  Other shapes (and plan == NULL) run
  * exactly as rb_krylov_solve.  Same arguments, workspace, results and flags.
  */
 int rb_krylov_solve_planned(int64_t n, int64_t nnz, const int32_t* row_ptr, const int32_t* col_idx,

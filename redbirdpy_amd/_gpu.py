@@ -594,12 +594,12 @@ SWEEP_MAX_BYTES = 1 << 31
 
 def lds_plan_sets(nsys, k, is_complex, precision):
     """Record value sets an LDS-staged SpMM plan needs for this solve shape (0: it does not apply):
-    for 64-column solves of one system with 8-byte values (complex64 mixed, float64 real) two, A and
-    A D^-1 (the polynomial preconditioner's product); for mixed solves of an even number of real
-    64-column systems one per pair, then one per pair for the pairs' A D^-1."""
+    for 64-column solves of one system with 8-byte or 4-byte inner values (complex64 or float32 mixed,
+    float64 real) two, A and A D^-1 (the polynomial preconditioner's product); for mixed solves of an
+    even number of real 64-column systems one per pair, then one per pair for the pairs' A D^-1."""
     if k != 64:
         return 0
-    if nsys == 1 and is_complex == (precision == "mixed"):
+    if nsys == 1 and (precision == "mixed" or not is_complex):
         return 2
     if precision == "mixed" and not is_complex and nsys >= 2 and nsys % 2 == 0:
         return nsys
@@ -649,8 +649,8 @@ def solve_system(sysm, B, k, precision="fp64", tol=1e-12, maxiter=2000, X0=None,
         tuned = tune.lookup(sysm, k, precision)
         if tuned is None and os.environ.get("RB_AUTOTUNE") == "1":
             tuned = tune.tune_system(sysm, B, k, precision, tol if tol > 0 else 1e-10, maxiter)
-    # the LDS-staged SpMM serves 64-column solves of one system with 8-byte inner values
-    # (complex64 mixed, float64 plain) and mixed solves of real 64-column systems in pairs
+    # the LDS-staged SpMM serves 64-column solves of one system with 8-byte or 4-byte inner values
+    # (complex64 or float32 mixed, float64 plain) and mixed solves of real 64-column systems in pairs
     lds_sets = lds_plan_sets(sysm.nsys, k, sysm.is_complex, precision)
     lds = lds_sets > 0 and (tuned is None or bool(tuned.get("lds", True)))
     check_every = int(tuned.get("check_every", 16)) if tuned else 16

@@ -1023,31 +1023,38 @@ __global__ void __launch_bounds__(KB) __attribute__((amdgpu_waves_per_eu(5))) k_
 
 // block limits and buffer ring of one kernel configuration; a plan is built
 // for one (UMAX, RECMAX) pair and runs on the kernel configured for it
-template <int UMAX_, int RECMAX_, int NBUF_>
+template <int UMAX_, int RECMAX_, int NBUF_, int VB_ = 8>
 struct LdsCfg {
   static constexpr int UMAX = UMAX_;      // neighbour rows per block
   static constexpr int RECMAX = RECMAX_;  // records per block
   static constexpr int NBUF = NBUF_;      // block buffers in the ring (copies run NBUF-1 blocks ahead)
-  static constexpr int ROWB = (UMAX + 2) * 512;  // neighbour rows (+2 rows the odd copy may fill)
-  static constexpr int LOFB = RECMAX * 4;        // record offsets
-  static constexpr int VALB = RECMAX * 8;        // record values
+  static constexpr int VB = VB_;          // bytes per value: a row is 64 of them
+  static constexpr int RB = 64 * VB;      // row bytes (512, or 256 for 4-byte values)
+  static constexpr int RPC = 1024 / RB;   // rows per 1 KB copy
+  static constexpr int LSH = RB == 256 ? 1 : 0;  // plan offsets count 512-byte row slots
+  static constexpr int ROWB = (UMAX + RPC) * RB;  // neighbour rows (+ the rows a last copy may fill)
+  static constexpr int LOFB = RECMAX * 4;         // record offsets
+  static constexpr int VALB = RECMAX * VB;        // record values
   static constexpr int BUFB = ROWB + LOFB + VALB + LDS_RINB;
   static constexpr int TRASH = NBUF * BUFB;      // 1 KB written by spare copies, never read
   static constexpr int HDR = TRASH + 1024;       // (first row, rows) of each buffer's block
   static constexpr int BYTES = HDR + 8 * NBUF;
-  static constexpr int JB = (UMAX / 2 + LDS_NP - 1) / LDS_NP;  // row copies per producer and block
-  static constexpr int JR = (LOFB + VALB + LDS_RINB) / 1024;   // record copies per block
-  static constexpr int JRP = (JR + LDS_NP - 1) / LDS_NP;       // ... per producer
+  static constexpr int JB = (UMAX / RPC + LDS_NP - 1) / LDS_NP;  // row copies per producer and block
+  static constexpr int JR = (LOFB + VALB + LDS_RINB) / 1024;     // record copies per block
+  static constexpr int JRP = (JR + LDS_NP - 1) / LDS_NP;         // ... per producer
+  static_assert(VB == 8 || VB == 4, "rows of 64 values of 8 or 4 bytes");
   static_assert(BYTES <= 160 * 1024, "the block buffers must fit the 160 KB LDS of a gfx950 CU");
   static_assert((LOFB + VALB) % 1024 == 0, "records are copied in 1 KB pieces");
   static_assert(2 * (RB_LDS_RMAX + 1) * 4 <= LDS_RINB, "row table of a block");
-  static_assert(2 * LDS_NP * JB <= UMAX + RB_LDS_UPAD, "producers read union index runs within the padding");
+  static_assert(RPC * LDS_NP * JB <= UMAX + RB_LDS_UPAD, "producers read union index runs within the padding");
   static_assert((JB + JRP) * (NBUF - 2) < 64, "copies in flight behind the awaited block fit vmcnt");
 };
 // RB_LDS_UMAX / RB_LDS_RECMAX, two buffers: each p row crosses the L2 ~4 times per SpMM.  A ring of
 // three buffers with blocks of 80 rows (copies two blocks ahead) measured slower in the solve (68.8
 // against 63.5 us per launch at 115k nodes): the smaller unions load ~35% more rows.
 typedef LdsCfg<RB_LDS_UMAX, RB_LDS_RECMAX, 2> LdsWide;
+// one real float32 system of 64 columns: 256-byte rows, four to a copy, on the same plans
+typedef LdsCfg<RB_LDS_UMAX, RB_LDS_RECMAX, 2, 4> LdsHalf;
 
 // s_waitcnt vmcnt(N), the other counters untouched (gfx9 encoding: vmcnt in bits 3:0 and 15:14)
 template <int N>
@@ -1136,7 +1143,7 @@ k_cg_spmm_lds(int64_t nblk, const int32_t* __restrict__ brow, const int32_t* __r
               const T* __restrict__ recv, const T* __restrict__ P, T* __restrict__ Q, int64_t rs, int64_t coff,
               int cbase, ColState<typename LdsAcc<T, RPAIR>::type> st, const void* __restrict__ dinvp,
               int64_t dstride, const float* __restrict__ lbound, int C) {
-  static_assert(sizeof(T) == 8, "512-byte rows: 64 lanes of 8 bytes");
+  static_assert(sizeof(T) == CF::VB, "rows of 64 values of the configuration's size");
   static_assert(!RPAIR || std::is_same<T, cplx<float>>::value, "a real pair is staged as 8-byte (float, float)");
   typedef typename sc<T>::acc A;
   typedef typename LdsAcc<T, RPAIR>::type AS;
@@ -1170,26 +1177,27 @@ k_cg_spmm_lds(int64_t nblk, const int32_t* __restrict__ brow, const int32_t* __r
     // then the union's row indices and first record); they are made one block ahead, while the
     // previous copies are in flight, so a block's copies start right after its barrier
     const int pw = wave - LDS_NW;
+    constexpr int RPC = CF::RPC;
     struct Meta {
       int ni, r0, nr, e0;
-      int g[2 * JB];
+      int g[RPC * JB];
     };
     auto meta = [&](int kb) {
       Meta mt;
       const bool real = kb < nmine;
       const int64_t bb = real ? bfirst + (int64_t)kb * stride : 0;
       const int u0 = real ? bu[bb] : 0;
-      mt.ni = real ? (bu[bb + 1] - u0 + 1) / 2 : 0;  // row copies (two rows each)
+      mt.ni = real ? (bu[bb + 1] - u0 + RPC - 1) / RPC : 0;  // row copies (RPC rows each)
       mt.r0 = real ? brow[bb] : 0;
       mt.nr = real ? brow[bb + 1] - mt.r0 : 0;
       // rinfo_m is rinfo again: rinfo is also a source of the copies, which the compiler takes as
       // possibly written, so its own loads could not use the scalar cache and would make the wave
       // wait on its copies in flight
       mt.e0 = real ? rinfo_m[2 * mt.r0] : 0;
-      const int* ix = ulist + u0 + 2 * pw * JB;  // the padding keeps these runs readable
+      const int* ix = ulist + u0 + RPC * pw * JB;  // the padding keeps these runs readable
       // the row indices are uniform: scalar loads, kept apart from the copies' counter
 #pragma unroll
-      for (int j = 0; j < 2 * JB; ++j) mt.g[j] = __builtin_amdgcn_readfirstlane(ix[j]);
+      for (int j = 0; j < RPC * JB; ++j) mt.g[j] = __builtin_amdgcn_readfirstlane(ix[j]);
       return mt;
     };
     auto issue = [&](const Meta& mt, int kb) {
@@ -1213,9 +1221,17 @@ k_cg_spmm_lds(int64_t nblk, const int32_t* __restrict__ brow, const int32_t* __r
 #pragma unroll
       for (int j = 0; j < JB; ++j) {
         const int i = pw * JB + j;
-        const int g = lane < 32 ? mt.g[2 * j] : mt.g[2 * j + 1];
-        const char* src = (const char*)(P + (int64_t)g * rs + coff) + (lane & 31) * 16;
-        dma16(src, i < mt.ni ? base + 2 * i * 512 : base + CF::UMAX * 512);
+        // lanes [q * 64 / RPC, (q + 1) * 64 / RPC) copy row q of the RPC rows of this copy
+        int g;
+        if constexpr (RPC == 2) {
+          g = lane < 32 ? mt.g[2 * j] : mt.g[2 * j + 1];
+        } else {
+          const int ga = lane < 16 ? mt.g[4 * j] : mt.g[4 * j + 1];
+          const int gb = lane < 48 ? mt.g[4 * j + 2] : mt.g[4 * j + 3];
+          g = lane < 32 ? ga : gb;
+        }
+        const char* src = (const char*)(P + (int64_t)g * rs + coff) + (lane & (64 / RPC - 1)) * 16;
+        dma16(src, i < mt.ni ? base + i * 1024 : base + CF::UMAX * CF::RB);
       }
     };
     for (int kb = 0; kb < NBUF - 1; ++kb) issue(meta(kb), kb);
@@ -1250,7 +1266,7 @@ k_cg_spmm_lds(int64_t nblk, const int32_t* __restrict__ brow, const int32_t* __r
             va[u] = lv[e + u];
           }
 #pragma unroll
-          for (int u = 0; u < 8; ++u) pa[u] = *(const T*)(cb + la[u]);
+          for (int u = 0; u < 8; ++u) pa[u] = *(const T*)(cb + (la[u] >> CF::LSH));
           if constexpr (RPAIR) {
             // select between values read into registers (a select between the fields' addresses
             // would index va dynamically and put it in scratch memory)
@@ -1266,7 +1282,7 @@ k_cg_spmm_lds(int64_t nblk, const int32_t* __restrict__ brow, const int32_t* __r
             for (int u = 0; u < 8; ++u) acc = acc + va[u] * pa[u];
           }
         }
-        const T pown = *(const T*)(cb + ri[2 * i + 1]);
+        const T pown = *(const T*)(cb + (ri[2 * i + 1] >> CF::LSH));
         if constexpr (POLY && RPAIR) {  // pown: this row's r in columns 2l, 2l+1; acc: (A D^-1 r)
           const float d = ((const float*)dinvp)[(int64_t)psys * dstride + r0 + i];
           T z;
@@ -2683,13 +2699,18 @@ static void launch_spmm_lds_poly(const LdsArgs& pl, dim3 grid, hipStream_t s, co
   if constexpr (sizeof(T) == 8) {
     launch_spmm_lds_cf<T, LdsWide, false, A, true>(pl, grid, s, (const T*)pl.recv_ad, R, Z, 64, 0, 0, st,
                                                    (const void*)dinv, 0, (const float*)lbound, 64);
+  } else if constexpr (sizeof(T) == 4) {
+    launch_spmm_lds_cf<T, LdsHalf, false, A, true>(pl, grid, s, (const T*)pl.recv_ad, R, Z, 64, 0, 0, st,
+                                                   (const void*)dinv, 0, (const float*)lbound, 64);
   }
 }
-// one 64-column system with 8-byte values (rows of 64 columns)
+// one 64-column system with 8-byte values (512-byte rows) or 4-byte values (256-byte rows)
 template <typename T, typename A>
 static void launch_spmm_lds(const LdsArgs& pl, dim3 grid, hipStream_t s, const T* P, T* Q, ColState<A> st) {
   if constexpr (sizeof(T) == 8) {
     launch_spmm_lds_cf<T, LdsWide, false>(pl, grid, s, (const T*)pl.recv, P, Q, 64, 0, 0, st);
+  } else if constexpr (sizeof(T) == 4) {
+    launch_spmm_lds_cf<T, LdsHalf, false>(pl, grid, s, (const T*)pl.recv, P, Q, 64, 0, 0, st);
   }
 }
 // real float32 systems of 64 columns each, in pairs: one launch per pair, record set p for pair p
@@ -2717,14 +2738,14 @@ static void launch_spmm_lds_pairs_poly(const LdsArgs& pl, int64_t nrec_set, dim3
         64 * p, 128 * p, st, (const void*)dinv, n, (const float*)lbound, C);
 }
 
-// a plan applies to 64-column solves of one system with 8-byte values
+// a plan applies to 64-column solves of one system with 8-byte or 4-byte values
 static bool lds_plan_ok(const rb_spmm_plan* plan) {
   const bool cfg_ok = plan && plan->umax == LdsWide::UMAX && plan->recmax == LdsWide::RECMAX;
   return cfg_ok && plan->nblk > 0 && plan->brow && plan->bu && plan->ulist && plan->rinfo && plan->rmap &&
          plan->loff && plan->recv && plan->recv_sets >= 1;
 }
 static bool lds_eligible(const rb_spmm_plan* plan, int nsys, int C, size_t value_bytes) {
-  return lds_plan_ok(plan) && nsys == 1 && C == 64 && value_bytes == 8;
+  return lds_plan_ok(plan) && nsys == 1 && C == 64 && (value_bytes == 8 || value_bytes == 4);
 }
 // ... and to real float32 systems of 64 columns each in pairs, one record set per pair
 static bool lds_pairs_eligible(const rb_spmm_plan* plan, int nsys, int k, size_t value_bytes) {

@@ -81,9 +81,10 @@ def femsolve(Amat, rhs, method="auto", **kwargs):
     # chunks of at most 512 columns; the column sweep's buffer descriptor (float64 vectors under
     # 2 GiB) may ask for narrower ones on large meshes
     step = max(1, 512 if use_bicgstab else _gpu.sweep_columns(n, min(ncol, 512), 1, is_complex))
-    # the LDS-staged SpMM serves 64-column chunks with 8-byte inner values (complex64 mixed, float64 plain)
+    # the LDS-staged SpMM serves 64-column chunks with 8-byte or 4-byte inner values (complex64 or
+    # float32 mixed, float64 plain)
     plan = None
-    if not use_bicgstab and is_complex == (precision == "mixed") and (step == 64 or ncol % step == 64):
+    if not use_bicgstab and (precision == "mixed" or not is_complex) and (step == 64 or ncol % step == 64):
         import os
 
         if os.environ.get("RB_LDS_SPMM", "1") != "0":

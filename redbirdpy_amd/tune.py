@@ -6,7 +6,7 @@ Here one multi-RHS solve covers every column, and what is left to choose is
 how the GPU runs it:
 
 * ``lds``: the LDS-staged SpMM (its plan exists for 64-column solves of one
-  system with 8-byte inner values) or the die-wide column sweep;
+  system with 8-byte or 4-byte inner values) or the die-wide column sweep;
 * ``grid``: the column sweep's workgroup cap (0 = one resident round of the
   device, else a multiple of 8; rb_krylov_set_sweep_grid);
 * ``check_every``: iterations between the host's convergence polls (at most;

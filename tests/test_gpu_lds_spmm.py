@@ -51,17 +51,19 @@ def test_fd_mixed_64col_against_oracle_and_gather(h, monkeypatch):
     assert _rel(f1, f2) < 1e-10 and abs(it1 - it2) <= 3
 
 
-def test_cw_fp64_64col_against_oracle_and_gather(monkeypatch):
-    """Plain float64 real solves of 64 columns (the Jacobian box's forward)."""
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_cw_64col_against_oracle_and_gather(precision, monkeypatch):
+    """Real solves of 64 columns (the Jacobian box's forward): plain float64 (512-byte rows), and
+    mixed with float32 inner solves (256-byte rows, four to a copy, on the same plan)."""
     from oracle import rb_oracle as orc
     from redbirdpy_amd import workloads
 
     cfg = workloads.jac_box(2.0)
-    (d1, f1, it1), (d2, f2, it2) = _forward_both(cfg, "fp64", monkeypatch)
+    (d1, f1, it1), (d2, f2, it2) = _forward_both(cfg, precision, monkeypatch)
     c3, _ = orc.meshprep(copy.deepcopy(cfg))
     d3, f3 = orc.runforward(c3)
     assert _rel(f1, f3) < 1e-10 and _relmax(d1, d3) < 1e-9
-    assert _rel(f1, f2) < 1e-11 and abs(it1 - it2) <= 2
+    assert _rel(f1, f2) < (1e-11 if precision == "fp64" else 1e-10) and abs(it1 - it2) <= 3
 
 
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
