@@ -165,3 +165,29 @@ def test_precond_setting_round_trip():
     _abi.call("rb_krylov_set_precond", _abi.RB_PRECOND_JACOBI)
     with pytest.raises(_abi.RbError):
         _abi.call("rb_krylov_set_precond", 7)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("omega", [0.0, 2 * np.pi * 1e8])
+def test_poly_femsolve_edge_columns_and_flags(omega, precision):
+    """Under the polynomial preconditioner, through femsolve (column sweep and the separate z kernel:
+    the columns are not 64 wide): a zero column stays zero, a warm start at the solution keeps it, and
+    an iteration cap reports flag 1."""
+    import redbirdpy_amd as rb
+    from redbirdpy_amd import tune
+    from oracle import rb_oracle as orc
+    from conftest import box_cfg
+
+    c2, sd2 = orc.meshprep(copy.deepcopy(box_cfg((16, 16, 8), 2.0, omega=omega)))
+    A = orc.femlhs(c2, c2["deldotdel"])
+    rhs, _, _, _ = orc.femrhs(c2, sd2)
+    xr, _ = orc.femsolve(A, rhs)
+    rhs_d = np.column_stack([rhs.toarray(), np.zeros(rhs.shape[0])])
+    with tune.applied({"precond": 1}):
+        x, flag = rb.femsolve(A, rhs_d, precision=precision)
+        assert flag == 0 and np.all(x[:, -1] == 0)
+        assert _rel(x[:, :-1], xr) < 1e-10
+        x2, flag2 = rb.femsolve(A, rhs_d, precision=precision, x0=x)
+        assert flag2 == 0 and _rel(x2[:, :-1], x[:, :-1]) < 1e-10 and np.all(x2[:, -1] == 0)
+        _, flag3 = rb.femsolve(A, rhs_d, precision=precision, maxiter=3)
+        assert flag3 == 1
