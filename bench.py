@@ -216,9 +216,9 @@ def _tuned(cfg, precision, tol, warm_from=None):
     t0 = time.perf_counter()
     out = tune.tune_forward(cfg, precision=precision, tol=tol, warm_from=warm_from)
     return {"lds": out["lds"], "grid": out["grid"], "check_every": out["check_every"],
-            "precond": ("jacobi", "poly")[out["precond"]], "tuning_s": time.perf_counter() - t0,
+            "precond": tune.PRECOND_NAMES[out["precond"]], "tuning_s": time.perf_counter() - t0,
             "ms_per_solve": {"%s, %s, poll %d" % ("lds" if c["lds"] else "sweep grid %d" % c["grid"],
-                                                  ("jacobi", "poly")[c["precond"]], c["check_every"]):
+                                                  tune.PRECOND_NAMES[c["precond"]], c["check_every"]):
                              c["seconds_per_solve"] * 1e3 for c in out["candidates"]}}
 
 
@@ -297,7 +297,7 @@ def batch_bench(a):
     import ctypes
     import torch
     import redbirdpy_amd as rb
-    from redbirdpy_amd import workloads, _abi
+    from redbirdpy_amd import workloads, _abi, tune
     from redbirdpy_amd.forward import BatchForward
 
     cfgs = workloads.batch_boxes(count=a.batch_meshes)
@@ -326,7 +326,7 @@ def batch_bench(a):
             t0 = time.perf_counter()
             tr = bf.tune(prec, 1e-10)
             out["autotune"][prec] = {"lds": tr["lds"], "grid": tr["grid"], "check_every": tr["check_every"],
-                                     "precond": ("jacobi", "poly")[tr["precond"]],
+                                     "precond": tune.PRECOND_NAMES[tr["precond"]],
                                      "tuning_s": time.perf_counter() - t0}
         bf.solve(prec, 1e-10, fields=False)
         torch.cuda.synchronize()

@@ -1101,6 +1101,11 @@ __device__ inline cplx<double> rscale(cplx<double> v, double s) { return mk<doub
 
 // slope of the polynomial preconditioner p(x) = (3 / L)(1 - POLY_K x / L)
 #define POLY_K 0.8
+// the degree-2 one: p(x) = (P2_0 - P2_1 t + P2_2 t^2) / L with t = x / L (positive on [0, 1], its
+// minimum 0.38 at t = 0.88)
+#define P2_0 7.0
+#define P2_1 15.0
+#define P2_2 8.5
 
 typedef __attribute__((address_space(3))) void lds_void_t;
 
@@ -1142,7 +1147,7 @@ k_cg_spmm_lds(int64_t nblk, const int32_t* __restrict__ brow, const int32_t* __r
               const int32_t* __restrict__ rinfo_m, const int32_t* __restrict__ loff,
               const T* __restrict__ recv, const T* __restrict__ P, T* __restrict__ Q, int64_t rs, int64_t coff,
               int cbase, ColState<typename LdsAcc<T, RPAIR>::type> st, const void* __restrict__ dinvp,
-              int64_t dstride, const float* __restrict__ lbound, int C) {
+              int64_t dstride, const float* __restrict__ lbound, int C, const T* __restrict__ Rg, int pdeg) {
   static_assert(sizeof(T) == CF::VB, "rows of 64 values of the configuration's size");
   static_assert(!RPAIR || std::is_same<T, cplx<float>>::value, "a real pair is staged as 8-byte (float, float)");
   typedef typename sc<T>::acc A;
@@ -1164,12 +1169,18 @@ k_cg_spmm_lds(int64_t nblk, const int32_t* __restrict__ brow, const int32_t* __r
   A pv = acc_zero(A());  // RPAIR: .x column 2l, .y column 2l+1; POLY: r.z
   A pv2 = acc_zero(A());  // POLY: r.r
   typedef typename RealOf<T>::type R1;
-  R1 pc0 = 0, pc1 = 0;
+  R1 pc0 = 0, pc1 = 0, pc2 = 0;
   const int psys = RPAIR ? cbase / 64 + (second ? 1 : 0) : 0;  // POLY: this lane's system
   if constexpr (POLY) {
     const R1 L = (R1)lbound[psys];
-    pc0 = (R1)3.0 / L;
-    pc1 = (R1)(-3.0 * POLY_K) / (L * L);
+    if (pdeg == 2) {
+      pc0 = (R1)P2_0 / L;
+      pc1 = (R1)(-P2_1) / (L * L);
+      pc2 = (R1)P2_2 / (L * L * L);
+    } else {
+      pc0 = (R1)3.0 / L;
+      pc1 = (R1)(-3.0 * POLY_K) / (L * L);
+    }
   }
   if (wave >= LDS_NW) {
     // producer: block k of this workgroup into buffer k % NBUF, JB + JRP copies every time.  A
@@ -1294,11 +1305,20 @@ k_cg_spmm_lds(int64_t nblk, const int32_t* __restrict__ brow, const int32_t* __r
           pv2.x += (double)pown.x * (double)pown.x;
           pv2.y += (double)pown.y * (double)pown.y;
           continue;
-        } else if constexpr (POLY) {  // pown is this row's r, acc its (A D^-1 r)
-          const T z = ((const T*)dinvp)[r0 + i] * (rscale(pown, pc0) + rscale(acc, pc1));
-          Q[(int64_t)(r0 + i) * rs + coff + lane] = z;
-          pv = acc_add(pv, amul(pown, z));
-          pv2 = acc_add(pv2, acc_from_real(sc<T>::norm2(pown), A()));
+        } else if constexpr (POLY) {
+          const int64_t ro = (int64_t)(r0 + i) * rs + coff + lane;
+          const T d = ((const T*)dinvp)[r0 + i];
+          T z, rv;
+          if (pdeg == 2) {  // pown is this row's w1 = (A D^-1 r), acc its (A D^-1 w1); r from HBM
+            rv = Rg[ro];
+            z = d * (rscale(rv, pc0) + rscale(pown, pc1) + rscale(acc, pc2));
+          } else {  // pown is this row's r, acc its (A D^-1 r)
+            rv = pown;
+            z = d * (rscale(pown, pc0) + rscale(acc, pc1));
+          }
+          Q[ro] = z;
+          pv = acc_add(pv, amul(rv, z));
+          pv2 = acc_add(pv2, acc_from_real(sc<T>::norm2(rv), A()));
           continue;
         }
         Q[(int64_t)(r0 + i) * rs + coff + lane] = acc;
@@ -2293,7 +2313,7 @@ struct SolveBufs {
 static std::atomic<int> g_precond{0};
 
 extern "C" int rb_krylov_set_precond(int kind) {
-  if (kind != RB_PRECOND_JACOBI && kind != RB_PRECOND_POLY) {
+  if (kind != RB_PRECOND_JACOBI && kind != RB_PRECOND_POLY && kind != RB_PRECOND_POLY2) {
     rb_set_error("rb_krylov_set_precond: unknown preconditioner %d", kind);
     return RB_ERR_ARG;
   }
@@ -2314,7 +2334,7 @@ static size_t ws_layout(int64_t n, int64_t C, int nsys, int mixed, size_t* o) {
     o[2] = L.take(n * C * sizeof(T64));
     o[3] = L.take(n * nsys * sizeof(T64));
     state_layout<A>(L, C, nb, o + 8);
-    if (g_precond.load() == RB_PRECOND_POLY) {
+    if (g_precond.load() != RB_PRECOND_JACOBI) {
       o[5] = L.take(n * C * sizeof(T64));               // y = D^-1 r, then z
       o[7] = L.take(std::max<int64_t>(64, nsys * 4));   // Gershgorin bound per system
     }
@@ -2327,7 +2347,7 @@ static size_t ws_layout(int64_t n, int64_t C, int nsys, int mixed, size_t* o) {
     state_layout<A>(L, C, nb, o + 8);
     state_layout<A>(L, C, nb, o + 20);
     o[6] = L.take(5 * MAXCHUNK * sizeof(int));  // replacement requests (2 parities) + chunk counters
-    if (g_precond.load() == RB_PRECOND_POLY) {
+    if (g_precond.load() != RB_PRECOND_JACOBI) {
       o[5] = L.take(n * C * sizeof(T32));
       o[7] = L.take(std::max<int64_t>(64, nsys * 4));
     }
@@ -2685,23 +2705,31 @@ static void launch_resid_sweep(int nch, dim3 grid, hipStream_t s, const Csr& m, 
 template <typename T, typename CF, bool RPAIR, typename A, bool POLY = false>
 static void launch_spmm_lds_cf(const LdsArgs& pl, dim3 grid, hipStream_t s, const T* recv, const T* P, T* Q,
                                int64_t rs, int64_t coff, int cbase, ColState<A> st, const void* dinvp = nullptr,
-                               int64_t dstride = 0, const float* lbound = nullptr, int C = 64) {
+                               int64_t dstride = 0, const float* lbound = nullptr, int C = 64,
+                               const T* Rg = nullptr, int pdeg = 1) {
   (void)hipFuncSetAttribute((const void*)k_cg_spmm_lds<T, CF, RPAIR, POLY>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, CF::BYTES);
   plaunch(k_cg_spmm_lds<T, CF, RPAIR, POLY>, grid, dim3((LDS_NW + LDS_NP) * 64), CF::BYTES, s, pl.nblk, pl.brow,
-          pl.bu, pl.ulist, pl.rinfo, pl.rinfo, pl.loff, recv, P, Q, rs, coff, cbase, st, dinvp, dstride, lbound, C);
+          pl.bu, pl.ulist, pl.rinfo, pl.rinfo, pl.loff, recv, P, Q, rs, coff, cbase, st, dinvp, dstride, lbound, C,
+          Rg, pdeg);
 }
 // the polynomial preconditioner's product with z formed in the epilogue (one 64-column system,
-// pl.recv_ad packed): the residual R in, z out to Z, r.z and r.r partials into st
+// pl.recv_ad packed): the residual R in, z out to Z, r.z and r.r partials into st.  W (degree 2):
+// first w1 = A D^-1 r into W (a plain launch on the A D^-1 records; its p.q partials are
+// overwritten by the second launch's), then the epilogue's launch on the staged w1 rows
 template <typename T, typename A>
 static void launch_spmm_lds_poly(const LdsArgs& pl, dim3 grid, hipStream_t s, const T* R, T* Z, const T* dinv,
-                                 const unsigned* lbound, ColState<A> st) {
-  if constexpr (sizeof(T) == 8) {
-    launch_spmm_lds_cf<T, LdsWide, false, A, true>(pl, grid, s, (const T*)pl.recv_ad, R, Z, 64, 0, 0, st,
-                                                   (const void*)dinv, 0, (const float*)lbound, 64);
-  } else if constexpr (sizeof(T) == 4) {
-    launch_spmm_lds_cf<T, LdsHalf, false, A, true>(pl, grid, s, (const T*)pl.recv_ad, R, Z, 64, 0, 0, st,
-                                                   (const void*)dinv, 0, (const float*)lbound, 64);
+                                 const unsigned* lbound, ColState<A> st, T* W = nullptr) {
+  typedef typename std::conditional<sizeof(T) == 8, LdsWide, LdsHalf>::type CF;
+  if constexpr (sizeof(T) == 8 || sizeof(T) == 4) {
+    if (W) {
+      launch_spmm_lds_cf<T, CF, false, A, false>(pl, grid, s, (const T*)pl.recv_ad, R, W, 64, 0, 0, st);
+      launch_spmm_lds_cf<T, CF, false, A, true>(pl, grid, s, (const T*)pl.recv_ad, (const T*)W, Z, 64, 0, 0, st,
+                                                (const void*)dinv, 0, (const float*)lbound, 64, R, 2);
+    } else {
+      launch_spmm_lds_cf<T, CF, false, A, true>(pl, grid, s, (const T*)pl.recv_ad, R, Z, 64, 0, 0, st,
+                                                (const void*)dinv, 0, (const float*)lbound, 64);
+    }
   }
 }
 // one 64-column system with 8-byte values (512-byte rows) or 4-byte values (256-byte rows)
@@ -2783,6 +2811,8 @@ static int run_cg(const Csr& m, int nsys, const T* vals, const T* dinv, const T*
   const bool poly = YZ && lb;
   const bool spmm_pair = pair_sweep<T>(nsys, C, k) && !poly;
   const bool lds_poly = poly && lds && lds->recv_ad;
+  // the degree-2 polynomial where the fused product applies (its w1 = A D^-1 r goes to YZ)
+  T* W2 = lds_poly && g_precond.load() == RB_PRECOND_POLY2 ? YZ : nullptr;
   st.poly = ss.poly = poly ? 1 : 0;
   auto spmm = [&](const T* Pin, T* Qout) {
     if (lds)
@@ -2799,7 +2829,7 @@ static int run_cg(const Csr& m, int nsys, const T* vals, const T* dinv, const T*
       if (rcf) return rcf;
       hipLaunchKernelGGL(k_gersh<T>, dim3((unsigned)ceil_div(m.n, 256), nsys), dim3(256), 0, s, m, vals, lb);
       if (lds_poly) {  // z formed in the LDS product's epilogue, into the spent q
-        launch_spmm_lds_poly<T>(*lds, dim3(nbs), s, (const T*)w.R, w.Q, dinv, lb, ss);
+        launch_spmm_lds_poly<T>(*lds, dim3(nbs), s, (const T*)w.R, w.Q, dinv, lb, ss, W2);
         hipLaunchKernelGGL(k_fin_rho<A>, dim3(C), dim3(FB), 0, s, C, ss);
         DISPATCH_GN(C, L_PUP, m.n, dinv, X, w.R, w.P, C, k, (TW*)nullptr, st, (const T*)w.Q);
       } else {
@@ -2835,7 +2865,7 @@ static int run_cg(const Csr& m, int nsys, const T* vals, const T* dinv, const T*
       bool zlds = false;       // its r.z, r.r partials on the LDS grid
       if (lds_poly) {          // w = A D^-1 r with z formed in the epilogue, into the spent q
         ProfScope ps(RB_PROF_POLY, s);
-        launch_spmm_lds_poly<T>(*lds, dim3(nbs), s, (const T*)w.R, w.Q, dinv, lb, ss);
+        launch_spmm_lds_poly<T>(*lds, dim3(nbs), s, (const T*)w.R, w.Q, dinv, lb, ss, W2);
         zin = w.Q;
         zlds = true;
       } else if (poly) {  // w = A y into the spent q, then z and the r.z, r.r partials
@@ -2903,7 +2933,7 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
       hipLaunchKernelGGL(k_lds_pack<T64>, dim3(ew_grid(nr)), dim3(256), 0, s, nr, plan->rmap, A64, (T64*)plan->recv);
       RB_CHECK_LAUNCH();
     }
-    const bool poly = g_precond.load() == RB_PRECOND_POLY;
+    const bool poly = g_precond.load() != RB_PRECOND_JACOBI;
     if (use_lds && poly && plan->recv_sets >= 2) {  // the second record set: A D^-1
       const int64_t nr = plan->nrec + RB_LDS_RECPAD;
       la.recv_ad = (const T64*)plan->recv + nr;
@@ -2961,7 +2991,7 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
   si.ru_cnt = ru_req + 2 * MAXCHUNK;
   // the polynomial preconditioner (real 64-column systems then run in the one-column layout
   // rather than in chunk pairs)
-  const bool poly = g_precond.load() == RB_PRECOND_POLY;
+  const bool poly = g_precond.load() != RB_PRECOND_JACOBI;
   si.poly = poly ? 1 : 0;
   T32* YZ = poly ? (T32*)(base + o[5]) : nullptr;
   unsigned* lb = poly ? (unsigned*)(base + o[7]) : nullptr;
@@ -3061,6 +3091,8 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
   };
   // with the plan's second record set (A D^-1) the preconditioner's product stages r itself
   const bool lds_poly = poly && lds_one && plan->recv_sets >= 2;
+  // the degree-2 polynomial where the fused product applies (its w1 = A D^-1 r goes to YZ)
+  T32* W2 = lds_poly && g_precond.load() == RB_PRECOND_POLY2 ? YZ : nullptr;
   if (lds_poly) {
     la.recv_ad = (const T32*)plan->recv + nrec_set;
     hipLaunchKernelGGL(k_lds_pack_ad<T32>, dim3(ew_grid(nrec_set)), dim3(256), 0, s, nrec_set, plan->rmap, A32,
@@ -3090,7 +3122,7 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
     if (lds_poly) {  // z formed in the LDS product's epilogue, into the spent q
       ColState<A> sz = si;
       sz.nb = (int)nbs;
-      launch_spmm_lds_poly<T32>(la, dim3(nbs), s, (const T32*)w.R, w.Q, (const T32*)w.dinv, lb, sz);
+      launch_spmm_lds_poly<T32>(la, dim3(nbs), s, (const T32*)w.R, w.Q, (const T32*)w.dinv, lb, sz, W2);
       hipLaunchKernelGGL(k_fin_rho<A>, dim3(C), dim3(FB), 0, s, C, sz);
       DISPATCH_GN(C, L_PUP, m.n, w.dinv, D, w.R, w.P, C, k, X, si, (const T32*)w.Q);
     } else if (pair_poly) {  // the same with the pairs' product
@@ -3155,7 +3187,7 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
       if (lds_poly) {            // w = A D^-1 r with z formed in the epilogue, into the spent q
         sz.nb = (int)nbs;
         ProfScope ps(RB_PROF_POLY, s);
-        launch_spmm_lds_poly<T32>(la, dim3(nbs), s, (const T32*)w.R, w.Q, (const T32*)w.dinv, lb, sz);
+        launch_spmm_lds_poly<T32>(la, dim3(nbs), s, (const T32*)w.R, w.Q, (const T32*)w.dinv, lb, sz, W2);
         zin = w.Q;
       } else if (pair_poly) {  // the same, one launch per system pair
         sz.nb = (int)nbs;

@@ -32,7 +32,8 @@ def test_tune_forward_caches_a_choice_and_results_stay_exact(tmp_path, monkeypat
     assert tune.lookup(sysm, k, "mixed") == {key: out[key] for key in ("lds", "grid", "check_every", "precond")}
     d1, _ = rb.runforward(c1, precision="mixed", tol=1e-12)
     assert np.max(np.abs(d1 - d2)) / np.max(np.abs(d2)) < 1e-9
-    tune.store(sysm, k, "mixed", {"lds": False, "grid": 256, "check_every": 8, "precond": 1 - out["precond"]})
+    other = _abi.RB_PRECOND_POLY if out["precond"] == _abi.RB_PRECOND_JACOBI else _abi.RB_PRECOND_JACOBI
+    tune.store(sysm, k, "mixed", {"lds": False, "grid": 256, "check_every": 8, "precond": other})
     d3, _ = rb.runforward(c1, precision="mixed", tol=1e-12)
     assert np.max(np.abs(d3 - d2)) / np.max(np.abs(d2)) < 1e-9
     assert _abi.lib().rb_krylov_get_sweep_grid() == 0
