@@ -39,7 +39,7 @@ extern "C" {
 /* preconditioner of the CG/COCG solves (rb_krylov_set_precond) */
 #define RB_PRECOND_JACOBI 0 /* z = D^-1 r */
 #define RB_PRECOND_POLY 1   /* z = p(D^-1 A) D^-1 r, p of degree 1 positive on a Gershgorin bound of the spectrum */
-#define RB_PRECOND_POLY2 2  /* p of degree 2 where the fused LDS product applies, else degree 1 */
+#define RB_PRECOND_POLY2 2  /* p of degree 2 where a fused LDS product applies, else degree 1 */
 
 /* dtype codes */
 #define RB_F64 0
@@ -313,8 +313,9 @@ int64_t rb_krylov_get_sweep_grid(void);
  * p(x) = (3 / L)(1 - 0.8 x / L), L the Gershgorin bound of D^-1 A per system:
  * one more q = A y product per iteration for ~1.8x fewer iterations on 3-D
  * FEM systems.  RB_PRECOND_POLY2: p(x) = (7 - 15 t + 8.5 t^2) / L, t = x / L,
- * two more products for ~2.7x fewer iterations, on the one-system LDS path
- * with the A D^-1 records (degree 1 on the others).  Real systems of 64 columns each then run in the one-column
+ * two more products for ~2.7x fewer iterations, on the LDS paths with the
+ * A D^-1 records (one system, or float32 systems in pairs; degree 1 on the
+ * others).  Real systems of 64 columns each then run in the one-column
  * layout instead of chunk pairs.  With an LDS plan holding two record sets
  * (rb_spmm_plan.recv_sets >= 2) the product is fused into the LDS-staged SpMM.
  * No counterpart in the reference (its iterative solvers take a caller's M,

@@ -1294,16 +1294,24 @@ k_cg_spmm_lds(int64_t nblk, const int32_t* __restrict__ brow, const int32_t* __r
           }
         }
         const T pown = *(const T*)(cb + (ri[2 * i + 1] >> CF::LSH));
-        if constexpr (POLY && RPAIR) {  // pown: this row's r in columns 2l, 2l+1; acc: (A D^-1 r)
+        if constexpr (POLY && RPAIR) {  // columns 2l, 2l+1 of this lane's system
+          const int64_t ro = (int64_t)(r0 + i) * rs + coff + lane;
           const float d = ((const float*)dinvp)[(int64_t)psys * dstride + r0 + i];
-          T z;
-          z.x = d * (pown.x * pc0 + acc.x * pc1);
-          z.y = d * (pown.y * pc0 + acc.y * pc1);
-          Q[(int64_t)(r0 + i) * rs + coff + lane] = z;
-          pv.x += (double)pown.x * (double)z.x;
-          pv.y += (double)pown.y * (double)z.y;
-          pv2.x += (double)pown.x * (double)pown.x;
-          pv2.y += (double)pown.y * (double)pown.y;
+          T z, rv;
+          if (pdeg == 2) {  // pown: this row's w1 = (A D^-1 r), acc: (A D^-1 w1); r from HBM
+            rv = Rg[ro];
+            z.x = d * (rv.x * pc0 + pown.x * pc1 + acc.x * pc2);
+            z.y = d * (rv.y * pc0 + pown.y * pc1 + acc.y * pc2);
+          } else {  // pown: this row's r, acc: (A D^-1 r)
+            rv = pown;
+            z.x = d * (pown.x * pc0 + acc.x * pc1);
+            z.y = d * (pown.y * pc0 + acc.y * pc1);
+          }
+          Q[ro] = z;
+          pv.x += (double)rv.x * (double)z.x;
+          pv.y += (double)rv.y * (double)z.y;
+          pv2.x += (double)rv.x * (double)rv.x;
+          pv2.y += (double)rv.y * (double)rv.y;
           continue;
         } else if constexpr (POLY) {
           const int64_t ro = (int64_t)(r0 + i) * rs + coff + lane;
@@ -2755,15 +2763,26 @@ static void launch_spmm_lds_pairs(const LdsArgs& pl, int64_t nrec_set, dim3 grid
 // ... and the polynomial preconditioner's product of such pairs (pl.recv_ad: the pairs' A D^-1
 // records, set p for pair p): the residual R in, z out to Z, r.z and r.r partials into st; dinv
 // holds every system's float D^-1, system s at dinv + s * n
+// (W: the degree-2 form, w1 = A D^-1 r of every pair into W first, as launch_spmm_lds_poly does)
 template <typename A>
 static void launch_spmm_lds_pairs_poly(const LdsArgs& pl, int64_t nrec_set, dim3 grid, hipStream_t s, int nsys,
                                        int64_t n, const float* R, float* Z, const float* dinv,
-                                       const unsigned* lbound, ColState<A> st) {
+                                       const unsigned* lbound, ColState<A> st, float* W = nullptr) {
+  typedef cplx<float> c2;
   const int C = 64 * nsys;
-  for (int p = 0; p < nsys / 2; ++p)
-    launch_spmm_lds_cf<cplx<float>, LdsWide, true, A, true>(
-        pl, grid, s, (const cplx<float>*)pl.recv_ad + p * nrec_set, (const cplx<float>*)R, (cplx<float>*)Z, C / 2,
-        64 * p, 128 * p, st, (const void*)dinv, n, (const float*)lbound, C);
+  for (int p = 0; p < nsys / 2; ++p) {
+    const c2* rec = (const c2*)pl.recv_ad + p * nrec_set;
+    if (W) {
+      launch_spmm_lds_cf<c2, LdsWide, true, A, false>(pl, grid, s, rec, (const c2*)R, (c2*)W, C / 2, 64 * p,
+                                                      128 * p, st);
+      launch_spmm_lds_cf<c2, LdsWide, true, A, true>(pl, grid, s, rec, (const c2*)W, (c2*)Z, C / 2, 64 * p, 128 * p,
+                                                     st, (const void*)dinv, n, (const float*)lbound, C,
+                                                     (const c2*)R, 2);
+    } else {
+      launch_spmm_lds_cf<c2, LdsWide, true, A, true>(pl, grid, s, rec, (const c2*)R, (c2*)Z, C / 2, 64 * p, 128 * p,
+                                                     st, (const void*)dinv, n, (const float*)lbound, C);
+    }
+  }
 }
 
 // a plan applies to 64-column solves of one system with 8-byte or 4-byte values
@@ -3109,11 +3128,13 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
                            (cplx<float>*)la.recv_ad + pp * nrec_set);
     }
   }
-  // the pair-form product of the polynomial preconditioner (float32 systems only)
+  // the pair-form product of the polynomial preconditioner (float32 systems only; degree 2 with
+  // its w1 in YZ)
   auto pair_poly_product = [&](ColState<A> sz) {
     if constexpr (std::is_same<T32, float>::value)
       launch_spmm_lds_pairs_poly(la, nrec_set, dim3(nbs), s, nsys, n, (const float*)w.R, (float*)w.Q,
-                                 (const float*)w.dinv, lb, sz);
+                                 (const float*)w.dinv, lb, sz,
+                                 g_precond.load() == RB_PRECOND_POLY2 ? (float*)YZ : (float*)nullptr);
   };
   if (poly) {  // z0 = M r0, rho = r0.z0, p0 = z0 (P is zero, alpha and beta 0)
     rc = rb_fill_i32((int32_t*)lb, 0, nsys, s);

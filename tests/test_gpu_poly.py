@@ -100,11 +100,15 @@ def test_poly_real_64_column_systems_in_pairs(precision, nw, monkeypatch):
                                                 [0.02, 1.2, 0, 1.37]]) for i in range(nw)}
     out = _both(cfg, precision, monkeypatch)
     d3, f3 = _oracle(cfg)
-    for pc in (0, 1):
+    for pc in (0, 1, 2):
         d, f, _ = out[pc]
         for wv in d3:
             assert _rel(f[wv], f3[wv]) < 1e-10 and _relmax(d[wv], d3[wv]) < 1e-8
     assert out[1][2] < 0.8 * out[0][2]
+    if precision == "mixed":  # degree 2 on the pair product
+        assert out[2][2] < 0.85 * out[1][2]
+    else:
+        assert out[2][2] == out[1][2]
     if precision == "mixed":
         monkeypatch.setenv("RB_LDS_SPMM", "0")
         sweep = _both(cfg, precision, monkeypatch)
