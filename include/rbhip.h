@@ -312,8 +312,9 @@ int64_t rb_krylov_get_sweep_grid(void);
  * solve's width).  RB_PRECOND_POLY applies z = p(D^-1 A) D^-1 r with
  * p(x) = (3 / L)(1 - 0.8 x / L), L the Gershgorin bound of D^-1 A per system:
  * one more q = A y product per iteration for ~1.8x fewer iterations on 3-D
- * FEM systems.  RB_PRECOND_POLY2: p(x) = (7 - 15 t + 8.5 t^2) / L, t = x / L,
- * two more products for ~2.7x fewer iterations, on the LDS paths with the
+ * FEM systems.  RB_PRECOND_POLY2: p(x) = (13.54 - 33.78 t + 21.86 t^2) / L,
+ * t = x / L (1 - t p(t) the Chebyshev polynomial of degree 3 smallest on
+ * [0.03, 1]), two more products for ~2.8x fewer iterations, on the LDS paths with the
  * A D^-1 records (one system, or float32 systems in pairs; degree 1 on the
  * others).  Real systems of 64 columns each then run in the one-column
  * layout instead of chunk pairs.  With an LDS plan holding two record sets

@@ -1101,11 +1101,12 @@ __device__ inline cplx<double> rscale(cplx<double> v, double s) { return mk<doub
 
 // slope of the polynomial preconditioner p(x) = (3 / L)(1 - POLY_K x / L)
 #define POLY_K 0.8
-// the degree-2 one: p(x) = (P2_0 - P2_1 t + P2_2 t^2) / L with t = x / L (positive on [0, 1], its
-// minimum 0.38 at t = 0.88)
-#define P2_0 7.0
-#define P2_1 15.0
-#define P2_2 8.5
+// the degree-2 one: p(x) = (P2_0 - P2_1 t + P2_2 t^2) / L with t = x / L, from the Chebyshev
+// residual polynomial 1 - t p(t) = T3((1.03 - 2t) / 0.97) / T3(1.03 / 0.97) (smallest on [0.03, 1]);
+// p stays positive on [0, 1] (its minimum 0.49 at t = 0.77)
+#define P2_0 13.540030
+#define P2_1 33.781406
+#define P2_2 21.864988
 
 typedef __attribute__((address_space(3))) void lds_void_t;
 
