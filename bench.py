@@ -564,10 +564,14 @@ def main():
     nbytes = _spmm_bytes(mesh.nn, mesh.nnz, C, 8)  # complex64 inner solve
     achieved = nbytes / spmm_avg_s / 1e9
     pk = per_kernel["poly"]
+    degree2 = False
     if pk["launches_per_step"] > 0:
         # the polynomial preconditioner's product, LDS-staged with its z epilogue: the records of
-        # A D^-1, r read once, z written once, plus D^-1
-        pbytes = nbytes + mesh.nn * 8
+        # A D^-1, r read once, z written once, plus D^-1.  Degree 2 takes two launches per iteration:
+        # w1 = A D^-1 r (the q = A p bytes), then the epilogue's launch on w1, which also reads r
+        degree2 = pk["launches_per_step"] > 1.5 * per_kernel["spmm"]["launches_per_step"]
+        pbytes = nbytes + (mesh.nn * C * 8 + mesh.nn * 8) / 2 if degree2 else nbytes + mesh.nn * 8
+        pk["degree"] = 2 if degree2 else 1
         pavg = pk["ms_per_step"] / 1e3 / pk["launches_per_step"]
         pk.update({"bytes_per_launch": pbytes, "avg_launch_us": pavg * 1e6, "achieved_GBs": pbytes / pavg / 1e9})
 
@@ -580,7 +584,7 @@ def main():
             traffic = rec["hbm_bytes_per_launch"]
             traffic_plain = rec.get("hbm_bytes_median_launch")
         prec_ = tj.get(SPMM_KERNEL.replace(",false,false>", ",false,true>"))
-        if prec_ and a.h == 1.0 and "bytes_per_launch" in per_kernel["poly"]:
+        if prec_ and a.h == 1.0 and "bytes_per_launch" in per_kernel["poly"] and not degree2:
             per_kernel["poly"]["traffic"] = prec_["hbm_bytes_per_launch"]
     out = {
         "metric": "RHS-solves/s + GN iters/s @115k nodes: FP32+refine vs FP64, 1 GPU, <=1e-6",
