@@ -21,6 +21,7 @@ RB_PREC_MIXED = 1
 RB_PRECOND_JACOBI = 0
 RB_PRECOND_POLY = 1
 RB_PRECOND_POLY2 = 2
+RB_PRECOND_POLY3 = 3
 RB_F64, RB_C128, RB_F32, RB_C64 = 0, 1, 2, 3
 RB_PROF_SPMM, RB_PROF_UPDATE, RB_PROF_PUPDATE, RB_PROF_RESID, RB_PROF_INIT, RB_PROF_POLY = 0, 1, 2, 3, 4, 5
 RB_WF_SINK, RB_WF_FACES = 0, 1

@@ -1107,6 +1107,11 @@ __device__ inline cplx<double> rscale(cplx<double> v, double s) { return mk<doub
 #define P2_0 13.540030
 #define P2_1 33.781406
 #define P2_2 21.864988
+// the degree-3 one, from the Chebyshev residual polynomial of degree 4 on [0.03, 1] (minimum 0.53)
+#define P3_0 20.445082
+#define P3_1 91.179713
+#define P3_2 138.505086
+#define P3_3 67.235478
 
 typedef __attribute__((address_space(3))) void lds_void_t;
 
@@ -1148,7 +1153,8 @@ k_cg_spmm_lds(int64_t nblk, const int32_t* __restrict__ brow, const int32_t* __r
               const int32_t* __restrict__ rinfo_m, const int32_t* __restrict__ loff,
               const T* __restrict__ recv, const T* __restrict__ P, T* __restrict__ Q, int64_t rs, int64_t coff,
               int cbase, ColState<typename LdsAcc<T, RPAIR>::type> st, const void* __restrict__ dinvp,
-              int64_t dstride, const float* __restrict__ lbound, int C, const T* __restrict__ Rg, int pdeg) {
+              int64_t dstride, const float* __restrict__ lbound, int C, const T* __restrict__ Rg, int pdeg,
+              const T* __restrict__ W1g) {
   static_assert(sizeof(T) == CF::VB, "rows of 64 values of the configuration's size");
   static_assert(!RPAIR || std::is_same<T, cplx<float>>::value, "a real pair is staged as 8-byte (float, float)");
   typedef typename sc<T>::acc A;
@@ -1170,11 +1176,16 @@ k_cg_spmm_lds(int64_t nblk, const int32_t* __restrict__ brow, const int32_t* __r
   A pv = acc_zero(A());  // RPAIR: .x column 2l, .y column 2l+1; POLY: r.z
   A pv2 = acc_zero(A());  // POLY: r.r
   typedef typename RealOf<T>::type R1;
-  R1 pc0 = 0, pc1 = 0, pc2 = 0;
+  R1 pc0 = 0, pc1 = 0, pc2 = 0, pc3 = 0;
   const int psys = RPAIR ? cbase / 64 + (second ? 1 : 0) : 0;  // POLY: this lane's system
   if constexpr (POLY) {
     const R1 L = (R1)lbound[psys];
-    if (pdeg == 2) {
+    if (pdeg == 3) {
+      pc0 = (R1)P3_0 / L;
+      pc1 = (R1)(-P3_1) / (L * L);
+      pc2 = (R1)P3_2 / (L * L * L);
+      pc3 = (R1)(-P3_3) / (L * L * L * L);
+    } else if (pdeg == 2) {
       pc0 = (R1)P2_0 / L;
       pc1 = (R1)(-P2_1) / (L * L);
       pc2 = (R1)P2_2 / (L * L * L);
@@ -1318,7 +1329,10 @@ k_cg_spmm_lds(int64_t nblk, const int32_t* __restrict__ brow, const int32_t* __r
           const int64_t ro = (int64_t)(r0 + i) * rs + coff + lane;
           const T d = ((const T*)dinvp)[r0 + i];
           T z, rv;
-          if (pdeg == 2) {  // pown is this row's w1 = (A D^-1 r), acc its (A D^-1 w1); r from HBM
+          if (pdeg == 3) {  // pown: w2 = (A D^-1 w1), acc: (A D^-1 w2); r and w1 from HBM
+            rv = Rg[ro];
+            z = d * (rscale(rv, pc0) + rscale(W1g[ro], pc1) + rscale(pown, pc2) + rscale(acc, pc3));
+          } else if (pdeg == 2) {  // pown is this row's w1 = (A D^-1 r), acc its (A D^-1 w1); r from HBM
             rv = Rg[ro];
             z = d * (rscale(rv, pc0) + rscale(pown, pc1) + rscale(acc, pc2));
           } else {  // pown is this row's r, acc its (A D^-1 r)
@@ -2322,7 +2336,8 @@ struct SolveBufs {
 static std::atomic<int> g_precond{0};
 
 extern "C" int rb_krylov_set_precond(int kind) {
-  if (kind != RB_PRECOND_JACOBI && kind != RB_PRECOND_POLY && kind != RB_PRECOND_POLY2) {
+  if (kind != RB_PRECOND_JACOBI && kind != RB_PRECOND_POLY && kind != RB_PRECOND_POLY2 &&
+      kind != RB_PRECOND_POLY3) {
     rb_set_error("rb_krylov_set_precond: unknown preconditioner %d", kind);
     return RB_ERR_ARG;
   }
@@ -2344,7 +2359,7 @@ static size_t ws_layout(int64_t n, int64_t C, int nsys, int mixed, size_t* o) {
     o[3] = L.take(n * nsys * sizeof(T64));
     state_layout<A>(L, C, nb, o + 8);
     if (g_precond.load() != RB_PRECOND_JACOBI) {
-      o[5] = L.take(n * C * sizeof(T64));               // y = D^-1 r, then z
+      o[5] = L.take(n * C * sizeof(T64) * (g_precond.load() == RB_PRECOND_POLY3 ? 2 : 1));  // y / w1 (, w2)
       o[7] = L.take(std::max<int64_t>(64, nsys * 4));   // Gershgorin bound per system
     }
   } else {
@@ -2357,7 +2372,7 @@ static size_t ws_layout(int64_t n, int64_t C, int nsys, int mixed, size_t* o) {
     state_layout<A>(L, C, nb, o + 20);
     o[6] = L.take(5 * MAXCHUNK * sizeof(int));  // replacement requests (2 parities) + chunk counters
     if (g_precond.load() != RB_PRECOND_JACOBI) {
-      o[5] = L.take(n * C * sizeof(T32));
+      o[5] = L.take(n * C * sizeof(T32) * (g_precond.load() == RB_PRECOND_POLY3 ? 2 : 1));
       o[7] = L.take(std::max<int64_t>(64, nsys * 4));
     }
   }
@@ -2715,12 +2730,12 @@ template <typename T, typename CF, bool RPAIR, typename A, bool POLY = false>
 static void launch_spmm_lds_cf(const LdsArgs& pl, dim3 grid, hipStream_t s, const T* recv, const T* P, T* Q,
                                int64_t rs, int64_t coff, int cbase, ColState<A> st, const void* dinvp = nullptr,
                                int64_t dstride = 0, const float* lbound = nullptr, int C = 64,
-                               const T* Rg = nullptr, int pdeg = 1) {
+                               const T* Rg = nullptr, int pdeg = 1, const T* W1g = nullptr) {
   (void)hipFuncSetAttribute((const void*)k_cg_spmm_lds<T, CF, RPAIR, POLY>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, CF::BYTES);
   plaunch(k_cg_spmm_lds<T, CF, RPAIR, POLY>, grid, dim3((LDS_NW + LDS_NP) * 64), CF::BYTES, s, pl.nblk, pl.brow,
           pl.bu, pl.ulist, pl.rinfo, pl.rinfo, pl.loff, recv, P, Q, rs, coff, cbase, st, dinvp, dstride, lbound, C,
-          Rg, pdeg);
+          Rg, pdeg, W1g);
 }
 // the polynomial preconditioner's product with z formed in the epilogue (one 64-column system,
 // pl.recv_ad packed): the residual R in, z out to Z, r.z and r.r partials into st.  W (degree 2):
@@ -2728,10 +2743,16 @@ static void launch_spmm_lds_cf(const LdsArgs& pl, dim3 grid, hipStream_t s, cons
 // overwritten by the second launch's), then the epilogue's launch on the staged w1 rows
 template <typename T, typename A>
 static void launch_spmm_lds_poly(const LdsArgs& pl, dim3 grid, hipStream_t s, const T* R, T* Z, const T* dinv,
-                                 const unsigned* lbound, ColState<A> st, T* W = nullptr) {
+                                 const unsigned* lbound, ColState<A> st, T* W = nullptr, T* W2 = nullptr) {
   typedef typename std::conditional<sizeof(T) == 8, LdsWide, LdsHalf>::type CF;
   if constexpr (sizeof(T) == 8 || sizeof(T) == 4) {
-    if (W) {
+    if (W && W2) {  // degree 3: w1, w2 = A D^-1 w1, then the epilogue's launch on w2 (r, w1 from HBM)
+      launch_spmm_lds_cf<T, CF, false, A, false>(pl, grid, s, (const T*)pl.recv_ad, R, W, 64, 0, 0, st);
+      launch_spmm_lds_cf<T, CF, false, A, false>(pl, grid, s, (const T*)pl.recv_ad, (const T*)W, W2, 64, 0, 0, st);
+      launch_spmm_lds_cf<T, CF, false, A, true>(pl, grid, s, (const T*)pl.recv_ad, (const T*)W2, Z, 64, 0, 0, st,
+                                                (const void*)dinv, 0, (const float*)lbound, 64, R, 3,
+                                                (const T*)W);
+    } else if (W) {
       launch_spmm_lds_cf<T, CF, false, A, false>(pl, grid, s, (const T*)pl.recv_ad, R, W, 64, 0, 0, st);
       launch_spmm_lds_cf<T, CF, false, A, true>(pl, grid, s, (const T*)pl.recv_ad, (const T*)W, Z, 64, 0, 0, st,
                                                 (const void*)dinv, 0, (const float*)lbound, 64, R, 2);
@@ -2832,7 +2853,10 @@ static int run_cg(const Csr& m, int nsys, const T* vals, const T* dinv, const T*
   const bool spmm_pair = pair_sweep<T>(nsys, C, k) && !poly;
   const bool lds_poly = poly && lds && lds->recv_ad;
   // the degree-2 polynomial where the fused product applies (its w1 = A D^-1 r goes to YZ)
-  T* W2 = lds_poly && g_precond.load() == RB_PRECOND_POLY2 ? YZ : nullptr;
+  const int pdeg = lds_poly ? (g_precond.load() == RB_PRECOND_POLY3 ? 3 : g_precond.load() == RB_PRECOND_POLY2 ? 2 : 1)
+                            : 1;
+  T* W2 = pdeg >= 2 ? YZ : nullptr;                  // w1
+  T* W3 = pdeg == 3 ? YZ + (int64_t)m.n * C : nullptr;  // w2
   st.poly = ss.poly = poly ? 1 : 0;
   auto spmm = [&](const T* Pin, T* Qout) {
     if (lds)
@@ -2849,7 +2873,7 @@ static int run_cg(const Csr& m, int nsys, const T* vals, const T* dinv, const T*
       if (rcf) return rcf;
       hipLaunchKernelGGL(k_gersh<T>, dim3((unsigned)ceil_div(m.n, 256), nsys), dim3(256), 0, s, m, vals, lb);
       if (lds_poly) {  // z formed in the LDS product's epilogue, into the spent q
-        launch_spmm_lds_poly<T>(*lds, dim3(nbs), s, (const T*)w.R, w.Q, dinv, lb, ss, W2);
+        launch_spmm_lds_poly<T>(*lds, dim3(nbs), s, (const T*)w.R, w.Q, dinv, lb, ss, W2, W3);
         hipLaunchKernelGGL(k_fin_rho<A>, dim3(C), dim3(FB), 0, s, C, ss);
         DISPATCH_GN(C, L_PUP, m.n, dinv, X, w.R, w.P, C, k, (TW*)nullptr, st, (const T*)w.Q);
       } else {
@@ -2885,7 +2909,7 @@ static int run_cg(const Csr& m, int nsys, const T* vals, const T* dinv, const T*
       bool zlds = false;       // its r.z, r.r partials on the LDS grid
       if (lds_poly) {          // w = A D^-1 r with z formed in the epilogue, into the spent q
         ProfScope ps(RB_PROF_POLY, s);
-        launch_spmm_lds_poly<T>(*lds, dim3(nbs), s, (const T*)w.R, w.Q, dinv, lb, ss, W2);
+        launch_spmm_lds_poly<T>(*lds, dim3(nbs), s, (const T*)w.R, w.Q, dinv, lb, ss, W2, W3);
         zin = w.Q;
         zlds = true;
       } else if (poly) {  // w = A y into the spent q, then z and the r.z, r.r partials
@@ -3112,7 +3136,10 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
   // with the plan's second record set (A D^-1) the preconditioner's product stages r itself
   const bool lds_poly = poly && lds_one && plan->recv_sets >= 2;
   // the degree-2 polynomial where the fused product applies (its w1 = A D^-1 r goes to YZ)
-  T32* W2 = lds_poly && g_precond.load() == RB_PRECOND_POLY2 ? YZ : nullptr;
+  const int pdeg = lds_poly ? (g_precond.load() == RB_PRECOND_POLY3 ? 3 : g_precond.load() == RB_PRECOND_POLY2 ? 2 : 1)
+                            : 1;
+  T32* W2 = pdeg >= 2 ? YZ : nullptr;              // w1
+  T32* W3 = pdeg == 3 ? YZ + n * C : nullptr;      // w2
   if (lds_poly) {
     la.recv_ad = (const T32*)plan->recv + nrec_set;
     hipLaunchKernelGGL(k_lds_pack_ad<T32>, dim3(ew_grid(nrec_set)), dim3(256), 0, s, nrec_set, plan->rmap, A32,
@@ -3135,7 +3162,7 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
     if constexpr (std::is_same<T32, float>::value)
       launch_spmm_lds_pairs_poly(la, nrec_set, dim3(nbs), s, nsys, n, (const float*)w.R, (float*)w.Q,
                                  (const float*)w.dinv, lb, sz,
-                                 g_precond.load() == RB_PRECOND_POLY2 ? (float*)YZ : (float*)nullptr);
+                                 g_precond.load() != RB_PRECOND_POLY ? (float*)YZ : (float*)nullptr);
   };
   if (poly) {  // z0 = M r0, rho = r0.z0, p0 = z0 (P is zero, alpha and beta 0)
     rc = rb_fill_i32((int32_t*)lb, 0, nsys, s);
@@ -3144,7 +3171,7 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
     if (lds_poly) {  // z formed in the LDS product's epilogue, into the spent q
       ColState<A> sz = si;
       sz.nb = (int)nbs;
-      launch_spmm_lds_poly<T32>(la, dim3(nbs), s, (const T32*)w.R, w.Q, (const T32*)w.dinv, lb, sz, W2);
+      launch_spmm_lds_poly<T32>(la, dim3(nbs), s, (const T32*)w.R, w.Q, (const T32*)w.dinv, lb, sz, W2, W3);
       hipLaunchKernelGGL(k_fin_rho<A>, dim3(C), dim3(FB), 0, s, C, sz);
       DISPATCH_GN(C, L_PUP, m.n, w.dinv, D, w.R, w.P, C, k, X, si, (const T32*)w.Q);
     } else if (pair_poly) {  // the same with the pairs' product
@@ -3209,7 +3236,7 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
       if (lds_poly) {            // w = A D^-1 r with z formed in the epilogue, into the spent q
         sz.nb = (int)nbs;
         ProfScope ps(RB_PROF_POLY, s);
-        launch_spmm_lds_poly<T32>(la, dim3(nbs), s, (const T32*)w.R, w.Q, (const T32*)w.dinv, lb, sz, W2);
+        launch_spmm_lds_poly<T32>(la, dim3(nbs), s, (const T32*)w.R, w.Q, (const T32*)w.dinv, lb, sz, W2, W3);
         zin = w.Q;
       } else if (pair_poly) {  // the same, one launch per system pair
         sz.nb = (int)nbs;

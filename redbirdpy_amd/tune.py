@@ -16,8 +16,8 @@ how the GPU runs it:
   (rb_krylov_set_precond): one more SpMM per iteration for ~1.8x fewer
   iterations on 3-D FEM systems (real 64-column systems keep the chunk-pair
   layout with the LDS-staged pair SpMM, else run in the one-column sweep), or
-  the degree-2 one on the fused one-system LDS product (two more products for
-  ~2.7x fewer iterations than Jacobi);
+  the degree-2 and degree-3 ones on the fused LDS products (two or three more
+  products for ~2.8x or ~3.7x fewer iterations than Jacobi);
 
 ``tune_system`` times each candidate on the system itself with full solves to
 the requested tolerance (columns converge at different iterations and the
@@ -48,8 +48,9 @@ _CACHE_FILE = None
 
 GRID_CAPS = (0, 512, 256)
 CHECK_EVERY = (8, 16, 32)
-PRECONDS = (_abi.RB_PRECOND_JACOBI, _abi.RB_PRECOND_POLY, _abi.RB_PRECOND_POLY2)
-PRECOND_NAMES = {_abi.RB_PRECOND_JACOBI: "jacobi", _abi.RB_PRECOND_POLY: "poly", _abi.RB_PRECOND_POLY2: "poly2"}
+PRECONDS = (_abi.RB_PRECOND_JACOBI, _abi.RB_PRECOND_POLY, _abi.RB_PRECOND_POLY2, _abi.RB_PRECOND_POLY3)
+PRECOND_NAMES = {_abi.RB_PRECOND_JACOBI: "jacobi", _abi.RB_PRECOND_POLY: "poly", _abi.RB_PRECOND_POLY2: "poly2",
+                 _abi.RB_PRECOND_POLY3: "poly3"}
 DEFAULT = {"lds": True, "grid": 0, "check_every": 16, "precond": _abi.RB_PRECOND_JACOBI}
 
 
@@ -188,8 +189,8 @@ def tune_system(sysm, B, k, precision="mixed", tol=1e-10, maxiter=2000, reps=2, 
             cands += [{"lds": True, "grid": 0, "precond": pc} for pc in (_abi.RB_PRECOND_POLY, _abi.RB_PRECOND_POLY2)]
     else:
         cands += [dict(c, precond=_abi.RB_PRECOND_POLY) for c in spmms]
-        if plan_ok:  # degree 2 exists on the fused LDS product only
-            cands.append({"lds": True, "grid": 0, "precond": _abi.RB_PRECOND_POLY2})
+        if plan_ok:  # degrees 2 and 3 exist on the fused LDS product only
+            cands += [{"lds": True, "grid": 0, "precond": pc} for pc in (_abi.RB_PRECOND_POLY2, _abi.RB_PRECOND_POLY3)]
     def solve_time(ch):
         with applied(ch):
             return _timed(lambda: _gpu.solve_system(sysm, B, k, precision, tol, maxiter, X0=X0, tuned=ch), reps)

@@ -27,7 +27,7 @@ def _both(cfg, precision, monkeypatch, tol=1e-12):
 
     monkeypatch.setenv("RB_TUNE_CACHE", "/nonexistent/dir/tune.json")  # no cached choices apply
     out = {}
-    for pc in (0, 1, 2):
+    for pc in (0, 1, 2, 3):
         c1, _ = rb.meshprep(copy.deepcopy(cfg))
         with tune.applied({"precond": pc}):
             d, f = rb.runforward(c1, precision=precision, tol=tol)
@@ -49,14 +49,14 @@ def test_poly_fd_64col(precision, monkeypatch):
     cfg = workloads.fd_box(2.0)
     out = _both(cfg, precision, monkeypatch)
     d3, f3 = _oracle(cfg)
-    for pc in (0, 1, 2):
+    for pc in (0, 1, 2, 3):
         d, f, _ = out[pc]
         assert _rel(f, f3) < 1e-10 and _relmax(d, d3) < 1e-8
     assert out[1][2] < 0.8 * out[0][2]
-    if precision == "mixed":  # complex64: the fused LDS product, where degree 2 applies
-        assert out[2][2] < 0.85 * out[1][2]
+    if precision == "mixed":  # complex64: the fused LDS product, where degrees 2 and 3 apply
+        assert out[2][2] < 0.85 * out[1][2] and out[3][2] < out[2][2]
     else:  # complex128 runs the sweep: degree 1
-        assert out[2][2] == out[1][2]
+        assert out[3][2] == out[2][2] == out[1][2]
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
@@ -66,11 +66,12 @@ def test_poly_cw_64col(precision, monkeypatch):
     cfg = workloads.jac_box(2.0)
     out = _both(cfg, precision, monkeypatch)
     d3, f3 = _oracle(cfg)
-    for pc in (1, 2):
+    for pc in (1, 2, 3):
         d, f, _ = out[pc]
         assert _rel(f, f3) < 1e-10 and _relmax(d, d3) < 1e-8
     assert out[1][2] < 0.8 * out[0][2]
-    assert out[2][2] < 0.85 * out[1][2]  # real float64 and float32: the fused LDS product
+    # real float64 and float32: the fused LDS product
+    assert out[2][2] < 0.85 * out[1][2] and out[3][2] < out[2][2]
 
 
 def test_poly_two_complex_systems_one_solve(monkeypatch):
@@ -100,15 +101,15 @@ def test_poly_real_64_column_systems_in_pairs(precision, nw, monkeypatch):
                                                 [0.02, 1.2, 0, 1.37]]) for i in range(nw)}
     out = _both(cfg, precision, monkeypatch)
     d3, f3 = _oracle(cfg)
-    for pc in (0, 1, 2):
+    for pc in (0, 1, 2, 3):
         d, f, _ = out[pc]
         for wv in d3:
             assert _rel(f[wv], f3[wv]) < 1e-10 and _relmax(d[wv], d3[wv]) < 1e-8
     assert out[1][2] < 0.8 * out[0][2]
-    if precision == "mixed":  # degree 2 on the pair product
-        assert out[2][2] < 0.85 * out[1][2]
+    if precision == "mixed":  # degree 2 on the pair product (and for a degree-3 request)
+        assert out[2][2] < 0.85 * out[1][2] and out[3][2] == out[2][2]
     else:
-        assert out[2][2] == out[1][2]
+        assert out[3][2] == out[2][2] == out[1][2]
     if precision == "mixed":
         monkeypatch.setenv("RB_LDS_SPMM", "0")
         sweep = _both(cfg, precision, monkeypatch)
