@@ -40,7 +40,7 @@ extern "C" {
 #define RB_PRECOND_JACOBI 0 /* z = D^-1 r */
 #define RB_PRECOND_POLY 1   /* z = p(D^-1 A) D^-1 r, p of degree 1 positive on a Gershgorin bound of the spectrum */
 #define RB_PRECOND_POLY2 2  /* p of degree 2 where a fused LDS product applies, else degree 1 */
-#define RB_PRECOND_POLY3 3  /* p of degree 3 on the one-system LDS product, 2 on the pair one, else 1 */
+#define RB_PRECOND_POLY3 3  /* p of degree 3 where a fused LDS product applies, else degree 1 */
 
 /* dtype codes */
 #define RB_F64 0

@@ -1310,7 +1310,12 @@ k_cg_spmm_lds(int64_t nblk, const int32_t* __restrict__ brow, const int32_t* __r
           const int64_t ro = (int64_t)(r0 + i) * rs + coff + lane;
           const float d = ((const float*)dinvp)[(int64_t)psys * dstride + r0 + i];
           T z, rv;
-          if (pdeg == 2) {  // pown: this row's w1 = (A D^-1 r), acc: (A D^-1 w1); r from HBM
+          if (pdeg == 3) {  // pown: w2 = (A D^-1 w1), acc: (A D^-1 w2); r and w1 from HBM
+            rv = Rg[ro];
+            const T w1 = W1g[ro];
+            z.x = d * (rv.x * pc0 + w1.x * pc1 + pown.x * pc2 + acc.x * pc3);
+            z.y = d * (rv.y * pc0 + w1.y * pc1 + pown.y * pc2 + acc.y * pc3);
+          } else if (pdeg == 2) {  // pown: this row's w1 = (A D^-1 r), acc: (A D^-1 w1); r from HBM
             rv = Rg[ro];
             z.x = d * (rv.x * pc0 + pown.x * pc1 + acc.x * pc2);
             z.y = d * (rv.y * pc0 + pown.y * pc1 + acc.y * pc2);
@@ -2789,12 +2794,21 @@ static void launch_spmm_lds_pairs(const LdsArgs& pl, int64_t nrec_set, dim3 grid
 template <typename A>
 static void launch_spmm_lds_pairs_poly(const LdsArgs& pl, int64_t nrec_set, dim3 grid, hipStream_t s, int nsys,
                                        int64_t n, const float* R, float* Z, const float* dinv,
-                                       const unsigned* lbound, ColState<A> st, float* W = nullptr) {
+                                       const unsigned* lbound, ColState<A> st, float* W = nullptr,
+                                       float* W2 = nullptr) {
   typedef cplx<float> c2;
   const int C = 64 * nsys;
   for (int p = 0; p < nsys / 2; ++p) {
     const c2* rec = (const c2*)pl.recv_ad + p * nrec_set;
-    if (W) {
+    if (W && W2) {  // degree 3
+      launch_spmm_lds_cf<c2, LdsWide, true, A, false>(pl, grid, s, rec, (const c2*)R, (c2*)W, C / 2, 64 * p,
+                                                      128 * p, st);
+      launch_spmm_lds_cf<c2, LdsWide, true, A, false>(pl, grid, s, rec, (const c2*)W, (c2*)W2, C / 2, 64 * p,
+                                                      128 * p, st);
+      launch_spmm_lds_cf<c2, LdsWide, true, A, true>(pl, grid, s, rec, (const c2*)W2, (c2*)Z, C / 2, 64 * p,
+                                                     128 * p, st, (const void*)dinv, n, (const float*)lbound, C,
+                                                     (const c2*)R, 3, (const c2*)W);
+    } else if (W) {
       launch_spmm_lds_cf<c2, LdsWide, true, A, false>(pl, grid, s, rec, (const c2*)R, (c2*)W, C / 2, 64 * p,
                                                       128 * p, st);
       launch_spmm_lds_cf<c2, LdsWide, true, A, true>(pl, grid, s, rec, (const c2*)W, (c2*)Z, C / 2, 64 * p, 128 * p,
@@ -3156,13 +3170,14 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
                            (cplx<float>*)la.recv_ad + pp * nrec_set);
     }
   }
-  // the pair-form product of the polynomial preconditioner (float32 systems only; degree 2 with
-  // its w1 in YZ)
+  // the pair-form product of the polynomial preconditioner (float32 systems only; degrees 2 and 3
+  // with w1 in YZ and w2 after it)
   auto pair_poly_product = [&](ColState<A> sz) {
     if constexpr (std::is_same<T32, float>::value)
       launch_spmm_lds_pairs_poly(la, nrec_set, dim3(nbs), s, nsys, n, (const float*)w.R, (float*)w.Q,
                                  (const float*)w.dinv, lb, sz,
-                                 g_precond.load() != RB_PRECOND_POLY ? (float*)YZ : (float*)nullptr);
+                                 g_precond.load() != RB_PRECOND_POLY ? (float*)YZ : (float*)nullptr,
+                                 g_precond.load() == RB_PRECOND_POLY3 ? (float*)YZ + n * C : (float*)nullptr);
   };
   if (poly) {  // z0 = M r0, rho = r0.z0, p0 = z0 (P is zero, alpha and beta 0)
     rc = rb_fill_i32((int32_t*)lb, 0, nsys, s);

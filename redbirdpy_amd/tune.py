@@ -186,7 +186,8 @@ def tune_system(sysm, B, k, precision="mixed", tol=1e-10, maxiter=2000, reps=2, 
     if pairs:
         cands += [{"lds": False, "grid": g, "precond": _abi.RB_PRECOND_POLY} for g in (GRID_CAPS if flat else (0,))]
         if plan_ok:
-            cands += [{"lds": True, "grid": 0, "precond": pc} for pc in (_abi.RB_PRECOND_POLY, _abi.RB_PRECOND_POLY2)]
+            cands += [{"lds": True, "grid": 0, "precond": pc}
+                      for pc in (_abi.RB_PRECOND_POLY, _abi.RB_PRECOND_POLY2, _abi.RB_PRECOND_POLY3)]
     else:
         cands += [dict(c, precond=_abi.RB_PRECOND_POLY) for c in spmms]
         if plan_ok:  # degrees 2 and 3 exist on the fused LDS product only
