@@ -1,4 +1,5 @@
 """Autotuner cache (no GPU): keys, store/lookup through the JSON file, clear."""
+import pytest
 
 import json
 from types import SimpleNamespace
@@ -33,3 +34,23 @@ def test_tune_key_names_every_field():
     from redbirdpy_amd import tune
 
     assert tune.key("G", 5, 7, 64, 4, False, "fp64") == "G|n=5|nnz=7|k=64|nsys=4|real|fp64"
+
+
+def test_applied_sets_and_restores_each_preconditioner():
+    """tune.applied sets the process-wide preconditioner a choice names (host-side settings, no GPU
+    work) and restores the previous one; every name the bench reports has a setting."""
+    from redbirdpy_amd import _abi, tune
+
+    L = _abi.lib()
+    assert set(tune.PRECOND_NAMES) == set(tune.PRECONDS)
+    before = (L.rb_krylov_get_precond(), L.rb_krylov_get_sweep_grid())
+    for pc in tune.PRECONDS:
+        with tune.applied({"precond": pc, "grid": 256}):
+            assert L.rb_krylov_get_precond() == pc
+            assert L.rb_krylov_get_sweep_grid() == 256
+        assert (L.rb_krylov_get_precond(), L.rb_krylov_get_sweep_grid()) == before
+    with tune.applied({"grid": 512}):  # a choice without a preconditioner leaves it alone
+        assert L.rb_krylov_get_precond() == before[0]
+    with pytest.raises(_abi.RbError):
+        _abi.call("rb_krylov_set_precond", max(tune.PRECONDS) + 1)
+    assert L.rb_krylov_get_precond() == before[0]
