@@ -1,8 +1,10 @@
-"""Autotuner cache (no GPU): keys, store/lookup through the JSON file, clear."""
-import pytest
+"""Autotuner cache (no GPU): keys, store/lookup through the JSON file, clear; the preconditioner
+settings it applies."""
 
 import json
 from types import SimpleNamespace
+
+import pytest
 
 
 def test_tune_cache_round_trip(tmp_path, monkeypatch):
