@@ -2353,7 +2353,8 @@ extern "C" int rb_krylov_set_precond(int kind) {
 extern "C" int rb_krylov_get_precond(void) { return g_precond.load(); }
 
 template <typename T64, typename T32>
-static size_t ws_layout(int64_t n, int64_t C, int nsys, int mixed, size_t* o) {
+// (precond: the preconditioner setting the solve runs with, read once per solve)
+static size_t ws_layout(int64_t n, int64_t C, int nsys, int mixed, size_t* o, int precond) {
   typedef typename sc<T64>::acc A;
   Layout L;
   int64_t nb = grid_for(n, (int)C);
@@ -2363,8 +2364,8 @@ static size_t ws_layout(int64_t n, int64_t C, int nsys, int mixed, size_t* o) {
     o[2] = L.take(n * C * sizeof(T64));
     o[3] = L.take(n * nsys * sizeof(T64));
     state_layout<A>(L, C, nb, o + 8);
-    if (g_precond.load() != RB_PRECOND_JACOBI) {
-      o[5] = L.take(n * C * sizeof(T64) * (g_precond.load() == RB_PRECOND_POLY3 ? 2 : 1));  // y / w1 (, w2)
+    if (precond != RB_PRECOND_JACOBI) {
+      o[5] = L.take(n * C * sizeof(T64) * (precond == RB_PRECOND_POLY3 ? 2 : 1));  // y / w1 (, w2)
       o[7] = L.take(std::max<int64_t>(64, nsys * 4));   // Gershgorin bound per system
     }
   } else {
@@ -2376,8 +2377,8 @@ static size_t ws_layout(int64_t n, int64_t C, int nsys, int mixed, size_t* o) {
     state_layout<A>(L, C, nb, o + 8);
     state_layout<A>(L, C, nb, o + 20);
     o[6] = L.take(5 * MAXCHUNK * sizeof(int));  // replacement requests (2 parities) + chunk counters
-    if (g_precond.load() != RB_PRECOND_JACOBI) {
-      o[5] = L.take(n * C * sizeof(T32) * (g_precond.load() == RB_PRECOND_POLY3 ? 2 : 1));
+    if (precond != RB_PRECOND_JACOBI) {
+      o[5] = L.take(n * C * sizeof(T32) * (precond == RB_PRECOND_POLY3 ? 2 : 1));
       o[7] = L.take(std::max<int64_t>(64, nsys * 4));
     }
   }
@@ -2849,7 +2850,8 @@ static LdsArgs lds_args(const rb_spmm_plan* plan) {
 template <typename T>
 static int run_cg(const Csr& m, int nsys, const T* vals, const T* dinv, const T* B, T* X, int use_x0,
                   SolveBufs<T> w, int C, int k, ColState<typename sc<T>::acc> st, int check_every, hipStream_t s,
-                  int* iters_done, const LdsArgs* lds = nullptr, T* YZ = nullptr, unsigned* lb = nullptr) {
+                  int* iters_done, const LdsArgs* lds = nullptr, T* YZ = nullptr, unsigned* lb = nullptr,
+                  int precond = RB_PRECOND_JACOBI) {
   typedef typename sc<T>::acc A;
   typedef T TW;
   int64_t nb = grid_for(m.n, C);
@@ -2867,8 +2869,7 @@ static int run_cg(const Csr& m, int nsys, const T* vals, const T* dinv, const T*
   const bool spmm_pair = pair_sweep<T>(nsys, C, k) && !poly;
   const bool lds_poly = poly && lds && lds->recv_ad;
   // the degree-2 polynomial where the fused product applies (its w1 = A D^-1 r goes to YZ)
-  const int pdeg = lds_poly ? (g_precond.load() == RB_PRECOND_POLY3 ? 3 : g_precond.load() == RB_PRECOND_POLY2 ? 2 : 1)
-                            : 1;
+  const int pdeg = lds_poly ? (precond == RB_PRECOND_POLY3 ? 3 : precond == RB_PRECOND_POLY2 ? 2 : 1) : 1;
   T* W2 = pdeg >= 2 ? YZ : nullptr;                  // w1
   T* W3 = pdeg == 3 ? YZ + (int64_t)m.n * C : nullptr;  // w2
   st.poly = ss.poly = poly ? 1 : 0;
@@ -2959,7 +2960,10 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
   const int C = nsys * k;
   const int64_t n = m.n;
   size_t o[32];
-  size_t need = ws_layout<T64, T32>(n, C, nsys, precision, o);
+  // the preconditioner setting, once: the workspace layout and every choice below follow it even
+  // when another thread changes the setting during the solve
+  const int precond = g_precond.load();
+  size_t need = ws_layout<T64, T32>(n, C, nsys, precision, o, precond);
   if (work_bytes < need || !work) {
     rb_set_error("rb_krylov_solve: workspace %zu bytes < required %zu", work_bytes, need);
     return RB_ERR_SIZE;
@@ -2991,7 +2995,7 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
       hipLaunchKernelGGL(k_lds_pack<T64>, dim3(ew_grid(nr)), dim3(256), 0, s, nr, plan->rmap, A64, (T64*)plan->recv);
       RB_CHECK_LAUNCH();
     }
-    const bool poly = g_precond.load() != RB_PRECOND_JACOBI;
+    const bool poly = precond != RB_PRECOND_JACOBI;
     if (use_lds && poly && plan->recv_sets >= 2) {  // the second record set: A D^-1
       const int64_t nr = plan->nrec + RB_LDS_RECPAD;
       la.recv_ad = (const T64*)plan->recv + nr;
@@ -3000,7 +3004,7 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
       RB_CHECK_LAUNCH();
     }
     rc = run_cg<T64>(m, nsys, A64, w.dinv, B, X, use_x0, w, C, k, st, check_every, s, &its, use_lds ? &la : nullptr,
-                     poly ? (T64*)(base + o[5]) : nullptr, poly ? (unsigned*)(base + o[7]) : nullptr);
+                     poly ? (T64*)(base + o[5]) : nullptr, poly ? (unsigned*)(base + o[7]) : nullptr, precond);
     if (rc) return rc;
     RB_HIP(hipMemcpyAsync(hrr, st.rr, C * sizeof(double), hipMemcpyDeviceToHost, s));
     RB_HIP(hipMemcpyAsync(hbb, st.bb, C * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -3049,7 +3053,7 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
   si.ru_cnt = ru_req + 2 * MAXCHUNK;
   // the polynomial preconditioner (real 64-column systems then run in the one-column layout
   // rather than in chunk pairs)
-  const bool poly = g_precond.load() != RB_PRECOND_JACOBI;
+  const bool poly = precond != RB_PRECOND_JACOBI;
   si.poly = poly ? 1 : 0;
   T32* YZ = poly ? (T32*)(base + o[5]) : nullptr;
   unsigned* lb = poly ? (unsigned*)(base + o[7]) : nullptr;
@@ -3150,8 +3154,7 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
   // with the plan's second record set (A D^-1) the preconditioner's product stages r itself
   const bool lds_poly = poly && lds_one && plan->recv_sets >= 2;
   // the degree-2 polynomial where the fused product applies (its w1 = A D^-1 r goes to YZ)
-  const int pdeg = lds_poly ? (g_precond.load() == RB_PRECOND_POLY3 ? 3 : g_precond.load() == RB_PRECOND_POLY2 ? 2 : 1)
-                            : 1;
+  const int pdeg = lds_poly ? (precond == RB_PRECOND_POLY3 ? 3 : precond == RB_PRECOND_POLY2 ? 2 : 1) : 1;
   T32* W2 = pdeg >= 2 ? YZ : nullptr;              // w1
   T32* W3 = pdeg == 3 ? YZ + n * C : nullptr;      // w2
   if (lds_poly) {
@@ -3176,8 +3179,8 @@ static int solve_impl(const Csr& m, int nsys, int k, const void* vals64, const v
     if constexpr (std::is_same<T32, float>::value)
       launch_spmm_lds_pairs_poly(la, nrec_set, dim3(nbs), s, nsys, n, (const float*)w.R, (float*)w.Q,
                                  (const float*)w.dinv, lb, sz,
-                                 g_precond.load() != RB_PRECOND_POLY ? (float*)YZ : (float*)nullptr,
-                                 g_precond.load() == RB_PRECOND_POLY3 ? (float*)YZ + n * C : (float*)nullptr);
+                                 precond != RB_PRECOND_POLY ? (float*)YZ : (float*)nullptr,
+                                 precond == RB_PRECOND_POLY3 ? (float*)YZ + n * C : (float*)nullptr);
   };
   if (poly) {  // z0 = M r0, rho = r0.z0, p0 = z0 (P is zero, alpha and beta 0)
     rc = rb_fill_i32((int32_t*)lb, 0, nsys, s);
@@ -3307,9 +3310,9 @@ extern "C" int rb_krylov_workspace(int64_t n, int nsys, int k, int is_complex, i
   if (!bytes || n <= 0 || nsys <= 0 || k <= 0) return RB_ERR_ARG;
   size_t o[32];
   if (is_complex)
-    *bytes = ws_layout<cplx<double>, cplx<float>>(n, (int64_t)nsys * k, nsys, precision, o);
+    *bytes = ws_layout<cplx<double>, cplx<float>>(n, (int64_t)nsys * k, nsys, precision, o, g_precond.load());
   else
-    *bytes = ws_layout<double, float>(n, (int64_t)nsys * k, nsys, precision, o);
+    *bytes = ws_layout<double, float>(n, (int64_t)nsys * k, nsys, precision, o, g_precond.load());
   return RB_OK;
 }
 
