@@ -308,21 +308,23 @@ int64_t rb_krylov_get_sweep_grid(void);
 
 /*
  * The preconditioner of every later CG/COCG solve (RB_PRECOND_*), process-wide
- * like the sweep grid; rb_krylov_workspace sizes the workspace for the one
- * set when it is called (the polynomial one takes one more vector of the
- * solve's width).  RB_PRECOND_POLY applies z = p(D^-1 A) D^-1 r with
+ * like the sweep grid.  A solve reads the setting once when it starts;
+ * rb_krylov_workspace sizes the workspace for the one set when it is called
+ * (the polynomial ones take one more vector of the solve's width, degree 3
+ * two).  RB_PRECOND_POLY applies z = p(D^-1 A) D^-1 r with
  * p(x) = (3 / L)(1 - 0.8 x / L), L the Gershgorin bound of D^-1 A per system:
- * one more q = A y product per iteration for ~1.8x fewer iterations on 3-D
- * FEM systems.  RB_PRECOND_POLY2: p(x) = (13.54 - 33.78 t + 21.86 t^2) / L,
- * t = x / L (1 - t p(t) the Chebyshev polynomial of degree 3 smallest on
- * [0.03, 1]), two more products for ~2.8x fewer iterations, on the LDS paths with the
- * A D^-1 records (one system, or float32 systems in pairs; degree 1 on the
- * others).  Real systems of 64 columns each then run in the one-column
- * layout instead of chunk pairs.  With an LDS plan holding two record sets
- * (rb_spmm_plan.recv_sets >= 2) the product is fused into the LDS-staged SpMM.
- * No counterpart in the reference (its iterative solvers take a caller's M,
- * pyamg's smoothed aggregation for 'cg+amg' / 'minres+amg', solver.py:409-414
- * and :697-700, or none); the autotuner chooses per shape.
+ * one more product per iteration for ~1.8x fewer iterations on 3-D FEM
+ * systems.  RB_PRECOND_POLY2 / POLY3: p of degree 2 / 3 in t = x / L with
+ * 1 - t p(t) the Chebyshev polynomial of degree 3 / 4 smallest on [0.03, 1]
+ * (13.54 - 33.78 t + 21.86 t^2; 20.45 - 91.18 t + 138.51 t^2 - 67.24 t^3,
+ * over L), two / three more products for ~2.8x / ~3.7x fewer iterations, on
+ * the fused LDS products; degree 1 elsewhere.  The fused products need the
+ * plan's A D^-1 records: rb_spmm_plan.recv_sets >= 2 for one system, >= nsys
+ * for float32 systems in pairs (without them, and for float64 systems of 64
+ * columns each, the pairs run in the one-column layout).  No counterpart in
+ * the reference (its iterative solvers take a caller's M, pyamg's smoothed
+ * aggregation for 'cg+amg' / 'minres+amg', solver.py:409-414 and :697-700,
+ * or none); the autotuner chooses per shape.
  */
 int rb_krylov_set_precond(int kind);
 int rb_krylov_get_precond(void);
