@@ -1053,7 +1053,8 @@ struct LdsCfg {
 // three buffers with blocks of 80 rows (copies two blocks ahead) measured slower in the solve (68.8
 // against 63.5 us per launch at 115k nodes): the smaller unions load ~35% more rows.
 typedef LdsCfg<RB_LDS_UMAX, RB_LDS_RECMAX, 2> LdsWide;
-// one real float32 system of 64 columns: 256-byte rows, four to a copy, on the same plans
+// one real float32 system of 64 columns: 256-byte rows, four to a copy, on the same plans (a third
+// buffer fits in the LDS here but measured 4.5% slower on the 2.6M-node batch, two runs on one box)
 typedef LdsCfg<RB_LDS_UMAX, RB_LDS_RECMAX, 2, 4> LdsHalf;
 
 // s_waitcnt vmcnt(N), the other counters untouched (gfx9 encoding: vmcnt in bits 3:0 and 15:14)
