@@ -129,9 +129,10 @@ def updateprop(cfg, wv=None, extin=None):
             else:
                 newp = np.array([[mua, musp, 0, nref]])
         elif hasattr(mua, "__len__"):
-            newp = np.empty((len(mua), tmpl.shape[1]), dtype=np.result_type(np.asarray(mua), tmpl))
+            # the template row repeated (a contiguous copy), then the mua column
+            dt = np.result_type(np.asarray(mua), tmpl)
+            newp = np.repeat(tmpl[1:2].astype(dt, copy=False), len(mua), axis=0)
             newp[:, 0] = mua
-            newp[:, 1:] = tmpl[1, 1:]
         else:
             newp = np.array([[mua] + list(tmpl[1, 1:])])
         out[w] = newp
